@@ -1,0 +1,207 @@
+/*
+ * tcad_hip.h -- C ABI of libtcadhip.so, the MI355X (gfx950) implementation
+ * of the coupled (psi, n, p) drift-diffusion Newton solve on structured
+ * 2D/3D tensor-product meshes.
+ *
+ * One handle ("batch") holds one or more independent devices that share
+ * every kernel launch: a single device is a batch of one.  All array
+ * arguments named `d_*` are DEVICE pointers (HIP global memory, e.g.
+ * torch tensors' data_ptr()); arguments named `h_*` are host pointers.
+ * Sizes are explicit.  No entry point frees caller memory; library
+ * workspaces live in the handle and are released by tcad_batch_destroy.
+ *
+ * Unknown layout at this boundary: node k = j*Nx + i (2D),
+ * k = kk*Nx*Ny + j*Nx + i (3D) (pytcad/pytcad/mesh2d.py:1-8,
+ * mesh3d.py:1-9); state vectors psi, n, p are separate FP64 planes of
+ * length N_total (the sum of the batch's node counts, devices
+ * concatenated in order); scaled units as in
+ * pytcad/pytcad/device2d.py:319-345.
+ *
+ * Return codes (pytcad/core/include/tcad/base/errors.hpp:21-67 mapping):
+ *   TCAD_OK                0
+ *   TCAD_E_INVALID        -1  invalid argument          -> ValueError
+ *   TCAD_E_INDEX          -2  index out of range        -> IndexError
+ *   TCAD_E_NONFINITE      -3  non-finite / singular     -> LinearSolveError
+ *   TCAD_E_NOT_CONVERGED  -4  Krylov did not converge   -> LinearSolveError
+ *   TCAD_E_HIP            -5  HIP runtime error         -> RuntimeError
+ *   TCAD_E_NO_DEVICE      -6  no HIP device             -> RuntimeError
+ * The message of the last failure on the calling thread is returned by
+ * tcad_last_error().
+ */
+#ifndef TCAD_HIP_H
+#define TCAD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCAD_OK 0
+#define TCAD_E_INVALID (-1)
+#define TCAD_E_INDEX (-2)
+#define TCAD_E_NONFINITE (-3)
+#define TCAD_E_NOT_CONVERGED (-4)
+#define TCAD_E_HIP (-5)
+#define TCAD_E_NO_DEVICE (-6)
+
+/* Linear-solve modes (pytcad/pytcad/linsolve.py:338 `_METHODS` analogue). */
+#define TCAD_LIN_BICGSTAB_F64 0   /* FP64 BiCGStab + node block-Jacobi       */
+#define TCAD_LIN_BICGSTAB_MIXED 1 /* FP32 BiCGStab + block-Jacobi in FP64 IR */
+
+typedef struct tcad_batch tcad_batch;
+
+/* Per-device description.  Host arrays are copied at creation. */
+typedef struct {
+  int32_t dim;              /* 2 or 3 */
+  int32_t nx, ny, nz;       /* nz = 1 in 2D */
+  /* scaled geometry (x/LD): spacings (n-1) and box widths (n) per axis */
+  const double *h_hx, *h_hy, *h_hz;
+  const double *h_dvx, *h_dvy, *h_dvz;
+  /* scalars, pytcad/pytcad/device2d.py:319-328 */
+  double VT, Ns, LD, J0, R0;
+  double Cn_auger, Cp_auger;
+  int32_t srh, auger;
+  /* lagged Lombardi surface mobility on the row-0 y-edges (2D only),
+   * pytcad/pytcad/device2d.py:612-642: T and the bulk mobilities of
+   * rows 0 and 1 (length nx each), NULL when off */
+  int32_t surface_mobility;
+  double T;
+  const double *h_mun_row0, *h_mun_row1, *h_mup_row0, *h_mup_row1;
+  /* contacts: ncontacts groups; node lists (device-local node index),
+   * ohmic values ln(n0/nie), n0, p0 per listed node (contacts.py:21-37) */
+  int32_t ncontacts;
+  const int32_t *h_contact_ptr;    /* ncontacts+1 offsets into the lists */
+  const int32_t *h_contact_nodes;
+  const double *h_contact_lnr, *h_contact_n0, *h_contact_p0;
+  /* gates: ngates groups; node lists, kappa*w and psi_b per listed node
+   * (device2d.py:1623-1637, device3d.py:939-947) */
+  int32_t ngates;
+  const int32_t *h_gate_ptr;
+  const int32_t *h_gate_nodes;
+  const double *h_gate_kw, *h_gate_psib;
+} tcad_device_desc;
+
+/* Static per-node / per-edge coefficient arrays of the whole batch, as
+ * DEVICE pointers, each of length N_total (edge arrays are node-shaped:
+ * entry k holds the edge from node k to its +axis neighbour, 0 where
+ * none).  The library keeps the pointers; the caller keeps the memory
+ * alive until tcad_batch_destroy. */
+typedef struct {
+  const double *d_C;        /* doping / Ns */
+  const double *d_nie_s;    /* n_ie / Ns */
+  const double *d_nie;      /* n_ie [cm^-3] */
+  const double *d_tau_n, *d_tau_p;
+  const double *d_dV;       /* scaled box volume */
+  double *d_an[3];          /* dn_edge / h per axis (x, y, z); row 0 of the y
+                               * array is rewritten each Newton iteration when
+                               * the surface-mobility model is on */
+  double *d_ap[3];          /* dp_edge / h per axis (same) */
+  const double *d_dlnnie[3];/* ln(nie_s[k+axis] / nie_s[k]) per axis */
+} tcad_static_fields;
+
+/* Create a batch of `ndev` devices; `stream` is a hipStream_t (NULL =
+ * default stream).  Replaces: Device2D.__init__ / Device3D.__init__
+ * device-resident setup (pytcad/pytcad/device2d.py:168-453,
+ * device3d.py:437-724). */
+int tcad_batch_create(const tcad_device_desc *h_devs, int32_t ndev,
+                      const tcad_static_fields *fields, void *stream,
+                      tcad_batch **out);
+int tcad_batch_destroy(tcad_batch *b);
+
+/* Total node count of the batch. */
+int64_t tcad_batch_nodes(const tcad_batch *b);
+
+/* Contact voltages [V] (sum of ncontacts over devices, batch order) and
+ * gate voltages [V] and flat-band voltages [V] (sum of ngates).
+ * Replaces the bc.V / bc.Vg assignment of solve_bias
+ * (pytcad/pytcad/device2d.py:1941-1947). */
+int tcad_set_bias(tcad_batch *b, const double *h_contact_V,
+                  const double *h_gate_Vg, const double *h_gate_Vfb);
+
+/* Coupled residual/Jacobian at (psi, n, p).  Writes F (3 planes of
+ * N_total, raw residual after contact/gate rows, as the reference's F)
+ * and, if d_Fnp != NULL, the raw pre-Dirichlet F_n + F_p per node.
+ * The eliminated, scaled Jacobian stays inside the handle.
+ * Replaces: Device2D._residual_jacobian (device2d.py:1402-1920),
+ * Device3D._residual_jacobian (device3d.py:1485-1978) and
+ * dirichlet.eliminate_csr (dirichlet.py:161-191). */
+int tcad_assemble(tcad_batch *b, const double *d_psi, const double *d_n,
+                  const double *d_p, double *d_F, double *d_Fnp);
+
+/* Copy the last assembled Jacobian to the host in stencil-block form:
+ * h_blocks[e * N_total + k] for e < 9 + 5*2*dim_max (entries 0..8: the
+ * 3x3 node block, then per neighbour direction x-,x+,y-,y+,z-,z+ the
+ * five coupling entries (psi,psi) (n,psi) (n,n) (p,psi) (p,p)), of the
+ * ELIMINATED matrix scaled as R * J * Cc; h_rowscale (3*N_total, R) and
+ * h_rhs (3*N_total, the scaled eliminated right-hand side R*(-F - J g)).
+ * Column scale Cc is (1, n, p) per node.  Test/inspection entry. */
+int tcad_export_jacobian(tcad_batch *b, double *h_blocks, double *h_rowscale,
+                         double *h_rhs, int32_t *h_nentries);
+
+/* Solve the last assembled Newton system; writes the unscaled update du
+ * (3 planes of N_total) to d_du.  h_iters / h_resid (length ndev)
+ * receive Krylov iterations and final relative residual per device.
+ * Replaces: linsolve.solve_linear (pytcad/pytcad/linsolve.py:725-985)
+ * for the node-block-Jacobi Krylov methods. */
+int tcad_solve_linear(tcad_batch *b, int32_t mode, double rtol, int32_t maxiter,
+                      double *d_du, int32_t *h_iters, double *h_resid);
+
+/* Equilibrium Poisson Newton from the bulk-neutral guess (or from the
+ * given psi if from_guess == 0).  Writes psi, n, p.  h_iters receives
+ * Newton iterations per device.  Replaces: Device2D.solve_equilibrium
+ * (device2d.py:1308-1373), Device3D.solve_equilibrium
+ * (device3d.py:1311-1454). */
+int tcad_solve_equilibrium(tcad_batch *b, int32_t max_iter, double tol_update,
+                           double max_dpsi, int32_t from_guess,
+                           double *d_psi, double *d_n, double *d_p,
+                           int32_t *h_iters, int32_t *h_converged);
+
+/* Coupled Newton at the biases set by tcad_set_bias, warm-started from
+ * (psi, n, p) with contact values stamped.  Writes the converged state
+ * in place; per device: Newton iterations, converged flag, total Krylov
+ * iterations.  Replaces: Device2D.solve_bias (device2d.py:1923-2191),
+ * Device3D.solve_bias (device3d.py:1981-2271). */
+int tcad_solve_bias(tcad_batch *b, int32_t max_iter, double tol_update,
+                    double max_dpsi, int32_t lin_mode, double lin_rtol,
+                    int32_t lin_maxiter, double *d_psi, double *d_n, double *d_p,
+                    int32_t *h_iters, int32_t *h_converged, int32_t *h_krylov);
+
+/* Terminal currents of every contact of every device at (psi, n, p):
+ * h_I[c] = sum over the contact's nodes of the raw F_n + F_p times
+ * J0*LD (2D, A/cm) or J0*LD^2 (3D, A).  Replaces:
+ * Device2D.terminal_current (device2d.py:2194-2227),
+ * Device3D.terminal_current (device3d.py:2274-2306). */
+int tcad_terminal_currents(tcad_batch *b, const double *d_psi, const double *d_n,
+                           const double *d_p, double *h_I);
+
+/* Scharfetter-Gummel edge currents (scaled) per axis, node-shaped
+ * planes d_J[axis*2*N_total + carrier*N_total + k] (carrier 0 = Jn,
+ * 1 = Jp), at (psi, n, p).  Replaces the Jn_x/Jn_y/Jp_x/Jp_y outputs of
+ * _residual_jacobian (device2d.py:1463-1479). */
+int tcad_edge_currents(tcad_batch *b, const double *d_psi, const double *d_n,
+                       const double *d_p, double *d_J);
+
+/* Lagged Lombardi surface-mobility update of the row-0 y-edge
+ * coefficients at psi (2D devices with the model on; no-op otherwise).
+ * solve_bias applies it at the start of every Newton iteration.
+ * Replaces: Device2D._update_surface_mobility (device2d.py:612-642). */
+int tcad_surface_mobility(tcad_batch *b, const double *d_psi);
+
+/* Bench instrumentation: average duration [microseconds] of `reps`
+ * back-to-back launches of one kernel on the last assembled system,
+ * timed with HIP events on the library stream.  which: 0 FP64 stencil
+ * SpMV, 1 FP32 stencil SpMV, 2 coupled assembly (FP32 copy on).  Has no
+ * reference counterpart. */
+int tcad_time_kernel(tcad_batch *b, int32_t which, int32_t reps, double *h_avg_us);
+
+/* Wall-clock-free counters of the last call: kernel launches issued. */
+int64_t tcad_launch_count(const tcad_batch *b);
+
+const char *tcad_last_error(void);
+const char *tcad_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TCAD_HIP_H */

@@ -432,6 +432,7 @@ class _DeviceND:
                 self._update_surface_mobility(psi)
             F, J, *_ = self._residual_jacobian(psi, n, p, cur)
             Jd, rhs = eliminate_csr(J, -F, self._dirichlet_rows)
+            self._cur_np = (n, p)
             du = self._linear_solve(Jd, rhs, opts)
             self.last_iterations += 1
             dpsi = du[0::3].reshape(self.shape)
@@ -469,6 +470,25 @@ class _DeviceND:
         return float((F_n[bc.idx] + F_p[bc.idx]).sum()) * self.J0 * self.LD ** (self.ndim - 1)
 
 
+def equilibrated_direct(Jd, rhs, n, p):
+    """Direct solve of the same Newton system after column scaling by
+    (1, n, p) per node and row equilibration (inverse row max).  Not a
+    reference method: the reference's SuperLU on the unscaled Jacobian
+    (linsolve.py:860) loses digits when minority densities span tens of
+    decades (deep reverse bias), and this variant is the exact-arithmetic
+    Newton step the GPU's equilibrated Krylov solve converges to."""
+    cs = np.stack([np.ones(n.size), n.ravel(), p.ravel()], 1).ravel()
+    A = Jd.tocsr() @ diags_(cs)
+    rs = 1.0 / abs(A).max(axis=1).toarray().ravel()
+    y = solve_linear((diags_(rs) @ A).tocsc(), rs * rhs, method="direct")[0]
+    return cs * y
+
+
+def diags_(v):
+    from scipy.sparse import diags
+    return diags(v, format="csr")
+
+
 class Device2D(_DeviceND):
     """Oracle twin of device2d.Device2D (structured, Boltzmann)."""
 
@@ -496,6 +516,8 @@ class Device2D(_DeviceND):
         """device2d.py:2057-2071: direct is spsolve(Jd.tocsc(), rhs)."""
         if opts.linsolve == "direct":
             return solve_linear(Jd.tocsc(), rhs, method="direct")[0]
+        if opts.linsolve == "direct_equilibrated":
+            return equilibrated_direct(Jd, rhs, *self._cur_np)
         x, info = solve_linear(Jd, rhs, method=opts.linsolve, rtol=opts.linsolve_rtol,
                                block_size=opts.block_size)
         self.last_krylov_iterations += info["iterations"]
@@ -533,6 +555,8 @@ class Device3D(_DeviceND):
         per-iteration fallback to direct on LinearSolveError."""
         if opts.linsolve == "direct":
             return solve_linear(Jd, rhs, method="direct")[0]
+        if opts.linsolve == "direct_equilibrated":
+            return equilibrated_direct(Jd, rhs, *self._cur_np)
         try:
             x, info = solve_linear(Jd, rhs, method=opts.linsolve,
                                    rtol=opts.linsolve_rtol, block_size=opts.block_size)

@@ -1,0 +1,815 @@
+// Newton drivers, per-node update kernels and the extern "C" entry points.
+//
+// The Newton loops run on the host side of this library; every per-node
+// operation (contact stamping, damping/clipping, the update measure, the
+// convergence test and masking of converged devices) runs on the GPU.
+// The host reads back one small per-device scalar record per Newton
+// iteration (and per Krylov chunk) to decide when every device is done.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "tcad_internal.h"
+
+namespace tcad {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+__device__ __forceinline__ double clip_np(double a, double lo, double hi) {
+  // numpy.clip = minimum(maximum(a, lo), hi), NaN propagating
+  if (a != a) return a;
+  return fmin(fmax(a, lo), hi);
+}
+
+struct UArgs {
+  const DevInfo *info;
+  const int *blk2dev;
+  long long N;
+  int nblk;
+  double *part;
+  Scal *scal;
+};
+
+__device__ __forceinline__ double ublock_max(double v, double *sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s = fmax(s, sh[w]);
+  return s;
+}
+
+// Damped Newton update of solve_bias (device2d.py:2079-2100,
+// device3d.py:2170-2185): du = Cc*y; psi += clip(du_psi, +-max_dpsi);
+// n, p clipped to [0.1x, 10x]; err = max(|dpsi|, rel dn, rel dp) with the
+// density floor 1e-10.  NaN in the update propagates to err.
+__global__ void __launch_bounds__(BS) k_newton_update(UArgs U, const double *y, double *psi,
+                                                      double *n, double *p, double max_dpsi) {
+  __shared__ double sh[BS / 64];
+  const int d = U.blk2dev[blockIdx.x];
+  const DevInfo &I = U.info[d];
+  if (!U.scal[d].active) return;
+  const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
+  double err = 0.0;
+  if (k < I.n) {
+    const long long g = I.off + k, N = U.N;
+    const double n0 = n[g], p0 = p[g];
+    const double dpsi = clip_np(y[g], -max_dpsi, max_dpsi);
+    const double dn = n0 * y[N + g];
+    const double dp = p0 * y[2 * N + g];
+    const double n1 = clip_np(n0 + dn, 0.1 * n0, 10.0 * n0);
+    const double p1 = clip_np(p0 + dp, 0.1 * p0, 10.0 * p0);
+    const double rn = fabs(n1 - n0) / fmax(n0, 1e-10);
+    const double rp = fabs(p1 - p0) / fmax(p0, 1e-10);
+    err = fmax(fmax(fabs(dpsi), rn), rp);
+    if (!(err == err) || !isfinite(n1) || !isfinite(p1) || !isfinite(dpsi)) err = INFINITY;
+    psi[g] = psi[g] + dpsi;
+    n[g] = n1;
+    p[g] = p1;
+  }
+  const double m = ublock_max(err, sh);
+  if (threadIdx.x == 0) U.part[blockIdx.x] = m;
+}
+
+// Equilibrium update (device2d.py:1335-1341): psi += clip(d), err = max|d|
+__global__ void __launch_bounds__(BS) k_poisson_update(UArgs U, const double *x, double *psi,
+                                                       double max_dpsi) {
+  __shared__ double sh[BS / 64];
+  const int d = U.blk2dev[blockIdx.x];
+  const DevInfo &I = U.info[d];
+  if (!U.scal[d].active) return;
+  const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
+  double err = 0.0;
+  if (k < I.n) {
+    const long long g = I.off + k;
+    const double dd = clip_np(x[g], -max_dpsi, max_dpsi);
+    err = fabs(dd);
+    if (!(err == err)) err = INFINITY;
+    psi[g] = psi[g] + dd;
+  }
+  const double m = ublock_max(err, sh);
+  if (threadIdx.x == 0) U.part[blockIdx.x] = m;
+}
+
+// One workgroup per device: err = max of its partials; convergence test
+// err < tol; Krylov iterations of this Newton step accumulated.
+__global__ void k_newton_finalize(UArgs U, double tol, int add_krylov) {
+  __shared__ double sh[BS / 64];
+  const int d = blockIdx.x;
+  const DevInfo &I = U.info[d];
+  Scal &S = U.scal[d];
+  if (!S.active) return;
+  double m = 0.0;
+  for (int t = threadIdx.x; t < I.nblk; t += blockDim.x) m = fmax(m, U.part[I.blk0 + t]);
+  m = ublock_max(m, sh);
+  if (threadIdx.x == 0) {
+    S.err = m;
+    S.newton_it += 1;
+    if (add_krylov) S.kry_total += S.it;
+    if (m < tol) {
+      S.newton_done = 1;
+      S.active = 0;
+    } else if (!isfinite(m)) {
+      S.status = 3;
+      S.active = 0;
+    }
+  }
+}
+
+__global__ void k_set_active(Scal *scal, int ndev) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndev) return;
+  Scal &S = scal[d];
+  S.active = 1;
+  S.newton_it = 0;
+  S.newton_done = 0;
+  S.kry_total = 0;
+  S.ir_it = 0;
+  S.status = 0;
+  S.err = 0.0;
+}
+
+__global__ void k_ir_reset(Scal *scal, int ndev) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndev) return;
+  scal[d].ir_it = 0;
+}
+
+// Stamp the contact values into the warm start (device2d.py:1949-1953).
+__global__ void __launch_bounds__(BS) k_stamp(const DevInfo *info, const int *blk2dev,
+                                              const int *cidx, const double *lnr,
+                                              const double *n0, const double *p0,
+                                              const int *ccontact, const double *V,
+                                              double *psi, double *n, double *p) {
+  const int d = blk2dev[blockIdx.x];
+  const DevInfo &I = info[d];
+  const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
+  if (k >= I.n) return;
+  const long long g = I.off + k;
+  const int t = cidx[g];
+  if (t < 0) return;
+  psi[g] = V[ccontact[t]] / I.VT + lnr[t];
+  n[g] = n0[t];
+  p[g] = p0[t];
+}
+
+// Bulk-neutral guess arcsinh(C / (2 nie_s)) (device2d.py:744-759).
+__global__ void __launch_bounds__(BS) k_bulk_guess(const DevInfo *info, const int *blk2dev,
+                                                   const double *C, const double *nie_s,
+                                                   double *psi) {
+  const int d = blk2dev[blockIdx.x];
+  const DevInfo &I = info[d];
+  const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
+  if (k >= I.n) return;
+  const long long g = I.off + k;
+  psi[g] = asinh(C[g] / (2.0 * nie_s[g]));
+}
+
+// Slaved equilibrium densities (device2d.py:1365-1367).
+__global__ void __launch_bounds__(BS) k_slaved(const DevInfo *info, const int *blk2dev,
+                                               const double *nie_s, const double *psi,
+                                               double *n, double *p) {
+  const int d = blk2dev[blockIdx.x];
+  const DevInfo &I = info[d];
+  const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
+  if (k >= I.n) return;
+  const long long g = I.off + k;
+  const double ps = psi[g];
+  n[g] = nie_s[g] * exp(fmin(fmax(ps, -700.0), 700.0));
+  p[g] = nie_s[g] * exp(fmin(fmax(-ps, -700.0), 700.0));
+}
+
+static UArgs uargs(Batch &b) {
+  UArgs U{};
+  U.info = b.d_info;
+  U.blk2dev = b.d_blk2dev;
+  U.N = b.ntot;
+  U.nblk = b.nblk;
+  U.part = b.part;
+  U.scal = b.d_scal;
+  return U;
+}
+
+int sync_scalars(Batch &b) {
+  TCAD_HIP_CHECK(hipMemcpyAsync(b.h_scal, b.d_scal, sizeof(Scal) * b.ndev,
+                                hipMemcpyDeviceToHost, b.stream));
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  return TCAD_OK;
+}
+
+int launch_set_active(Batch &b, int) {
+  hipLaunchKernelGGL(k_set_active, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
+                     b.ndev);
+  b.launches++;
+  TCAD_HIP_CHECK(hipGetLastError());
+  return TCAD_OK;
+}
+
+static int check_flag(Batch &b) {
+  int flag = 0;
+  TCAD_HIP_CHECK(hipMemcpyAsync(&flag, b.d_flag, sizeof(int), hipMemcpyDeviceToHost, b.stream));
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  if (flag) {
+    TCAD_HIP_CHECK(hipMemsetAsync(b.d_flag, 0, sizeof(int), b.stream));
+    set_error("assembly produced a non-finite or singular node block");
+    return TCAD_E_NONFINITE;
+  }
+  return TCAD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Newton drivers
+// ---------------------------------------------------------------------------
+static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, int mode,
+                           double rtol, int maxit, double *psi, double *n, double *p) {
+  int rc;
+  if ((rc = launch_set_active(b, 1))) return rc;
+  hipLaunchKernelGGL(k_stamp, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev, b.d_cidx,
+                     b.d_cn_lnr, b.d_cn_n0, b.d_cn_p0, b.d_cn_contact, b.d_contact_V, psi, n, p);
+  b.launches++;
+  UArgs U = uargs(b);
+  const bool mixed = mode == TCAD_LIN_BICGSTAB_MIXED;
+  for (int it = 0; it < max_iter; ++it) {
+    if ((rc = launch_surface_mobility(b, psi))) return rc;
+    if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, mixed))) return rc;
+    if (mixed) {
+      hipLaunchKernelGGL(k_ir_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
+                         b.ndev);
+      b.launches++;
+      if ((rc = ir_solve(b, rtol, 12, maxit, 1e-6))) return rc;
+    } else {
+      if ((rc = bicgstab(b, false, rtol, maxit, true, 16))) return rc;
+    }
+    hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
+                       max_dpsi);
+    hipLaunchKernelGGL(k_newton_finalize, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol,
+                       mixed ? 0 : 1);
+    b.launches += 2;
+    TCAD_HIP_CHECK(hipGetLastError());
+    if ((rc = check_flag(b))) return rc;
+    if ((rc = sync_scalars(b))) return rc;
+    int open = 0;
+    for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active;
+    if (open == 0) break;
+  }
+  return TCAD_OK;
+}
+
+static int solve_equilibrium_impl(Batch &b, int max_iter, double tol, double max_dpsi,
+                                  int from_guess, double *psi, double *n, double *p) {
+  int rc;
+  if ((rc = launch_set_active(b, 1))) return rc;
+  if (from_guess) {
+    hipLaunchKernelGGL(k_bulk_guess, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev,
+                       b.f.d_C, b.f.d_nie_s, psi);
+    b.launches++;
+  }
+  UArgs U = uargs(b);
+  for (int it = 0; it < max_iter; ++it) {
+    if ((rc = launch_assemble_poisson(b, psi, b.F))) return rc;
+    if ((rc = pcg_poisson(b, 1e-12, 20000))) return rc;
+    hipLaunchKernelGGL(k_poisson_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.veq[0], psi,
+                       max_dpsi);
+    hipLaunchKernelGGL(k_newton_finalize, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol, 1);
+    b.launches += 2;
+    TCAD_HIP_CHECK(hipGetLastError());
+    if ((rc = check_flag(b))) return rc;
+    if ((rc = sync_scalars(b))) return rc;
+    int open = 0;
+    for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active;
+    if (open == 0) break;
+  }
+  hipLaunchKernelGGL(k_slaved, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev,
+                     b.f.d_nie_s, psi, n, p);
+  b.launches++;
+  TCAD_HIP_CHECK(hipGetLastError());
+  return TCAD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batch construction
+// ---------------------------------------------------------------------------
+template <typename T>
+static int dalloc(T **ptr, size_t count) {
+  *ptr = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)ptr, count * sizeof(T));
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return TCAD_E_HIP;
+  }
+  return TCAD_OK;
+}
+
+template <typename T>
+static int upload(T **ptr, const std::vector<T> &v, hipStream_t s) {
+  int rc = dalloc(ptr, v.size());
+  if (rc) return rc;
+  if (!v.empty())
+    TCAD_HIP_CHECK(hipMemcpyAsync(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return TCAD_OK;
+}
+
+static void free_batch(Batch *b) {
+  if (!b) return;
+  void *ptrs[] = {b->d_info, b->d_blk2dev, b->d_h[0], b->d_h[1], b->d_h[2], b->d_dv[0],
+                  b->d_dv[1], b->d_dv[2], b->d_cidx, b->d_gidx, b->d_cn_lnr, b->d_cn_n0,
+                  b->d_cn_p0, b->d_cn_contact, b->d_gn_kw, b->d_gn_psib, b->d_gn_gate,
+                  b->d_contact_V, b->d_gate_Vg, b->d_gate_Vfb, b->d_cptr, b->d_cnodes,
+                  b->d_cscale, b->d_c2dev, b->d_surf[0], b->d_surf[1], b->d_surf[2],
+                  b->d_surf[3], b->A64, b->A32, b->Dinv64, b->Dinv32, b->bs, b->rowscale,
+                  b->colscale, b->y64, b->F, b->Fnp, b->part, b->d_scal, b->Aeq, b->beq,
+                  b->d_flag};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  for (int i = 0; i < 9; ++i)
+    if (b->v64[i]) (void)hipFree(b->v64[i]);
+  if (b->ir_r) (void)hipFree(b->ir_r);
+  if (b->ir_rhs) (void)hipFree(b->ir_rhs);
+  for (int i = 0; i < 6; ++i)
+    if (b->veq[i]) (void)hipFree(b->veq[i]);
+  if (b->h_scal) (void)hipHostFree(b->h_scal);
+  delete b;
+}
+
+static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static_fields *fields,
+                       void *stream, Batch **out) {
+  if (!devs || ndev <= 0 || !fields || !out) {
+    set_error("tcad_batch_create: null argument or ndev <= 0");
+    return TCAD_E_INVALID;
+  }
+  int ndevices = 0;
+  if (hipGetDeviceCount(&ndevices) != hipSuccess || ndevices == 0) {
+    set_error("no HIP device available");
+    return TCAD_E_NO_DEVICE;
+  }
+  Batch *b = new (std::nothrow) Batch();
+  if (!b) {
+    set_error("out of host memory");
+    return TCAD_E_INVALID;
+  }
+  b->ndev = ndev;
+  b->stream = (hipStream_t)stream;
+  b->f = *fields;
+  std::vector<double> hx[3], dvw[3];
+  std::vector<int> blk2dev;
+  std::vector<int> cidx, gidx, cn_contact, gn_gate, cptr{0}, cnodes;
+  std::vector<double> cn_lnr, cn_n0, cn_p0, gn_kw, gn_psib, cscale;
+  std::vector<double> surf[4];
+  long long off = 0;
+  int blk = 0, cglobal = 0, gglobal = 0;
+  b->info.resize(ndev);
+  for (int d = 0; d < ndev; ++d) {
+    const tcad_device_desc &D = devs[d];
+    if (D.dim != 2 && D.dim != 3) {
+      set_error("device " + std::to_string(d) + ": dim must be 2 or 3");
+      free_batch(b);
+      return TCAD_E_INVALID;
+    }
+    const int nz = D.dim == 3 ? D.nz : 1;
+    if (D.nx < 2 || D.ny < 2 || (D.dim == 3 && nz < 2)) {
+      set_error("device " + std::to_string(d) + ": every axis needs >= 2 nodes");
+      free_batch(b);
+      return TCAD_E_INVALID;
+    }
+    DevInfo &I = b->info[d];
+    I.dim = D.dim;
+    I.nx = D.nx;
+    I.ny = D.ny;
+    I.nz = nz;
+    I.off = off;
+    I.n = (long long)D.nx * D.ny * nz;
+    I.blk0 = blk;
+    I.nblk = (int)((I.n + BS - 1) / BS);
+    const int n_ax[3] = {D.nx, D.ny, nz};
+    const double *hs[3] = {D.h_hx, D.h_hy, D.h_hz};
+    const double *dvs[3] = {D.h_dvx, D.h_dvy, D.h_dvz};
+    for (int a = 0; a < 3; ++a) {
+      I.h_off[a] = (int)hx[a].size();
+      I.dv_off[a] = (int)dvw[a].size();
+      if (a < D.dim) {
+        if (!hs[a] || !dvs[a]) {
+          set_error("device " + std::to_string(d) + ": missing geometry array");
+          free_batch(b);
+          return TCAD_E_INVALID;
+        }
+        hx[a].insert(hx[a].end(), hs[a], hs[a] + n_ax[a] - 1);
+        dvw[a].insert(dvw[a].end(), dvs[a], dvs[a] + n_ax[a]);
+      }
+    }
+    I.VT = D.VT; I.Ns = D.Ns; I.LD = D.LD; I.J0 = D.J0; I.R0 = D.R0;
+    I.Cn = D.Cn_auger; I.Cp = D.Cp_auger;
+    I.srh = D.srh; I.auger = D.auger;
+    I.surf = (D.surface_mobility && D.dim == 2) ? 1 : 0;
+    I.T = D.T;
+    I.surf_off = (int)surf[0].size();
+    if (I.surf) {
+      if (!D.h_mun_row0 || !D.h_mun_row1 || !D.h_mup_row0 || !D.h_mup_row1) {
+        set_error("device " + std::to_string(d) + ": surface mobility rows missing");
+        free_batch(b);
+        return TCAD_E_INVALID;
+      }
+      surf[0].insert(surf[0].end(), D.h_mun_row0, D.h_mun_row0 + D.nx);
+      surf[1].insert(surf[1].end(), D.h_mun_row1, D.h_mun_row1 + D.nx);
+      surf[2].insert(surf[2].end(), D.h_mup_row0, D.h_mup_row0 + D.nx);
+      surf[3].insert(surf[3].end(), D.h_mup_row1, D.h_mup_row1 + D.nx);
+      b->any_surf = 1;
+    }
+    for (int t = 0; t < I.nblk; ++t) blk2dev.push_back(d);
+    // boundary tables
+    cidx.resize(off + I.n, -1);
+    gidx.resize(off + I.n, -1);
+    I.c0 = cglobal;
+    I.nc = D.ncontacts;
+    for (int c = 0; c < D.ncontacts; ++c) {
+      for (int t = D.h_contact_ptr[c]; t < D.h_contact_ptr[c + 1]; ++t) {
+        const int node = D.h_contact_nodes[t];
+        if (node < 0 || node >= I.n) {
+          set_error("device " + std::to_string(d) + ": contact node index out of range");
+          free_batch(b);
+          return TCAD_E_INDEX;
+        }
+        const long long gnode = off + node;
+        // A node listed by several contacts takes the Dirichlet value of
+        // the last one (the reference overwrites F3[kk] contact by contact,
+        // device2d.py:1801-1806), while every contact's current still sums
+        // its whole node list (terminal_current, device2d.py:2219-2226).
+        cidx[gnode] = (int)cn_lnr.size();
+        cn_lnr.push_back(D.h_contact_lnr[t]);
+        cn_n0.push_back(D.h_contact_n0[t]);
+        cn_p0.push_back(D.h_contact_p0[t]);
+        cn_contact.push_back(cglobal + c);
+        cnodes.push_back((int)gnode);
+      }
+      cptr.push_back((int)cnodes.size());
+      cscale.push_back(D.J0);
+      cscale.push_back(D.dim == 3 ? D.LD * D.LD : D.LD);
+    }
+    cglobal += D.ncontacts;
+    I.g0 = gglobal;
+    I.ng = D.ngates;
+    for (int gg = 0; gg < D.ngates; ++gg) {
+      for (int t = D.h_gate_ptr[gg]; t < D.h_gate_ptr[gg + 1]; ++t) {
+        const int node = D.h_gate_nodes[t];
+        if (node < 0 || node >= I.n) {
+          set_error("device " + std::to_string(d) + ": gate node index out of range");
+          free_batch(b);
+          return TCAD_E_INDEX;
+        }
+        const long long gnode = off + node;
+        if (gidx[gnode] >= 0) {
+          set_error("device " + std::to_string(d) + ": node " + std::to_string(node) +
+                    " belongs to two gates");
+          free_batch(b);
+          return TCAD_E_INVALID;
+        }
+        gidx[gnode] = (int)gn_kw.size();
+        gn_kw.push_back(D.h_gate_kw[t]);
+        gn_psib.push_back(D.h_gate_psib[t]);
+        gn_gate.push_back(gglobal + gg);
+      }
+    }
+    gglobal += D.ngates;
+    b->dim_max = std::max(b->dim_max, D.dim);
+    off += I.n;
+    blk += I.nblk;
+  }
+  b->ntot = off;
+  b->nblk = blk;
+  b->ne = nentries(b->dim_max);
+  b->ncontact = cglobal;
+  b->ngate = gglobal;
+  const hipStream_t s = b->stream;
+  const size_t N = (size_t)b->ntot;
+  int rc = TCAD_OK;
+#define TRY(x)                   \
+  do {                           \
+    if ((rc = (x))) {            \
+      free_batch(b);             \
+      return rc;                 \
+    }                            \
+  } while (0)
+  TRY(upload(&b->d_info, b->info, s));
+  TRY(upload(&b->d_blk2dev, blk2dev, s));
+  for (int a = 0; a < 3; ++a) {
+    TRY(upload(&b->d_h[a], hx[a], s));
+    TRY(upload(&b->d_dv[a], dvw[a], s));
+  }
+  TRY(upload(&b->d_cidx, cidx, s));
+  TRY(upload(&b->d_gidx, gidx, s));
+  TRY(upload(&b->d_cn_lnr, cn_lnr, s));
+  TRY(upload(&b->d_cn_n0, cn_n0, s));
+  TRY(upload(&b->d_cn_p0, cn_p0, s));
+  TRY(upload(&b->d_cn_contact, cn_contact, s));
+  TRY(upload(&b->d_gn_kw, gn_kw, s));
+  TRY(upload(&b->d_gn_psib, gn_psib, s));
+  TRY(upload(&b->d_gn_gate, gn_gate, s));
+  TRY(upload(&b->d_cptr, cptr, s));
+  TRY(upload(&b->d_cnodes, cnodes, s));
+  TRY(upload(&b->d_cscale, cscale, s));
+  for (int i = 0; i < 4; ++i) TRY(upload(&b->d_surf[i], surf[i], s));
+  TRY(upload(&b->d_contact_V, std::vector<double>(std::max(1, cglobal), 0.0), s));
+  TRY(upload(&b->d_gate_Vg, std::vector<double>(std::max(1, gglobal), 0.0), s));
+  TRY(upload(&b->d_gate_Vfb, std::vector<double>(std::max(1, gglobal), 0.0), s));
+  TRY(dalloc(&b->A64, (size_t)b->ne * N));
+  TRY(dalloc(&b->A32, (size_t)b->ne * N));
+  TRY(dalloc(&b->Dinv64, 9 * N));
+  TRY(dalloc(&b->Dinv32, 9 * N));
+  TRY(dalloc(&b->bs, 3 * N));
+  TRY(dalloc(&b->rowscale, 3 * N));
+  TRY(dalloc(&b->y64, 3 * N));
+  TRY(dalloc(&b->F, 3 * N));
+  TRY(dalloc(&b->Fnp, N));
+  for (int i = 0; i < 9; ++i) TRY(dalloc(&b->v64[i], 3 * N));
+  TRY(dalloc(&b->ir_r, 3 * N));
+  TRY(dalloc(&b->ir_rhs, 3 * N));
+  TRY(dalloc(&b->part, 4 * (size_t)b->nblk));
+  TRY(dalloc(&b->d_scal, (size_t)ndev));
+  TRY(dalloc(&b->Aeq, 7 * N));
+  TRY(dalloc(&b->beq, N));
+  for (int i = 0; i < 5; ++i) TRY(dalloc(&b->veq[i], N));
+  TRY(dalloc(&b->d_flag, (size_t)ndev + 1));
+#undef TRY
+  TCAD_HIP_CHECK(hipMemsetAsync(b->d_flag, 0, sizeof(int) * (ndev + 1), s));
+  TCAD_HIP_CHECK(hipMemsetAsync(b->d_scal, 0, sizeof(Scal) * ndev, s));
+  TCAD_HIP_CHECK(hipMemsetAsync(b->A32, 0, sizeof(float) * b->ne * N, s));
+  TCAD_HIP_CHECK(hipMemsetAsync(b->A64, 0, sizeof(double) * b->ne * N, s));
+  if (hipHostMalloc((void **)&b->h_scal, sizeof(Scal) * ndev, hipHostMallocDefault) != hipSuccess) {
+    free_batch(b);
+    set_error("hipHostMalloc failed");
+    return TCAD_E_HIP;
+  }
+  TCAD_HIP_CHECK(hipStreamSynchronize(s));
+  *out = b;
+  return TCAD_OK;
+}
+
+}  // namespace tcad
+
+// ---------------------------------------------------------------------------
+// extern "C" boundary
+// ---------------------------------------------------------------------------
+using tcad::Batch;
+
+struct tcad_batch {
+  Batch *impl;
+};
+
+#define GUARD(bp)                                 \
+  if (!(bp) || !(bp)->impl) {                     \
+    tcad::set_error("null tcad_batch handle");    \
+    return TCAD_E_INVALID;                        \
+  }
+
+extern "C" {
+
+const char *tcad_last_error(void) { return tcad::g_last_error.c_str(); }
+
+const char *tcad_version(void) { return "tcadhip 0.1.0 (gfx950)"; }
+
+int tcad_batch_create(const tcad_device_desc *h_devs, int32_t ndev,
+                      const tcad_static_fields *fields, void *stream, tcad_batch **out) {
+  if (!out) {
+    tcad::set_error("tcad_batch_create: out is NULL");
+    return TCAD_E_INVALID;
+  }
+  *out = nullptr;
+  Batch *b = nullptr;
+  int rc = tcad::create_impl(h_devs, ndev, fields, stream, &b);
+  if (rc) return rc;
+  *out = new tcad_batch{b};
+  return TCAD_OK;
+}
+
+int tcad_batch_destroy(tcad_batch *b) {
+  if (!b) return TCAD_OK;
+  if (b->impl) (void)hipStreamSynchronize(b->impl->stream);
+  tcad::free_batch(b->impl);
+  delete b;
+  return TCAD_OK;
+}
+
+int64_t tcad_batch_nodes(const tcad_batch *b) { return (b && b->impl) ? b->impl->ntot : -1; }
+
+int64_t tcad_launch_count(const tcad_batch *b) { return (b && b->impl) ? b->impl->launches : -1; }
+
+int tcad_set_bias(tcad_batch *bp, const double *h_contact_V, const double *h_gate_Vg,
+                  const double *h_gate_Vfb) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (b.ncontact && h_contact_V)
+    TCAD_HIP_CHECK(hipMemcpyAsync(b.d_contact_V, h_contact_V, sizeof(double) * b.ncontact,
+                                  hipMemcpyHostToDevice, b.stream));
+  if (b.ngate && h_gate_Vg)
+    TCAD_HIP_CHECK(hipMemcpyAsync(b.d_gate_Vg, h_gate_Vg, sizeof(double) * b.ngate,
+                                  hipMemcpyHostToDevice, b.stream));
+  if (b.ngate && h_gate_Vfb)
+    TCAD_HIP_CHECK(hipMemcpyAsync(b.d_gate_Vfb, h_gate_Vfb, sizeof(double) * b.ngate,
+                                  hipMemcpyHostToDevice, b.stream));
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  return TCAD_OK;
+}
+
+int tcad_assemble(tcad_batch *bp, const double *psi, const double *n, const double *p, double *F,
+                  double *Fnp) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi || !n || !p || !F) {
+    tcad::set_error("tcad_assemble: null state or output pointer");
+    return TCAD_E_INVALID;
+  }
+  int rc = tcad::launch_assemble(b, psi, n, p, F, Fnp, true);
+  if (rc) return rc;
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  return TCAD_OK;
+}
+
+int tcad_export_jacobian(tcad_batch *bp, double *h_blocks, double *h_rowscale, double *h_rhs,
+                         int32_t *h_nentries) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  const size_t N = (size_t)b.ntot;
+  if (h_nentries) *h_nentries = b.ne;
+  if (h_blocks)
+    TCAD_HIP_CHECK(hipMemcpy(h_blocks, b.A64, sizeof(double) * b.ne * N, hipMemcpyDeviceToHost));
+  if (h_rowscale)
+    TCAD_HIP_CHECK(hipMemcpy(h_rowscale, b.rowscale, sizeof(double) * 3 * N, hipMemcpyDeviceToHost));
+  if (h_rhs) TCAD_HIP_CHECK(hipMemcpy(h_rhs, b.bs, sizeof(double) * 3 * N, hipMemcpyDeviceToHost));
+  return TCAD_OK;
+}
+
+int tcad_solve_linear(tcad_batch *bp, int32_t mode, double rtol, int32_t maxiter, double *d_du,
+                      int32_t *h_iters, double *h_resid) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (mode != TCAD_LIN_BICGSTAB_F64 && mode != TCAD_LIN_BICGSTAB_MIXED) {
+    tcad::set_error("tcad_solve_linear: unknown mode");
+    return TCAD_E_INVALID;
+  }
+  if (!(rtol > 0.0) || maxiter <= 0) {
+    tcad::set_error("tcad_solve_linear: rtol must be > 0 and maxiter > 0");
+    return TCAD_E_INVALID;
+  }
+  int rc = tcad::launch_set_active(b, 1);
+  if (rc) return rc;
+  if (mode == TCAD_LIN_BICGSTAB_MIXED) rc = tcad::ir_solve(b, rtol, 12, maxiter, 1e-6);
+  else rc = tcad::bicgstab(b, false, rtol, maxiter, true, 16);
+  if (rc) return rc;
+  if ((rc = tcad::sync_scalars(b))) return rc;
+  int worst = 0;
+  for (int d = 0; d < b.ndev; ++d) {
+    const tcad::Scal &S = b.h_scal[d];
+    if (h_iters) h_iters[d] = mode == TCAD_LIN_BICGSTAB_MIXED ? S.kry_total : S.it;
+    if (h_resid) h_resid[d] = S.bb > 0 ? std::sqrt(S.rr / S.bb) : 0.0;
+    if (S.status) worst = 1;
+  }
+  if (d_du) {
+    if (!b.last_n || !b.last_p) {
+      tcad::set_error("tcad_solve_linear: no assembled system");
+      return TCAD_E_INVALID;
+    }
+    if ((rc = tcad::launch_unscale(b, b.last_n, b.last_p, d_du))) return rc;
+  }
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  if (worst) {
+    tcad::set_error("Krylov solve did not reach the requested tolerance");
+    return TCAD_E_NOT_CONVERGED;
+  }
+  return TCAD_OK;
+}
+
+int tcad_solve_equilibrium(tcad_batch *bp, int32_t max_iter, double tol_update, double max_dpsi,
+                           int32_t from_guess, double *psi, double *n, double *p,
+                           int32_t *h_iters, int32_t *h_converged) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi || !n || !p || max_iter <= 0) {
+    tcad::set_error("tcad_solve_equilibrium: invalid argument");
+    return TCAD_E_INVALID;
+  }
+  int rc = tcad::solve_equilibrium_impl(b, max_iter, tol_update, max_dpsi, from_guess, psi, n, p);
+  if (rc) return rc;
+  if ((rc = tcad::sync_scalars(b))) return rc;
+  for (int d = 0; d < b.ndev; ++d) {
+    if (h_iters) h_iters[d] = b.h_scal[d].newton_it;
+    if (h_converged) h_converged[d] = b.h_scal[d].newton_done;
+  }
+  return TCAD_OK;
+}
+
+int tcad_solve_bias(tcad_batch *bp, int32_t max_iter, double tol_update, double max_dpsi,
+                    int32_t lin_mode, double lin_rtol, int32_t lin_maxiter, double *psi,
+                    double *n, double *p, int32_t *h_iters, int32_t *h_converged,
+                    int32_t *h_krylov) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi || !n || !p || max_iter <= 0 || lin_maxiter <= 0 || !(lin_rtol > 0.0)) {
+    tcad::set_error("tcad_solve_bias: invalid argument");
+    return TCAD_E_INVALID;
+  }
+  if (lin_mode != TCAD_LIN_BICGSTAB_F64 && lin_mode != TCAD_LIN_BICGSTAB_MIXED) {
+    tcad::set_error("tcad_solve_bias: unknown linear-solve mode");
+    return TCAD_E_INVALID;
+  }
+  int rc = tcad::solve_bias_impl(b, max_iter, tol_update, max_dpsi, lin_mode, lin_rtol,
+                                 lin_maxiter, psi, n, p);
+  if (rc) return rc;
+  if ((rc = tcad::sync_scalars(b))) return rc;
+  for (int d = 0; d < b.ndev; ++d) {
+    if (h_iters) h_iters[d] = b.h_scal[d].newton_it;
+    if (h_converged) h_converged[d] = b.h_scal[d].newton_done;
+    if (h_krylov) h_krylov[d] = b.h_scal[d].kry_total;
+  }
+  return TCAD_OK;
+}
+
+int tcad_terminal_currents(tcad_batch *bp, const double *psi, const double *n, const double *p,
+                           double *h_I) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi || !n || !p || !h_I) {
+    tcad::set_error("tcad_terminal_currents: null argument");
+    return TCAD_E_INVALID;
+  }
+  int rc = tcad::launch_assemble(b, psi, n, p, b.F, b.Fnp, false);
+  if (rc) return rc;
+  double *d_I = b.v64[0];
+  if ((rc = tcad::launch_contact_sums(b, b.Fnp, d_I))) return rc;
+  if (b.ncontact)
+    TCAD_HIP_CHECK(hipMemcpyAsync(h_I, d_I, sizeof(double) * b.ncontact, hipMemcpyDeviceToHost,
+                                  b.stream));
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  return TCAD_OK;
+}
+
+int tcad_time_kernel(tcad_batch *bp, int32_t which, int32_t reps, double *h_avg_us) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (which < 0 || which > 2 || reps <= 0 || !h_avg_us) {
+    tcad::set_error("tcad_time_kernel: invalid argument");
+    return TCAD_E_INVALID;
+  }
+  if (!b.last_psi) {
+    tcad::set_error("tcad_time_kernel: no assembled system");
+    return TCAD_E_INVALID;
+  }
+  int rc = tcad::clear_done(b);
+  if (rc) return rc;
+  // one untimed launch, then the timed burst
+  auto launch = [&]() -> int {
+    if (which == 2) return tcad::launch_assemble(b, b.last_psi, b.last_n, b.last_p, b.F, nullptr, true);
+    return tcad::launch_spmv_only(b, which == 1);
+  };
+  if ((rc = launch())) return rc;
+  hipEvent_t e0, e1;
+  TCAD_HIP_CHECK(hipEventCreate(&e0));
+  TCAD_HIP_CHECK(hipEventCreate(&e1));
+  TCAD_HIP_CHECK(hipEventRecord(e0, b.stream));
+  for (int r = 0; r < reps; ++r)
+    if ((rc = launch())) return rc;
+  TCAD_HIP_CHECK(hipEventRecord(e1, b.stream));
+  TCAD_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  TCAD_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *h_avg_us = 1000.0 * ms / reps;
+  return TCAD_OK;
+}
+
+int tcad_surface_mobility(tcad_batch *bp, const double *psi) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi) {
+    tcad::set_error("tcad_surface_mobility: null psi");
+    return TCAD_E_INVALID;
+  }
+  int rc = tcad::launch_surface_mobility(b, psi);
+  if (rc) return rc;
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  return TCAD_OK;
+}
+
+int tcad_edge_currents(tcad_batch *bp, const double *psi, const double *n, const double *p,
+                       double *d_J) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi || !n || !p || !d_J) {
+    tcad::set_error("tcad_edge_currents: null argument");
+    return TCAD_E_INVALID;
+  }
+  int rc = tcad::launch_edge_currents(b, psi, n, p, d_J);
+  if (rc) return rc;
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  return TCAD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,164 @@
+// Internal declarations shared by the libtcadhip translation units.
+//
+// Data layout in HBM (all planes length N_total = sum of the batch's node
+// counts, devices concatenated):
+//   state        psi, n, p                 FP64, 3 planes (caller-owned)
+//   static       C, nie_s, nie, tau_n, tau_p, dV, an[3], ap[3], dlnnie[3]
+//                FP64 (caller-owned), edge arrays node-shaped
+//   matrix       A64[e*N + k], A32[e*N + k] stencil-block form, e < ne:
+//                0..8 the 3x3 node block (row-major), then per neighbour
+//                direction x-,x+,y-,y+,z-,z+ five coupling entries
+//                (psi<-psi, n<-psi, n<-n, p<-psi, p<-p); the matrix is the
+//                Dirichlet-ELIMINATED Jacobian scaled as R*J*Cc with
+//                Cc = (1, n, p) per node and R the inverse row max.
+//   precond      Dinv64/Dinv32[e*N + k], e < 9: inverse of the scaled node
+//                block (node block-Jacobi)
+//   vectors      3 planes of N_total (v[c*N + k])
+// Kernels map one thread to one node; each workgroup covers one device's
+// node range (blocks never straddle devices), so per-device reductions
+// are over a contiguous block range and deterministic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tcad_hip.h"
+
+namespace tcad {
+
+constexpr int BS = 256;             // threads per node block
+constexpr int NDIAG = 9;
+constexpr int NOFF = 5;
+
+inline int nentries(int dim) { return NDIAG + NOFF * 2 * dim; }
+
+// Per-device metadata, device resident.
+struct DevInfo {
+  int dim, nx, ny, nz;
+  long long off;                    // first node in the batch planes
+  long long n;                      // node count
+  int blk0, nblk;                   // node-kernel block range
+  int h_off[3];                     // offsets into the concatenated spacing arrays
+  int dv_off[3];                    // offsets into the concatenated width arrays
+  double VT, Ns, LD, J0, R0, Cn, Cp;
+  int srh, auger, surf;
+  double T;
+  int surf_off;                     // offset into surface-mobility row arrays
+  int c0, nc;                       // global contact id range
+  int g0, ng;                       // global gate id range
+};
+
+// Per-device Krylov / Newton scalars, device resident.
+struct Scal {
+  double rho[2];                    // parity slots
+  double alpha, omega;
+  double bb;                        // ||b||^2
+  double rr;                        // last ||r||^2
+  double nrm;                       // IR residual norm (rhs normalisation)
+  double err;                       // Newton update measure
+  int it;                           // Krylov iterations of the current solve
+  int done;                         // Krylov finished (converged or stopped)
+  int status;                       // 0 ok, 1 breakdown / non-finite
+  int active;                       // device takes part in the current solve
+  int newton_it, newton_done, kry_total, ir_it;
+};
+
+struct Batch {
+  int ndev = 0;
+  long long ntot = 0;
+  int nblk = 0;
+  int dim_max = 2;
+  int ne = 0;                       // stencil entries per node
+  hipStream_t stream = nullptr;
+  long long launches = 0;
+  std::vector<DevInfo> info;        // host copy
+  DevInfo *d_info = nullptr;
+  int *d_blk2dev = nullptr;
+  tcad_static_fields f{};
+  double *d_h[3] = {nullptr, nullptr, nullptr};
+  double *d_dv[3] = {nullptr, nullptr, nullptr};
+  // boundary tables
+  int *d_cidx = nullptr, *d_gidx = nullptr;      // per node: table index or -1
+  double *d_cn_lnr = nullptr, *d_cn_n0 = nullptr, *d_cn_p0 = nullptr;
+  int *d_cn_contact = nullptr;                   // table row -> global contact id
+  double *d_gn_kw = nullptr, *d_gn_psib = nullptr;
+  int *d_gn_gate = nullptr;
+  int ncontact = 0, ngate = 0;
+  double *d_contact_V = nullptr, *d_gate_Vg = nullptr, *d_gate_Vfb = nullptr;
+  int *d_cptr = nullptr;                         // contact node lists (global nodes)
+  int *d_cnodes = nullptr;
+  double *d_cscale = nullptr;                    // per contact J0*LD^(dim-1)
+  int *d_c2dev = nullptr;
+  double *d_surf[4] = {nullptr, nullptr, nullptr, nullptr};
+  int any_surf = 0;
+  // coupled system
+  double *A64 = nullptr;
+  float *A32 = nullptr;
+  double *Dinv64 = nullptr;
+  float *Dinv32 = nullptr;
+  double *bs = nullptr;                          // scaled rhs
+  double *rowscale = nullptr;
+  double *colscale = nullptr;                    // (1, n, p) at assembly
+  double *y64 = nullptr;                         // scaled solution
+  double *F = nullptr;                           // internal residual buffer
+  double *Fnp = nullptr;
+  // Krylov work vectors (FP64) and the refinement buffers
+  double *v64[9] = {};
+  double *ir_r = nullptr, *ir_rhs = nullptr;
+  double *part = nullptr;                        // nblk * 4 partial sums
+  Scal *d_scal = nullptr;
+  Scal *h_scal = nullptr;                        // pinned host mirror
+  // equilibrium Poisson system
+  double *Aeq = nullptr;                         // (1 + 2*dim) entries per node
+  double *beq = nullptr;
+  double *veq[6] = {};
+  int *d_flag = nullptr;                         // device-wide error flag
+  const double *last_n = nullptr, *last_p = nullptr;   // state of the last assembly
+  const double *last_psi = nullptr;
+  std::vector<int> dev_ncontact, dev_ngate;
+};
+
+// error handling (tcad_capi.cpp)
+void set_error(const std::string &msg);
+#define TCAD_HIP_CHECK(expr)                                               \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess) {                                                \
+      ::tcad::set_error(std::string("HIP error: ") + hipGetErrorString(e_) \
+                        + " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+      return TCAD_E_HIP;                                                   \
+    }                                                                      \
+  } while (0)
+
+// assemble.hip
+int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
+                    double *F, double *Fnp, bool want_fp32);
+int launch_surface_mobility(Batch &b, const double *psi);
+int launch_assemble_poisson(Batch &b, const double *psi, double *F);
+int launch_edge_currents(Batch &b, const double *psi, const double *n, const double *p,
+                         double *J);
+int launch_contact_sums(Batch &b, const double *Fnp, double *d_out);
+
+// krylov.hip
+int bicgstab(Batch &b, bool fp32, double rtol, int maxit, bool zero_guess, int chunk);
+int ir_solve(Batch &b, double rtol, int maxit, int inner_maxit, double inner_rtol);
+int pcg_poisson(Batch &b, double rtol, int maxit);
+int launch_unscale(Batch &b, const double *n, const double *p, double *du);
+int launch_unscale_state(Batch &b, const double *n, const double *p, double *du);
+int launch_spmv_only(Batch &b, bool fp32);
+int clear_done(Batch &b);
+
+// newton.hip
+int launch_newton_update(Batch &b, double *psi, double *n, double *p, double max_dpsi,
+                         double tol);
+int launch_poisson_update(Batch &b, double *psi, double max_dpsi, double tol);
+int launch_stamp_contacts(Batch &b, double *psi, double *n, double *p, int zero_bias);
+int launch_bulk_guess(Batch &b, double *psi);
+int launch_slaved_densities(Batch &b, const double *psi, double *n, double *p);
+int launch_set_active(Batch &b, int all);
+int sync_scalars(Batch &b);
+
+}  // namespace tcad

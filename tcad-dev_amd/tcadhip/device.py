@@ -1,0 +1,611 @@
+"""GPU-backed Device2D / Device3D with the reference's operator API.
+
+Mirrors pytcad/pytcad/device2d.py (Device2D: __init__ :168-453,
+add_contact/add_gate :672-680, solve_equilibrium :1308-1373,
+_residual_jacobian :1402-1920, solve_bias :1923-2191, terminal_current
+:2194-2227) and pytcad/pytcad/device3d.py (same methods, :437-2306) for
+the structured Boltzmann path: one semiconductor, ohmic contacts, Robin
+gates, SRH/Auger, BGN, Caughey-Thomas mobility, lagged Lombardi surface
+mobility (2D).  Every Newton step runs on the GPU through libtcadhip.so;
+the host side here only computes the one-time coefficient arrays
+(the reference's __init__ arithmetic) and moves scalars.
+
+State (psi, n, p) stays resident on the GPU between bias points (the
+warm start of solve_bias); the `psi`, `n`, `p` attributes copy it to
+the host on access.
+"""
+import ctypes
+import warnings
+from collections import OrderedDict
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from . import physics as ph
+from .mesh import control_volume_widths
+
+_MODEL_FLAGS = ("doping_mobility", "srh", "auger", "bgn", "surface_mobility")
+
+
+@dataclass
+class NewtonOptions:
+    """pytcad/pytcad/device.py:539-628 for the fields this path reads.
+
+    linsolve: "auto" | "direct" | "bicgstab" -> FP64 BiCGStab with node
+    block-Jacobi (no direct factorisation runs on the GPU);
+    "bicgstab_mixed" -> FP32 BiCGStab + block-Jacobi inside FP64
+    iterative refinement.  linsolve_rtol is the relative residual of the
+    (row-equilibrated) Newton system."""
+    max_iter: int = 100
+    tol_update: float = 1e-8
+    max_dpsi: float = 5.0
+    linsolve: str = "auto"
+    linsolve_rtol: float = 1e-10
+    lin_maxiter: int = 20000
+    verbose: bool = False
+
+    def lin_mode(self):
+        m = self.linsolve
+        if m in ("auto", "direct", "bicgstab", "gmres"):
+            return _lib.LIN_BICGSTAB_F64
+        if m in ("bicgstab_mixed", "mixed"):
+            return _lib.LIN_BICGSTAB_MIXED
+        raise ValueError(f"unknown linsolve {m!r}")
+
+
+class DirichletBC:
+    def __init__(self, idx, V):
+        self.idx = idx
+        self.V = float(V)
+
+
+class GateBC:
+    def __init__(self, idx, kappa, Vfb, Vg, normal_axis):
+        self.idx = idx
+        self.kappa = float(kappa)
+        self.Vfb = float(Vfb)
+        self.Vg = float(Vg)
+        self.normal_axis = normal_axis
+
+
+def _hmean(lo, hi):
+    return 2.0 * lo * hi / (lo + hi)
+
+
+class DeviceSetup:
+    """One device's coefficient arrays, computed once on the host with the
+    reference's __init__ arithmetic (device2d.py:319-445,
+    device3d.py:537-695), in array order (Ny,Nx) / (Nz,Ny,Nx)."""
+
+    def __init__(self, coords, doping, Ntotal=None, T=300.0, material=ph.SILICON,
+                 models=None):
+        models = models or ph.Models()
+        if isinstance(models, dict):
+            models = ph.Models(**models)
+        self.dim = len(coords)
+        if self.dim not in (2, 3):
+            raise ValueError("coords must be [x, y] or [x, y, z]")
+        if self.dim == 3 and models.surface_mobility:
+            raise NotImplementedError("surface_mobility is a 2D model (device.py:431-434)")
+        self.coords = [np.asarray(c, dtype=float) for c in coords]
+        self.shape = tuple(c.size for c in self.coords[::-1])
+        self.N = int(np.prod(self.shape))
+        self.models = models
+        self.mat = material
+        self.T = float(T)
+        self.doping = np.asarray(doping, dtype=float).reshape(self.shape)
+        self.Ntot = (np.abs(self.doping) if Ntotal is None
+                     else np.asarray(Ntotal, dtype=float).reshape(self.shape))
+        if self.Ntot.max() > 1e19:
+            warnings.warn("Doping exceeds ~1e19 cm^-3: Boltzmann statistics used here "
+                          "overestimate the carrier density.")
+        self.VT = ph.thermal_voltage(T)
+        self.eps = material.eps_r * ph.EPS0
+        self.ni = material.ni(T)
+        self.Ns = max(float(np.abs(self.doping).max()), self.ni)
+        self.LD = np.sqrt(self.eps * self.VT / (ph.Q * self.Ns))
+        self.J0 = ph.Q * ph.D0_REF * self.Ns / self.LD
+        self.R0 = ph.D0_REF * self.Ns / self.LD ** 2
+        self.h = [np.diff(c / self.LD) for c in self.coords]
+        self.dVw = [control_volume_widths(h) for h in self.h]
+        if self.dim == 2:
+            self.dV = np.outer(self.dVw[1], self.dVw[0])
+        else:
+            self.dV = (self.dVw[2][:, None, None] * self.dVw[1][None, :, None]
+                       * self.dVw[0][None, None, :])
+        self.C = self.doping / self.Ns
+        self.nie = ph.nie_effective(self.Ntot, material, T, models.bgn)
+        self.nie_s = self.nie / self.Ns
+        if models.doping_mobility:
+            self.mu_n0 = ph.mobility_caughey_thomas(self.Ntot, material, T, "n")
+            self.mu_p0 = ph.mobility_caughey_thomas(self.Ntot, material, T, "p")
+        else:
+            self.mu_n0 = np.full(self.shape, material.mu_n_max)
+            self.mu_p0 = np.full(self.shape, material.mu_p_max)
+        self.tau_n = ph.lifetime_scharfetter(self.Ntot, material.tau_n0, material.tau_Nref)
+        self.tau_p = ph.lifetime_scharfetter(self.Ntot, material.tau_p0, material.tau_Nref)
+        # node-shaped edge arrays: entry k is the edge k -> k + axis stride
+        self.an, self.ap, self.dln = [], [], []
+        for a in range(self.dim):
+            ax = self.dim - 1 - a
+            lo = [slice(None)] * self.dim
+            hi = [slice(None)] * self.dim
+            lo[ax] = slice(None, -1)
+            hi[ax] = slice(1, None)
+            lo, hi = tuple(lo), tuple(hi)
+            hb = self.h[a].reshape([-1 if b == ax else 1 for b in range(self.dim)])
+            dn = _hmean(self.mu_n0[lo], self.mu_n0[hi]) * self.VT / ph.D0_REF
+            dp = _hmean(self.mu_p0[lo], self.mu_p0[hi]) * self.VT / ph.D0_REF
+            for arr, val in ((self.an, dn / hb), (self.ap, dp / hb),
+                             (self.dln, np.log(self.nie_s[hi] / self.nie_s[lo]))):
+                full = np.zeros(self.shape)
+                full[lo] = val
+                arr.append(full)
+        self.bcs = OrderedDict()
+
+    def flat(self, idx):
+        return np.ravel_multi_index(tuple(np.asarray(i, dtype=np.int64) for i in idx),
+                                    self.shape).astype(np.int32)
+
+    def gate_face_weight(self, bc):
+        """device3d.py:939-947; 2D gates are y-normal, weight dVx[i]."""
+        a = "xyz".index(bc.normal_axis)
+        if a >= self.dim:
+            raise ValueError(f"normal_axis {bc.normal_axis!r} invalid in {self.dim}D")
+        others = [b for b in range(self.dim) if b != a]
+        w = self.dVw[others[0]][bc.idx[self.dim - 1 - others[0]]]
+        for b in others[1:]:
+            w = w * self.dVw[b][bc.idx[self.dim - 1 - b]]
+        return w
+
+    def contact_values(self, bc, V):
+        """contacts.py:21-37 at the contact's nodes: (psi0, n0, p0, ln(n0/nie))."""
+        return ph.ohmic_values(self.C[bc.idx], self.nie_s[bc.idx], V, self.VT)
+
+    def contacts(self):
+        return [(nm, bc) for nm, bc in self.bcs.items() if isinstance(bc, DirichletBC)]
+
+    def gates(self):
+        return [(nm, bc) for nm, bc in self.bcs.items() if isinstance(bc, GateBC)]
+
+    def description(self, keep):
+        """A DeviceDesc; host arrays it points to are appended to `keep`."""
+        def arr(a, dt=np.float64):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+
+        D = _lib.DeviceDesc()
+        D.dim = self.dim
+        D.nx = self.coords[0].size
+        D.ny = self.coords[1].size
+        D.nz = self.coords[2].size if self.dim == 3 else 1
+        hs = [arr(h) for h in self.h] + [None] * (3 - self.dim)
+        dvs = [arr(v) for v in self.dVw] + [None] * (3 - self.dim)
+        D.h_hx, D.h_hy, D.h_hz = (_lib.dptr(x) for x in hs)
+        D.h_dvx, D.h_dvy, D.h_dvz = (_lib.dptr(x) for x in dvs)
+        D.VT, D.Ns, D.LD, D.J0, D.R0 = self.VT, self.Ns, self.LD, self.J0, self.R0
+        D.Cn_auger, D.Cp_auger = self.mat.Cn_auger, self.mat.Cp_auger
+        D.srh, D.auger = int(self.models.srh), int(self.models.auger)
+        D.surface_mobility = int(self.models.surface_mobility and self.dim == 2)
+        D.T = self.T
+        if D.surface_mobility:
+            D.h_mun_row0 = _lib.dptr(arr(self.mu_n0[0, :]))
+            D.h_mun_row1 = _lib.dptr(arr(self.mu_n0[1, :]))
+            D.h_mup_row0 = _lib.dptr(arr(self.mu_p0[0, :]))
+            D.h_mup_row1 = _lib.dptr(arr(self.mu_p0[1, :]))
+        cs = self.contacts()
+        ptr, nodes, lnr, n0, p0 = [0], [], [], [], []
+        for _, bc in cs:
+            kk = self.flat(bc.idx)
+            _, nn0, pp0, ll = self.contact_values(bc, 0.0)
+            nodes.append(kk); lnr.append(ll); n0.append(nn0); p0.append(pp0)
+            ptr.append(ptr[-1] + kk.size)
+        D.ncontacts = len(cs)
+        cat = (lambda xs, dt: arr(np.concatenate(xs) if xs else np.zeros(0), dt))
+        D.h_contact_ptr = _lib.iptr(arr(ptr, np.int32))
+        D.h_contact_nodes = _lib.iptr(cat(nodes, np.int32))
+        D.h_contact_lnr = _lib.dptr(cat(lnr, np.float64))
+        D.h_contact_n0 = _lib.dptr(cat(n0, np.float64))
+        D.h_contact_p0 = _lib.dptr(cat(p0, np.float64))
+        gs = self.gates()
+        ptr, nodes, kw, psib = [0], [], [], []
+        for _, bc in gs:
+            kk = self.flat(bc.idx)
+            nodes.append(kk)
+            kw.append(np.broadcast_to(bc.kappa * self.gate_face_weight(bc), kk.shape))
+            psib.append(np.arcsinh(self.C[bc.idx] / (2.0 * self.nie_s[bc.idx])))
+            ptr.append(ptr[-1] + kk.size)
+        D.ngates = len(gs)
+        D.h_gate_ptr = _lib.iptr(arr(ptr, np.int32))
+        D.h_gate_nodes = _lib.iptr(cat(nodes, np.int32))
+        D.h_gate_kw = _lib.dptr(cat(kw, np.float64))
+        D.h_gate_psib = _lib.dptr(cat(psib, np.float64))
+        return D
+
+    def node_fields(self):
+        """Per-node static arrays (flattened) keyed as in StaticFields."""
+        f = {"C": self.C, "nie_s": self.nie_s, "nie": self.nie, "tau_n": self.tau_n,
+             "tau_p": self.tau_p, "dV": self.dV}
+        for a in range(3):
+            z = np.zeros(self.shape)
+            f[f"an{a}"] = self.an[a] if a < self.dim else z
+            f[f"ap{a}"] = self.ap[a] if a < self.dim else z
+            f[f"dln{a}"] = self.dln[a] if a < self.dim else z
+        return {k: np.ascontiguousarray(v, dtype=np.float64).ravel() for k, v in f.items()}
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("tcadhip needs a HIP (ROCm) GPU; torch.cuda.is_available() is False")
+    return torch
+
+
+class GPUBatch:
+    """A libtcadhip batch: the devices of `setups` share every launch."""
+
+    FIELD_KEYS = ("C", "nie_s", "nie", "tau_n", "tau_p", "dV",
+                  "an0", "an1", "an2", "ap0", "ap1", "ap2", "dln0", "dln1", "dln2")
+
+    def __init__(self, setups):
+        torch = _torch()
+        L = _lib.lib()
+        self.setups = list(setups)
+        self.offsets = np.cumsum([0] + [s.N for s in self.setups])
+        self.N = int(self.offsets[-1])
+        host = {k: np.concatenate([s.node_fields()[k] for s in self.setups])
+                for k in self.FIELD_KEYS}
+        self._t = {k: torch.from_numpy(v).to("cuda") for k, v in host.items()}
+        F = _lib.StaticFields()
+        F.d_C, F.d_nie_s, F.d_nie = (self._t[k].data_ptr() for k in ("C", "nie_s", "nie"))
+        F.d_tau_n, F.d_tau_p, F.d_dV = (self._t[k].data_ptr() for k in ("tau_n", "tau_p", "dV"))
+        for a in range(3):
+            F.d_an[a] = self._t[f"an{a}"].data_ptr()
+            F.d_ap[a] = self._t[f"ap{a}"].data_ptr()
+            F.d_dlnnie[a] = self._t[f"dln{a}"].data_ptr()
+        keep = []
+        descs = (_lib.DeviceDesc * len(self.setups))(
+            *[s.description(keep) for s in self.setups])
+        self._handle = ctypes.c_void_p()
+        _lib.check(L.tcad_batch_create(descs, len(self.setups), ctypes.byref(F), None,
+                                       ctypes.byref(self._handle)), "tcad_batch_create")
+        self._keep = keep
+        self.contacts = [(d, nm) for d, s in enumerate(self.setups) for nm, _ in s.contacts()]
+        self.gates = [(d, nm) for d, s in enumerate(self.setups) for nm, _ in s.gates()]
+        self.psi = torch.zeros(self.N, dtype=torch.float64, device="cuda")
+        self.n = torch.zeros_like(self.psi)
+        self.p = torch.zeros_like(self.psi)
+        self.ndev = len(self.setups)
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().tcad_batch_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def set_bias(self):
+        Vc = np.array([s.bcs[nm].V for d, nm in self.contacts for s in [self.setups[d]]],
+                      dtype=np.float64)
+        Vg = np.array([s.bcs[nm].Vg for d, nm in self.gates for s in [self.setups[d]]],
+                      dtype=np.float64)
+        Vfb = np.array([s.bcs[nm].Vfb for d, nm in self.gates for s in [self.setups[d]]],
+                       dtype=np.float64)
+        _lib.check(_lib.lib().tcad_set_bias(
+            self._handle, _lib.dptr(Vc) if Vc.size else None,
+            _lib.dptr(Vg) if Vg.size else None,
+            _lib.dptr(Vfb) if Vfb.size else None), "tcad_set_bias")
+
+    def solve_equilibrium(self, opts):
+        its = np.zeros(self.ndev, dtype=np.int32)
+        conv = np.zeros(self.ndev, dtype=np.int32)
+        self.set_bias()
+        _lib.check(_lib.lib().tcad_solve_equilibrium(
+            self._handle, opts.max_iter, opts.tol_update, opts.max_dpsi, 1,
+            self.psi.data_ptr(), self.n.data_ptr(), self.p.data_ptr(),
+            _lib.iptr(its), _lib.iptr(conv)), "tcad_solve_equilibrium")
+        return its, conv
+
+    def solve_bias(self, opts):
+        its = np.zeros(self.ndev, dtype=np.int32)
+        conv = np.zeros(self.ndev, dtype=np.int32)
+        kry = np.zeros(self.ndev, dtype=np.int32)
+        self.set_bias()
+        _lib.check(_lib.lib().tcad_solve_bias(
+            self._handle, opts.max_iter, opts.tol_update, opts.max_dpsi, opts.lin_mode(),
+            opts.linsolve_rtol, opts.lin_maxiter, self.psi.data_ptr(), self.n.data_ptr(),
+            self.p.data_ptr(), _lib.iptr(its), _lib.iptr(conv), _lib.iptr(kry)),
+            "tcad_solve_bias")
+        return its, conv, kry
+
+    def terminal_currents(self):
+        out = np.zeros(max(1, len(self.contacts)), dtype=np.float64)
+        _lib.check(_lib.lib().tcad_terminal_currents(
+            self._handle, self.psi.data_ptr(), self.n.data_ptr(), self.p.data_ptr(),
+            _lib.dptr(out)), "tcad_terminal_currents")
+        return out[:len(self.contacts)]
+
+    def device_slice(self, d):
+        return slice(int(self.offsets[d]), int(self.offsets[d + 1]))
+
+
+class _GPUDevice:
+    """Shared implementation of the GPU Device2D / Device3D."""
+
+    def __init__(self, coords, doping, Ntotal=None, T=300.0, material=ph.SILICON,
+                 models=None):
+        self._s = DeviceSetup(coords, doping, Ntotal, T, material, models)
+        for k in ("dim", "shape", "N", "VT", "Ns", "LD", "J0", "R0", "C", "nie_s", "nie",
+                  "mu_n0", "mu_p0", "tau_n", "tau_p", "dV", "eps", "models"):
+            setattr(self, k, getattr(self._s, k))
+        self.bcs = self._s.bcs
+        self._batch = None
+        self._solved = False
+        self.last_converged = None
+        self.last_iterations = None
+        self.last_krylov_iterations = None
+        self.last_eq_iterations = None
+
+    # -- construction ----------------------------------------------------
+    def _invalidate(self):
+        self._batch = None
+        self._solved = False
+
+    def _gpu(self):
+        if self._batch is None:
+            self._batch = GPUBatch([self._s])
+        return self._batch
+
+    def _set_voltages(self, voltages):
+        for name, V in (voltages or {}).items():
+            bc = self.bcs[name]
+            if isinstance(bc, DirichletBC):
+                bc.V = float(V)
+            else:
+                bc.Vg = float(V)
+
+    # -- solves -------------------------------------------------------------
+    def solve_equilibrium(self, opts=None):
+        opts = opts or NewtonOptions()
+        B = self._gpu()
+        its, conv = B.solve_equilibrium(opts)
+        self.last_eq_iterations = int(its[0])
+        if not conv[0]:
+            warnings.warn("equilibrium Poisson solve did not converge.")
+        self._solved = True
+        return self
+
+    def solve_bias(self, voltages=None, opts=None):
+        opts = opts or NewtonOptions()
+        if not self._solved:
+            self.solve_equilibrium(opts)
+        self._set_voltages(voltages)
+        its, conv, kry = self._gpu().solve_bias(opts)
+        self.last_iterations = int(its[0])
+        self.last_converged = bool(conv[0])
+        self.last_krylov_iterations = int(kry[0])
+        if not self.last_converged:
+            warnings.warn("Newton did not converge")
+        return self
+
+    def terminal_current(self, name):
+        bc = self.bcs[name]
+        if not isinstance(bc, DirichletBC):
+            raise ValueError(f"terminal_current: '{name}' is not a Dirichlet contact")
+        if not self._solved:
+            raise RuntimeError("terminal_current: solve the device first")
+        B = self._gpu()
+        I = B.terminal_currents()
+        return float(I[[nm for _, nm in B.contacts].index(name)])
+
+    def terminal_currents(self):
+        """All contacts' currents in contact order (one GPU reduction)."""
+        B = self._gpu()
+        return dict(zip([nm for _, nm in B.contacts], B.terminal_currents()))
+
+    # -- state ---------------------------------------------------------------
+    def _state(self, t):
+        if not self._solved:
+            return None
+        return t.cpu().numpy().reshape(self.shape)
+
+    @property
+    def psi(self):
+        return self._state(self._gpu().psi)
+
+    @property
+    def n(self):
+        return self._state(self._gpu().n)
+
+    @property
+    def p(self):
+        return self._state(self._gpu().p)
+
+    def set_state(self, psi, n, p):
+        torch = _torch()
+        B = self._gpu()
+        for t, v in ((B.psi, psi), (B.n, n), (B.p, p)):
+            t.copy_(torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64).ravel()))
+        self._solved = True
+
+    @property
+    def psi_V(self):
+        return self.psi * self.VT
+
+    @property
+    def n_cm3(self):
+        return self.n * self.Ns
+
+    @property
+    def p_cm3(self):
+        return self.p * self.Ns
+
+    # -- residual / Jacobian boundary (tests and inspection) -----------------
+    def _update_surface_mobility(self, psi):
+        """Apply the lagged Lombardi update at `psi` (device2d.py:612-642)."""
+        torch = _torch()
+        B = self._gpu()
+        t = torch.from_numpy(np.ascontiguousarray(psi, dtype=np.float64).ravel()).to("cuda")
+        _lib.check(_lib.lib().tcad_surface_mobility(B.handle, t.data_ptr()),
+                   "tcad_surface_mobility")
+
+    def _residual_jacobian(self, psi, n, p, voltages):
+        """GPU assembly at (psi, n, p).  Returns (F, Jd, Jn, Jp, Fnp):
+        F interleaved (3N) exactly as the reference's first output; Jd the
+        Dirichlet-eliminated Jacobian (what eliminate_csr returns,
+        dirichlet.py:161-191) as scipy CSR; per-axis scaled SG edge
+        currents (node-shaped, last entry along each axis 0); and the raw
+        pre-Dirichlet F_n + F_p per node."""
+        torch = _torch()
+        self._set_voltages(voltages)
+        B = self._gpu()
+        B.set_bias()
+        N = self.N
+        dev = [torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64).ravel()).to("cuda")
+               for v in (psi, n, p)]
+        F = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+        Fnp = torch.zeros(N, dtype=torch.float64, device="cuda")
+        L = _lib.lib()
+        _lib.check(L.tcad_assemble(B.handle, dev[0].data_ptr(), dev[1].data_ptr(),
+                                   dev[2].data_ptr(), F.data_ptr(), Fnp.data_ptr()),
+                   "tcad_assemble")
+        ne = ctypes.c_int32()
+        _lib.check(L.tcad_export_jacobian(B.handle, None, None, None, ctypes.byref(ne)),
+                   "tcad_export_jacobian")
+        blocks = np.zeros(ne.value * N)
+        rs = np.zeros(3 * N)
+        _lib.check(L.tcad_export_jacobian(B.handle, _lib.dptr(blocks), _lib.dptr(rs), None,
+                                          None), "tcad_export_jacobian")
+        Jd = stencil_to_csr(blocks.reshape(ne.value, N), rs,
+                            np.asarray(n, dtype=float).ravel(),
+                            np.asarray(p, dtype=float).ravel(), self._s)
+        J = torch.zeros(2 * self.dim * N, dtype=torch.float64, device="cuda")
+        _lib.check(L.tcad_edge_currents(B.handle, dev[0].data_ptr(), dev[1].data_ptr(),
+                                        dev[2].data_ptr(), J.data_ptr()), "tcad_edge_currents")
+        Jh = J.cpu().numpy().reshape(self.dim, 2, N)
+        Fh = F.cpu().numpy().reshape(3, N).T.ravel()
+        return (Fh, Jd, [Jh[a, 0].reshape(self.shape) for a in range(self.dim)],
+                [Jh[a, 1].reshape(self.shape) for a in range(self.dim)],
+                Fnp.cpu().numpy().reshape(self.shape))
+
+
+def stencil_to_csr(blocks, rowscale, n, p, setup):
+    """Rebuild the unscaled eliminated Jacobian Jd = R^-1 (A_s) Cc^-1 from
+    the exported stencil blocks (interleaved 3k+c numbering)."""
+    import scipy.sparse as sp
+    N = setup.N
+    shape = setup.shape
+    idx = np.arange(N).reshape(shape)
+    colscale = np.stack([np.ones(N), n, p], axis=0)      # (3, N)
+    rows, cols, vals = [], [], []
+    k = np.arange(N)
+    for r in range(3):
+        for c in range(3):
+            rows.append(3 * k + r); cols.append(3 * k + c)
+            vals.append(blocks[3 * r + c] / rowscale[r * N + k] / colscale[c, k])
+    pattern = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 2)]
+    for a in range(setup.dim):
+        ax = setup.dim - 1 - a
+        for side in (0, 1):          # 0: -axis neighbour, 1: +axis neighbour
+            dirn = 2 * a + side
+            sl_self = [slice(None)] * setup.dim
+            sl_nb = [slice(None)] * setup.dim
+            if side == 0:
+                sl_self[ax] = slice(1, None); sl_nb[ax] = slice(None, -1)
+            else:
+                sl_self[ax] = slice(None, -1); sl_nb[ax] = slice(1, None)
+            ks = idx[tuple(sl_self)].ravel()
+            kn = idx[tuple(sl_nb)].ravel()
+            for m, (r, c) in enumerate(pattern):
+                v = blocks[9 + 5 * dirn + m][ks] / rowscale[r * N + ks] / colscale[c, kn]
+                rows.append(3 * ks + r); cols.append(3 * kn + c); vals.append(v)
+    rows = np.concatenate(rows); cols = np.concatenate(cols); vals = np.concatenate(vals)
+    keep = vals != 0.0
+    return sp.csr_matrix((vals[keep], (rows[keep], cols[keep])), shape=(3 * N, 3 * N))
+
+
+class Device2D(_GPUDevice):
+    """GPU twin of pytcad.device2d.Device2D on a structured mesh
+    (device2d.py:138-453)."""
+
+    def __init__(self, mesh, doping, Ntotal=None, T=300.0, material=ph.SILICON, models=None):
+        super().__init__([mesh.x, mesh.y], doping, Ntotal, T, material, models)
+        self.mesh = mesh
+        self.Ny, self.Nx = self.shape
+
+    def add_contact(self, name, i, j, V=0.0):
+        """device2d.py:672-674."""
+        i, j = np.broadcast_arrays(np.atleast_1d(np.asarray(i, dtype=int)),
+                                   np.atleast_1d(np.asarray(j, dtype=int)))
+        self.bcs[name] = DirichletBC((j.copy(), i.copy()), V)
+        self._invalidate()
+        return self.bcs[name]
+
+    def add_gate(self, name, i, j, tox_cm, Vfb, Vg=0.0):
+        """device2d.py:676-680: kappa = eps_ox LD / (eps_si tox)."""
+        i, j = np.broadcast_arrays(np.atleast_1d(np.asarray(i, dtype=int)),
+                                   np.atleast_1d(np.asarray(j, dtype=int)))
+        kappa = ph.EPS_OX_R * ph.EPS0 * self.LD / (self.eps * tox_cm)
+        self.bcs[name] = GateBC((j.copy(), i.copy()), kappa, Vfb, Vg, "y")
+        self._invalidate()
+        return self.bcs[name]
+
+
+class Device3D(_GPUDevice):
+    """GPU twin of pytcad.device3d.Device3D on a structured mesh
+    (device3d.py:411-724)."""
+
+    def __init__(self, mesh, doping, Ntotal=None, T=300.0, material=ph.SILICON, models=None):
+        super().__init__([mesh.x, mesh.y, mesh.z], doping, Ntotal, T, material, models)
+        self.mesh = mesh
+        self.Nz, self.Ny, self.Nx = self.shape
+
+    def add_contact(self, name, i, j, k, V=0.0):
+        """device3d.py:748-750."""
+        i, j, k = np.broadcast_arrays(*(np.atleast_1d(np.asarray(a, dtype=int))
+                                        for a in (i, j, k)))
+        self.bcs[name] = DirichletBC((k.copy(), j.copy(), i.copy()), V)
+        self._invalidate()
+        return self.bcs[name]
+
+    def add_gate(self, name, i, j, k, tox_cm, Vfb, Vg=0.0, normal_axis="z"):
+        """device3d.py:752-756."""
+        if normal_axis not in ("x", "y", "z"):
+            raise ValueError(f"normal_axis must be 'x', 'y', or 'z', got {normal_axis!r}")
+        i, j, k = np.broadcast_arrays(*(np.atleast_1d(np.asarray(a, dtype=int))
+                                        for a in (i, j, k)))
+        kappa = ph.EPS_OX_R * ph.EPS0 * self.LD / (self.eps * tox_cm)
+        self.bcs[name] = GateBC((k.copy(), j.copy(), i.copy()), kappa, Vfb, Vg, normal_axis)
+        self._invalidate()
+        return self.bcs[name]
+
+
+def device_from_spec(spec):
+    """Build a GPU Device2D/Device3D from a configs.DeviceSpec."""
+    from .mesh import Mesh2D, Mesh3D
+    mat = ph.SILICON.with_overrides(spec.material)
+    models = ph.Models(**spec.models)
+    if spec.dim == 2:
+        dev = Device2D(Mesh2D(*spec.coords), spec.doping, spec.Ntotal, material=mat,
+                       models=models)
+        for c in spec.contacts:
+            dev.add_contact(c.name, c.idx[1], c.idx[0], V=c.V)
+        for g in spec.gates:
+            dev.add_gate(g.name, g.idx[1], g.idx[0], g.tox_cm, g.Vfb, g.Vg)
+    else:
+        dev = Device3D(Mesh3D(*spec.coords), spec.doping, spec.Ntotal, material=mat,
+                       models=models)
+        for c in spec.contacts:
+            dev.add_contact(c.name, c.idx[2], c.idx[1], c.idx[0], V=c.V)
+        for g in spec.gates:
+            dev.add_gate(g.name, g.idx[2], g.idx[1], g.idx[0], g.tox_cm, g.Vfb, g.Vg,
+                         normal_axis=g.normal_axis)
+    return dev

@@ -1,0 +1,188 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's
+golden fixtures and the CPU oracle.
+
+Tolerances (stated per quantity):
+  * residual F at a given state: max |F_gpu - F_ref| <= 1e-12 * max |F_ref|
+    (same arithmetic; only expm1/log/exp differ by a few ulp from glibc);
+  * eliminated Jacobian: max |Jd_gpu - Jd_ref| <= 1e-12 * max |Jd_ref| per row;
+  * converged states: psi atol 1e-9 (scaled, i.e. 2.6e-11 V), n and p
+    relative 1e-7 (the reference's own bars, tests/test_native_linsolve.py:118-137);
+  * terminal currents: |I_gpu - I_ref| <= 1e-6 * max_c |I_ref| + floor, per
+    bias point, with floor = 10 x the reference's own current-conservation
+    error |sum_c I_ref| at that point (currents below the reference's
+    rounding noise are not resolvable by either side);
+  * Newton iteration counts within +-1 of the reference's.
+"""
+import os
+import warnings
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from golden.fixtures import load
+from oracle.linsolve import eliminate_csr
+from oracle_build import oracle_device, oracle_sweep
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a HIP device")
+    from tcadhip.device import NewtonOptions, device_from_spec
+    return NewtonOptions, device_from_spec
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def current_tol(ref_row):
+    ref_row = np.asarray(ref_row)
+    return 1e-6 * np.abs(ref_row).max() + 10.0 * abs(ref_row.sum())
+
+
+@pytest.mark.parametrize("name", ["diode_14x9_fj.npz", "mosfet_16x12_fj.npz"])
+def test_residual_jacobian_vs_reference(name):
+    NewtonOptions, device_from_spec = _gpu()
+    _say(f"[F/J] {name}")
+    spec, d = load(name)
+    dev = device_from_spec(spec)
+    volts = dict(zip([str(k) for k in d["volt_keys"]], d["volt_vals"]))
+    dev._set_voltages(volts)
+    if spec.models["surface_mobility"]:
+        dev._update_surface_mobility(d["in_psi"])
+    F, Jd, Jn, Jp, Fnp = dev._residual_jacobian(d["in_psi"], d["in_n"], d["in_p"], volts)
+    Fref = d["F"]
+    assert np.abs(F - Fref).max() <= 1e-12 * np.abs(Fref).max()
+    Jr = sp.csr_matrix((d["J_data"], d["J_indices"], d["J_indptr"]), shape=(F.size, F.size))
+    Jdr, rhs = eliminate_csr(Jr, -Fref, d["dirichlet_rows"])
+    diff = abs(Jd - Jdr).max(axis=1).toarray().ravel()
+    scale = abs(Jdr).max(axis=1).toarray().ravel()
+    assert np.all(diff <= 1e-12 * scale), (diff / scale).max()
+    for a, key in enumerate(("Jnx", "Jny")):
+        ref = d[key]
+        got = Jn[a][tuple(slice(0, s) for s in ref.shape)]
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max() + 1e-300
+    Fn_p_ref = d["Fn"] + d["Fp"]
+    assert np.abs(Fnp - Fn_p_ref).max() <= 1e-12 * (np.abs(d["Fn"]).max() + np.abs(d["Fp"]).max())
+
+
+def _run_sweep(spec, opts):
+    NewtonOptions, device_from_spec = _gpu()
+    dev = device_from_spec(spec)
+    dev.solve_equilibrium(opts)
+    its, cur, states = [], [], []
+    for bias in spec.sweep:
+        dev.solve_bias(bias, opts)
+        its.append(dev.last_iterations)
+        cur.append([dev.terminal_current(c.name) for c in spec.contacts])
+        states.append((dev.psi, dev.n, dev.p))
+    return dev, np.array(its), np.array(cur), states
+
+
+SWEEPS = ["diode_60x40_sweep.npz", "mosfet_48x32_sweep.npz", "diode_24x16_defaults_sweep.npz"]
+
+
+def _equilibrated_counts(spec):
+    from oracle import physics as oph
+    _, its, _, _ = oracle_sweep(spec, opts=oph.NewtonOptions(linsolve="direct_equilibrated"))
+    return its
+
+
+@pytest.mark.parametrize("mode", ["bicgstab", "bicgstab_mixed"])
+@pytest.mark.parametrize("name", SWEEPS)
+def test_sweep_vs_reference(name, mode):
+    NewtonOptions, _ = _gpu()
+    _say(f"[sweep] {name} {mode}")
+    spec, d = load(name)
+    opts = NewtonOptions(linsolve=mode)
+    dev, its, cur, states = _run_sweep(spec, opts)
+    assert abs(dev.last_eq_iterations - int(d["eq_iterations"])) <= 1
+    ref_its = d["newton_iterations"]
+    # Where the reference's SuperLU on the unscaled Jacobian loses digits
+    # (deep reverse bias), its Newton count drifts from the exact-step
+    # count; there the GPU is held to the equilibrated direct oracle
+    # (same algorithm, well-conditioned solve) and to never needing more
+    # iterations than the reference.
+    eq_its = _equilibrated_counts(spec)
+    for p in range(len(its)):
+        if abs(eq_its[p] - ref_its[p]) <= 1:
+            assert abs(its[p] - ref_its[p]) <= 1, (p, its, ref_its)
+        else:
+            assert abs(its[p] - eq_its[p]) <= 1 and its[p] <= ref_its[p], (p, its, eq_its)
+    ref = d["currents"]
+    for p in range(len(spec.sweep)):
+        tol = current_tol(ref[p])
+        assert np.abs(cur[p] - ref[p]).max() <= tol, (p, cur[p], ref[p], tol)
+    for k in d.files:
+        if k.startswith("psi_") and k != "psi_eq":
+            p = int(k.split("_")[1])
+            psi, n, pp = states[p]
+            assert np.abs(psi - d[f"psi_{p}"]).max() <= 1e-9
+            assert (np.abs(n - d[f"n_{p}"]) / d[f"n_{p}"]).max() <= 1e-7
+            assert (np.abs(pp - d[f"p_{p}"]) / d[f"p_{p}"]).max() <= 1e-7
+
+
+def test_equilibrium_vs_reference():
+    NewtonOptions, device_from_spec = _gpu()
+    _say("[eq] diode_60x40 and mosfet_48x32")
+    for name in ("diode_60x40_sweep.npz", "mosfet_48x32_sweep.npz"):
+        spec, d = load(name)
+        dev = device_from_spec(spec)
+        dev.solve_equilibrium(NewtonOptions())
+        assert abs(dev.last_eq_iterations - int(d["eq_iterations"])) <= 1
+        assert np.abs(dev.psi - d["psi_eq"]).max() <= 1e-9
+
+
+def test_trigate_small_vs_oracle():
+    """3D tri-gate (reduced mesh) against the live CPU oracle."""
+    NewtonOptions, _ = _gpu()
+    from tcadhip import configs
+    _say("[3d] trigate 14x10x8 Vd=0.05, Vg 0,0.4,0.8")
+    spec = configs.trigate_3d(nx=14, ny=10, nz=8, vg_list=[0.0, 0.4, 0.8])
+    odev, oits, ocur, ost = oracle_sweep(spec, keep=(2,))
+    dev, its, cur, states = _run_sweep(spec, NewtonOptions())
+    assert np.all(np.abs(its - oits) <= 1), (its, oits)
+    for p in range(len(spec.sweep)):
+        assert np.abs(cur[p] - ocur[p]).max() <= current_tol(ocur[p]), (p, cur[p], ocur[p])
+    psi, n, pp = states[2]
+    assert np.abs(psi - ost[2][0]).max() <= 1e-9
+    assert (np.abs(n - ost[2][1]) / ost[2][1]).max() <= 1e-7
+
+
+def test_batch_equals_alone():
+    """Devices solved together in one batch give the same results as alone."""
+    NewtonOptions, device_from_spec = _gpu()
+    from tcadhip import configs
+    from tcadhip.device import DeviceSetup, GPUBatch
+    from tcadhip import physics as ph
+    _say("[batch] 3 diodes together vs alone")
+    specs = configs.batch_diodes(count=3, nx_range=(20, 40), ny_range=(12, 20))
+    opts = NewtonOptions()
+    alone = []
+    for s in specs:
+        dev = device_from_spec(s)
+        dev.solve_equilibrium(opts)
+        dev.solve_bias(s.sweep[0], opts)
+        alone.append((dev.last_iterations, dev.terminal_currents(), dev.psi))
+    setups = []
+    for s in specs:
+        dev = device_from_spec(s)
+        dev._set_voltages(s.sweep[0])
+        setups.append(dev._s)
+    B = GPUBatch(setups)
+    B.solve_equilibrium(opts)
+    its, conv, _ = B.solve_bias(opts)
+    I = B.terminal_currents()
+    psi = B.psi.cpu().numpy()
+    c = 0
+    for d, (ait, aI, apsi) in enumerate(alone):
+        assert its[d] == ait
+        got = I[c:c + len(aI)]
+        ref = np.array(list(aI.values()))
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max() + 1e-300
+        assert np.array_equal(psi[B.device_slice(d)].reshape(apsi.shape), apsi)
+        c += len(aI)
