@@ -126,6 +126,7 @@ def test_fd_jacobian_3d():
 def test_terminal_current_conservation_3d():
     dev = _gated_3d(False)
     dev.solve_equilibrium()
-    dev.solve_bias({"d": 0.3, "g": 0.4})
+    dev.solve_bias({"d": -0.4, "g": 0.4})          # forward-biased junction
     Is, Id = dev.terminal_current("s"), dev.terminal_current("d")
+    assert abs(Id) > 1e-12
     assert abs(Is + Id) <= 1e-6 * max(abs(Is), abs(Id))
