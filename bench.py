@@ -48,10 +48,10 @@ def run_sweep(dev, spec, opts, lat=None):
     return out
 
 
-def measure(spec, linsolve, steps, warmup):
+def measure(spec, linsolve, steps, warmup, rtol=1e-10):
     import torch
     from tcadhip.device import NewtonOptions, device_from_spec
-    opts = NewtonOptions(linsolve=linsolve)
+    opts = NewtonOptions(linsolve=linsolve, linsolve_rtol=rtol)
     dev = device_from_spec(spec)
     for _ in range(warmup):
         run_sweep(dev, spec, opts)
@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--points", type=int, default=None,
                     help="first K bias points only (short profiling runs)")
     ap.add_argument("--modes", default="bicgstab,bicgstab_mixed")
+    ap.add_argument("--rtol", type=float, default=1e-10,
+                    help="relative residual of each Newton linear solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every MI355X box here has one, and this "
@@ -134,7 +136,7 @@ def main():
     for mode in modes:
         print(f"[bench] {mode}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
               flush=True)
-        res[mode] = measure(spec, mode, args.steps, args.warmup)
+        res[mode] = measure(spec, mode, args.steps, args.warmup, args.rtol)
         r = res[mode]
         print(f"[bench] {mode}: {npts * args.steps / r['seconds']:.3f} points/s "
               f"newton={r['newton']} krylov={r['krylov']}", file=sys.stderr, flush=True)

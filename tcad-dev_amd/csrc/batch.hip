@@ -50,9 +50,13 @@ __device__ __forceinline__ double ublock_max(double v, double *sh) {
 // Damped Newton update of solve_bias (device2d.py:2079-2100,
 // device3d.py:2170-2185): du = Cc*y; psi += clip(du_psi, +-max_dpsi);
 // n, p clipped to [0.1x, 10x]; err = max(|dpsi|, rel dn, rel dp) with the
-// density floor 1e-10.  NaN in the update propagates to err.
+// density floor 1e-10.  NaN in the update propagates to err.  With
+// apply == 0 only the measure err is formed (the state is untouched), so
+// a step that passes the test can have its linear solve refined before it
+// is applied; apply == 1 writes the state.
 __global__ void __launch_bounds__(BS) k_newton_update(UArgs U, const double *y, double *psi,
-                                                      double *n, double *p, double max_dpsi) {
+                                                      double *n, double *p, double max_dpsi,
+                                                      int apply) {
   __shared__ double sh[BS / 64];
   const int d = U.blk2dev[blockIdx.x];
   const DevInfo &I = U.info[d];
@@ -71,12 +75,62 @@ __global__ void __launch_bounds__(BS) k_newton_update(UArgs U, const double *y, 
     const double rp = fabs(p1 - p0) / fmax(p0, 1e-10);
     err = fmax(fmax(fabs(dpsi), rn), rp);
     if (!(err == err) || !isfinite(n1) || !isfinite(p1) || !isfinite(dpsi)) err = INFINITY;
-    psi[g] = psi[g] + dpsi;
-    n[g] = n1;
-    p[g] = p1;
+    if (apply) {
+      psi[g] = psi[g] + dpsi;
+      n[g] = n1;
+      p[g] = p1;
+    }
   }
+  if (apply) return;
   const double m = ublock_max(err, sh);
   if (threadIdx.x == 0) U.part[blockIdx.x] = m;
+}
+
+// One workgroup per device: the update measure of this step and whether it
+// passes the convergence test (the step is applied afterwards).
+__global__ void k_newton_check(UArgs U, double tol) {
+  __shared__ double sh[BS / 64];
+  const int d = blockIdx.x;
+  const DevInfo &I = U.info[d];
+  Scal &S = U.scal[d];
+  if (!S.active) return;
+  double m = 0.0;
+  for (int t = threadIdx.x; t < I.nblk; t += blockDim.x) m = fmax(m, U.part[I.blk0 + t]);
+  m = ublock_max(m, sh);
+  if (threadIdx.x == 0) {
+    S.err = m;
+    S.conv_now = m < tol ? 1 : 0;
+  }
+}
+
+// Refinement mask: only the devices whose step just converged stay active.
+__global__ void k_mask_converging(Scal *scal, int ndev) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndev) return;
+  Scal &S = scal[d];
+  S.newton_active = S.active;
+  S.active = S.active && S.conv_now;
+}
+
+__global__ void k_unmask(Scal *scal, int ndev) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < ndev) scal[d].active = scal[d].newton_active;
+}
+
+// After the step is applied: count it, retire converged or failed devices.
+__global__ void k_newton_finish(Scal *scal, int ndev) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndev) return;
+  Scal &S = scal[d];
+  if (!S.active) return;
+  S.newton_it += 1;
+  if (S.conv_now) {
+    S.newton_done = 1;
+    S.active = 0;
+  } else if (!isfinite(S.err)) {
+    S.status = 3;
+    S.active = 0;
+  }
 }
 
 // Equilibrium update (device2d.py:1335-1341): psi += clip(d), err = max|d|
@@ -237,24 +291,40 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
   b.launches++;
   UArgs U = uargs(b);
   const bool mixed = mode == TCAD_LIN_BICGSTAB_MIXED;
+  // inner BiCGStab tolerance per refinement sweep: the FP64-stored matrix
+  // reaches the outer target in one sweep; the FP32-stored one is limited
+  // by its storage rounding
+  const double inner = mixed ? 1e-6 : rtol;
+  const dim3 gd((b.ndev + 63) / 64), td(64);
   for (int it = 0; it < max_iter; ++it) {
     if ((rc = launch_surface_mobility(b, psi))) return rc;
     if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, mixed))) return rc;
-    if (mixed) {
-      hipLaunchKernelGGL(k_ir_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
-                         b.ndev);
+    if ((rc = ir_solve(b, mixed, true, rtol, 16, maxit, inner))) return rc;
+    // measure the step, and before applying a step that passes the test
+    // refine its solve to a true residual of 1e-14: the converged state is
+    // then the reference's (exact Newton steps converge quadratically, so
+    // its final state is exact to rounding), which the terminal current
+    // needs where it is a cancellation of much larger fluxes (reverse bias)
+    hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
+                       max_dpsi, 0);
+    hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol);
+    b.launches += 2;
+    if ((rc = check_flag(b))) return rc;
+    if ((rc = sync_scalars(b))) return rc;
+    int conv = 0;
+    for (int d = 0; d < b.ndev; ++d) conv += b.h_scal[d].active && b.h_scal[d].conv_now;
+    if (conv && rtol > 1e-14) {
+      hipLaunchKernelGGL(k_mask_converging, gd, td, 0, b.stream, b.d_scal, b.ndev);
       b.launches++;
-      if ((rc = ir_solve(b, rtol, 12, maxit, 1e-6))) return rc;
-    } else {
-      if ((rc = bicgstab(b, false, rtol, maxit, true, 16))) return rc;
+      if ((rc = ir_solve(b, mixed, false, 1e-14, 16, maxit, inner))) return rc;
+      hipLaunchKernelGGL(k_unmask, gd, td, 0, b.stream, b.d_scal, b.ndev);
+      b.launches++;
     }
     hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
-                       max_dpsi);
-    hipLaunchKernelGGL(k_newton_finalize, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol,
-                       mixed ? 0 : 1);
+                       max_dpsi, 1);
+    hipLaunchKernelGGL(k_newton_finish, gd, td, 0, b.stream, b.d_scal, b.ndev);
     b.launches += 2;
     TCAD_HIP_CHECK(hipGetLastError());
-    if ((rc = check_flag(b))) return rc;
     if ((rc = sync_scalars(b))) return rc;
     int open = 0;
     for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active;
@@ -660,14 +730,14 @@ int tcad_solve_linear(tcad_batch *bp, int32_t mode, double rtol, int32_t maxiter
   }
   int rc = tcad::launch_set_active(b, 1);
   if (rc) return rc;
-  if (mode == TCAD_LIN_BICGSTAB_MIXED) rc = tcad::ir_solve(b, rtol, 12, maxiter, 1e-6);
-  else rc = tcad::bicgstab(b, false, rtol, maxiter, true, 16);
+  const bool mixed = mode == TCAD_LIN_BICGSTAB_MIXED;
+  rc = tcad::ir_solve(b, mixed, true, rtol, 16, maxiter, mixed ? 1e-6 : rtol);
   if (rc) return rc;
   if ((rc = tcad::sync_scalars(b))) return rc;
   int worst = 0;
   for (int d = 0; d < b.ndev; ++d) {
     const tcad::Scal &S = b.h_scal[d];
-    if (h_iters) h_iters[d] = mode == TCAD_LIN_BICGSTAB_MIXED ? S.kry_total : S.it;
+    if (h_iters) h_iters[d] = S.kry_total;
     if (h_resid) h_resid[d] = S.bb > 0 ? std::sqrt(S.rr / S.bb) : 0.0;
     if (S.status) worst = 1;
   }

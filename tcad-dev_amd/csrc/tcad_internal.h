@@ -29,7 +29,10 @@
 
 namespace tcad {
 
-constexpr int BS = 256;             // threads per node block
+#ifndef TCAD_BS
+#define TCAD_BS 256
+#endif
+constexpr int BS = TCAD_BS;         // threads per node workgroup (multiple of 64)
 constexpr int NDIAG = 9;
 constexpr int NOFF = 5;
 
@@ -53,17 +56,23 @@ struct DevInfo {
 
 // Per-device Krylov / Newton scalars, device resident.
 struct Scal {
-  double rho[2];                    // parity slots
-  double alpha, omega;
+  double rho2[2];                   // rho by iteration parity
+  double rho, alpha, omega, beta;   // Krylov scalars of the current iteration
   double bb;                        // ||b||^2
   double rr;                        // last ||r||^2
   double nrm;                       // IR residual norm (rhs normalisation)
   double err;                       // Newton update measure
+  double tol2;                      // stop test rr <= tol2 * bb of the current solve
+  int maxit;                        // iteration cap of the current solve
   int it;                           // Krylov iterations of the current solve
   int done;                         // Krylov finished (converged or stopped)
-  int status;                       // 0 ok, 1 breakdown / non-finite
+  int status;                       // 0 ok, 1 breakdown / non-finite / not converged
   int active;                       // device takes part in the current solve
-  int newton_it, newton_done, kry_total, ir_it;
+  int newton_it, newton_done, kry_total, ir_it, ir_maxsweep;
+  int conv_now;                     // this Newton step's update passed the test
+  int saved_active;                 // active flag kept across one refinement
+  int newton_active;                // Newton-level flag kept across the final-step refinement
+  int cnt;                          // arrival counter of the last-workgroup finalize
 };
 
 struct Batch {
@@ -144,7 +153,10 @@ int launch_contact_sums(Batch &b, const double *Fnp, double *d_out);
 
 // krylov.hip
 int bicgstab(Batch &b, bool fp32, double rtol, int maxit, bool zero_guess, int chunk);
-int ir_solve(Batch &b, double rtol, int maxit, int inner_maxit, double inner_rtol);
+// FP64 iterative refinement of the scaled Newton system A64 y = bs around
+// BiCGStab on A32 (fp32) or A64; restart: y = 0 first, else continue from y.
+int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int inner_maxit,
+             double inner_rtol);
 int pcg_poisson(Batch &b, double rtol, int maxit);
 int launch_unscale(Batch &b, const double *n, const double *p, double *du);
 int launch_unscale_state(Batch &b, const double *n, const double *p, double *du);

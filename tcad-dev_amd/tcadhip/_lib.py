@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtcadhip.so")
+# TCADHIP_LIB lets a developer point at another build of the same library
+# (e.g. a launch-shape variant); the default is the in-tree build.
+LIB_PATH = os.environ.get("TCADHIP_LIB") or os.path.join(HERE, "libtcadhip.so")
 
 TCAD_OK = 0
 TCAD_E_INVALID = -1
