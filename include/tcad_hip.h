@@ -13,9 +13,11 @@
  * Unknown layout at this boundary: node k = j*Nx + i (2D),
  * k = kk*Nx*Ny + j*Nx + i (3D) (pytcad/pytcad/mesh2d.py:1-8,
  * mesh3d.py:1-9); state vectors psi, n, p are separate FP64 planes of
- * length N_total (the sum of the batch's node counts, devices
- * concatenated in order); scaled units as in
- * pytcad/pytcad/device2d.py:319-345.
+ * length N_total; each device occupies the node range
+ * [node_offset, node_offset + n) of every plane (packed in order by
+ * default, or at explicit offsets so that a resident pool of devices can
+ * be re-batched without moving their state); N_total is the end of the
+ * last range.  Scaled units as in pytcad/pytcad/device2d.py:319-345.
  *
  * Return codes (pytcad/core/include/tcad/base/errors.hpp:21-67 mapping):
  *   TCAD_OK                0
@@ -49,6 +51,12 @@ extern "C" {
 #define TCAD_LIN_BICGSTAB_F64 0   /* FP64 BiCGStab + node block-Jacobi       */
 #define TCAD_LIN_BICGSTAB_MIXED 1 /* FP32 BiCGStab + block-Jacobi in FP64 IR */
 
+/* Per-device outcome of a Newton solve (h_status). */
+#define TCAD_NEWTON_OK 0             /* converged (or masked out)            */
+#define TCAD_NEWTON_MAX_ITER 1       /* not converged within max_iter        */
+#define TCAD_NEWTON_LINEAR_FAIL 2    /* a linear solve ended above 1e-6      */
+#define TCAD_NEWTON_NONFINITE 3      /* non-finite or singular assembly/step */
+
 typedef struct tcad_batch tcad_batch;
 
 /* Per-device description.  Host arrays are copied at creation. */
@@ -80,6 +88,9 @@ typedef struct {
   const int32_t *h_gate_ptr;
   const int32_t *h_gate_nodes;
   const double *h_gate_kw, *h_gate_psib;
+  /* first node of this device in the batch planes; < 0 packs it right
+   * after the previous device.  Ranges must not overlap. */
+  int64_t node_offset;
 } tcad_device_desc;
 
 /* Static per-node / per-edge coefficient arrays of the whole batch, as
@@ -109,7 +120,7 @@ int tcad_batch_create(const tcad_device_desc *h_devs, int32_t ndev,
                       tcad_batch **out);
 int tcad_batch_destroy(tcad_batch *b);
 
-/* Total node count of the batch. */
+/* Plane length N_total of the batch (end of the last device range). */
 int64_t tcad_batch_nodes(const tcad_batch *b);
 
 /* Contact voltages [V] (sum of ncontacts over devices, batch order) and
@@ -148,24 +159,35 @@ int tcad_solve_linear(tcad_batch *b, int32_t mode, double rtol, int32_t maxiter,
                       double *d_du, int32_t *h_iters, double *h_resid);
 
 /* Equilibrium Poisson Newton from the bulk-neutral guess (or from the
- * given psi if from_guess == 0).  Writes psi, n, p.  h_iters receives
- * Newton iterations per device.  Replaces: Device2D.solve_equilibrium
- * (device2d.py:1308-1373), Device3D.solve_equilibrium
- * (device3d.py:1311-1454). */
+ * given psi if from_guess == 0).  Writes psi, n, p.  h_active (length
+ * ndev, NULL = all) selects the devices solved; the others' state is
+ * not touched.  Per device: Newton iterations, converged flag and a
+ * TCAD_NEWTON_* outcome (each output may be NULL).  Replaces:
+ * Device2D.solve_equilibrium (device2d.py:1308-1373),
+ * Device3D.solve_equilibrium (device3d.py:1311-1454). */
 int tcad_solve_equilibrium(tcad_batch *b, int32_t max_iter, double tol_update,
                            double max_dpsi, int32_t from_guess,
+                           const int32_t *h_active,
                            double *d_psi, double *d_n, double *d_p,
-                           int32_t *h_iters, int32_t *h_converged);
+                           int32_t *h_iters, int32_t *h_converged, int32_t *h_status);
 
 /* Coupled Newton at the biases set by tcad_set_bias, warm-started from
- * (psi, n, p) with contact values stamped.  Writes the converged state
- * in place; per device: Newton iterations, converged flag, total Krylov
- * iterations.  Replaces: Device2D.solve_bias (device2d.py:1923-2191),
- * Device3D.solve_bias (device3d.py:1981-2271). */
+ * (psi, n, p) with contact values stamped.  Writes the state in place.
+ * h_active (length ndev, NULL = all) selects the devices solved; the
+ * others' state is not touched.  Per device (outputs may be NULL):
+ * Newton iterations, converged flag, total Krylov iterations and a
+ * TCAD_NEWTON_* outcome.  A device whose linear solve ends with a
+ * relative residual above 1e-6 stops with TCAD_NEWTON_LINEAR_FAIL
+ * before the step is applied (the reference raises LinearSolveError
+ * there, linsolve.py:973-982); the other devices go on.  Replaces:
+ * Device2D.solve_bias (device2d.py:1923-2191), Device3D.solve_bias
+ * (device3d.py:1981-2271). */
 int tcad_solve_bias(tcad_batch *b, int32_t max_iter, double tol_update,
                     double max_dpsi, int32_t lin_mode, double lin_rtol,
-                    int32_t lin_maxiter, double *d_psi, double *d_n, double *d_p,
-                    int32_t *h_iters, int32_t *h_converged, int32_t *h_krylov);
+                    int32_t lin_maxiter, const int32_t *h_active,
+                    double *d_psi, double *d_n, double *d_p,
+                    int32_t *h_iters, int32_t *h_converged, int32_t *h_krylov,
+                    int32_t *h_status);
 
 /* Terminal currents of every contact of every device at (psi, n, p):
  * h_I[c] = sum over the contact's nodes of the raw F_n + F_p times

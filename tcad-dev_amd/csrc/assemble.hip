@@ -39,7 +39,8 @@ struct AsmArgs {
   int want32;
   double *F, *Fnp, *A64, *Dinv64, *bs, *rowscale;
   float *A32, *Dinv32;
-  int *flag;
+  int *flag;                        // [0] any, [1 + d] per device
+  const Scal *scal;                 // non-NULL: skip devices not active
 };
 
 struct EdgeVals {
@@ -106,7 +107,7 @@ __device__ __forceinline__ void contact_target(const AsmArgs &A, const DevInfo &
 }
 
 template <int DIM>
-__device__ void assemble_node(const AsmArgs &A, const DevInfo &I, long long k) {
+__device__ void assemble_node(const AsmArgs &A, const DevInfo &I, long long k, int dev) {
   const long long N = A.N;
   const long long g = I.off + k;
   int c[3];
@@ -332,7 +333,10 @@ __device__ void assemble_node(const AsmArgs &A, const DevInfo &I, long long k) {
   double Di[9] = {c0 * id, (a2 * a7 - a1 * a8) * id, (a1 * a5 - a2 * a4) * id,
                   c1 * id, (a0 * a8 - a2 * a6) * id, (a2 * a3 - a0 * a5) * id,
                   c2 * id, (a1 * a6 - a0 * a7) * id, (a0 * a4 - a1 * a3) * id};
-  if (bad) atomicOr(A.flag, 1);       // error flag only; never part of a sum
+  if (bad) {                          // error flags only; never part of a sum
+    atomicOr(A.flag, 1);
+    atomicOr(A.flag + 1 + dev, 1);
+  }
 
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -363,11 +367,12 @@ __device__ void assemble_node(const AsmArgs &A, const DevInfo &I, long long k) {
 
 __global__ void __launch_bounds__(BS) k_assemble(AsmArgs A) {
   const int d = A.blk2dev[blockIdx.x];
+  if (A.scal && !A.scal[d].active) return;
   const DevInfo &I = A.info[d];
   const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
   if (k >= I.n) return;
-  if (I.dim == 2) assemble_node<2>(A, I, k);
-  else assemble_node<3>(A, I, k);
+  if (I.dim == 2) assemble_node<2>(A, I, k, d);
+  else assemble_node<3>(A, I, k, d);
 }
 
 static AsmArgs make_args(Batch &b, const double *psi, const double *n, const double *p) {
@@ -393,8 +398,9 @@ static AsmArgs make_args(Batch &b, const double *psi, const double *n, const dou
 }
 
 int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
-                    double *F, double *Fnp, bool want_fp32) {
+                    double *F, double *Fnp, bool want_fp32, bool only_active) {
   AsmArgs A = make_args(b, psi, n, p);
+  A.scal = only_active ? b.d_scal : nullptr;
   A.want32 = want_fp32 ? 1 : 0;
   A.F = F; A.Fnp = Fnp;
   A.A64 = b.A64; A.A32 = b.A32; A.Dinv64 = b.Dinv64; A.Dinv32 = b.Dinv32;
@@ -416,7 +422,7 @@ int launch_assemble(Batch &b, const double *psi, const double *n, const double *
 // centre, x-, x+, y-, y+, z-, z+.
 // ---------------------------------------------------------------------------
 template <int DIM>
-__device__ void assemble_poisson_node(const AsmArgs &A, const DevInfo &I, long long k,
+__device__ void assemble_poisson_node(const AsmArgs &A, const DevInfo &I, long long k, int dev,
                                       double *Aeq, double *beq, double *F) {
   const long long N = A.N;
   const long long g = I.off + k;
@@ -497,7 +503,10 @@ __device__ void assemble_poisson_node(const AsmArgs &A, const DevInfo &I, long l
     for (int dir = 0; dir < 6; ++dir) M[1 + dir] = -offv[dir];
     rhs = -rhs;
   }
-  if (!(M[0] > 0.0) || !isfinite(M[0])) atomicOr(A.flag, 1);
+  if (!(M[0] > 0.0) || !isfinite(M[0])) {
+    atomicOr(A.flag, 1);
+    atomicOr(A.flag + 1 + dev, 1);
+  }
   F[g] = Fv;
   beq[g] = rhs;
   const int nent = 1 + 2 * DIM;
@@ -507,15 +516,17 @@ __device__ void assemble_poisson_node(const AsmArgs &A, const DevInfo &I, long l
 __global__ void __launch_bounds__(BS) k_assemble_poisson(AsmArgs A, double *Aeq, double *beq,
                                                          double *F) {
   const int d = A.blk2dev[blockIdx.x];
+  if (!A.scal[d].active) return;
   const DevInfo &I = A.info[d];
   const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
   if (k >= I.n) return;
-  if (I.dim == 2) assemble_poisson_node<2>(A, I, k, Aeq, beq, F);
-  else assemble_poisson_node<3>(A, I, k, Aeq, beq, F);
+  if (I.dim == 2) assemble_poisson_node<2>(A, I, k, d, Aeq, beq, F);
+  else assemble_poisson_node<3>(A, I, k, d, Aeq, beq, F);
 }
 
 int launch_assemble_poisson(Batch &b, const double *psi, double *F) {
   AsmArgs A = make_args(b, psi, nullptr, nullptr);
+  A.scal = b.d_scal;
   hipLaunchKernelGGL(k_assemble_poisson, dim3(b.nblk), dim3(BS), 0, b.stream, A, b.Aeq,
                      b.beq, F);
   b.launches++;
@@ -528,12 +539,13 @@ int launch_assemble_poisson(Batch &b, const double *psi, double *F) {
 // (device2d.py:612-642): one thread per surface column of each 2D device
 // with the model on.  Rewrites an_y / ap_y of row 0.
 // ---------------------------------------------------------------------------
-__global__ void k_surface_mobility(const DevInfo *info, int ndev, const double *psi,
-                                   const double *hy, double *an_y, double *ap_y,
-                                   const double *mun0, const double *mun1, const double *mup0,
-                                   const double *mup1) {
+__global__ void k_surface_mobility(const DevInfo *info, int ndev, const Scal *scal,
+                                   const double *psi, const double *hy, double *an_y,
+                                   double *ap_y, const double *mun0, const double *mun1,
+                                   const double *mup0, const double *mup1) {
   const int d = blockIdx.y;
   if (d >= ndev) return;
+  if (scal && !scal[d].active) return;
   const DevInfo &I = info[d];
   if (!I.surf || I.dim != 2) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -550,12 +562,13 @@ __global__ void k_surface_mobility(const DevInfo *info, int ndev, const double *
   ap_y[g] = dp / hy0;
 }
 
-int launch_surface_mobility(Batch &b, const double *psi) {
+int launch_surface_mobility(Batch &b, const double *psi, bool only_active) {
   if (!b.any_surf) return TCAD_OK;
   int maxnx = 1;
   for (auto &I : b.info) maxnx = I.nx > maxnx ? I.nx : maxnx;
   dim3 grid((maxnx + 127) / 128, b.ndev);
-  hipLaunchKernelGGL(k_surface_mobility, grid, dim3(128), 0, b.stream, b.d_info, b.ndev, psi,
+  hipLaunchKernelGGL(k_surface_mobility, grid, dim3(128), 0, b.stream, b.d_info, b.ndev,
+                     only_active ? b.d_scal : nullptr, psi,
                      b.d_h[1], const_cast<double *>(b.f.d_an[1]),
                      const_cast<double *>(b.f.d_ap[1]), b.d_surf[0], b.d_surf[1], b.d_surf[2],
                      b.d_surf[3]);

@@ -87,8 +87,10 @@ __global__ void __launch_bounds__(BS) k_newton_update(UArgs U, const double *y, 
 }
 
 // One workgroup per device: the update measure of this step and whether it
-// passes the convergence test (the step is applied afterwards).
-__global__ void k_newton_check(UArgs U, double tol) {
+// passes the convergence test (the step is applied afterwards).  A device
+// whose linear solve ended above the acceptance residual stops here, before
+// its step is applied (linsolve.py:973-982 refuses such an iterate).
+__global__ void k_newton_check(UArgs U, double tol, double accept2) {
   __shared__ double sh[BS / 64];
   const int d = blockIdx.x;
   const DevInfo &I = U.info[d];
@@ -100,7 +102,24 @@ __global__ void k_newton_check(UArgs U, double tol) {
   if (threadIdx.x == 0) {
     S.err = m;
     S.conv_now = m < tol ? 1 : 0;
+    if (!(S.rr <= accept2 * S.bb)) {
+      S.conv_now = 0;
+      S.active = 0;
+      S.result = TCAD_NEWTON_LINEAR_FAIL;
+    }
   }
+}
+
+// Per-device assembly error flags -> the device stops as non-finite.
+__global__ void k_flags_to_status(Scal *scal, int *flag, int ndev) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndev) return;
+  if (flag[1 + d]) {
+    flag[1 + d] = 0;
+    scal[d].active = 0;
+    scal[d].result = TCAD_NEWTON_NONFINITE;
+  }
+  if (d == 0) flag[0] = 0;
 }
 
 // Refinement mask: only the devices whose step just converged stay active.
@@ -128,7 +147,7 @@ __global__ void k_newton_finish(Scal *scal, int ndev) {
     S.newton_done = 1;
     S.active = 0;
   } else if (!isfinite(S.err)) {
-    S.status = 3;
+    S.result = TCAD_NEWTON_NONFINITE;
     S.active = 0;
   }
 }
@@ -172,17 +191,21 @@ __global__ void k_newton_finalize(UArgs U, double tol, int add_krylov) {
       S.newton_done = 1;
       S.active = 0;
     } else if (!isfinite(m)) {
-      S.status = 3;
+      S.result = TCAD_NEWTON_NONFINITE;
       S.active = 0;
     }
   }
 }
 
-__global__ void k_set_active(Scal *scal, int ndev) {
+// Select the devices of the next Newton solve (mask NULL = all) and reset
+// their counters.
+__global__ void k_set_active(Scal *scal, int ndev, const int *mask) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndev) return;
   Scal &S = scal[d];
-  S.active = 1;
+  S.sel = mask ? (mask[d] != 0) : 1;
+  S.active = S.sel;
+  S.result = 0;
   S.newton_it = 0;
   S.newton_done = 0;
   S.kry_total = 0;
@@ -202,8 +225,10 @@ __global__ void __launch_bounds__(BS) k_stamp(const DevInfo *info, const int *bl
                                               const int *cidx, const double *lnr,
                                               const double *n0, const double *p0,
                                               const int *ccontact, const double *V,
-                                              double *psi, double *n, double *p) {
+                                              const Scal *scal, double *psi, double *n,
+                                              double *p) {
   const int d = blk2dev[blockIdx.x];
+  if (!scal[d].sel) return;
   const DevInfo &I = info[d];
   const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
   if (k >= I.n) return;
@@ -218,8 +243,9 @@ __global__ void __launch_bounds__(BS) k_stamp(const DevInfo *info, const int *bl
 // Bulk-neutral guess arcsinh(C / (2 nie_s)) (device2d.py:744-759).
 __global__ void __launch_bounds__(BS) k_bulk_guess(const DevInfo *info, const int *blk2dev,
                                                    const double *C, const double *nie_s,
-                                                   double *psi) {
+                                                   const Scal *scal, double *psi) {
   const int d = blk2dev[blockIdx.x];
+  if (!scal[d].sel) return;
   const DevInfo &I = info[d];
   const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
   if (k >= I.n) return;
@@ -229,9 +255,10 @@ __global__ void __launch_bounds__(BS) k_bulk_guess(const DevInfo *info, const in
 
 // Slaved equilibrium densities (device2d.py:1365-1367).
 __global__ void __launch_bounds__(BS) k_slaved(const DevInfo *info, const int *blk2dev,
-                                               const double *nie_s, const double *psi,
-                                               double *n, double *p) {
+                                               const double *nie_s, const Scal *scal,
+                                               const double *psi, double *n, double *p) {
   const int d = blk2dev[blockIdx.x];
+  if (!scal[d].sel) return;
   const DevInfo &I = info[d];
   const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
   if (k >= I.n) return;
@@ -259,9 +286,20 @@ int sync_scalars(Batch &b) {
   return TCAD_OK;
 }
 
-int launch_set_active(Batch &b, int) {
+int launch_set_active(Batch &b, const int32_t *h_mask) {
+  if (h_mask)
+    TCAD_HIP_CHECK(hipMemcpyAsync(b.d_mask, h_mask, sizeof(int) * b.ndev, hipMemcpyHostToDevice,
+                                  b.stream));
   hipLaunchKernelGGL(k_set_active, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
-                     b.ndev);
+                     b.ndev, h_mask ? (const int *)b.d_mask : nullptr);
+  b.launches++;
+  TCAD_HIP_CHECK(hipGetLastError());
+  return TCAD_OK;
+}
+
+static int launch_flags_to_status(Batch &b) {
+  hipLaunchKernelGGL(k_flags_to_status, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream,
+                     b.d_scal, b.d_flag, b.ndev);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
   return TCAD_OK;
@@ -272,7 +310,7 @@ static int check_flag(Batch &b) {
   TCAD_HIP_CHECK(hipMemcpyAsync(&flag, b.d_flag, sizeof(int), hipMemcpyDeviceToHost, b.stream));
   TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
   if (flag) {
-    TCAD_HIP_CHECK(hipMemsetAsync(b.d_flag, 0, sizeof(int), b.stream));
+    TCAD_HIP_CHECK(hipMemsetAsync(b.d_flag, 0, sizeof(int) * (b.ndev + 1), b.stream));
     set_error("assembly produced a non-finite or singular node block");
     return TCAD_E_NONFINITE;
   }
@@ -283,11 +321,13 @@ static int check_flag(Batch &b) {
 // Newton drivers
 // ---------------------------------------------------------------------------
 static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, int mode,
-                           double rtol, int maxit, double *psi, double *n, double *p) {
+                           double rtol, int maxit, const int32_t *mask, double *psi, double *n,
+                           double *p) {
   int rc;
-  if ((rc = launch_set_active(b, 1))) return rc;
+  if ((rc = launch_set_active(b, mask))) return rc;
   hipLaunchKernelGGL(k_stamp, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev, b.d_cidx,
-                     b.d_cn_lnr, b.d_cn_n0, b.d_cn_p0, b.d_cn_contact, b.d_contact_V, psi, n, p);
+                     b.d_cn_lnr, b.d_cn_n0, b.d_cn_p0, b.d_cn_contact, b.d_contact_V, b.d_scal,
+                     psi, n, p);
   b.launches++;
   UArgs U = uargs(b);
   const bool mixed = mode == TCAD_LIN_BICGSTAB_MIXED;
@@ -295,10 +335,12 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
   // reaches the outer target in one sweep; the FP32-stored one is limited
   // by its storage rounding
   const double inner = mixed ? 1e-6 : rtol;
+  const double accept = rtol > 1e-6 ? rtol : 1e-6;
   const dim3 gd((b.ndev + 63) / 64), td(64);
   for (int it = 0; it < max_iter; ++it) {
-    if ((rc = launch_surface_mobility(b, psi))) return rc;
-    if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, mixed))) return rc;
+    if ((rc = launch_surface_mobility(b, psi, true))) return rc;
+    if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, mixed, true))) return rc;
+    if ((rc = launch_flags_to_status(b))) return rc;
     if ((rc = ir_solve(b, mixed, true, rtol, 16, maxit, inner))) return rc;
     // measure the step, and before applying a step that passes the test
     // refine its solve to a true residual of 1e-14: the converged state is
@@ -307,9 +349,10 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
     // needs where it is a cancellation of much larger fluxes (reverse bias)
     hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
                        max_dpsi, 0);
-    hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol);
+    hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol,
+                       accept * accept);
     b.launches += 2;
-    if ((rc = check_flag(b))) return rc;
+    TCAD_HIP_CHECK(hipGetLastError());
     if ((rc = sync_scalars(b))) return rc;
     int conv = 0;
     for (int d = 0; d < b.ndev; ++d) conv += b.h_scal[d].active && b.h_scal[d].conv_now;
@@ -334,31 +377,32 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
 }
 
 static int solve_equilibrium_impl(Batch &b, int max_iter, double tol, double max_dpsi,
-                                  int from_guess, double *psi, double *n, double *p) {
+                                  int from_guess, const int32_t *mask, double *psi, double *n,
+                                  double *p) {
   int rc;
-  if ((rc = launch_set_active(b, 1))) return rc;
+  if ((rc = launch_set_active(b, mask))) return rc;
   if (from_guess) {
     hipLaunchKernelGGL(k_bulk_guess, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev,
-                       b.f.d_C, b.f.d_nie_s, psi);
+                       b.f.d_C, b.f.d_nie_s, b.d_scal, psi);
     b.launches++;
   }
   UArgs U = uargs(b);
   for (int it = 0; it < max_iter; ++it) {
     if ((rc = launch_assemble_poisson(b, psi, b.F))) return rc;
+    if ((rc = launch_flags_to_status(b))) return rc;
     if ((rc = pcg_poisson(b, 1e-12, 20000))) return rc;
     hipLaunchKernelGGL(k_poisson_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.veq[0], psi,
                        max_dpsi);
     hipLaunchKernelGGL(k_newton_finalize, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol, 1);
     b.launches += 2;
     TCAD_HIP_CHECK(hipGetLastError());
-    if ((rc = check_flag(b))) return rc;
     if ((rc = sync_scalars(b))) return rc;
     int open = 0;
     for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active;
     if (open == 0) break;
   }
   hipLaunchKernelGGL(k_slaved, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev,
-                     b.f.d_nie_s, psi, n, p);
+                     b.f.d_nie_s, b.d_scal, psi, n, p);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
   return TCAD_OK;
@@ -397,7 +441,7 @@ static void free_batch(Batch *b) {
                   b->d_cscale, b->d_c2dev, b->d_surf[0], b->d_surf[1], b->d_surf[2],
                   b->d_surf[3], b->A64, b->A32, b->Dinv64, b->Dinv32, b->bs, b->rowscale,
                   b->colscale, b->y64, b->F, b->Fnp, b->part, b->d_scal, b->Aeq, b->beq,
-                  b->d_flag};
+                  b->d_flag, b->d_mask};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (int i = 0; i < 9; ++i)
@@ -434,7 +478,8 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   std::vector<int> cidx, gidx, cn_contact, gn_gate, cptr{0}, cnodes;
   std::vector<double> cn_lnr, cn_n0, cn_p0, gn_kw, gn_psib, cscale;
   std::vector<double> surf[4];
-  long long off = 0;
+  long long off = 0, end_max = 0;
+  std::vector<std::pair<long long, long long>> ranges;
   int blk = 0, cglobal = 0, gglobal = 0;
   b->info.resize(ndev);
   for (int d = 0; d < ndev; ++d) {
@@ -455,8 +500,11 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
     I.nx = D.nx;
     I.ny = D.ny;
     I.nz = nz;
+    if (D.node_offset >= 0) off = D.node_offset;
     I.off = off;
     I.n = (long long)D.nx * D.ny * nz;
+    ranges.emplace_back(I.off, I.off + I.n);
+    end_max = std::max(end_max, I.off + I.n);
     I.blk0 = blk;
     I.nblk = (int)((I.n + BS - 1) / BS);
     const int n_ax[3] = {D.nx, D.ny, nz};
@@ -495,8 +543,10 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
     }
     for (int t = 0; t < I.nblk; ++t) blk2dev.push_back(d);
     // boundary tables
-    cidx.resize(off + I.n, -1);
-    gidx.resize(off + I.n, -1);
+    if ((long long)cidx.size() < off + I.n) {
+      cidx.resize(off + I.n, -1);
+      gidx.resize(off + I.n, -1);
+    }
     I.c0 = cglobal;
     I.nc = D.ncontacts;
     for (int c = 0; c < D.ncontacts; ++c) {
@@ -552,7 +602,14 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
     off += I.n;
     blk += I.nblk;
   }
-  b->ntot = off;
+  std::sort(ranges.begin(), ranges.end());
+  for (size_t r = 1; r < ranges.size(); ++r)
+    if (ranges[r].first < ranges[r - 1].second) {
+      set_error("tcad_batch_create: device node ranges overlap");
+      free_batch(b);
+      return TCAD_E_INVALID;
+    }
+  b->ntot = end_max;
   b->nblk = blk;
   b->ne = nentries(b->dim_max);
   b->ncontact = cglobal;
@@ -607,6 +664,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   TRY(dalloc(&b->beq, N));
   for (int i = 0; i < 5; ++i) TRY(dalloc(&b->veq[i], N));
   TRY(dalloc(&b->d_flag, (size_t)ndev + 1));
+  TRY(dalloc(&b->d_mask, (size_t)ndev));
 #undef TRY
   TCAD_HIP_CHECK(hipMemsetAsync(b->d_flag, 0, sizeof(int) * (ndev + 1), s));
   TCAD_HIP_CHECK(hipMemsetAsync(b->d_scal, 0, sizeof(Scal) * ndev, s));
@@ -698,8 +756,7 @@ int tcad_assemble(tcad_batch *bp, const double *psi, const double *n, const doub
   }
   int rc = tcad::launch_assemble(b, psi, n, p, F, Fnp, true);
   if (rc) return rc;
-  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
-  return TCAD_OK;
+  return tcad::check_flag(b);
 }
 
 int tcad_export_jacobian(tcad_batch *bp, double *h_blocks, double *h_rowscale, double *h_rhs,
@@ -728,7 +785,7 @@ int tcad_solve_linear(tcad_batch *bp, int32_t mode, double rtol, int32_t maxiter
     tcad::set_error("tcad_solve_linear: rtol must be > 0 and maxiter > 0");
     return TCAD_E_INVALID;
   }
-  int rc = tcad::launch_set_active(b, 1);
+  int rc = tcad::launch_set_active(b, nullptr);
   if (rc) return rc;
   const bool mixed = mode == TCAD_LIN_BICGSTAB_MIXED;
   rc = tcad::ir_solve(b, mixed, true, rtol, 16, maxiter, mixed ? 1e-6 : rtol);
@@ -756,29 +813,39 @@ int tcad_solve_linear(tcad_batch *bp, int32_t mode, double rtol, int32_t maxiter
   return TCAD_OK;
 }
 
+// Per-device TCAD_NEWTON_* outcome after a Newton solve.
+static int newton_outcome(const tcad::Scal &S) {
+  if (!S.sel || S.newton_done) return TCAD_NEWTON_OK;
+  return S.result ? S.result : TCAD_NEWTON_MAX_ITER;
+}
+
 int tcad_solve_equilibrium(tcad_batch *bp, int32_t max_iter, double tol_update, double max_dpsi,
-                           int32_t from_guess, double *psi, double *n, double *p,
-                           int32_t *h_iters, int32_t *h_converged) {
+                           int32_t from_guess, const int32_t *h_active, double *psi, double *n,
+                           double *p, int32_t *h_iters, int32_t *h_converged,
+                           int32_t *h_status) {
   GUARD(bp);
   Batch &b = *bp->impl;
   if (!psi || !n || !p || max_iter <= 0) {
     tcad::set_error("tcad_solve_equilibrium: invalid argument");
     return TCAD_E_INVALID;
   }
-  int rc = tcad::solve_equilibrium_impl(b, max_iter, tol_update, max_dpsi, from_guess, psi, n, p);
+  int rc = tcad::solve_equilibrium_impl(b, max_iter, tol_update, max_dpsi, from_guess, h_active,
+                                        psi, n, p);
   if (rc) return rc;
   if ((rc = tcad::sync_scalars(b))) return rc;
   for (int d = 0; d < b.ndev; ++d) {
     if (h_iters) h_iters[d] = b.h_scal[d].newton_it;
     if (h_converged) h_converged[d] = b.h_scal[d].newton_done;
+    if (h_status) h_status[d] = newton_outcome(b.h_scal[d]);
   }
   return TCAD_OK;
 }
 
 int tcad_solve_bias(tcad_batch *bp, int32_t max_iter, double tol_update, double max_dpsi,
-                    int32_t lin_mode, double lin_rtol, int32_t lin_maxiter, double *psi,
-                    double *n, double *p, int32_t *h_iters, int32_t *h_converged,
-                    int32_t *h_krylov) {
+                    int32_t lin_mode, double lin_rtol, int32_t lin_maxiter,
+                    const int32_t *h_active, double *psi, double *n, double *p,
+                    int32_t *h_iters, int32_t *h_converged, int32_t *h_krylov,
+                    int32_t *h_status) {
   GUARD(bp);
   Batch &b = *bp->impl;
   if (!psi || !n || !p || max_iter <= 0 || lin_maxiter <= 0 || !(lin_rtol > 0.0)) {
@@ -790,13 +857,14 @@ int tcad_solve_bias(tcad_batch *bp, int32_t max_iter, double tol_update, double 
     return TCAD_E_INVALID;
   }
   int rc = tcad::solve_bias_impl(b, max_iter, tol_update, max_dpsi, lin_mode, lin_rtol,
-                                 lin_maxiter, psi, n, p);
+                                 lin_maxiter, h_active, psi, n, p);
   if (rc) return rc;
   if ((rc = tcad::sync_scalars(b))) return rc;
   for (int d = 0; d < b.ndev; ++d) {
     if (h_iters) h_iters[d] = b.h_scal[d].newton_it;
     if (h_converged) h_converged[d] = b.h_scal[d].newton_done;
     if (h_krylov) h_krylov[d] = b.h_scal[d].kry_total;
+    if (h_status) h_status[d] = newton_outcome(b.h_scal[d]);
   }
   return TCAD_OK;
 }

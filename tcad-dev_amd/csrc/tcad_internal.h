@@ -72,7 +72,8 @@ struct Scal {
   int conv_now;                     // this Newton step's update passed the test
   int saved_active;                 // active flag kept across one refinement
   int newton_active;                // Newton-level flag kept across the final-step refinement
-  int cnt;                          // arrival counter of the last-workgroup finalize
+  int sel;                          // device selected for the current Newton solve
+  int result;                       // TCAD_NEWTON_* outcome of the current Newton solve
 };
 
 struct Batch {
@@ -124,7 +125,8 @@ struct Batch {
   double *Aeq = nullptr;                         // (1 + 2*dim) entries per node
   double *beq = nullptr;
   double *veq[6] = {};
-  int *d_flag = nullptr;                         // device-wide error flag
+  int *d_flag = nullptr;                         // [0] any, [1 + d] per-device error flag
+  int *d_mask = nullptr;                         // per-device selection of the next solve
   const double *last_n = nullptr, *last_p = nullptr;   // state of the last assembly
   const double *last_psi = nullptr;
   std::vector<int> dev_ncontact, dev_ngate;
@@ -143,9 +145,10 @@ void set_error(const std::string &msg);
   } while (0)
 
 // assemble.hip
+// only_active: skip devices whose Newton solve is not running (Scal.active)
 int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
-                    double *F, double *Fnp, bool want_fp32);
-int launch_surface_mobility(Batch &b, const double *psi);
+                    double *F, double *Fnp, bool want_fp32, bool only_active = false);
+int launch_surface_mobility(Batch &b, const double *psi, bool only_active = false);
 int launch_assemble_poisson(Batch &b, const double *psi, double *F);
 int launch_edge_currents(Batch &b, const double *psi, const double *n, const double *p,
                          double *J);
@@ -170,7 +173,8 @@ int launch_poisson_update(Batch &b, double *psi, double max_dpsi, double tol);
 int launch_stamp_contacts(Batch &b, double *psi, double *n, double *p, int zero_bias);
 int launch_bulk_guess(Batch &b, double *psi);
 int launch_slaved_densities(Batch &b, const double *psi, double *n, double *p);
-int launch_set_active(Batch &b, int all);
+// select the devices of the next Newton solve (h_mask NULL = all)
+int launch_set_active(Batch &b, const int32_t *h_mask);
 int sync_scalars(Batch &b);
 
 }  // namespace tcad

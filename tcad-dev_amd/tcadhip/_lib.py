@@ -25,6 +25,11 @@ TCAD_E_NO_DEVICE = -6
 LIN_BICGSTAB_F64 = 0
 LIN_BICGSTAB_MIXED = 1
 
+NEWTON_OK = 0
+NEWTON_MAX_ITER = 1
+NEWTON_LINEAR_FAIL = 2
+NEWTON_NONFINITE = 3
+
 c_double_p = ctypes.POINTER(ctypes.c_double)
 c_int32_p = ctypes.POINTER(ctypes.c_int32)
 
@@ -51,6 +56,7 @@ class DeviceDesc(ctypes.Structure):
         ("h_contact_n0", c_double_p), ("h_contact_p0", c_double_p),
         ("ngates", ctypes.c_int32), ("h_gate_ptr", c_int32_p), ("h_gate_nodes", c_int32_p),
         ("h_gate_kw", c_double_p), ("h_gate_psib", c_double_p),
+        ("node_offset", ctypes.c_int64),
     ]
 
 
@@ -90,11 +96,12 @@ def lib():
         "tcad_solve_linear": ([vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, vp,
                                c_int32_p, c_double_p], ctypes.c_int),
         "tcad_solve_equilibrium": ([vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
-                                    ctypes.c_int32, vp, vp, vp, c_int32_p, c_int32_p],
-                                   ctypes.c_int),
+                                    ctypes.c_int32, c_int32_p, vp, vp, vp, c_int32_p,
+                                    c_int32_p, c_int32_p], ctypes.c_int),
         "tcad_solve_bias": ([vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
-                             ctypes.c_int32, ctypes.c_double, ctypes.c_int32, vp, vp, vp,
-                             c_int32_p, c_int32_p, c_int32_p], ctypes.c_int),
+                             ctypes.c_int32, ctypes.c_double, ctypes.c_int32, c_int32_p,
+                             vp, vp, vp, c_int32_p, c_int32_p, c_int32_p, c_int32_p],
+                            ctypes.c_int),
         "tcad_terminal_currents": ([vp, vp, vp, vp, c_double_p], ctypes.c_int),
         "tcad_edge_currents": ([vp, vp, vp, vp, vp], ctypes.c_int),
         "tcad_surface_mobility": ([vp, vp], ctypes.c_int),

@@ -169,8 +169,9 @@ class DeviceSetup:
     def gates(self):
         return [(nm, bc) for nm, bc in self.bcs.items() if isinstance(bc, GateBC)]
 
-    def description(self, keep):
-        """A DeviceDesc; host arrays it points to are appended to `keep`."""
+    def description(self, keep, node_offset=-1):
+        """A DeviceDesc; host arrays it points to are appended to `keep`.
+        node_offset < 0 packs the device after the previous one."""
         def arr(a, dt=np.float64):
             a = np.ascontiguousarray(a, dtype=dt)
             keep.append(a)
@@ -222,6 +223,7 @@ class DeviceSetup:
         D.h_gate_nodes = _lib.iptr(cat(nodes, np.int32))
         D.h_gate_kw = _lib.dptr(cat(kw, np.float64))
         D.h_gate_psib = _lib.dptr(cat(psib, np.float64))
+        D.node_offset = int(node_offset)
         return D
 
     def node_fields(self):
@@ -244,20 +246,36 @@ def _torch():
 
 
 class GPUBatch:
-    """A libtcadhip batch: the devices of `setups` share every launch."""
+    """A libtcadhip batch: the devices of `setups` share every launch.
+
+    By default the devices are packed into planes this object allocates.
+    With `planes` (a dict of CUDA float64 tensors for FIELD_KEYS and
+    psi, n, p) and `offsets` (first node of each device), the batch works
+    in place on caller-owned planes, where each device's static fields
+    are already written at its range: a resident pool re-batches its
+    devices this way without moving their state."""
 
     FIELD_KEYS = ("C", "nie_s", "nie", "tau_n", "tau_p", "dV",
                   "an0", "an1", "an2", "ap0", "ap1", "ap2", "dln0", "dln1", "dln2")
 
-    def __init__(self, setups):
+    def __init__(self, setups, offsets=None, planes=None):
         torch = _torch()
         L = _lib.lib()
         self.setups = list(setups)
-        self.offsets = np.cumsum([0] + [s.N for s in self.setups])
-        self.N = int(self.offsets[-1])
-        host = {k: np.concatenate([s.node_fields()[k] for s in self.setups])
-                for k in self.FIELD_KEYS}
-        self._t = {k: torch.from_numpy(v).to("cuda") for k, v in host.items()}
+        if planes is None:
+            self.offsets = np.cumsum([0] + [s.N for s in self.setups])[:-1]
+            host = {k: np.concatenate([s.node_fields()[k] for s in self.setups])
+                    for k in self.FIELD_KEYS}
+            self._t = {k: torch.from_numpy(v).to("cuda") for k, v in host.items()}
+            N = sum(s.N for s in self.setups)
+            self.psi = torch.zeros(N, dtype=torch.float64, device="cuda")
+            self.n = torch.zeros_like(self.psi)
+            self.p = torch.zeros_like(self.psi)
+        else:
+            self.offsets = np.asarray(offsets, dtype=np.int64)
+            self._t = {k: planes[k] for k in self.FIELD_KEYS}
+            self.psi, self.n, self.p = planes["psi"], planes["n"], planes["p"]
+        self.N = int(max(o + s.N for o, s in zip(self.offsets, self.setups)))
         F = _lib.StaticFields()
         F.d_C, F.d_nie_s, F.d_nie = (self._t[k].data_ptr() for k in ("C", "nie_s", "nie"))
         F.d_tau_n, F.d_tau_p, F.d_dV = (self._t[k].data_ptr() for k in ("tau_n", "tau_p", "dV"))
@@ -267,16 +285,13 @@ class GPUBatch:
             F.d_dlnnie[a] = self._t[f"dln{a}"].data_ptr()
         keep = []
         descs = (_lib.DeviceDesc * len(self.setups))(
-            *[s.description(keep) for s in self.setups])
+            *[s.description(keep, int(o)) for s, o in zip(self.setups, self.offsets)])
         self._handle = ctypes.c_void_p()
         _lib.check(L.tcad_batch_create(descs, len(self.setups), ctypes.byref(F), None,
                                        ctypes.byref(self._handle)), "tcad_batch_create")
         self._keep = keep
         self.contacts = [(d, nm) for d, s in enumerate(self.setups) for nm, _ in s.contacts()]
         self.gates = [(d, nm) for d, s in enumerate(self.setups) for nm, _ in s.gates()]
-        self.psi = torch.zeros(self.N, dtype=torch.float64, device="cuda")
-        self.n = torch.zeros_like(self.psi)
-        self.p = torch.zeros_like(self.psi)
         self.ndev = len(self.setups)
 
     def __del__(self):
@@ -304,27 +319,43 @@ class GPUBatch:
             _lib.dptr(Vg) if Vg.size else None,
             _lib.dptr(Vfb) if Vfb.size else None), "tcad_set_bias")
 
-    def solve_equilibrium(self, opts):
+    def _mask(self, active):
+        if active is None:
+            return None
+        m = np.ascontiguousarray(active, dtype=np.int32)
+        if m.shape != (self.ndev,):
+            raise ValueError(f"active mask must have {self.ndev} entries")
+        return m
+
+    def solve_equilibrium(self, opts, active=None):
+        """Equilibrium of the selected devices; per device (iterations,
+        converged, outcome code)."""
         its = np.zeros(self.ndev, dtype=np.int32)
         conv = np.zeros(self.ndev, dtype=np.int32)
+        status = np.zeros(self.ndev, dtype=np.int32)
+        m = self._mask(active)
         self.set_bias()
         _lib.check(_lib.lib().tcad_solve_equilibrium(
-            self._handle, opts.max_iter, opts.tol_update, opts.max_dpsi, 1,
+            self._handle, opts.max_iter, opts.tol_update, opts.max_dpsi, 1, _lib.iptr(m),
             self.psi.data_ptr(), self.n.data_ptr(), self.p.data_ptr(),
-            _lib.iptr(its), _lib.iptr(conv)), "tcad_solve_equilibrium")
-        return its, conv
+            _lib.iptr(its), _lib.iptr(conv), _lib.iptr(status)), "tcad_solve_equilibrium")
+        return its, conv, status
 
-    def solve_bias(self, opts):
+    def solve_bias(self, opts, active=None):
+        """Coupled Newton of the selected devices; per device (iterations,
+        converged, Krylov iterations, outcome code)."""
         its = np.zeros(self.ndev, dtype=np.int32)
         conv = np.zeros(self.ndev, dtype=np.int32)
         kry = np.zeros(self.ndev, dtype=np.int32)
+        status = np.zeros(self.ndev, dtype=np.int32)
+        m = self._mask(active)
         self.set_bias()
         _lib.check(_lib.lib().tcad_solve_bias(
             self._handle, opts.max_iter, opts.tol_update, opts.max_dpsi, opts.lin_mode(),
-            opts.linsolve_rtol, opts.lin_maxiter, self.psi.data_ptr(), self.n.data_ptr(),
-            self.p.data_ptr(), _lib.iptr(its), _lib.iptr(conv), _lib.iptr(kry)),
-            "tcad_solve_bias")
-        return its, conv, kry
+            opts.linsolve_rtol, opts.lin_maxiter, _lib.iptr(m), self.psi.data_ptr(),
+            self.n.data_ptr(), self.p.data_ptr(), _lib.iptr(its), _lib.iptr(conv),
+            _lib.iptr(kry), _lib.iptr(status)), "tcad_solve_bias")
+        return its, conv, kry, status
 
     def terminal_currents(self):
         out = np.zeros(max(1, len(self.contacts)), dtype=np.float64)
@@ -334,7 +365,18 @@ class GPUBatch:
         return out[:len(self.contacts)]
 
     def device_slice(self, d):
-        return slice(int(self.offsets[d]), int(self.offsets[d + 1]))
+        o = int(self.offsets[d])
+        return slice(o, o + self.setups[d].N)
+
+
+def _raise_outcome(code, what):
+    """A failed linear solve raises, as the reference's solve_linear does
+    (linsolve.py:973-982); so does a non-finite assembly or step."""
+    if code == _lib.NEWTON_LINEAR_FAIL:
+        raise _lib.LinearSolveError(f"{what}: the Newton linear solve did not reach a "
+                                    "relative residual of 1e-6")
+    if code == _lib.NEWTON_NONFINITE:
+        raise _lib.LinearSolveError(f"{what}: non-finite or singular Newton system")
 
 
 class _GPUDevice:
@@ -376,8 +418,9 @@ class _GPUDevice:
     def solve_equilibrium(self, opts=None):
         opts = opts or NewtonOptions()
         B = self._gpu()
-        its, conv = B.solve_equilibrium(opts)
+        its, conv, status = B.solve_equilibrium(opts)
         self.last_eq_iterations = int(its[0])
+        _raise_outcome(int(status[0]), "solve_equilibrium")
         if not conv[0]:
             warnings.warn("equilibrium Poisson solve did not converge.")
         self._solved = True
@@ -388,10 +431,11 @@ class _GPUDevice:
         if not self._solved:
             self.solve_equilibrium(opts)
         self._set_voltages(voltages)
-        its, conv, kry = self._gpu().solve_bias(opts)
+        its, conv, kry, status = self._gpu().solve_bias(opts)
         self.last_iterations = int(its[0])
         self.last_converged = bool(conv[0])
         self.last_krylov_iterations = int(kry[0])
+        _raise_outcome(int(status[0]), "solve_bias")
         if not self.last_converged:
             warnings.warn("Newton did not converge")
         return self
@@ -609,3 +653,9 @@ def device_from_spec(spec):
             dev.add_gate(g.name, g.idx[2], g.idx[1], g.idx[0], g.tox_cm, g.Vfb, g.Vg,
                          normal_axis=g.normal_axis)
     return dev
+
+
+def setup_from_spec(spec):
+    """The DeviceSetup (boundary conditions attached) of a DeviceSpec, for
+    building batches."""
+    return device_from_spec(spec)._s

@@ -175,7 +175,7 @@ def test_batch_equals_alone():
         setups.append(dev._s)
     B = GPUBatch(setups)
     B.solve_equilibrium(opts)
-    its, conv, _ = B.solve_bias(opts)
+    its, conv, _, _ = B.solve_bias(opts)
     I = B.terminal_currents()
     psi = B.psi.cpu().numpy()
     c = 0
