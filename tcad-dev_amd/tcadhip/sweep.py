@@ -26,7 +26,7 @@ from .device import GPUBatch, NewtonOptions, _torch
 
 @dataclass
 class SweepResult:
-    """Per-device outcome of a batched sweep."""
+    """Per-device outcome of a sweep."""
     points: list = field(default_factory=list)      # requested bias dicts reached
     currents: list = field(default_factory=list)    # {contact: I} after each point
     newton: list = field(default_factory=list)      # Newton iterations per point (last attempt)
@@ -35,10 +35,6 @@ class SweepResult:
     backoffs: int = 0                               # failed attempts
     failed: str = ""                                # reason when the sweep stopped early
     t_done: float = 0.0                             # wall time when the last point finished
-
-
-def _bias_vector(setup, bias, names):
-    return np.array([float(bias.get(nm, _bc_value(setup, nm))) for nm in names])
 
 
 def _bc_value(setup, name):
@@ -55,115 +51,161 @@ def _apply_bias(setup, names, vec):
             bc.Vg = float(v)
 
 
+class Track:
+    """Continuation state of one device through its list of bias points."""
+
+    def __init__(self, setup, points, max_step=None, min_step=None, growth=1.5, shrink=0.5,
+                 max_attempts=64):
+        self.setup = setup
+        self.points = list(points)
+        self.names = list(setup.bcs)
+        self.max_step, self.min_step = max_step, min_step
+        self.growth, self.shrink = growth, shrink
+        self.max_attempts = max_attempts
+        self.cur = self._vec({})         # last converged bias (terminals' present values)
+        self.next = 0                    # index of the next requested point
+        self.step = None                 # current step length (max-norm, V)
+        self.attempts = 0
+        self.kry = 0
+        self.trial = None
+        self.result = SweepResult()
+
+    def _vec(self, bias):
+        return np.array([float(bias.get(nm, _bc_value(self.setup, nm))) for nm in self.names])
+
+    @property
+    def open(self):
+        return not self.result.failed and self.next < len(self.points)
+
+    def propose(self):
+        """Set the device's terminals to the next trial bias."""
+        target = self._vec(self.points[self.next])
+        dist = float(np.max(np.abs(target - self.cur))) if target.size else 0.0
+        lim = dist
+        if self.max_step is not None:
+            lim = min(lim, self.max_step)
+        if self.step is not None:
+            lim = min(lim, self.step)
+        frac = 1.0 if dist <= lim or dist == 0.0 else lim / dist
+        self.trial = (self.cur + frac * (target - self.cur), frac >= 1.0, lim)
+        _apply_bias(self.setup, self.names, self.trial[0])
+
+    def accept(self, its, kry):
+        """The trial converged; returns True when it was a requested point."""
+        vec, final, _ = self.trial
+        R = self.result
+        self.kry += int(kry)
+        self.cur = vec
+        R.steps += 1
+        self.attempts = 0
+        if self.step is not None:
+            self.step = self.step * self.growth
+            if self.max_step is not None:
+                self.step = min(self.step, self.max_step)
+        if final:
+            R.points.append(self.points[self.next])
+            R.newton.append(int(its))
+            R.krylov.append(self.kry)
+            self.kry = 0
+            self.next += 1
+        return final
+
+    def reject(self, status, kry):
+        """The trial failed: back off (the caller restores the state)."""
+        _, _, lim = self.trial
+        R = self.result
+        self.kry += int(kry)
+        _apply_bias(self.setup, self.names, self.cur)
+        R.backoffs += 1
+        self.attempts += 1
+        self.step = lim * self.shrink
+        floor = self.min_step if self.min_step is not None else 1e-4
+        if self.step < floor or self.attempts > self.max_attempts:
+            why = {_lib.NEWTON_LINEAR_FAIL: "linear solve failed",
+                   _lib.NEWTON_NONFINITE: "non-finite Newton step"}.get(
+                       int(status), "Newton did not converge")
+            R.failed = (f"{why}; step shrank below {floor:g} V before "
+                        f"{self.points[self.next]}")
+
+
+def solve_round(B, tracks, opts):
+    """One masked Newton solve over the open tracks (tracks[d] belongs to
+    device d of batch B); returns the indices that reached a requested
+    point.  Failed devices get their state restored."""
+    todo = [d for d, t in enumerate(tracks) if t is not None and t.open]
+    if not todo:
+        return []
+    for d in todo:
+        tracks[d].propose()
+    mask = np.zeros(B.ndev, dtype=np.int32)
+    mask[todo] = 1
+    # whole-plane snapshot: three copies, cheaper in launches than one
+    # per device; only failed devices' ranges are restored from it
+    snap = (B.psi.clone(), B.n.clone(), B.p.clone())
+    its, conv, kry, status = B.solve_bias(opts, active=mask)
+    reached = []
+    for d in todo:
+        t = tracks[d]
+        if conv[d] and status[d] == _lib.NEWTON_OK:
+            if t.accept(its[d], kry[d]):
+                reached.append(d)
+        else:
+            sl = B.device_slice(d)
+            B.psi[sl].copy_(snap[0][sl])
+            B.n[sl].copy_(snap[1][sl])
+            B.p[sl].copy_(snap[2][sl])
+            t.reject(status[d], kry[d])
+    return reached
+
+
+def record_currents(B, tracks, reached, now):
+    """Terminal currents of the devices in `reached` after their point."""
+    if not reached:
+        return
+    I = B.terminal_currents()
+    c = 0
+    per_dev = {}
+    for d, nm in B.contacts:
+        per_dev.setdefault(d, []).append((nm, I[c]))
+        c += 1
+    for d in reached:
+        tracks[d].result.currents.append(dict(per_dev.get(d, [])))
+        tracks[d].result.t_done = now
+
+
 class BatchedSweep:
     """Run the bias lists `sweeps[d]` (lists of {terminal: V} dicts) of the
     devices `setups[d]` (DeviceSetup, boundary conditions attached) in one
-    batch.  `batch` may be passed to work on an existing GPUBatch (e.g. a
-    resident pool's); its devices must be `setups` in order."""
+    batch.  `batch` may be passed to work on an existing GPUBatch; its
+    devices must be `setups` in order."""
 
     def __init__(self, setups, sweeps, opts=None, batch=None, max_step=None, min_step=None,
                  growth=1.5, shrink=0.5, max_attempts=64):
         self.setups = list(setups)
-        self.sweeps = [list(s) for s in sweeps]
-        if len(self.sweeps) != len(self.setups):
+        if len(sweeps) != len(self.setups):
             raise ValueError("one bias list per device")
         self.opts = opts or NewtonOptions()
         self.B = batch if batch is not None else GPUBatch(self.setups)
-        self.max_step = max_step
-        self.min_step = min_step
-        self.growth, self.shrink = growth, shrink
-        self.max_attempts = max_attempts
-        nd = len(self.setups)
-        # terminals whose bias a sweep moves, per device (contacts and gates)
-        self.names = [[nm for nm in s.bcs] for s in self.setups]
-        self.results = [SweepResult() for _ in range(nd)]
+        self.tracks = [Track(s, pts, max_step, min_step, growth, shrink, max_attempts)
+                       for s, pts in zip(self.setups, sweeps)]
+        self.results = [t.result for t in self.tracks]
         self.rounds = 0
 
     def run(self, equilibrium=True):
         torch = _torch()
         B, opts = self.B, self.opts
-        nd = len(self.setups)
         t0 = time.perf_counter()
-        # the sweep starts from the terminals' present values (equilibrium
-        # is solved at them, as Device2D.solve_equilibrium is)
-        cur = [_bias_vector(s, {}, nm) for s, nm in zip(self.setups, self.names)]
         if equilibrium:
+            # solved at the terminals' present values, as
+            # Device2D.solve_equilibrium is
             its, conv, status = B.solve_equilibrium(opts)
-            for d in range(nd):
+            for d, t in enumerate(self.tracks):
                 if status[d] != _lib.NEWTON_OK or not conv[d]:
-                    self.results[d].failed = "equilibrium did not converge"
-        nxt = [0] * nd                        # index of the next requested point
-        step = [None] * nd                    # current step length (max-norm, V)
-        attempts = [0] * nd
-        kry_acc = [0] * nd
-        while True:
-            todo = [d for d in range(nd)
-                    if not self.results[d].failed and nxt[d] < len(self.sweeps[d])]
-            if not todo:
-                break
+                    t.result.failed = "equilibrium did not converge"
+        while any(t.open for t in self.tracks):
             self.rounds += 1
-            trial = {}
-            for d in todo:
-                target = _bias_vector(self.setups[d], self.sweeps[d][nxt[d]], self.names[d])
-                dist = float(np.max(np.abs(target - cur[d]))) if target.size else 0.0
-                lim = dist
-                if self.max_step is not None:
-                    lim = min(lim, self.max_step)
-                if step[d] is not None:
-                    lim = min(lim, step[d])
-                frac = 1.0 if dist <= lim or dist == 0.0 else lim / dist
-                trial[d] = (cur[d] + frac * (target - cur[d]), frac >= 1.0, lim)
-                _apply_bias(self.setups[d], self.names[d], trial[d][0])
-            mask = np.zeros(nd, dtype=np.int32)
-            mask[todo] = 1
-            snap = (B.psi.clone(), B.n.clone(), B.p.clone())
-            its, conv, kry, status = B.solve_bias(opts, active=mask)
-            reached = []
-            for d in todo:
-                R = self.results[d]
-                kry_acc[d] += int(kry[d])
-                vec, final, lim = trial[d]
-                if conv[d] and status[d] == _lib.NEWTON_OK:
-                    cur[d] = vec
-                    R.steps += 1
-                    if final:
-                        reached.append(d)
-                        R.points.append(self.sweeps[d][nxt[d]])
-                        R.newton.append(int(its[d]))
-                        R.krylov.append(kry_acc[d])
-                        kry_acc[d] = 0
-                        nxt[d] += 1
-                    if step[d] is not None:
-                        step[d] = step[d] * self.growth
-                        if self.max_step is not None:
-                            step[d] = min(step[d], self.max_step)
-                    attempts[d] = 0
-                else:
-                    sl = B.device_slice(d)
-                    B.psi[sl].copy_(snap[0][sl])
-                    B.n[sl].copy_(snap[1][sl])
-                    B.p[sl].copy_(snap[2][sl])
-                    _apply_bias(self.setups[d], self.names[d], cur[d])
-                    R.backoffs += 1
-                    attempts[d] += 1
-                    step[d] = lim * self.shrink
-                    floor = self.min_step if self.min_step is not None else 1e-4
-                    if step[d] < floor or attempts[d] > self.max_attempts:
-                        code = int(status[d])
-                        why = {_lib.NEWTON_LINEAR_FAIL: "linear solve failed",
-                               _lib.NEWTON_NONFINITE: "non-finite Newton step"}.get(
-                                   code, "Newton did not converge")
-                        R.failed = (f"{why}; step shrank below {floor:g} V before "
-                                    f"{self.sweeps[d][nxt[d]]}")
+            reached = solve_round(B, self.tracks, opts)
             if reached:
-                I = B.terminal_currents()
                 torch.cuda.synchronize()
-                now = time.perf_counter() - t0
-                c = 0
-                for d in range(nd):
-                    names = [nm for dd, nm in B.contacts if dd == d]
-                    if d in reached:
-                        self.results[d].currents.append(dict(zip(names, I[c:c + len(names)])))
-                        self.results[d].t_done = now
-                    c += len(names)
+                record_currents(B, self.tracks, reached, time.perf_counter() - t0)
         return self.results

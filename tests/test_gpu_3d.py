@@ -1,0 +1,82 @@
+"""3D tri-gate on the GPU: a mid-size mesh against the live CPU oracle, and
+the full 64x48x48 mesh (442,368 unknowns, where the oracle's direct solve
+is out of reach) through invariants.
+
+Bars: Newton counts within +-1 of the oracle's; terminal currents within
+1e-6 * max_c |I_ref| per point (the tri-gate's currents are not
+cancellation-limited; its overlapping source/drain/body node lists make
+sum_c I nonzero by construction, so no conservation floor is used);
+psi within 1e-9 (scaled).  Full size: the device is mirror-symmetric in z
+(symmetric mesh, doping and side gates), so the converged psi must be too,
+to 1e-8 (scaled, i.e. 2.6e-10 V); the FP32-matrix path must give the FP64
+path's drain current to 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle_build import oracle_sweep
+
+pytestmark = pytest.mark.gpu
+
+
+def _mods():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a HIP device")
+    from tcadhip import configs
+    from tcadhip.device import NewtonOptions, device_from_spec
+    return configs, NewtonOptions, device_from_spec
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _sweep(spec, opts):
+    configs, NewtonOptions, device_from_spec = _mods()
+    dev = device_from_spec(spec)
+    dev.solve_equilibrium(NewtonOptions(max_iter=400))
+    its, cur = [], []
+    for bias in spec.sweep:
+        dev.solve_bias(bias, opts)
+        assert dev.last_converged
+        its.append(dev.last_iterations)
+        cur.append([dev.terminal_current(c.name) for c in spec.contacts])
+    return dev, np.array(its), np.array(cur)
+
+
+@pytest.mark.parametrize("mode", ["bicgstab", "bicgstab_mixed"])
+def test_trigate_mid_vs_oracle(mode):
+    configs, NewtonOptions, _ = _mods()
+    _say(f"[3d] trigate 24x18x14 Vd=0.05 Vg 0,0.8 {mode} vs oracle")
+    spec = configs.trigate_3d(nx=24, ny=18, nz=14, Vd=0.05, vg_list=[0.0, 0.8])
+    odev, oits, ocur, ost = oracle_sweep(spec, keep=(1,))
+    dev, its, cur = _sweep(spec, NewtonOptions(linsolve=mode))
+    assert np.all(np.abs(its - oits) <= 1), (its, oits)
+    for p in range(len(spec.sweep)):
+        err = np.abs(cur[p] - ocur[p]).max() / np.abs(ocur[p]).max()
+        _say(f"  point {p}: max rel current error {err:.2e}")
+        assert err <= 1e-6, (p, cur[p], ocur[p])
+    assert np.abs(dev.psi - ost[1][0]).max() <= 1e-9
+
+
+def test_trigate_full_size_invariants():
+    configs, NewtonOptions, _ = _mods()
+    _say("[3d] trigate 64x48x48 Vd=0.05 Vg 0,0.4: z-mirror symmetry, FP32 vs FP64")
+    spec = configs.config3_trigate(0.05)
+    spec.sweep = [spec.sweep[0], spec.sweep[8]]
+    dev64, its64, cur64 = _sweep(spec, NewtonOptions(linsolve="bicgstab"))
+    psi = dev64.psi
+    assert psi.shape == (48, 48, 64)
+    asym = np.abs(psi - psi[::-1]).max()
+    _say(f"  z-mirror asymmetry of psi {asym:.2e}")
+    assert asym <= 1e-8
+    n = dev64.n
+    assert (np.abs(n - n[::-1]) / n).max() <= 1e-7
+    dev32, its32, cur32 = _sweep(spec, NewtonOptions(linsolve="bicgstab_mixed"))
+    names = [c.name for c in spec.contacts]
+    d = names.index("drain")
+    rel = np.abs(cur32[:, d] - cur64[:, d]) / np.abs(cur64[:, d])
+    _say(f"  FP32-matrix vs FP64 drain current {rel.max():.2e}; Newton {its64} {its32}")
+    assert rel.max() <= 1e-6
+    assert np.all(np.abs(its32 - its64) <= 1)

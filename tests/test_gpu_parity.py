@@ -147,7 +147,9 @@ def test_trigate_small_vs_oracle():
     dev, its, cur, states = _run_sweep(spec, NewtonOptions())
     assert np.all(np.abs(its - oits) <= 1), (its, oits)
     for p in range(len(spec.sweep)):
-        assert np.abs(cur[p] - ocur[p]).max() <= current_tol(ocur[p]), (p, cur[p], ocur[p])
+        # overlapping contact node lists: no conservation floor here
+        tol = 1e-6 * np.abs(ocur[p]).max()
+        assert np.abs(cur[p] - ocur[p]).max() <= tol, (p, cur[p], ocur[p])
     psi, n, pp = states[2]
     assert np.abs(psi - ost[2][0]).max() <= 1e-9
     assert (np.abs(n - ost[2][1]) / ost[2][1]).max() <= 1e-7
