@@ -48,10 +48,10 @@ def run_sweep(dev, spec, opts, lat=None):
     return out
 
 
-def measure(spec, linsolve, steps, warmup, rtol=1e-10):
+def measure(spec, linsolve, steps, warmup, rtol=1e-10, forcing=False):
     import torch
     from tcadhip.device import NewtonOptions, device_from_spec
-    opts = NewtonOptions(linsolve=linsolve, linsolve_rtol=rtol)
+    opts = NewtonOptions(linsolve=linsolve, linsolve_rtol=rtol, forcing=forcing)
     dev = device_from_spec(spec)
     for _ in range(warmup):
         run_sweep(dev, spec, opts)
@@ -114,6 +114,11 @@ def main():
     ap.add_argument("--modes", default="bicgstab,bicgstab_mixed")
     ap.add_argument("--rtol", type=float, default=1e-10,
                     help="relative residual of each Newton linear solve")
+    ap.add_argument("--forcing", action="store_true",
+                    help="Newton forcing terms (inexact early solves)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (the reference "
+                         "kernel for scaling HBM traffic counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every MI355X box here has one, and this "
@@ -123,6 +128,16 @@ def main():
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_calibration:
+        n = args.copy_calibration << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        a.fill_(1)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copies": 5}))
+        return
     import warnings
     warnings.simplefilter("ignore")
     from tcadhip import configs
@@ -136,7 +151,8 @@ def main():
     for mode in modes:
         print(f"[bench] {mode}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
               flush=True)
-        res[mode] = measure(spec, mode, args.steps, args.warmup, args.rtol)
+        res[mode] = measure(spec, mode, args.steps, args.warmup, args.rtol,
+                            forcing=args.forcing)
         r = res[mode]
         print(f"[bench] {mode}: {npts * args.steps / r['seconds']:.3f} points/s "
               f"newton={r['newton']} krylov={r['krylov']}", file=sys.stderr, flush=True)

@@ -50,6 +50,13 @@ extern "C" {
 /* Linear-solve modes (pytcad/pytcad/linsolve.py:338 `_METHODS` analogue). */
 #define TCAD_LIN_BICGSTAB_F64 0   /* FP64 BiCGStab + node block-Jacobi       */
 #define TCAD_LIN_BICGSTAB_MIXED 1 /* FP32 BiCGStab + block-Jacobi in FP64 IR */
+/* Flag OR'd into lin_mode of tcad_solve_bias: Newton forcing terms.  Each
+ * Newton solve's relative residual target is clamp(0.01 * err, lin_rtol,
+ * max(1e-4, lin_rtol)) with err the previous step's update measure (1e-4
+ * for the first step); without the flag every solve aims at lin_rtol.
+ * Either way the step that passes the convergence test is refined to a
+ * true relative residual of 1e-14 before it is applied. */
+#define TCAD_LIN_FORCING 0x100
 
 /* Per-device outcome of a Newton solve (h_status). */
 #define TCAD_NEWTON_OK 0             /* converged (or masked out)            */

@@ -90,7 +90,8 @@ __global__ void __launch_bounds__(BS) k_newton_update(UArgs U, const double *y, 
 // passes the convergence test (the step is applied afterwards).  A device
 // whose linear solve ended above the acceptance residual stops here, before
 // its step is applied (linsolve.py:973-982 refuses such an iterate).
-__global__ void k_newton_check(UArgs U, double tol, double accept2) {
+__global__ void k_newton_check(UArgs U, double tol, double accept2, double rtol, double eta_max,
+                               int forcing) {
   __shared__ double sh[BS / 64];
   const int d = blockIdx.x;
   const DevInfo &I = U.info[d];
@@ -102,10 +103,17 @@ __global__ void k_newton_check(UArgs U, double tol, double accept2) {
   if (threadIdx.x == 0) {
     S.err = m;
     S.conv_now = m < tol ? 1 : 0;
-    if (!(S.rr <= accept2 * S.bb)) {
+    if (!(S.rr <= fmax(S.eta2, accept2) * S.bb)) {
       S.conv_now = 0;
       S.active = 0;
       S.result = TCAD_NEWTON_LINEAR_FAIL;
+    }
+    // forcing term of the next solve: far from the solution the step only
+    // needs a few digits; near it the target tightens with the update
+    // (the converging step is refined to 1e-14 before it is applied)
+    if (forcing) {
+      const double eta = fmin(fmax(0.01 * m, rtol), eta_max);
+      S.eta2 = eta * eta;
     }
   }
 }
@@ -199,10 +207,11 @@ __global__ void k_newton_finalize(UArgs U, double tol, int add_krylov) {
 
 // Select the devices of the next Newton solve (mask NULL = all) and reset
 // their counters.
-__global__ void k_set_active(Scal *scal, int ndev, const int *mask) {
+__global__ void k_set_active(Scal *scal, int ndev, const int *mask, double eta0) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndev) return;
   Scal &S = scal[d];
+  S.eta2 = eta0 * eta0;
   S.sel = mask ? (mask[d] != 0) : 1;
   S.active = S.sel;
   S.result = 0;
@@ -286,12 +295,12 @@ int sync_scalars(Batch &b) {
   return TCAD_OK;
 }
 
-int launch_set_active(Batch &b, const int32_t *h_mask) {
+int launch_set_active(Batch &b, const int32_t *h_mask, double eta0) {
   if (h_mask)
     TCAD_HIP_CHECK(hipMemcpyAsync(b.d_mask, h_mask, sizeof(int) * b.ndev, hipMemcpyHostToDevice,
                                   b.stream));
   hipLaunchKernelGGL(k_set_active, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
-                     b.ndev, h_mask ? (const int *)b.d_mask : nullptr);
+                     b.ndev, h_mask ? (const int *)b.d_mask : nullptr, eta0);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
   return TCAD_OK;
@@ -324,7 +333,10 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
                            double rtol, int maxit, const int32_t *mask, double *psi, double *n,
                            double *p) {
   int rc;
-  if ((rc = launch_set_active(b, mask))) return rc;
+  const bool forcing = (mode & TCAD_LIN_FORCING) != 0;
+  mode &= ~TCAD_LIN_FORCING;
+  const double eta_max = rtol > 1e-4 ? rtol : 1e-4;
+  if ((rc = launch_set_active(b, mask, forcing ? eta_max : rtol))) return rc;
   hipLaunchKernelGGL(k_stamp, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev, b.d_cidx,
                      b.d_cn_lnr, b.d_cn_n0, b.d_cn_p0, b.d_cn_contact, b.d_contact_V, b.d_scal,
                      psi, n, p);
@@ -336,12 +348,15 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
   // by its storage rounding
   const double inner = mixed ? 1e-6 : rtol;
   const double accept = rtol > 1e-6 ? rtol : 1e-6;
+  // the final-step refinement only has to take the residual from the
+  // solve's target down to 1e-14: a few digits per sweep suffice
+  const double inner_ref = mixed ? 1e-6 : 1e-4;
   const dim3 gd((b.ndev + 63) / 64), td(64);
   for (int it = 0; it < max_iter; ++it) {
     if ((rc = launch_surface_mobility(b, psi, true))) return rc;
     if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, mixed, true))) return rc;
     if ((rc = launch_flags_to_status(b))) return rc;
-    if ((rc = ir_solve(b, mixed, true, rtol, 16, maxit, inner))) return rc;
+    if ((rc = ir_solve(b, mixed, true, rtol, 16, maxit, inner, true))) return rc;
     // measure the step, and before applying a step that passes the test
     // refine its solve to a true residual of 1e-14: the converged state is
     // then the reference's (exact Newton steps converge quadratically, so
@@ -350,7 +365,7 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
     hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
                        max_dpsi, 0);
     hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol,
-                       accept * accept);
+                       accept * accept, rtol, eta_max, forcing ? 1 : 0);
     b.launches += 2;
     TCAD_HIP_CHECK(hipGetLastError());
     if ((rc = sync_scalars(b))) return rc;
@@ -359,7 +374,7 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
     if (conv && rtol > 1e-14) {
       hipLaunchKernelGGL(k_mask_converging, gd, td, 0, b.stream, b.d_scal, b.ndev);
       b.launches++;
-      if ((rc = ir_solve(b, mixed, false, 1e-14, 16, maxit, inner))) return rc;
+      if ((rc = ir_solve(b, mixed, false, 1e-14, 16, maxit, inner_ref))) return rc;
       hipLaunchKernelGGL(k_unmask, gd, td, 0, b.stream, b.d_scal, b.ndev);
       b.launches++;
     }
@@ -852,7 +867,8 @@ int tcad_solve_bias(tcad_batch *bp, int32_t max_iter, double tol_update, double 
     tcad::set_error("tcad_solve_bias: invalid argument");
     return TCAD_E_INVALID;
   }
-  if (lin_mode != TCAD_LIN_BICGSTAB_F64 && lin_mode != TCAD_LIN_BICGSTAB_MIXED) {
+  const int base_mode = lin_mode & ~TCAD_LIN_FORCING;
+  if (base_mode != TCAD_LIN_BICGSTAB_F64 && base_mode != TCAD_LIN_BICGSTAB_MIXED) {
     tcad::set_error("tcad_solve_bias: unknown linear-solve mode");
     return TCAD_E_INVALID;
   }

@@ -353,8 +353,10 @@ __global__ void __launch_bounds__(BS) k_bicg_5(KArgs<T> K) {
 }
 
 // Reset the per-device Krylov state for a new solve; devices not active
-// are marked done.
-__global__ void k_krylov_reset(Scal *scal, int ndev, double tol2, int maxit) {
+// are marked done.  per_dev: each device's own target S.eta2 (not below
+// floor2) instead of tol2.
+__global__ void k_krylov_reset(Scal *scal, int ndev, double tol2, int maxit, int per_dev,
+                               double floor2) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndev) return;
   Scal &S = scal[d];
@@ -362,7 +364,7 @@ __global__ void k_krylov_reset(Scal *scal, int ndev, double tol2, int maxit) {
   S.alpha = 1.0;
   S.omega = 1.0;
   S.it = 0;
-  S.tol2 = tol2;
+  S.tol2 = per_dev ? fmax(S.eta2, floor2) : tol2;
   S.maxit = maxit;
   if (S.active) {
     S.status = 0;
@@ -417,9 +419,10 @@ static void launch_iteration(Batch &b, KArgs<T> K, int parity) {
 // read-back per chunk until every device is done or maxit is reached
 // (one extra K1 then records the final stop test).
 template <typename T>
-static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int chunk) {
+static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int chunk,
+                        bool per_dev = false, double floor = 0.0) {
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
-                     b.ndev, rtol * rtol, maxit);
+                     b.ndev, rtol * rtol, maxit, per_dev ? 1 : 0, floor * floor);
   hipLaunchKernelGGL(k_bicg_init<T>, dim3(b.nblk), dim3(BS), 0, b.stream, K);
   b.launches += 2;
   int it = 0;
@@ -527,7 +530,7 @@ __global__ void __launch_bounds__(BS) k_ir_residual(IRArgs R, int first) {
 }
 
 // One workgroup per device: refinement stop test; nrm for the inner rhs.
-__global__ void k_ir_check(IRArgs R, int ndev, double tol2) {
+__global__ void k_ir_check(IRArgs R, int ndev, double tol2, int per_dev) {
   __shared__ double sh[BS / 64];
   const int d = blockIdx.x;
   if (d >= ndev) return;
@@ -540,9 +543,10 @@ __global__ void k_ir_check(IRArgs R, int ndev, double tol2) {
     S.nrm = sqrt(rr);
     S.rr = rr;
     S.bb = bb;
-    if (rr <= tol2 * bb || S.ir_it >= S.ir_maxsweep || !isfinite(rr)) {
+    const double t2 = per_dev ? S.eta2 : tol2;
+    if (rr <= t2 * bb || S.ir_it >= S.ir_maxsweep || !isfinite(rr)) {
       S.active = 0;
-      S.status = (rr <= tol2 * bb) ? 0 : 2;
+      S.status = (rr <= t2 * bb) ? 0 : 2;
     }
   }
 }
@@ -579,7 +583,7 @@ __global__ void k_ir_restore(Scal *scal, int ndev) {
 }
 
 int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int inner_maxit,
-             double inner_rtol) {
+             double inner_rtol, bool per_dev) {
   IRArgs R{};
   R.W = make_where(b);
   R.N = b.ntot; R.nblk = b.nblk;
@@ -592,7 +596,8 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
   for (int sweep = 0; sweep <= maxsweep; ++sweep) {
     hipLaunchKernelGGL(k_ir_residual, dim3(b.nblk), dim3(BS), 0, b.stream, R,
                        (restart && sweep == 0) ? 1 : 0);
-    hipLaunchKernelGGL(k_ir_check, dim3(b.ndev), dim3(BS), 0, b.stream, R, b.ndev, rtol * rtol);
+    hipLaunchKernelGGL(k_ir_check, dim3(b.ndev), dim3(BS), 0, b.stream, R, b.ndev, rtol * rtol,
+                       per_dev ? 1 : 0);
     b.launches += 2;
     TCAD_HIP_CHECK(hipGetLastError());
     int rc = sync_scalars(b);
@@ -600,8 +605,12 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
     int open = 0;
     for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active ? 1 : 0;
     if (open == 0) break;
-    rc = fp32 ? run_bicgstab<float>(b, make_kargs<float>(b, true), inner_rtol, inner_maxit, 8)
-              : run_bicgstab<double>(b, make_kargs<double>(b, false), inner_rtol, inner_maxit, 8);
+    // per-device targets: the inner solve aims at the device's own target,
+    // not below inner_rtol (the FP32-stored matrix's limit)
+    rc = fp32 ? run_bicgstab<float>(b, make_kargs<float>(b, true), inner_rtol, inner_maxit, 8,
+                                    per_dev, inner_rtol)
+              : run_bicgstab<double>(b, make_kargs<double>(b, false), inner_rtol, inner_maxit, 8,
+                                     per_dev, 0.0);
     if (rc) return rc;
     hipLaunchKernelGGL(k_ir_update, dim3(b.nblk), dim3(BS), 0, b.stream, R);
     b.launches++;
@@ -775,7 +784,7 @@ int pcg_poisson(Batch &b, double rtol, int maxit) {
   P.x = b.veq[0]; P.r = b.veq[1]; P.z = b.veq[2]; P.p = b.veq[3]; P.q = b.veq[4];
   P.part = b.part; P.scal = b.d_scal;
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
-                     b.ndev, rtol * rtol, maxit);
+                     b.ndev, rtol * rtol, maxit, 0, 0.0);
   hipLaunchKernelGGL(k_pcg_init, dim3(b.nblk), dim3(BS), 0, b.stream, P);
   b.launches += 2;
   int it = 0, chunk = 16;

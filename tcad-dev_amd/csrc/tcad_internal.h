@@ -63,6 +63,7 @@ struct Scal {
   double nrm;                       // IR residual norm (rhs normalisation)
   double err;                       // Newton update measure
   double tol2;                      // stop test rr <= tol2 * bb of the current solve
+  double eta2;                      // squared forcing term of this device's next Newton solve
   int maxit;                        // iteration cap of the current solve
   int it;                           // Krylov iterations of the current solve
   int done;                         // Krylov finished (converged or stopped)
@@ -158,8 +159,10 @@ int launch_contact_sums(Batch &b, const double *Fnp, double *d_out);
 int bicgstab(Batch &b, bool fp32, double rtol, int maxit, bool zero_guess, int chunk);
 // FP64 iterative refinement of the scaled Newton system A64 y = bs around
 // BiCGStab on A32 (fp32) or A64; restart: y = 0 first, else continue from y.
+// per_dev: every device's own target S.eta2 replaces rtol (and the inner
+// BiCGStab aims at it, not below inner_rtol for fp32)
 int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int inner_maxit,
-             double inner_rtol);
+             double inner_rtol, bool per_dev = false);
 int pcg_poisson(Batch &b, double rtol, int maxit);
 int launch_unscale(Batch &b, const double *n, const double *p, double *du);
 int launch_unscale_state(Batch &b, const double *n, const double *p, double *du);
@@ -173,8 +176,9 @@ int launch_poisson_update(Batch &b, double *psi, double max_dpsi, double tol);
 int launch_stamp_contacts(Batch &b, double *psi, double *n, double *p, int zero_bias);
 int launch_bulk_guess(Batch &b, double *psi);
 int launch_slaved_densities(Batch &b, const double *psi, double *n, double *p);
-// select the devices of the next Newton solve (h_mask NULL = all)
-int launch_set_active(Batch &b, const int32_t *h_mask);
+// select the devices of the next Newton solve (h_mask NULL = all); eta0 is
+// the forcing term of each device's first Newton solve
+int launch_set_active(Batch &b, const int32_t *h_mask, double eta0 = 0.0);
 int sync_scalars(Batch &b);
 
 }  // namespace tcad

@@ -205,8 +205,6 @@ class Server:
             t_round = time.perf_counter()
             reached = solve_round(B, tracks, self.opts)
             self.rounds += 1
-            if reached:
-                torch.cuda.synchronize()
             if not realtime:
                 vt += time.perf_counter() - t_round
             record_currents(B, tracks, reached, now())

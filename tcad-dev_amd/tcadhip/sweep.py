@@ -139,22 +139,24 @@ def solve_round(B, tracks, opts):
         tracks[d].propose()
     mask = np.zeros(B.ndev, dtype=np.int32)
     mask[todo] = 1
-    # whole-plane snapshot: three copies, cheaper in launches than one
-    # per device; only failed devices' ranges are restored from it
-    snap = (B.psi.clone(), B.n.clone(), B.p.clone())
-    its, conv, kry, status = B.solve_bias(opts, active=mask)
-    reached = []
-    for d in todo:
-        t = tracks[d]
-        if conv[d] and status[d] == _lib.NEWTON_OK:
-            if t.accept(its[d], kry[d]):
-                reached.append(d)
-        else:
-            sl = B.device_slice(d)
-            B.psi[sl].copy_(snap[0][sl])
-            B.n[sl].copy_(snap[1][sl])
-            B.p[sl].copy_(snap[2][sl])
-            t.reject(status[d], kry[d])
+    with B.on_stream():
+        # whole-plane snapshot: three copies, cheaper in launches than one
+        # per device; only failed devices' ranges are restored from it
+        snap = (B.psi.clone(), B.n.clone(), B.p.clone())
+        its, conv, kry, status = B.solve_bias(opts, active=mask)
+        reached = []
+        for d in todo:
+            t = tracks[d]
+            if conv[d] and status[d] == _lib.NEWTON_OK:
+                if t.accept(its[d], kry[d]):
+                    reached.append(d)
+            else:
+                sl = B.device_slice(d)
+                B.psi[sl].copy_(snap[0][sl])
+                B.n[sl].copy_(snap[1][sl])
+                B.p[sl].copy_(snap[2][sl])
+                t.reject(status[d], kry[d])
+        B.stream_sync()
     return reached
 
 
@@ -205,7 +207,43 @@ class BatchedSweep:
         while any(t.open for t in self.tracks):
             self.rounds += 1
             reached = solve_round(B, self.tracks, opts)
-            if reached:
-                torch.cuda.synchronize()
-                record_currents(B, self.tracks, reached, time.perf_counter() - t0)
+            record_currents(B, self.tracks, reached, time.perf_counter() - t0)
         return self.results
+
+
+def run_lanes(setups, sweeps, opts=None, lanes=1, max_step=None):
+    """BatchedSweep over `lanes` independent batches, each on its own HIP
+    stream driven by its own host thread, so that their launches overlap
+    on the GPU.  Devices are grouped by node count (similar sizes share a
+    batch, so a batch waits less for its slowest device).  Returns the
+    per-device results in the input order and the per-lane sweeps."""
+    import threading
+    torch = _torch()
+    order = np.argsort([s.N for s in setups], kind="stable")
+    groups = [list(g) for g in np.array_split(order, lanes) if len(g)]
+    results = [None] * len(setups)
+    sweeps_out = [None] * len(groups)
+    errors = []
+
+    def lane(k, idx):
+        try:
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                B = GPUBatch([setups[i] for i in idx], stream=stream)
+                bs = BatchedSweep([setups[i] for i in idx], [sweeps[i] for i in idx], opts,
+                                  batch=B, max_step=max_step)
+                res = bs.run()
+            for i, r in zip(idx, res):
+                results[i] = r
+            sweeps_out[k] = bs
+        except Exception as e:             # re-raised in the caller's thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=lane, args=(k, g)) for k, g in enumerate(groups)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return results, sweeps_out
