@@ -74,6 +74,19 @@ def measure(spec, linsolve, steps, warmup, rtol=1e-10, forcing=False):
                 newton=its, krylov=kry)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(REPO, "profiles", "mosfet_200x120_spmv_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == f"`{kernel}`":
+            return float(cells[6]) * 1e6
+    return None
+
+
 def kernel_timing(dev, which, reps=200):
     import ctypes
     from tcadhip import _lib
@@ -81,6 +94,128 @@ def kernel_timing(dev, which, reps=200):
     _lib.check(_lib.lib().tcad_time_kernel(dev._gpu().handle, which, reps, ctypes.byref(out)),
                "tcad_time_kernel")
     return out.value
+
+
+def bench_trigate(vds=(0.05, 0.7)):
+    """3D tri-gate 64x48x48 (147,456 nodes, 442,368 unknowns): equilibrium
+    + 17 Vg points at each Vd, FP64 path; Newton counts per point."""
+    import torch
+    from tcadhip import configs
+    from tcadhip.device import NewtonOptions, device_from_spec
+    out = {}
+    total_pts, total_s = 0, 0.0
+    for Vd in vds:
+        spec = configs.config3_trigate(Vd)
+        dev = device_from_spec(spec)
+        opts = NewtonOptions()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.solve_equilibrium(NewtonOptions(max_iter=400))
+        its, ids = [], []
+        for bias in spec.sweep:
+            dev.solve_bias(bias, opts)
+            its.append(dev.last_iterations)
+            ids.append(dev.terminal_current("drain"))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        total_pts += len(spec.sweep)
+        total_s += dt
+        out[f"vd{Vd}"] = {"points": len(spec.sweep), "seconds": round(dt, 3),
+                          "newton_per_point": its,
+                          "id_first_last_A": [ids[0], ids[-1]]}
+    out["points_per_s"] = round(total_pts / total_s, 4)
+    out["nodes"] = 147456
+    return out
+
+
+def bench_batch(lanes=4, sample=8):
+    """256 seeded diodes (40x30..160x100 nodes, -2.0..0.8 V, 0.1 V
+    continuation steps), batched in `lanes` streams, both precisions;
+    the plain FP64 path (one device at a time) on an evenly spaced
+    sample.  Unit of work: one device converged at its target bias."""
+    import torch
+    from tcadhip import configs, sweep
+    from tcadhip.device import NewtonOptions, setup_from_spec
+    specs = configs.batch_diodes()
+    res = {}
+    for mode in ("bicgstab_mixed", "bicgstab"):
+        setups = [setup_from_spec(s) for s in specs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r, _ = sweep.run_lanes(setups, [s.sweep for s in specs], NewtonOptions(linsolve=mode),
+                               lanes=lanes)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[mode] = dict(seconds=dt, results=r)
+        print(f"[bench] batch {mode}: {len(specs) / dt:.3f} devices/s", file=sys.stderr,
+              flush=True)
+    idx = list(range(0, len(specs), len(specs) // sample))[:sample]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    alone = {}
+    for d in idx:
+        alone[d] = sweep.BatchedSweep([setup_from_spec(specs[d])], [specs[d].sweep],
+                                      NewtonOptions()).run()[0]
+    torch.cuda.synchronize()
+    dt_alone = time.perf_counter() - t0
+    plain = len(idx) / dt_alone
+
+    def final(r):
+        return np.array([list(c.values()) for c in r.currents[-1:]]).ravel()
+
+    out = {"devices": len(specs), "nodes": int(sum(s.N for s in specs)),
+           "continuation_solves": int(sum(len(s.sweep) for s in specs)), "lanes": lanes,
+           "plain_fp64_devices_per_s": round(plain, 4),
+           "plain_sample": f"{len(idx)} evenly spaced devices solved one at a time"}
+    for mode, r in res.items():
+        fails = sum(1 for x in r["results"] if x.failed)
+        err = max(float(np.max(np.abs(final(r["results"][d]) - final(alone[d])) /
+                                np.abs(final(alone[d])))) for d in idx)
+        out[mode] = {"devices_per_s": round(len(specs) / r["seconds"], 4),
+                     "seconds": round(r["seconds"], 3), "failed": fails,
+                     "vs_plain_fp64": round(len(specs) / r["seconds"] / plain, 2),
+                     "max_rel_current_err_vs_fp64_on_sample": err}
+    return out
+
+
+def bench_serve(count=30, rate=2.0, check=3, lanes=2):
+    """Serving: seeded Poisson arrivals of mixed requests (2D diode and
+    MOSFET sweeps, 3D tri-gate sweeps, 1k-200k nodes, 5-40 points each),
+    continuous batching in a 32 GiB pool split into `lanes` independent
+    lanes (own HIP stream each); `check` of them re-solved alone must give
+    the same bits."""
+    import torch
+    from tcadhip import serve
+    from tcadhip.device import NewtonOptions
+    reqs = serve.request_stream(count, rate=rate, seed=20261203)
+    S = serve.Server(serve.nodes_for_budget(32 << 30), opts=NewtonOptions(), lanes=lanes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    served = S.run(reqs, realtime=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    lat = np.array([o.latency for o in served])
+    pts = sum(len(o.result.currents) for o in served)
+    same = True
+    t1 = time.perf_counter()
+    alone_pts = 0
+    for o, r in list(zip(served, reqs))[:check]:
+        a = serve.serve_alone(r.spec, NewtonOptions())
+        alone_pts += len(a.currents)
+        same = same and a.newton == o.result.newton and all(
+            x[k] == y[k] for x, y in zip(a.currents, o.result.currents) for k in x)
+    torch.cuda.synchronize()
+    dt_alone = time.perf_counter() - t1
+    return {"requests": count, "arrival_rate_per_s": rate, "lanes": lanes,
+            "kinds": {k: sum(r.kind == k for r in reqs) for k in ("diode", "mosfet", "trigate")},
+            "bias_points": pts, "wall_s": round(dt, 3),
+            "points_per_s": round(pts / dt, 4),
+            "request_latency_p50_s": round(float(np.percentile(lat, 50)), 3),
+            "request_latency_p99_s": round(float(np.percentile(lat, 99)), 3),
+            "failed": sum(1 for o in served if o.result.failed),
+            "rounds": S.rounds, "batch_rebuilds": S.rebuilds, "max_resident": S.max_resident,
+            "alone_equals_mix_on_first": check, "alone_equals_mix": bool(same),
+            "plain_fp64_points_per_s_on_those": round(alone_pts / dt_alone, 4)}
 
 
 def cpu_baseline(spec, npoints, budget_s=30.0):
@@ -116,6 +251,8 @@ def main():
                     help="relative residual of each Newton linear solve")
     ap.add_argument("--forcing", action="store_true",
                     help="Newton forcing terms (inexact early solves)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 3D tri-gate, batched-diode and serving workloads")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (the reference "
                          "kernel for scaling HBM traffic counters) and exit")
@@ -220,7 +357,8 @@ def main():
                       "meets_1e-6": v["ok"]} for m, v in res.items()},
         "roofline": {"bound": "hbm", "kernel": "k_bicg_spmv<%s>" % ("float" if w == 4 else "double"),
                      "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / 8000.0, 5), "traffic": None,
+                     "frac": round(achieved / 8000.0, 5),
+                     "traffic": measured_traffic("k_bicg_spmv<double>") if w == 8 else None,
                      "bytes_per_launch": spmv_bytes, "avg_launch_us": round(us, 3)},
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["points"] / cpu["seconds"], 5), "unit": "bias points/s",
@@ -229,6 +367,13 @@ def main():
                       f"{cpu['points']} of {npts} Vg points of the same 200x120 sweep, "
                       f"{cpu['seconds']:.1f} s"},
     }
+    if not args.no_extras:
+        print("[bench] 3D tri-gate 64x48x48", file=sys.stderr, flush=True)
+        line["trigate_3d_64x48x48"] = bench_trigate()
+        print("[bench] 256 batched diodes", file=sys.stderr, flush=True)
+        line["batched_diodes_256"] = bench_batch()
+        print("[bench] serving", file=sys.stderr, flush=True)
+        line["serving"] = bench_serve()
     print(json.dumps(line), flush=True)
 
 

@@ -349,8 +349,11 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
   const double inner = mixed ? 1e-6 : rtol;
   const double accept = rtol > 1e-6 ? rtol : 1e-6;
   // the final-step refinement only has to take the residual from the
-  // solve's target down to 1e-14: a few digits per sweep suffice
-  const double inner_ref = mixed ? 1e-6 : 1e-4;
+  // solve's target down to 1e-14: a few digits per sweep suffice.  It runs
+  // on the FP64 matrix in both modes: where the Jacobian's conditioning
+  // times FP32 rounding approaches 1, refinement around the FP32-stored
+  // matrix stalls short of 1e-14 (measured on reverse-biased 1e19 diodes)
+  const double inner_ref = 1e-4;
   const dim3 gd((b.ndev + 63) / 64), td(64);
   for (int it = 0; it < max_iter; ++it) {
     if ((rc = launch_surface_mobility(b, psi, true))) return rc;
@@ -374,7 +377,7 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
     if (conv && rtol > 1e-14) {
       hipLaunchKernelGGL(k_mask_converging, gd, td, 0, b.stream, b.d_scal, b.ndev);
       b.launches++;
-      if ((rc = ir_solve(b, mixed, false, 1e-14, 16, maxit, inner_ref))) return rc;
+      if ((rc = ir_solve(b, false, false, 1e-14, 16, maxit, inner_ref))) return rc;
       hipLaunchKernelGGL(k_unmask, gd, td, 0, b.stream, b.d_scal, b.ndev);
       b.launches++;
     }

@@ -132,94 +132,214 @@ class Served:
         return self.done - self.arrival
 
 
-class Server:
-    """Continuous-batching solver over one DevicePool.  `opts` is used for
-    every bias point; equilibrium uses `eq_opts`."""
+class _Lane:
+    """One continuous-batching loop: its own pool share, HIP stream (None =
+    the default stream) and, with several lanes, its own host thread."""
 
-    def __init__(self, capacity_nodes, opts=None, eq_opts=None, max_step=None):
-        self.pool = DevicePool(capacity_nodes)
+    def __init__(self, capacity, server, stream):
+        self.server = server
+        self.stream = stream
+        with self._ctx():
+            self.pool = DevicePool(capacity)
+        self.queue = []                 # routed here, not admitted yet
+        self.members, self.live = [], []
+        self.B = None
+        self.rounds = self.rebuilds = self.max_resident = 0
+        self.load = 0                   # outstanding node-points routed here
+
+    def _ctx(self):
+        import contextlib
+        torch = _torch()
+        return contextlib.nullcontext() if self.stream is None else torch.cuda.stream(self.stream)
+
+    @property
+    def busy(self):
+        return bool(self.queue or self.live)
+
+    def step(self, now, out, lock=None):
+        """Admit what fits, then one round over the resident devices.
+        Returns the number of devices that reached a point.  `lock` guards
+        the queue and load against the dispatcher thread."""
+        import contextlib
+        guard = lock if lock is not None else contextlib.nullcontext()
+        srv = self.server
+        new = []
+        with guard:
+            routed = list(self.queue)
+        for r in routed:
+            off = self.pool.alloc(r.spec.N)
+            if off is None:
+                continue
+            setup = setup_from_spec(r.spec)
+            self.pool.write(setup, off)
+            slot = dict(req=r, setup=setup, off=off,
+                        track=Track(setup, r.spec.sweep, max_step=srv.max_step))
+            out[r.rid] = Served(r.rid, r.kind, r.spec.N, len(r.spec.sweep), r.arrival,
+                                admitted=now(), result=slot["track"].result)
+            self.live.append(slot)
+            new.append(slot)
+            with guard:
+                self.queue.remove(r)
+        if not self.live:
+            self.B, self.members = None, []
+            return 0
+        if new:
+            self.B = None                         # release the old workspace first
+            self.members = list(self.live)
+            self.B = GPUBatch([m["setup"] for m in self.members],
+                              offsets=[m["off"] for m in self.members], planes=self.pool.planes,
+                              stream=self.stream)
+            self.rebuilds += 1
+            self.max_resident = max(self.max_resident, len(self.members))
+            mask = np.array([1 if m in new else 0 for m in self.members], dtype=np.int32)
+            its, conv, status = self.B.solve_equilibrium(srv.eq_opts, active=mask)
+            for d, m in enumerate(self.members):
+                if mask[d] and (status[d] != _lib.NEWTON_OK or not conv[d]):
+                    m["track"].result.failed = "equilibrium did not converge"
+        tracks = [m["track"] for m in self.members]
+        reached = solve_round(self.B, tracks, srv.opts)
+        self.rounds += 1
+        record_currents(self.B, tracks, reached, now())
+        keep = []
+        for m in self.live:
+            if m["track"].open:
+                keep.append(m)
+                continue
+            out[m["req"].rid].done = now()
+            self.pool.release(m["off"], m["setup"].N)
+            with guard:
+                self.load -= m["req"].spec.N * len(m["req"].spec.sweep)
+        self.live = keep
+        return len(reached)
+
+
+class Server:
+    """Continuous-batching solver.  `opts` is used for every bias point;
+    equilibrium uses `eq_opts`.  With lanes > 1 the pool is split into
+    that many lanes, each an independent loop on its own HIP stream and
+    host thread, so that their launches overlap on the GPU; an arriving
+    request goes to the lane with the least outstanding work (node count
+    x points) that can hold it."""
+
+    def __init__(self, capacity_nodes, opts=None, eq_opts=None, max_step=None, lanes=1):
         self.opts = opts or NewtonOptions()
         self.eq_opts = eq_opts or NewtonOptions(max_iter=400)
         self.max_step = max_step
-        self.rounds = 0
-        self.rebuilds = 0
-        self.max_resident = 0
+        torch = _torch()
+        self.lanes = [_Lane(int(capacity_nodes) // lanes, self,
+                            None if lanes == 1 else torch.cuda.Stream())
+                      for _ in range(lanes)]
+
+    @property
+    def rounds(self):
+        return sum(L.rounds for L in self.lanes)
+
+    @property
+    def rebuilds(self):
+        return sum(L.rebuilds for L in self.lanes)
+
+    @property
+    def max_resident(self):
+        return max(L.max_resident for L in self.lanes)
+
+    def _route(self, r):
+        fits = [L for L in self.lanes if r.spec.N <= L.pool.capacity]
+        if not fits:
+            raise ValueError(f"request {r.rid}: {r.spec.N} nodes exceed the lane capacity "
+                             f"{self.lanes[0].pool.capacity}")
+        L = min(fits, key=lambda L: L.load)
+        L.load += r.spec.N * len(r.spec.sweep)
+        L.queue.append(r)
 
     def run(self, requests, realtime=True, log=None):
-        torch = _torch()
         pending = sorted(requests, key=lambda r: r.arrival)
         for r in pending:
-            if r.spec.N > self.pool.capacity:
-                raise ValueError(f"request {r.rid}: {r.spec.N} nodes exceed the pool "
-                                 f"capacity {self.pool.capacity}")
-        queue, members, live, out = [], [], [], {}
-        B = None
+            if not any(r.spec.N <= L.pool.capacity for L in self.lanes):
+                raise ValueError(f"request {r.rid}: {r.spec.N} nodes exceed the lane "
+                                 f"capacity {self.lanes[0].pool.capacity}")
+        out = {}
+        if len(self.lanes) == 1:
+            self._run_single(pending, realtime, out, log)
+        else:
+            if not realtime:
+                raise ValueError("several lanes run in real time only")
+            self._run_lanes(pending, out)
+        return [out[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+
+    def _run_single(self, pending, realtime, out, log):
+        L = self.lanes[0]
         t0 = time.perf_counter()
-        clock = (lambda: time.perf_counter() - t0) if realtime else None
-        vt = 0.0                      # virtual clock when not real time
+        vt = [0.0]                                # virtual clock when not real time
 
         def now():
-            return clock() if realtime else vt
+            return time.perf_counter() - t0 if realtime else vt[0]
 
-        while pending or queue or live:
-            t = now()
-            while pending and pending[0].arrival <= t:
-                queue.append(pending.pop(0))
-            # admission by HBM budget, arrival order, first fit
-            new = []
-            for r in list(queue):
-                off = self.pool.alloc(r.spec.N)
-                if off is None:
-                    continue
-                setup = setup_from_spec(r.spec)
-                self.pool.write(setup, off)
-                slot = dict(req=r, setup=setup, off=off,
-                            track=Track(setup, r.spec.sweep, max_step=self.max_step))
-                out[r.rid] = Served(r.rid, r.kind, r.spec.N, len(r.spec.sweep), r.arrival,
-                                    admitted=t, result=slot["track"].result)
-                live.append(slot)
-                new.append(slot)
-                queue.remove(r)
-            if not live:
-                B, members = None, []
-                if pending:
-                    wait = pending[0].arrival - now()
-                    if realtime and wait > 0:
-                        time.sleep(wait)
-                    elif not realtime:
-                        vt = max(vt, pending[0].arrival)
+        while pending or L.busy:
+            while pending and pending[0].arrival <= now():
+                self._route(pending.pop(0))
+            if not L.busy:
+                wait = pending[0].arrival - now()
+                if realtime and wait > 0:
+                    time.sleep(wait)
+                elif not realtime:
+                    vt[0] = max(vt[0], pending[0].arrival)
                 continue
-            if new:
-                B = None                      # release the old workspace first
-                members = list(live)
-                B = GPUBatch([s["setup"] for s in members],
-                             offsets=[s["off"] for s in members], planes=self.pool.planes)
-                self.rebuilds += 1
-                self.max_resident = max(self.max_resident, len(members))
-                mask = np.array([1 if s in new else 0 for s in members], dtype=np.int32)
-                its, conv, status = B.solve_equilibrium(self.eq_opts, active=mask)
-                for d, s in enumerate(members):
-                    if mask[d] and (status[d] != _lib.NEWTON_OK or not conv[d]):
-                        s["track"].result.failed = "equilibrium did not converge"
-            tracks = [s["track"] for s in members]
             t_round = time.perf_counter()
-            reached = solve_round(B, tracks, self.opts)
-            self.rounds += 1
+            reached = L.step(now, out)
             if not realtime:
-                vt += time.perf_counter() - t_round
-            record_currents(B, tracks, reached, now())
-            keep = []
-            for s in live:
-                if s["track"].open:
-                    keep.append(s)
-                    continue
-                out[s["req"].rid].done = now()
-                self.pool.release(s["off"], s["setup"].N)
-            live = keep
+                vt[0] += time.perf_counter() - t_round
             if log is not None and reached:
-                log(f"t={now():.2f}s live={len(live)} queue={len(queue)} "
-                    f"pool={self.pool.used}/{self.pool.capacity}")
-        return [out[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+                log(f"t={now():.2f}s live={len(L.live)} queue={len(L.queue)} "
+                    f"pool={L.pool.used}/{L.pool.capacity}")
+
+    def _run_lanes(self, pending, out):
+        import threading
+        torch = _torch()
+        lock = threading.Condition()
+        state = {"arrivals_done": False}
+        errors = []
+        t0 = time.perf_counter()
+
+        def now():
+            return time.perf_counter() - t0
+
+        def lane_loop(L):
+            try:
+                with torch.cuda.stream(L.stream):
+                    while True:
+                        with lock:
+                            while not L.busy and not state["arrivals_done"]:
+                                lock.wait(0.05)
+                            if not L.busy and state["arrivals_done"]:
+                                return
+                        L.step(now, out, lock)
+            except Exception as e:                     # re-raised by the caller
+                errors.append(e)
+                with lock:
+                    state["arrivals_done"] = True
+                    lock.notify_all()
+
+        threads = [threading.Thread(target=lane_loop, args=(L,)) for L in self.lanes]
+        for t in threads:
+            t.start()
+        try:
+            for r in pending:
+                wait = r.arrival - now()
+                if wait > 0:
+                    time.sleep(wait)
+                if errors:
+                    break
+                with lock:
+                    self._route(r)
+                    lock.notify_all()
+        finally:
+            with lock:
+                state["arrivals_done"] = True
+                lock.notify_all()
+            for t in threads:
+                t.join()
+        if errors:
+            raise errors[0]
 
 
 def request_stream(count, rate, seed, mix=("diode", "mosfet", "trigate"), max_nodes=200_000,

@@ -42,3 +42,20 @@ def test_serving_alone_equals_mix(mode):
             assert a.keys() == b.keys()
             assert all(a[k] == b[k] for k in a), (r.spec.name, a, b)
         assert sv.done >= sv.admitted >= sv.arrival
+
+
+def test_serving_lanes_alone_equals_mix():
+    """Two lanes (own HIP stream and host thread each), real-time arrivals."""
+    serve, NewtonOptions = _mods()
+    reqs = serve.request_stream(6, rate=30.0, seed=29, small=True, max_points=6)
+    print(f"[serve-lanes] {len(reqs)} requests, 2 lanes", flush=True)
+    opts = NewtonOptions()
+    S = serve.Server(capacity_nodes=2 * max(r.spec.N for r in reqs) + 100, opts=opts, lanes=2)
+    served = S.run(reqs, realtime=True)
+    assert all(L.rounds > 0 for L in S.lanes)
+    for r, sv in zip(reqs, served):
+        alone = serve.serve_alone(r.spec, opts)
+        assert sv.result.failed == alone.failed
+        assert sv.result.newton == alone.newton
+        for a, b in zip(sv.result.currents, alone.currents):
+            assert all(a[k] == b[k] for k in a), (r.spec.name, a, b)
