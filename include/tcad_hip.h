@@ -54,8 +54,9 @@ extern "C" {
  * Newton solve's relative residual target is clamp(0.01 * err, lin_rtol,
  * max(1e-4, lin_rtol)) with err the previous step's update measure (1e-4
  * for the first step); without the flag every solve aims at lin_rtol.
- * Either way the step that passes the convergence test is refined to a
- * true relative residual of 1e-14 before it is applied. */
+ * Either way the step that passes the convergence test is refined to the
+ * FP64 rounding floor (true relative residual 1e-16, or until a sweep no
+ * longer halves it) before it is applied. */
 #define TCAD_LIN_FORCING 0x100
 
 /* Per-device outcome of a Newton solve (h_status). */

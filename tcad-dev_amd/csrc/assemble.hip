@@ -37,6 +37,7 @@ struct AsmArgs {
   const double *gate_Vg, *gate_Vfb;
   int ne;
   int want32;
+  int res_only;                     // only the raw F_n + F_p (no rows, no matrix)
   double *F, *Fnp, *A64, *Dinv64, *bs, *rowscale;
   float *A32, *Dinv32;
   int *flag;                        // [0] any, [1 + d] per device
@@ -227,6 +228,7 @@ __device__ void assemble_node(const AsmArgs &A, const DevInfo &I, long long k, i
   double rhs[3];
   const int ct = A.cidx[g];
   if (A.Fnp) A.Fnp[g] = Fn + Fp;      // raw, pre-Dirichlet (terminal current)
+  if (A.res_only) return;
   if (ct >= 0) {
     // contact node: Dirichlet rows (device2d.py:1800-1852)
     double psi0, n0, p0;
@@ -369,8 +371,9 @@ __global__ void __launch_bounds__(BS) k_assemble(AsmArgs A) {
   const int d = A.blk2dev[blockIdx.x];
   if (A.scal && !A.scal[d].active) return;
   const DevInfo &I = A.info[d];
-  const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
-  if (k >= I.n) return;
+  const NodeAt P = node_at(I, blockIdx.x);
+  if (!P.valid) return;
+  const long long k = P.k;
   if (I.dim == 2) assemble_node<2>(A, I, k, d);
   else assemble_node<3>(A, I, k, d);
 }
@@ -398,16 +401,19 @@ static AsmArgs make_args(Batch &b, const double *psi, const double *n, const dou
 }
 
 int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
-                    double *F, double *Fnp, bool want_fp32, bool only_active) {
+                    double *F, double *Fnp, bool want_fp32, bool only_active, bool res_only) {
   AsmArgs A = make_args(b, psi, n, p);
   A.scal = only_active ? b.d_scal : nullptr;
+  A.res_only = res_only ? 1 : 0;
   A.want32 = want_fp32 ? 1 : 0;
   A.F = F; A.Fnp = Fnp;
   A.A64 = b.A64; A.A32 = b.A32; A.Dinv64 = b.Dinv64; A.Dinv32 = b.Dinv32;
   A.bs = b.bs; A.rowscale = b.rowscale;
-  b.last_n = n;
-  b.last_p = p;
-  b.last_psi = psi;
+  if (!res_only) {
+    b.last_n = n;
+    b.last_p = p;
+    b.last_psi = psi;
+  }
   hipLaunchKernelGGL(k_assemble, dim3(b.nblk), dim3(BS), 0, b.stream, A);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
@@ -518,8 +524,9 @@ __global__ void __launch_bounds__(BS) k_assemble_poisson(AsmArgs A, double *Aeq,
   const int d = A.blk2dev[blockIdx.x];
   if (!A.scal[d].active) return;
   const DevInfo &I = A.info[d];
-  const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
-  if (k >= I.n) return;
+  const NodeAt P = node_at(I, blockIdx.x);
+  if (!P.valid) return;
+  const long long k = P.k;
   if (I.dim == 2) assemble_poisson_node<2>(A, I, k, d, Aeq, beq, F);
   else assemble_poisson_node<3>(A, I, k, d, Aeq, beq, F);
 }
@@ -583,8 +590,9 @@ int launch_surface_mobility(Batch &b, const double *psi, bool only_active) {
 __global__ void __launch_bounds__(BS) k_edge_currents(AsmArgs A, double *J) {
   const int d = A.blk2dev[blockIdx.x];
   const DevInfo &I = A.info[d];
-  const long long k = (long long)(blockIdx.x - I.blk0) * BS + threadIdx.x;
-  if (k >= I.n) return;
+  const NodeAt P = node_at(I, blockIdx.x);
+  if (!P.valid) return;
+  const long long k = P.k;
   const long long N = A.N, g = I.off + k;
   int c[3];
   node_coords(I, k, c);

@@ -90,14 +90,10 @@ struct NodeHood {
 
 __device__ __forceinline__ NodeHood hood(const DevInfo &I, int blk) {
   NodeHood H;
-  const int k = (blk - I.blk0) * BS + (int)threadIdx.x;
-  H.valid = k < (int)I.n;
-  H.g = I.off + k;
-  const unsigned uk = (unsigned)k, unx = (unsigned)I.nx, uny = (unsigned)I.ny;
-  const unsigned t = uk / unx;
-  const int i = (int)(uk - t * unx);
-  const unsigned kk = t / uny;
-  const int j = (int)(t - kk * uny);
+  const NodeAt P = node_at(I, blk);
+  H.valid = P.valid;
+  H.g = I.off + P.k;
+  const int i = P.i, j = P.j, kk = P.kk;
   const long long sy = I.nx, sz = (long long)I.nx * I.ny;
   H.nb[0] = i > 0;          H.nbg[0] = H.g - 1;
   H.nb[1] = i < I.nx - 1;   H.nbg[1] = H.g + 1;

@@ -14,9 +14,13 @@
 //   precond      Dinv64/Dinv32[e*N + k], e < 9: inverse of the scaled node
 //                block (node block-Jacobi)
 //   vectors      3 planes of N_total (v[c*N + k])
-// Kernels map one thread to one node; each workgroup covers one device's
-// node range (blocks never straddle devices), so per-device reductions
-// are over a contiguous block range and deterministic.
+// Kernels map one thread to one node.  Each device is cut into patches of
+// PX x PY nodes (x fastest) in every z-plane, one workgroup per patch:
+// thread t is node (i0 + t % PX, j0 + t / PX), so every wavefront reads one
+// contiguous row segment of PX nodes (coalesced), and a patch holds whole
+// 4 x 4 node tiles (the tile preconditioner inverts them in LDS).
+// Workgroups never straddle devices, so per-device reductions run over a
+// contiguous block range in a fixed order and are deterministic.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,6 +37,9 @@ namespace tcad {
 #define TCAD_BS 256
 #endif
 constexpr int BS = TCAD_BS;         // threads per node workgroup (multiple of 64)
+constexpr int PX = 64;              // patch width: one wavefront per patch row
+constexpr int PY = BS / PX;         // patch height
+static_assert(BS % PX == 0, "workgroup must hold whole patch rows");
 constexpr int NDIAG = 9;
 constexpr int NOFF = 5;
 
@@ -43,7 +50,8 @@ struct DevInfo {
   int dim, nx, ny, nz;
   long long off;                    // first node in the batch planes
   long long n;                      // node count
-  int blk0, nblk;                   // node-kernel block range
+  int blk0, nblk;                   // node-kernel block range (patches)
+  int px, py;                       // patches along x and y
   int h_off[3];                     // offsets into the concatenated spacing arrays
   int dv_off[3];                    // offsets into the concatenated width arrays
   double VT, Ns, LD, J0, R0, Cn, Cp;
@@ -60,6 +68,7 @@ struct Scal {
   double rho, alpha, omega, beta;   // Krylov scalars of the current iteration
   double bb;                        // ||b||^2
   double rr;                        // last ||r||^2
+  double rr_ref;                    // final refinement: ||r||^2 before the last sweep
   double nrm;                       // IR residual norm (rhs normalisation)
   double err;                       // Newton update measure
   double tol2;                      // stop test rr <= tol2 * bb of the current solve
@@ -76,6 +85,27 @@ struct Scal {
   int sel;                          // device selected for the current Newton solve
   int result;                       // TCAD_NEWTON_* outcome of the current Newton solve
 };
+
+// Node of thread threadIdx.x in workgroup blk of device I.
+struct NodeAt {
+  bool valid;
+  long long k;                      // device-local node index
+  int i, j, kk;
+};
+
+__device__ __forceinline__ NodeAt node_at(const DevInfo &I, int blk) {
+  NodeAt P;
+  const unsigned local = (unsigned)(blk - I.blk0);
+  const unsigned upx = (unsigned)I.px, upy = (unsigned)I.py;
+  const unsigned bx = local % upx, t = local / upx;
+  const unsigned by = t % upy, bz = t / upy;
+  P.i = (int)(bx * PX + threadIdx.x % PX);
+  P.j = (int)(by * PY + threadIdx.x / PX);
+  P.kk = (int)bz;
+  P.valid = P.i < I.nx && P.j < I.ny;
+  P.k = ((long long)P.kk * I.ny + P.j) * I.nx + P.i;
+  return P;
+}
 
 struct Batch {
   int ndev = 0;
@@ -116,6 +146,8 @@ struct Batch {
   double *y64 = nullptr;                         // scaled solution
   double *F = nullptr;                           // internal residual buffer
   double *Fnp = nullptr;
+  double *cand = nullptr;                        // candidate state psi, n, p (3 planes)
+  double *d_Icand = nullptr;                     // contact currents at the candidate
   // Krylov work vectors (FP64) and the refinement buffers
   double *v64[9] = {};
   double *ir_r = nullptr, *ir_rhs = nullptr;
@@ -146,9 +178,11 @@ void set_error(const std::string &msg);
   } while (0)
 
 // assemble.hip
-// only_active: skip devices whose Newton solve is not running (Scal.active)
+// only_active: skip devices whose Newton solve is not running (Scal.active);
+// res_only: write only the raw F_n + F_p per node (Fnp), nothing else
 int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
-                    double *F, double *Fnp, bool want_fp32, bool only_active = false);
+                    double *F, double *Fnp, bool want_fp32, bool only_active = false,
+                    bool res_only = false);
 int launch_surface_mobility(Batch &b, const double *psi, bool only_active = false);
 int launch_assemble_poisson(Batch &b, const double *psi, double *F);
 int launch_edge_currents(Batch &b, const double *psi, const double *n, const double *p,
