@@ -58,6 +58,12 @@ extern "C" {
  * FP64 rounding floor (true relative residual 1e-16, or until a sweep no
  * longer halves it) before it is applied. */
 #define TCAD_LIN_FORCING 0x100
+/* Flag OR'd into lin_mode (tcad_solve_bias, tcad_solve_linear): precondition
+ * with the reference's node block-Jacobi (3x3 block per node,
+ * linsolve.py:363-427) instead of the default 4x4-tile block-Jacobi (the
+ * exact inverse of the matrix restricted to each 4 x 4 node tile of a
+ * plane, 48 unknowns, stored in FP32). */
+#define TCAD_PRECOND_NODE 0x200
 
 /* Per-device outcome of a Newton solve (h_status). */
 #define TCAD_NEWTON_OK 0             /* converged (or masked out)            */

@@ -397,7 +397,8 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
                            double *p) {
   int rc;
   const bool forcing = (mode & TCAD_LIN_FORCING) != 0;
-  mode &= ~TCAD_LIN_FORCING;
+  b.tile = (mode & TCAD_PRECOND_NODE) ? 0 : 1;
+  mode &= ~(TCAD_LIN_FORCING | TCAD_PRECOND_NODE);
   const double eta_max = rtol > 1e-4 ? rtol : 1e-4;
   if ((rc = launch_set_active(b, mask, forcing ? eta_max : rtol))) return rc;
   hipLaunchKernelGGL(k_stamp, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev, b.d_cidx,
@@ -427,6 +428,7 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
   for (int it = 0; it < max_iter; ++it) {
     if ((rc = launch_surface_mobility(b, psi, true))) return rc;
     if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, mixed, true))) return rc;
+    if (b.tile && (rc = launch_tile_setup(b, true))) return rc;
     if ((rc = launch_flags_to_status(b))) return rc;
     if ((rc = ir_solve(b, mixed, true, rtol, 16, maxit, inner, true))) return rc;
     // measure the step, and before applying a step that passes the test
@@ -549,7 +551,7 @@ static void free_batch(Batch *b) {
                   b->d_cscale, b->d_c2dev, b->d_surf[0], b->d_surf[1], b->d_surf[2],
                   b->d_surf[3], b->A64, b->A32, b->Dinv64, b->Dinv32, b->bs, b->rowscale,
                   b->colscale, b->y64, b->F, b->Fnp, b->part, b->d_scal, b->Aeq, b->beq,
-                  b->d_flag, b->d_mask, b->cand, b->d_Icand};
+                  b->d_flag, b->d_mask, b->cand, b->d_Icand, b->T4};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (int i = 0; i < 9; ++i)
@@ -775,6 +777,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   for (int i = 0; i < 5; ++i) TRY(dalloc(&b->veq[i], N));
   TRY(dalloc(&b->d_flag, (size_t)ndev + 1));
   TRY(dalloc(&b->d_mask, (size_t)ndev));
+  TRY(dalloc(&b->T4, 36 * N));
   TRY(dalloc(&b->cand, 3 * N));
   TRY(dalloc(&b->d_Icand, (size_t)std::max(1, cglobal)));
 #undef TRY
@@ -868,6 +871,7 @@ int tcad_assemble(tcad_batch *bp, const double *psi, const double *n, const doub
   }
   int rc = tcad::launch_assemble(b, psi, n, p, F, Fnp, true);
   if (rc) return rc;
+  if ((rc = tcad::launch_tile_setup(b, false))) return rc;
   return tcad::check_flag(b);
 }
 
@@ -889,6 +893,8 @@ int tcad_solve_linear(tcad_batch *bp, int32_t mode, double rtol, int32_t maxiter
                       int32_t *h_iters, double *h_resid) {
   GUARD(bp);
   Batch &b = *bp->impl;
+  b.tile = (mode & TCAD_PRECOND_NODE) ? 0 : 1;
+  mode &= ~TCAD_PRECOND_NODE;
   if (mode != TCAD_LIN_BICGSTAB_F64 && mode != TCAD_LIN_BICGSTAB_MIXED) {
     tcad::set_error("tcad_solve_linear: unknown mode");
     return TCAD_E_INVALID;
@@ -964,7 +970,7 @@ int tcad_solve_bias(tcad_batch *bp, int32_t max_iter, double tol_update, double 
     tcad::set_error("tcad_solve_bias: invalid argument");
     return TCAD_E_INVALID;
   }
-  const int base_mode = lin_mode & ~TCAD_LIN_FORCING;
+  const int base_mode = lin_mode & ~(TCAD_LIN_FORCING | TCAD_PRECOND_NODE);
   if (base_mode != TCAD_LIN_BICGSTAB_F64 && base_mode != TCAD_LIN_BICGSTAB_MIXED) {
     tcad::set_error("tcad_solve_bias: unknown linear-solve mode");
     return TCAD_E_INVALID;

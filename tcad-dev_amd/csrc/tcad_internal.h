@@ -140,6 +140,8 @@ struct Batch {
   float *A32 = nullptr;
   double *Dinv64 = nullptr;
   float *Dinv32 = nullptr;
+  float4 *T4 = nullptr;                          // 4x4-tile inverses, 36 planes
+  int tile = 1;                                  // preconditioner of the next solves
   double *bs = nullptr;                          // scaled rhs
   double *rowscale = nullptr;
   double *colscale = nullptr;                    // (1, n, p) at assembly
@@ -198,6 +200,8 @@ int bicgstab(Batch &b, bool fp32, double rtol, int maxit, bool zero_guess, int c
 int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int inner_maxit,
              double inner_rtol, bool per_dev = false);
 int pcg_poisson(Batch &b, double rtol, int maxit);
+// inverses of the 4x4-tile diagonal blocks of the last assembled matrix
+int launch_tile_setup(Batch &b, bool only_active);
 int launch_unscale(Batch &b, const double *n, const double *p, double *du);
 int launch_unscale_state(Batch &b, const double *n, const double *p, double *du);
 int launch_spmv_only(Batch &b, bool fp32);

@@ -25,6 +25,7 @@ TCAD_E_NO_DEVICE = -6
 LIN_BICGSTAB_F64 = 0
 LIN_BICGSTAB_MIXED = 1
 LIN_FORCING = 0x100
+PRECOND_NODE = 0x200
 
 NEWTON_OK = 0
 NEWTON_MAX_ITER = 1

@@ -38,7 +38,10 @@ class NewtonOptions:
     iterative refinement.  linsolve_rtol is the relative residual of the
     (row-equilibrated) Newton system.  forcing: inexact early Newton
     solves (target clamp(0.01 * previous update, linsolve_rtol, 1e-4));
-    the converging step is refined to the FP64 rounding floor either way.  Off by default:
+    the converging step is refined to the FP64 rounding floor either way.
+    precond: "tile" (default) -- exact inverse of every 4 x 4 node tile's
+    48 x 48 diagonal block, FP32 -- or "node", the reference's 3x3 node
+    block-Jacobi.  Off by default:
     on the 200x120 MOSFET sweep it added a Newton iteration per point and
     cost more Krylov work than it saved."""
     max_iter: int = 100
@@ -48,6 +51,7 @@ class NewtonOptions:
     linsolve_rtol: float = 1e-10
     lin_maxiter: int = 20000
     forcing: bool = False
+    precond: str = "tile"
     verbose: bool = False
 
     def lin_mode(self):
@@ -58,7 +62,10 @@ class NewtonOptions:
             mode = _lib.LIN_BICGSTAB_MIXED
         else:
             raise ValueError(f"unknown linsolve {m!r}")
-        return mode | (_lib.LIN_FORCING if self.forcing else 0)
+        if self.precond not in ("tile", "node"):
+            raise ValueError(f"unknown precond {self.precond!r}")
+        return (mode | (_lib.LIN_FORCING if self.forcing else 0)
+                | (_lib.PRECOND_NODE if self.precond == "node" else 0))
 
 
 class DirichletBC:
