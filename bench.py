@@ -128,7 +128,7 @@ def bench_trigate(vds=(0.05, 0.7)):
     return out
 
 
-def bench_batch(lanes=4, sample=8):
+def bench_batch(lanes=1, sample=8):
     """256 seeded diodes (40x30..160x100 nodes, -2.0..0.8 V, 0.1 V
     continuation steps), batched in `lanes` streams, both precisions;
     the plain FP64 path (one device at a time) on an evenly spaced
@@ -169,12 +169,21 @@ def bench_batch(lanes=4, sample=8):
            "plain_sample": f"{len(idx)} evenly spaced devices solved one at a time"}
     for mode, r in res.items():
         fails = sum(1 for x in r["results"] if x.failed)
-        err = max(float(np.max(np.abs(final(r["results"][d]) - final(alone[d])) /
-                                np.abs(final(alone[d])))) for d in idx)
+        err, ok = 0.0, True
+        for d in idx:
+            got, ref = final(r["results"][d]), final(alone[d])
+            err = max(err, float(np.max(np.abs(got - ref) / np.abs(ref))))
+            # reverse-bias currents are resolvable only to the FP64
+            # solve's own conservation error |sum_c I| (the CPU oracle's
+            # direct solve shows the same imbalance): 1e-6 of the largest
+            # current plus 10x that floor, as the parity tests use
+            ok = ok and bool(np.all(np.abs(got - ref) <= 1e-6 * np.abs(ref).max()
+                                    + 10.0 * abs(ref.sum())))
         out[mode] = {"devices_per_s": round(len(specs) / r["seconds"], 4),
                      "seconds": round(r["seconds"], 3), "failed": fails,
                      "vs_plain_fp64": round(len(specs) / r["seconds"] / plain, 2),
-                     "max_rel_current_err_vs_fp64_on_sample": err}
+                     "max_rel_current_err_vs_fp64_on_sample": err,
+                     "meets_1e-6_with_conservation_floor": ok}
     return out
 
 

@@ -59,10 +59,10 @@ extern "C" {
  * longer halves it) before it is applied. */
 #define TCAD_LIN_FORCING 0x100
 /* Flag OR'd into lin_mode (tcad_solve_bias, tcad_solve_linear): precondition
- * with the reference's node block-Jacobi (3x3 block per node,
- * linsolve.py:363-427) instead of the default 4x4-tile block-Jacobi (the
- * exact inverse of the matrix restricted to each 4 x 4 node tile of a
- * plane, 48 unknowns, stored in FP32). */
+ * every device with the reference's node block-Jacobi (3x3 block per node,
+ * linsolve.py:363-427).  Without it, 2D devices use 4x4-tile block-Jacobi
+ * (the exact inverse of the matrix restricted to each 4 x 4 node tile, 48
+ * unknowns, stored in FP32) and 3D devices node block-Jacobi. */
 #define TCAD_PRECOND_NODE 0x200
 
 /* Per-device outcome of a Newton solve (h_status). */

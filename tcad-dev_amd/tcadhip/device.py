@@ -39,9 +39,10 @@ class NewtonOptions:
     (row-equilibrated) Newton system.  forcing: inexact early Newton
     solves (target clamp(0.01 * previous update, linsolve_rtol, 1e-4));
     the converging step is refined to the FP64 rounding floor either way.
-    precond: "tile" (default) -- exact inverse of every 4 x 4 node tile's
-    48 x 48 diagonal block, FP32 -- or "node", the reference's 3x3 node
-    block-Jacobi.  Off by default:
+    precond: "tile" (default; 2D devices get the exact inverse of every
+    4 x 4 node tile's 48 x 48 diagonal block, FP32, 3D devices the node
+    block-Jacobi) or "node" (the reference's 3x3 node block-Jacobi for
+    every device).  Off by default:
     on the 200x120 MOSFET sweep it added a Newton iteration per point and
     cost more Krylov work than it saved."""
     max_iter: int = 100
