@@ -36,3 +36,37 @@ def test_error_mapping():
         _lib.check(_lib.TCAD_E_INDEX, "x")
     with pytest.raises(_lib.LinearSolveError):
         _lib.check(_lib.TCAD_E_NOT_CONVERGED, "x")
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """The ctypes mirrors of tcad_device_desc / tcad_static_fields have the
+    C layout of include/tcad_hip.h (size and every field offset), checked
+    with the host C compiler."""
+    import ctypes
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        import pytest
+        pytest.skip("no host C compiler")
+    fields = [f for f, _ in _lib.DeviceDesc._fields_]
+    sfields = ["d_C", "d_nie_s", "d_nie", "d_tau_n", "d_tau_p", "d_dV", "d_an", "d_ap",
+               "d_dlnnie"]
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tcad_hip.h"',
+             'int main(void) {', 'printf("%zu\\n", sizeof(tcad_device_desc));']
+    lines += [f'printf("%zu\\n", offsetof(tcad_device_desc, {f}));' for f in fields]
+    lines += ['printf("%zu\\n", sizeof(tcad_static_fields));']
+    lines += [f'printf("%zu\\n", offsetof(tcad_static_fields, {f}));' for f in sfields]
+    lines += ['return 0; }']
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run([cc, "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    want = [ctypes.sizeof(_lib.DeviceDesc)]
+    want += [getattr(_lib.DeviceDesc, f).offset for f in fields]
+    want += [ctypes.sizeof(_lib.StaticFields)]
+    want += [getattr(_lib.StaticFields, f).offset for f in sfields]
+    assert got == want
