@@ -10,16 +10,18 @@ It restates, in NumPy/SciPy, the reference's structured 2D/3D
 drift-diffusion path (Boltzmann statistics, single semiconductor,
 ohmic Dirichlet contacts and Robin gates):
 
-  physics.py  constants, silicon parameters, Bernoulli, recombination,
-              mobility (Caughey-Thomas, Lombardi CVT), BGN, lifetimes,
-              ohmic contact values, control-volume widths
-  dd2d.py     Device2D: equilibrium Poisson Newton, coupled bias Newton,
-              residual/Jacobian assembly, terminal current
-  dd3d.py     Device3D: the same on the 7-point stencil with
-              normal-axis gates
-  linsolve.py direct (SuperLU) and node-block-Jacobi GMRES/BiCGStab
-              solves with the reference's acceptance contract
-  dirichlet.py symmetric Dirichlet elimination
+  physics.py    constants, silicon parameters, Bernoulli, recombination,
+                mobility (Caughey-Thomas, Lombardi CVT), BGN, lifetimes,
+                ohmic contact values, control-volume widths
+  dd.py         Device2D / Device3D (dimension-generic core): equilibrium
+                Poisson Newton, coupled bias Newton, residual/Jacobian
+                assembly with the 5-/7-point stencils and normal-axis
+                gates, terminal current
+  linsolve.py   symmetric Dirichlet elimination, direct (SuperLU) and
+                node-block-Jacobi GMRES/BiCGStab solves with the
+                reference's acceptance contract
+  transient.py  theta-scheme transient steps (storage term, line-search
+                Newton, adaptive dt) and the voltage waveforms
 
 Every function cites the reference file:line it follows.  How each
 function is pinned (golden fixtures generated from the reference,
