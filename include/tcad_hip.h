@@ -203,6 +203,38 @@ int tcad_solve_bias(tcad_batch *b, int32_t max_iter, double tol_update,
                     int32_t *h_iters, int32_t *h_converged, int32_t *h_krylov,
                     int32_t *h_status);
 
+/* One implicit time step of the theta scheme, from the state at the start
+ * of the step to the new time, whose contact voltages are set by
+ * tcad_set_bias.  Per non-contact node the continuity rows become
+ *   theta F_n + (1 - theta) F_n(old) - dV/dt (n - n_old)   (electrons)
+ *   theta F_p + (1 - theta) F_p(old) + dV/dt (p - p_old)   (holes)
+ * with the Poisson and contact rows unchanged; Newton with Armijo
+ * backtracking on 0.5 |F|^2 (up to 40 halvings), psi clip, density clip
+ * and the update test; in 3D a Newton iteration whose line search finds no
+ * step ends the step unconverged.  h_dt_s: per device, the scaled step
+ * dt * R0 / Ns as the reference forms it; d_Fn_old / d_Fp_old: the raw
+ * continuity residuals at the old state and old voltages
+ * (tcad_continuity_residuals), needed when theta != 1.  lin_mode must
+ * select the FP64 path (TCAD_LIN_BICGSTAB_F64, flags allowed).  Writes the
+ * new state in place; per device iterations, converged flag and outcome.
+ * Replaces: transient2d._step_residual_jacobian / _newton_step
+ * (pytcad/pytcad/transient2d.py:104-183), transient3d.py:118-219. */
+int tcad_transient_step(tcad_batch *b, int32_t max_iter, double tol_update,
+                        double max_dpsi, int32_t lin_mode, double lin_rtol,
+                        int32_t lin_maxiter, const int32_t *h_active,
+                        const double *h_dt_s, double theta,
+                        const double *d_n_old, const double *d_p_old,
+                        const double *d_Fn_old, const double *d_Fp_old,
+                        double *d_psi, double *d_n, double *d_p,
+                        int32_t *h_iters, int32_t *h_converged, int32_t *h_status);
+
+/* Raw box-integration continuity residuals F_n, F_p per node (before the
+ * contact rows), the last two outputs of _residual_jacobian
+ * (device2d.py:1920, device3d.py:1978), at the voltages set by
+ * tcad_set_bias. */
+int tcad_continuity_residuals(tcad_batch *b, const double *d_psi, const double *d_n,
+                              const double *d_p, double *d_Fn, double *d_Fp);
+
 /* Terminal currents of every contact of every device at (psi, n, p):
  * h_I[c] = sum over the contact's nodes of the raw F_n + F_p times
  * J0*LD (2D, A/cm) or J0*LD^2 (3D, A).  Replaces:

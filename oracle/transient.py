@@ -17,7 +17,7 @@ import numpy as np
 import scipy.sparse as sp
 from scipy.sparse.linalg import spsolve
 
-from .dd import DirichletBC
+from .dd import DirichletBC, equilibrated_direct
 from .physics import NewtonOptions
 
 
@@ -141,18 +141,22 @@ def step_residual_jacobian(device, psi, n, p, voltages_new, n_old, p_old, F_old_
 
 
 def newton_step(device, psi0, n0, p0, voltages_new, F_old_n, F_old_p, dV, dt_s, theta, k_free,
-                opts):
+                opts, equilibrated=False):
     """transient2d.py:138-183 / transient3d.py:152-219: Newton on the step
     residual with a direct solve, psi clip, Armijo backtracking on
     0.5 |F|^2 (up to 40 halvings), density clip; err from the clipped
     (not damped) psi update and the damped density updates.  In 3D a step
-    whose line search rejects everything ends the Newton loop."""
+    whose line search rejects everything ends the Newton loop.
+    equilibrated=True (not a reference method) solves the same system
+    after column scaling (1, n, p) and row equilibration: the exact step
+    the GPU's equilibrated Krylov solve converges to, where the
+    reference's SuperLU on the unscaled Jacobian loses digits."""
     psi, n, p = psi0.copy(), n0.copy(), p0.copy()
     shape = psi.shape
     for it in range(opts.max_iter):
         F, J, _, _ = step_residual_jacobian(device, psi, n, p, voltages_new, n0, p0, F_old_n,
                                             F_old_p, dV, dt_s, theta, k_free)
-        du = spsolve(J.tocsc(), -F)
+        du = equilibrated_direct(J, -F, n, p) if equilibrated else spsolve(J.tocsc(), -F)
         dpsi = np.clip(du[0::3].reshape(shape), -opts.max_dpsi, opts.max_dpsi)
         dn = du[1::3].reshape(shape)
         dp = du[2::3].reshape(shape)
@@ -185,7 +189,7 @@ def newton_step(device, psi0, n0, p0, voltages_new, F_old_n, F_old_p, dV, dt_s, 
 
 
 def solve_transient(device, waveforms, t_end, dt0, theta=1.0, opts=None, dt_min=None,
-                    dt_max=None, growth=1.5, shrink=0.5, output_times=None):
+                    dt_max=None, growth=1.5, shrink=0.5, output_times=None, equilibrated=False):
     """transient2d.py:186-306 / transient3d.py:234-363: step the device
     from its current state to t_end with adaptive dt (grow by `growth`
     after a step that took <= max(3, max_iter // 4) Newton iterations,
@@ -237,7 +241,7 @@ def solve_transient(device, waveforms, t_end, dt0, theta=1.0, opts=None, dt_min=
         if theta != 1.0:
             *_, F_old_n, F_old_p = device._residual_jacobian(psi, n, p, voltages_at(t))
         psi1, n1, p1, ok, nit = newton_step(device, psi, n, p, v_new, F_old_n, F_old_p, dV, step,
-                                            theta, k_free, opts)
+                                            theta, k_free, opts, equilibrated)
         if ok:
             t = t_new
             psi, n, p = psi1, n1, p1

@@ -37,7 +37,14 @@ struct AsmArgs {
   const double *gate_Vg, *gate_Vfb;
   int ne;
   int want32;
-  int res_only;                     // only the raw F_n + F_p (no rows, no matrix)
+  int res_only;                     // 1: only the raw F_n + F_p; 2: only F (all rows)
+  double *Fn_raw, *Fp_raw;          // optional raw continuity residuals per node
+  // transient step (transient2d.py:104-135): storage term on the
+  // non-contact continuity rows; theta-scheme with the old residuals
+  int transient;
+  const double *n_old, *p_old, *Fn_old, *Fp_old;
+  const double *dt_s;               // per device, scaled time step
+  double theta;
   double *F, *Fnp, *A64, *Dinv64, *bs, *rowscale;
   float *A32, *Dinv32;
   int *flag;                        // [0] any, [1 + d] per device
@@ -228,7 +235,31 @@ __device__ void assemble_node(const AsmArgs &A, const DevInfo &I, long long k, i
   double rhs[3];
   const int ct = A.cidx[g];
   if (A.Fnp) A.Fnp[g] = Fn + Fp;      // raw, pre-Dirichlet (terminal current)
-  if (A.res_only) return;
+  if (A.Fn_raw) {
+    A.Fn_raw[g] = Fn;
+    A.Fp_raw[g] = Fp;
+  }
+  if (A.res_only == 1) return;
+  if (A.transient && ct < 0) {
+    // F_n <- theta F_n + (1 - theta) F_n(old) - dV/dt (n - n_old), rows
+    // scaled by theta, storage on the diagonal; same order of operations
+    // as transient2d.py:118-135 (theta == 1 leaves F and J untouched)
+    const double th = A.theta, dts = A.dt_s[dev];
+    if (th != 1.0) {
+      F3[1] = th * F3[1] + (1.0 - th) * A.Fn_old[g];
+      F3[2] = th * F3[2] + (1.0 - th) * A.Fp_old[g];
+#pragma unroll
+      for (int e = 3; e < 9; ++e) D[e] = th * D[e];
+#pragma unroll
+      for (int dir = 0; dir < 6; ++dir)
+#pragma unroll
+        for (int m = 1; m < 5; ++m) off[dir][m] = th * off[dir][m];
+    }
+    F3[1] = F3[1] - dV / dts * (nn - A.n_old[g]);
+    F3[2] = F3[2] + dV / dts * (pp - A.p_old[g]);
+    D[4] = D[4] + -dV / dts;
+    D[8] = D[8] + dV / dts;
+  }
   if (ct >= 0) {
     // contact node: Dirichlet rows (device2d.py:1800-1852)
     double psi0, n0, p0;
@@ -244,7 +275,17 @@ __device__ void assemble_node(const AsmArgs &A, const DevInfo &I, long long k, i
       for (int m = 0; m < 5; ++m) off[dir][m] = 0.0;
 #pragma unroll
     for (int r = 0; r < 3; ++r) rhs[r] = -F3[r];
+    if (A.res_only == 2) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) A.F[r * N + g] = F3[r];
+      return;
+    }
   } else {
+    if (A.res_only == 2) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) A.F[r * N + g] = F3[r];
+      return;
+    }
     // symmetric elimination of contact columns (dirichlet.py:161-191):
     // rhs -= J[:, contact] * g with g = -F(contact), in column order
     double corr[3] = {0.0, 0.0, 0.0};
@@ -401,10 +442,22 @@ static AsmArgs make_args(Batch &b, const double *psi, const double *n, const dou
 }
 
 int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
-                    double *F, double *Fnp, bool want_fp32, bool only_active, bool res_only) {
+                    double *F, double *Fnp, bool want_fp32, bool only_active, int res_only,
+                    const TransientArgs *tr) {
   AsmArgs A = make_args(b, psi, n, p);
   A.scal = only_active ? b.d_scal : nullptr;
-  A.res_only = res_only ? 1 : 0;
+  A.res_only = res_only;
+  if (tr) {
+    A.transient = tr->active;
+    A.n_old = tr->n_old;
+    A.p_old = tr->p_old;
+    A.Fn_old = tr->Fn_old;
+    A.Fp_old = tr->Fp_old;
+    A.dt_s = tr->dt_s;
+    A.theta = tr->theta;
+    A.Fn_raw = tr->Fn_raw;
+    A.Fp_raw = tr->Fp_raw;
+  }
   A.want32 = want_fp32 ? 1 : 0;
   A.F = F; A.Fnp = Fnp;
   A.A64 = b.A64; A.A32 = b.A32; A.Dinv64 = b.Dinv64; A.Dinv32 = b.Dinv32;

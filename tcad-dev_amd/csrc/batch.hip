@@ -122,9 +122,11 @@ __global__ void k_newton_check(UArgs U, double tol, double accept2, double rtol,
   }
 }
 
-// Partial sums over the non-contact nodes of the raw F_n + F_p (at a
-// candidate state): minus the imbalance of the contact currents.
-__global__ void __launch_bounds__(BS) k_interior_sum(UArgs U, const int *cidx, const double *Fnp) {
+// Partial sums over the non-contact nodes of the continuity rows F_n + F_p
+// (at a candidate state; storage terms included in a transient step):
+// minus the imbalance of the contact currents (plus, in a transient step,
+// the displacement of mobile charge, which the rows already contain).
+__global__ void __launch_bounds__(BS) k_interior_sum(UArgs U, const int *cidx, const double *F) {
   __shared__ double sh[BS / 64];
   const int d = U.blk2dev[blockIdx.x];
   const DevInfo &I = U.info[d];
@@ -133,7 +135,7 @@ __global__ void __launch_bounds__(BS) k_interior_sum(UArgs U, const int *cidx, c
   double v = 0.0;
   if (P.valid) {
     const long long g = I.off + P.k;
-    if (cidx[g] < 0) v = Fnp[g];
+    if (cidx[g] < 0) v = F[U.N + g] + F[2 * U.N + g];
   }
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -175,6 +177,145 @@ __global__ void k_refine_check(UArgs U, const double *I_c, const double *cscale,
   const bool stalled = !first && S.rr > 0.25 * S.rr_ref;
   if (balanced || stalled || !isfinite(delta)) S.active = 0;
   S.rr_ref = S.rr;
+}
+
+// ---------------------------------------------------------------------------
+// Transient step (transient2d.py:138-183, transient3d.py:152-219): Newton on
+// the step residual with Armijo backtracking on the merit 0.5 |F|^2.
+// ---------------------------------------------------------------------------
+// Partial sums of |F|^2 (all three rows) per workgroup.
+__global__ void __launch_bounds__(BS) k_merit_part(UArgs U, const double *F) {
+  __shared__ double sh[BS / 64];
+  const int d = U.blk2dev[blockIdx.x];
+  const DevInfo &I = U.info[d];
+  if (!U.scal[d].active) return;
+  const NodeAt P = node_at(I, blockIdx.x);
+  double v = 0.0;
+  if (P.valid) {
+    const long long g = I.off + P.k;
+    const double f0 = F[g], f1 = F[U.N + g], f2 = F[2 * U.N + g];
+    v = f0 * f0 + f1 * f1 + f2 * f2;
+  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < BS / 64; ++w) s += sh[w];
+    U.part[blockIdx.x] = s;
+  }
+}
+
+// One workgroup per device.  phase 0: the merit at the iterate starts the
+// line search at lambda = 1.  phase 1: the trial at lambda is accepted if
+// finite and merit <= merit0 (1 - 1e-4 lambda), else lambda halves; after
+// 40 rejections lambda = 0.  A settled device leaves the active set.
+__global__ void k_ls_merit(UArgs U, int phase) {
+  __shared__ double sh[BS / 64];
+  const int d = blockIdx.x;
+  const DevInfo &I = U.info[d];
+  Scal &S = U.scal[d];
+  if (!S.active) return;
+  double m = 0.0;
+  for (int t = threadIdx.x; t < I.nblk; t += blockDim.x) m += U.part[I.blk0 + t];
+  for (int o = 32; o > 0; o >>= 1) m += __shfl_down(m, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+  const double merit = 0.5 * s;
+  if (phase == 0) {
+    S.merit0 = merit;
+    S.lam = 1.0;
+    S.ls_done = 0;
+    S.ls_it = 0;
+    return;
+  }
+  if (isfinite(merit) && merit <= S.merit0 * (1.0 - 1e-4 * S.lam)) {
+    S.ls_done = 1;
+    S.active = 0;
+  } else {
+    S.lam = S.lam * 0.5;
+    S.ls_it += 1;
+    if (S.ls_it >= 40) {
+      S.lam = 0.0;
+      S.ls_done = 1;
+      S.active = 0;
+    }
+  }
+}
+
+__global__ void k_ls_begin(Scal *scal, int ndev) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < ndev) scal[d].newton_active = scal[d].active;
+}
+
+// Step at the device's lambda: psi + lambda clip(dpsi), densities clipped
+// to [0.1x, 10x].  apply == 0: into the candidate buffers; apply == 1: into
+// the state, with the update measure err = max(|clip(dpsi)|, rel dn,
+// rel dp) (psi undamped, densities damped, transient2d.py:172-181).
+__global__ void __launch_bounds__(BS) k_tr_step(UArgs U, const double *y, double *psi, double *n,
+                                                double *p, double max_dpsi, int apply,
+                                                double *cp, double *cn, double *cq) {
+  __shared__ double sh[BS / 64];
+  const int d = U.blk2dev[blockIdx.x];
+  const DevInfo &I = U.info[d];
+  const Scal &S = U.scal[d];
+  if (!S.active) return;
+  const NodeAt P = node_at(I, blockIdx.x);
+  double err = 0.0;
+  if (P.valid) {
+    const long long g = I.off + P.k, N = U.N;
+    const double lam = S.lam;
+    const double n0 = n[g], p0 = p[g];
+    const double dpsi = clip_np(y[g], -max_dpsi, max_dpsi);
+    const double psi1 = psi[g] + lam * dpsi;
+    const double n1 = clip_np(n0 + lam * (n0 * y[N + g]), 0.1 * n0, 10.0 * n0);
+    const double p1 = clip_np(p0 + lam * (p0 * y[2 * N + g]), 0.1 * p0, 10.0 * p0);
+    if (apply) {
+      err = fmax(fmax(fabs(dpsi), fabs(n1 - n0) / fmax(n0, 1e-10)), fabs(p1 - p0) / fmax(p0, 1e-10));
+      if (!(err == err) || !isfinite(psi1) || !isfinite(n1) || !isfinite(p1)) err = INFINITY;
+      psi[g] = psi1;
+      n[g] = n1;
+      p[g] = p1;
+    } else {
+      cp[g] = psi1;
+      cn[g] = n1;
+      cq[g] = p1;
+    }
+  }
+  if (!apply) return;
+  const double m = ublock_max(err, sh);
+  if (threadIdx.x == 0) U.part[blockIdx.x] = m;
+}
+
+// After the step: count it; converged, failed (3D: line search found no
+// step, transient3d.py:216) or non-finite devices retire.
+__global__ void k_tr_finish(UArgs U, double tol) {
+  __shared__ double sh[BS / 64];
+  const int d = blockIdx.x;
+  const DevInfo &I = U.info[d];
+  Scal &S = U.scal[d];
+  if (!S.active) return;
+  double m = 0.0;
+  for (int t = threadIdx.x; t < I.nblk; t += blockDim.x) m = fmax(m, U.part[I.blk0 + t]);
+  m = ublock_max(m, sh);
+  if (threadIdx.x != 0) return;
+  S.err = m;
+  S.newton_it += 1;
+  if (m < tol) {
+    S.newton_done = 1;
+    S.active = 0;
+  } else if (!isfinite(m)) {
+    S.result = TCAD_NEWTON_NONFINITE;
+    S.active = 0;
+  } else if (I.dim == 3 && S.lam == 0.0) {
+    S.result = TCAD_NEWTON_MAX_ITER;
+    S.active = 0;
+  }
 }
 
 // Per-device assembly error flags -> the device stops as non-finite.
@@ -392,6 +533,49 @@ static int check_flag(Batch &b) {
 // ---------------------------------------------------------------------------
 // Newton drivers
 // ---------------------------------------------------------------------------
+// Refine the solves of the devices whose step just passed the convergence
+// test (Scal.conv_now), one 1e-6 inner sweep at a time, until the contact
+// currents at the candidate state balance to 1e-8 of the largest (toward
+// the FP64 rounding floor, 1e-16, if need be).  A reverse-bias terminal
+// current is a cancellation of fluxes many decades larger: with the
+// residual at 1e-14 it still moved by percents with the summation order
+// (measured on a 24x16 diode at -2 V), while a forward-biased or on-state
+// device usually needs no sweep at all.  Runs on the FP64 matrix in both
+// modes: where the Jacobian's conditioning times FP32 rounding approaches
+// 1, refinement around the FP32-stored matrix stalls early.
+static int refine_converging(Batch &b, const UArgs &U, const double *psi, const double *n,
+                             const double *p, double max_dpsi, int maxit,
+                             const TransientArgs *tr) {
+  int rc;
+  const double inner_ref = 1e-6, final_tol = 1e-16;
+  const dim3 gd((b.ndev + 63) / 64), td(64);
+  hipLaunchKernelGGL(k_mask_converging, gd, td, 0, b.stream, b.d_scal, b.ndev);
+  b.launches++;
+  double *cp = b.cand, *cn = b.cand + b.ntot, *cq = b.cand + 2 * b.ntot;
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
+                       max_dpsi, 1, cp, cn, cq);
+    b.launches++;
+    if ((rc = launch_assemble(b, cp, cn, cq, b.F, b.Fnp, false, true, 2, tr))) return rc;
+    hipLaunchKernelGGL(k_interior_sum, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.d_cidx, b.F);
+    b.launches++;
+    if ((rc = launch_contact_sums(b, b.Fnp, b.d_Icand))) return rc;
+    hipLaunchKernelGGL(k_refine_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, b.d_Icand,
+                       b.d_cscale, sweep == 0 ? 1 : 0);
+    b.launches++;
+    TCAD_HIP_CHECK(hipGetLastError());
+    if ((rc = sync_scalars(b))) return rc;
+    int open = 0;
+    for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active;
+    if (open == 0) break;
+    if ((rc = ir_solve(b, false, false, final_tol, 1, maxit, inner_ref))) return rc;
+  }
+  hipLaunchKernelGGL(k_unmask, gd, td, 0, b.stream, b.d_scal, b.ndev);
+  b.launches++;
+  TCAD_HIP_CHECK(hipGetLastError());
+  return TCAD_OK;
+}
+
 static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, int mode,
                            double rtol, int maxit, const int32_t *mask, double *psi, double *n,
                            double *p) {
@@ -412,18 +596,6 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
   // by its storage rounding
   const double inner = mixed ? 1e-6 : rtol;
   const double accept = rtol > 1e-6 ? rtol : 1e-6;
-  // the final-step refinement goes on, one 1e-6 inner sweep at a time,
-  // until the contact currents at the candidate state balance to 1e-8 of
-  // the largest (toward the FP64 rounding floor, 1e-16, if need be): a
-  // reverse-bias terminal current is a cancellation of fluxes many decades
-  // larger, and with the residual at 1e-14 it still moved by percents
-  // with the summation order (measured on a 24x16 diode at -2 V), while a
-  // forward-biased or on-state device usually needs no sweep at all.  It
-  // runs on the FP64 matrix in both modes: where the Jacobian's
-  // conditioning times FP32 rounding approaches 1, refinement around the
-  // FP32-stored matrix stalls early.
-  const double inner_ref = 1e-6;
-  const double final_tol = 1e-16;
   const dim3 gd((b.ndev + 63) / 64), td(64);
   for (int it = 0; it < max_iter; ++it) {
     if ((rc = launch_surface_mobility(b, psi, true))) return rc;
@@ -445,38 +617,79 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
     if ((rc = sync_scalars(b))) return rc;
     int conv = 0;
     for (int d = 0; d < b.ndev; ++d) conv += b.h_scal[d].active && b.h_scal[d].conv_now;
-    if (conv && rtol > final_tol) {
-      // refine the converging steps one sweep at a time until the contact
-      // currents at the candidate state balance (or refinement stalls)
-      hipLaunchKernelGGL(k_mask_converging, gd, td, 0, b.stream, b.d_scal, b.ndev);
-      b.launches++;
-      double *cp = b.cand, *cn = b.cand + b.ntot, *cq = b.cand + 2 * b.ntot;
-      for (int sweep = 0; sweep < 16; ++sweep) {
-        hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi,
-                           n, p, max_dpsi, 1, cp, cn, cq);
-        b.launches++;
-        if ((rc = launch_assemble(b, cp, cn, cq, b.F, b.Fnp, false, true, true))) return rc;
-        hipLaunchKernelGGL(k_interior_sum, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.d_cidx,
-                           b.Fnp);
-        b.launches++;
-        if ((rc = launch_contact_sums(b, b.Fnp, b.d_Icand))) return rc;
-        hipLaunchKernelGGL(k_refine_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, b.d_Icand,
-                           b.d_cscale, sweep == 0 ? 1 : 0);
-        b.launches++;
-        TCAD_HIP_CHECK(hipGetLastError());
-        if ((rc = sync_scalars(b))) return rc;
-        int open = 0;
-        for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active;
-        if (open == 0) break;
-        if ((rc = ir_solve(b, false, false, final_tol, 1, maxit, inner_ref))) return rc;
-      }
-      hipLaunchKernelGGL(k_unmask, gd, td, 0, b.stream, b.d_scal, b.ndev);
-      b.launches++;
+    if (conv && rtol > 1e-16) {
+      if ((rc = refine_converging(b, U, psi, n, p, max_dpsi, maxit, nullptr))) return rc;
     }
     hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
                        max_dpsi, 1, psi, n, p);
     hipLaunchKernelGGL(k_newton_finish, gd, td, 0, b.stream, b.d_scal, b.ndev);
     b.launches += 2;
+    TCAD_HIP_CHECK(hipGetLastError());
+    if ((rc = sync_scalars(b))) return rc;
+    int open = 0;
+    for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active;
+    if (open == 0) break;
+  }
+  return TCAD_OK;
+}
+
+static int transient_impl(Batch &b, int max_iter, double tol, double max_dpsi, int mode,
+                          double rtol, int maxit, const int32_t *mask, const TransientArgs &tr,
+                          double *psi, double *n, double *p) {
+  int rc;
+  b.tile = (mode & TCAD_PRECOND_NODE) ? 0 : 1;
+  mode &= ~(TCAD_LIN_FORCING | TCAD_PRECOND_NODE);
+  const bool mixed = mode == TCAD_LIN_BICGSTAB_MIXED;
+  if ((rc = launch_set_active(b, mask, rtol))) return rc;
+  UArgs U = uargs(b);
+  const double inner = mixed ? 1e-6 : rtol;
+  const double accept = rtol > 1e-6 ? rtol : 1e-6;
+  const dim3 gd((b.ndev + 63) / 64), td(64);
+  double *cp = b.cand, *cn = b.cand + b.ntot, *cq = b.cand + 2 * b.ntot;
+  for (int it = 0; it < max_iter; ++it) {
+    // F (all rows, step terms included) and the eliminated, scaled matrix;
+    // the contact rows move the contacts to the new time's values
+    if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, mixed, true, 0, &tr))) return rc;
+    if (b.tile && (rc = launch_tile_setup(b, true))) return rc;
+    if ((rc = launch_flags_to_status(b))) return rc;
+    hipLaunchKernelGGL(k_merit_part, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.F);
+    hipLaunchKernelGGL(k_ls_merit, dim3(b.ndev), dim3(BS), 0, b.stream, U, 0);
+    b.launches += 2;
+    if ((rc = ir_solve(b, mixed, true, rtol, 16, maxit, inner, true))) return rc;
+    // a full step that would pass the test is refined first (as in
+    // solve_bias), so the state it lands on is exact to rounding
+    hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
+                       max_dpsi, 0, psi, n, p);
+    hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol,
+                       accept * accept, rtol, 1.0, 0);
+    b.launches += 2;
+    TCAD_HIP_CHECK(hipGetLastError());
+    if ((rc = sync_scalars(b))) return rc;
+    int conv = 0;
+    for (int d = 0; d < b.ndev; ++d) conv += b.h_scal[d].active && b.h_scal[d].conv_now;
+    if (conv && (rc = refine_converging(b, U, psi, n, p, max_dpsi, maxit, &tr))) return rc;
+    // Armijo backtracking, every device at its own lambda
+    hipLaunchKernelGGL(k_ls_begin, gd, td, 0, b.stream, b.d_scal, b.ndev);
+    b.launches++;
+    for (int ls = 0; ls < 40; ++ls) {
+      hipLaunchKernelGGL(k_tr_step, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
+                         max_dpsi, 0, cp, cn, cq);
+      b.launches++;
+      if ((rc = launch_assemble(b, cp, cn, cq, b.F, nullptr, false, true, 2, &tr))) return rc;
+      hipLaunchKernelGGL(k_merit_part, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.F);
+      hipLaunchKernelGGL(k_ls_merit, dim3(b.ndev), dim3(BS), 0, b.stream, U, 1);
+      b.launches += 2;
+      TCAD_HIP_CHECK(hipGetLastError());
+      if ((rc = sync_scalars(b))) return rc;
+      int open = 0;
+      for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active;
+      if (open == 0) break;
+    }
+    hipLaunchKernelGGL(k_unmask, gd, td, 0, b.stream, b.d_scal, b.ndev);
+    hipLaunchKernelGGL(k_tr_step, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
+                       max_dpsi, 1, cp, cn, cq);
+    hipLaunchKernelGGL(k_tr_finish, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol);
+    b.launches += 3;
     TCAD_HIP_CHECK(hipGetLastError());
     if ((rc = sync_scalars(b))) return rc;
     int open = 0;
@@ -551,7 +764,7 @@ static void free_batch(Batch *b) {
                   b->d_cscale, b->d_c2dev, b->d_surf[0], b->d_surf[1], b->d_surf[2],
                   b->d_surf[3], b->A64, b->A32, b->Dinv64, b->Dinv32, b->bs, b->rowscale,
                   b->colscale, b->y64, b->F, b->Fnp, b->part, b->d_scal, b->Aeq, b->beq,
-                  b->d_flag, b->d_mask, b->cand, b->d_Icand, b->T4};
+                  b->d_flag, b->d_mask, b->cand, b->d_Icand, b->T4, b->d_dts};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (int i = 0; i < 9; ++i)
@@ -777,6 +990,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   for (int i = 0; i < 5; ++i) TRY(dalloc(&b->veq[i], N));
   TRY(dalloc(&b->d_flag, (size_t)ndev + 1));
   TRY(dalloc(&b->d_mask, (size_t)ndev));
+  TRY(dalloc(&b->d_dts, (size_t)ndev));
   TRY(dalloc(&b->T4, 36 * N));
   TRY(dalloc(&b->cand, 3 * N));
   TRY(dalloc(&b->d_Icand, (size_t)std::max(1, cglobal)));
@@ -985,6 +1199,71 @@ int tcad_solve_bias(tcad_batch *bp, int32_t max_iter, double tol_update, double 
     if (h_krylov) h_krylov[d] = b.h_scal[d].kry_total;
     if (h_status) h_status[d] = newton_outcome(b.h_scal[d]);
   }
+  return TCAD_OK;
+}
+
+int tcad_transient_step(tcad_batch *bp, int32_t max_iter, double tol_update, double max_dpsi,
+                        int32_t lin_mode, double lin_rtol, int32_t lin_maxiter,
+                        const int32_t *h_active, const double *h_dt_s, double theta,
+                        const double *n_old, const double *p_old, const double *Fn_old,
+                        const double *Fp_old, double *psi, double *n, double *p,
+                        int32_t *h_iters, int32_t *h_converged, int32_t *h_status) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi || !n || !p || !n_old || !p_old || !h_dt_s || max_iter <= 0 || lin_maxiter <= 0 ||
+      !(lin_rtol > 0.0) || !(theta > 0.0 && theta <= 1.0) ||
+      (theta != 1.0 && (!Fn_old || !Fp_old))) {
+    tcad::set_error("tcad_transient_step: invalid argument");
+    return TCAD_E_INVALID;
+  }
+  const int base_mode = lin_mode & ~(TCAD_LIN_FORCING | TCAD_PRECOND_NODE);
+  if (base_mode != TCAD_LIN_BICGSTAB_F64) {
+    // the FP32-matrix path took a different Newton path on a reverse-biased
+    // Crank-Nicolson ramp and ended 5e-5 away from the reference's state
+    tcad::set_error("tcad_transient_step: only the FP64 linear path is supported");
+    return TCAD_E_INVALID;
+  }
+  for (int d = 0; d < b.ndev; ++d)
+    if (!(h_dt_s[d] > 0.0) && (!h_active || h_active[d])) {
+      tcad::set_error("tcad_transient_step: every selected device needs dt_s > 0");
+      return TCAD_E_INVALID;
+    }
+  TCAD_HIP_CHECK(hipMemcpyAsync(b.d_dts, h_dt_s, sizeof(double) * b.ndev, hipMemcpyHostToDevice,
+                                b.stream));
+  tcad::TransientArgs tr{};
+  tr.active = 1;
+  tr.n_old = n_old;
+  tr.p_old = p_old;
+  tr.Fn_old = Fn_old;
+  tr.Fp_old = Fp_old;
+  tr.dt_s = b.d_dts;
+  tr.theta = theta;
+  int rc = tcad::transient_impl(b, max_iter, tol_update, max_dpsi, lin_mode, lin_rtol,
+                                lin_maxiter, h_active, tr, psi, n, p);
+  if (rc) return rc;
+  if ((rc = tcad::sync_scalars(b))) return rc;
+  for (int d = 0; d < b.ndev; ++d) {
+    if (h_iters) h_iters[d] = b.h_scal[d].newton_it;
+    if (h_converged) h_converged[d] = b.h_scal[d].newton_done;
+    if (h_status) h_status[d] = newton_outcome(b.h_scal[d]);
+  }
+  return TCAD_OK;
+}
+
+int tcad_continuity_residuals(tcad_batch *bp, const double *psi, const double *n, const double *p,
+                              double *Fn, double *Fp) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi || !n || !p || !Fn || !Fp) {
+    tcad::set_error("tcad_continuity_residuals: null argument");
+    return TCAD_E_INVALID;
+  }
+  tcad::TransientArgs tr{};
+  tr.Fn_raw = Fn;
+  tr.Fp_raw = Fp;
+  int rc = tcad::launch_assemble(b, psi, n, p, b.F, nullptr, false, false, 1, &tr);
+  if (rc) return rc;
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
   return TCAD_OK;
 }
 

@@ -73,6 +73,10 @@ struct Scal {
   double err;                       // Newton update measure
   double tol2;                      // stop test rr <= tol2 * bb of the current solve
   double eta2;                      // squared forcing term of this device's next Newton solve
+  double lam;                       // transient line search: current step fraction
+  double merit0;                    // transient line search: 0.5 |F|^2 at the iterate
+  int ls_done;                      // transient line search: step fraction settled
+  int ls_it;                        // transient line search: rejected trials
   int maxit;                        // iteration cap of the current solve
   int it;                           // Krylov iterations of the current solve
   int done;                         // Krylov finished (converged or stopped)
@@ -149,6 +153,7 @@ struct Batch {
   double *F = nullptr;                           // internal residual buffer
   double *Fnp = nullptr;
   double *cand = nullptr;                        // candidate state psi, n, p (3 planes)
+  double *d_dts = nullptr;                       // per-device scaled time step
   double *d_Icand = nullptr;                     // contact currents at the candidate
   // Krylov work vectors (FP64) and the refinement buffers
   double *v64[9] = {};
@@ -180,11 +185,23 @@ void set_error(const std::string &msg);
   } while (0)
 
 // assemble.hip
+// Transient-step terms of the assembly (transient2d.py:104-135) and the
+// optional raw continuity residual outputs.
+struct TransientArgs {
+  int active;                       // add the storage / theta terms
+  const double *n_old, *p_old;      // state at the start of the step
+  const double *Fn_old, *Fp_old;    // raw continuity residuals there (theta != 1)
+  const double *dt_s;               // per device, scaled step
+  double theta;
+  double *Fn_raw, *Fp_raw;          // if set: raw F_n, F_p per node
+};
+
 // only_active: skip devices whose Newton solve is not running (Scal.active);
-// res_only: write only the raw F_n + F_p per node (Fnp), nothing else
+// res_only: 1 = write only the raw F_n + F_p per node (Fnp); 2 = only F
+// (all rows, boundary and transient terms included), no matrix
 int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
                     double *F, double *Fnp, bool want_fp32, bool only_active = false,
-                    bool res_only = false);
+                    int res_only = 0, const TransientArgs *tr = nullptr);
 int launch_surface_mobility(Batch &b, const double *psi, bool only_active = false);
 int launch_assemble_poisson(Batch &b, const double *psi, double *F);
 int launch_edge_currents(Batch &b, const double *psi, const double *n, const double *p,

@@ -104,6 +104,11 @@ def lib():
                              ctypes.c_int32, ctypes.c_double, ctypes.c_int32, c_int32_p,
                              vp, vp, vp, c_int32_p, c_int32_p, c_int32_p, c_int32_p],
                             ctypes.c_int),
+        "tcad_transient_step": ([vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_int32, ctypes.c_double, ctypes.c_int32, c_int32_p,
+                                 c_double_p, ctypes.c_double, vp, vp, vp, vp, vp, vp, vp,
+                                 c_int32_p, c_int32_p, c_int32_p], ctypes.c_int),
+        "tcad_continuity_residuals": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "tcad_terminal_currents": ([vp, vp, vp, vp, c_double_p], ctypes.c_int),
         "tcad_edge_currents": ([vp, vp, vp, vp, vp], ctypes.c_int),
         "tcad_surface_mobility": ([vp, vp], ctypes.c_int),
@@ -155,4 +160,5 @@ def exported_symbols():
             "tcad_assemble", "tcad_export_jacobian", "tcad_solve_linear",
             "tcad_solve_equilibrium", "tcad_solve_bias", "tcad_terminal_currents",
             "tcad_edge_currents", "tcad_surface_mobility", "tcad_time_kernel",
+            "tcad_transient_step", "tcad_continuity_residuals",
             "tcad_launch_count", "tcad_last_error", "tcad_version"]

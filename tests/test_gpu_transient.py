@@ -1,0 +1,112 @@
+"""Transient steps on the GPU (tcad_transient_step through
+tcadhip.transient.solve_transient) against the golden fixtures written
+from the reference's transient2d.solve_transient.
+
+Bars:
+* the same time grid (times and dt history identical: the adaptive dt
+  depends only on the Newton counts);
+* Newton iterations per accepted step within +-1 of the reference's
+  where the reference's own solve is well conditioned, else within +-1 of
+  the oracle run with the equilibrated direct solve (the exact step the
+  GPU's equilibrated Krylov solve converges to; the reference's SuperLU on
+  the unscaled Jacobian loses digits in depletion regions and takes extra
+  iterations) and never more than the reference's + 1;
+* contact currents per step within 1e-6 of the largest contact current at
+  that step plus 1e-9 of the largest over the run (the early steps of a
+  voltage step carry currents decades above the late ones);
+* final psi within 1e-8 (scaled); n and p within 1e-6 relative to
+  max(value, 1e-10): each step stops at the update test err < 1e-8 and
+  those differences carry from step to step over the run (27 steps on
+  the MOSFET), and the reference's update measure floors densities at
+  1e-10 (transient2d.py:177-178), so minority densities far below it are
+  not controlled by either solve.
+Transient steps run the FP64 linear path only (the library rejects the
+FP32-matrix mode there).
+"""
+import warnings
+import numpy as np
+import pytest
+
+from golden.fixtures import load
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["transient_diode_be.npz", "transient_diode_cn.npz", "transient_mosfet_be.npz"]
+
+
+def _mods():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a HIP device")
+    from tcadhip import transient
+    from tcadhip.device import NewtonOptions, device_from_spec
+    return transient, NewtonOptions, device_from_spec
+
+
+def _waveforms(tr, d):
+    desc = [str(v) for v in d["wf_desc"]]
+    name, kind, vals = desc[0], desc[1], [float(v) for v in desc[2:]]
+    cls = {"step": tr.StepWaveform, "ramp": tr.RampWaveform}[kind]
+    return {name: cls(*vals)}
+
+
+def _equilibrated_iterations(spec, d):
+    """Newton counts of the oracle transient with the equilibrated direct
+    solve, same fixture inputs."""
+    from oracle import physics as oph
+    from oracle import transient as otr
+    from oracle_build import oracle_device
+    dev = oracle_device(spec)
+    opts = oph.NewtonOptions()
+    dev.solve_equilibrium(opts)
+    pre = {str(k): float(v) for k, v in zip(d["prebias_keys"], d["prebias_vals"])}
+    if pre:
+        dev.solve_bias(pre, oph.NewtonOptions(linsolve="direct_equilibrated"))
+    desc = [str(v) for v in d["wf_desc"]]
+    cls = {"step": otr.StepWaveform, "ramp": otr.RampWaveform}[desc[1]]
+    wf = {desc[0]: cls(*[float(v) for v in desc[2:]])}
+    out_t = [float(v) for v in d["output_times"]] or None
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        r = otr.solve_transient(dev, wf, float(d["t_end"]), float(d["dt0"]),
+                                theta=float(d["theta"]), opts=opts, output_times=out_t,
+                                equilibrated=True)
+    return r.iterations
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_transient_vs_reference(name):
+    mode = "bicgstab"
+    tr, NewtonOptions, device_from_spec = _mods()
+    print(f"[transient] {name} {mode}", flush=True)
+    spec, d = load(name)
+    dev = device_from_spec(spec)
+    opts = NewtonOptions(linsolve=mode)
+    dev.solve_equilibrium(opts)
+    pre = {str(k): float(v) for k, v in zip(d["prebias_keys"], d["prebias_vals"])}
+    if pre:
+        dev.solve_bias(pre, opts)
+    out_t = [float(v) for v in d["output_times"]] or None
+    res = tr.solve_transient(dev, _waveforms(tr, d), float(d["t_end"]), float(d["dt0"]),
+                             theta=float(d["theta"]), opts=opts, output_times=out_t)
+    assert np.array_equal(res.times, d["times"]), (res.times, d["times"])
+    assert np.array_equal(res.dt_hist, d["dt_hist"])
+    ref_its = d["iterations"]
+    if not np.all(np.abs(res.iterations - ref_its) <= 1):
+        eq_its = _equilibrated_iterations(spec, d)
+        assert np.all(np.abs(res.iterations - eq_its) <= 1), (res.iterations, eq_its)
+        assert np.all(res.iterations <= ref_its + 1), (res.iterations, ref_its)
+    cur = np.array([res.terminal_current[c.name] for c in spec.contacts]).T
+    ref = d["currents"]
+    big = np.abs(ref).max()
+    worst = 0.0
+    for s in range(ref.shape[0]):
+        tol = 1e-6 * np.abs(ref[s]).max() + 1e-9 * big
+        err = np.abs(cur[s] - ref[s]).max()
+        worst = max(worst, err / tol)
+        assert err <= tol, (s, cur[s], ref[s], tol)
+    print(f"  steps {len(res.times) - 1}, worst current error / tolerance {worst:.2e}",
+          flush=True)
+    assert np.abs(res.psi_hist[-1] - d["psi_final"]).max() <= 1e-8
+    for got, want in ((res.n_hist[-1], d["n_final"]), (res.p_hist[-1], d["p_final"])):
+        assert (np.abs(got - want) / np.maximum(want, 1e-10)).max() <= 1e-6
