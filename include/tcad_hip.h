@@ -259,8 +259,10 @@ int tcad_surface_mobility(tcad_batch *b, const double *d_psi);
 /* Bench instrumentation: average duration [microseconds] of `reps`
  * back-to-back launches of one kernel on the last assembled system,
  * timed with HIP events on the library stream.  which: 0 FP64 stencil
- * SpMV, 1 FP32 stencil SpMV, 2 coupled assembly (FP32 copy on).  Has no
- * reference counterpart. */
+ * SpMV, 1 FP32 stencil SpMV, 2 coupled assembly (FP32 copy on), 3 and 4
+ * the FP64 BiCGStab vector kernels that apply the preconditioner (each
+ * launch preceded by a one-thread state reset, included in the time).
+ * Has no reference counterpart. */
 int tcad_time_kernel(tcad_batch *b, int32_t which, int32_t reps, double *h_avg_us);
 
 /* Wall-clock-free counters of the last call: kernel launches issued. */

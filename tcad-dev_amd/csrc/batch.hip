@@ -1289,7 +1289,7 @@ int tcad_terminal_currents(tcad_batch *bp, const double *psi, const double *n, c
 int tcad_time_kernel(tcad_batch *bp, int32_t which, int32_t reps, double *h_avg_us) {
   GUARD(bp);
   Batch &b = *bp->impl;
-  if (which < 0 || which > 2 || reps <= 0 || !h_avg_us) {
+  if (which < 0 || which > 4 || reps <= 0 || !h_avg_us) {
     tcad::set_error("tcad_time_kernel: invalid argument");
     return TCAD_E_INVALID;
   }
@@ -1302,6 +1302,7 @@ int tcad_time_kernel(tcad_batch *bp, int32_t which, int32_t reps, double *h_avg_
   // one untimed launch, then the timed burst
   auto launch = [&]() -> int {
     if (which == 2) return tcad::launch_assemble(b, b.last_psi, b.last_n, b.last_p, b.F, nullptr, true);
+    if (which >= 3) return tcad::launch_vector_kernel_only(b, which);
     return tcad::launch_spmv_only(b, which == 1);
   };
   if ((rc = launch())) return rc;

@@ -180,10 +180,26 @@ __device__ __forceinline__ void tile_slot(int &tile, int &l) {
   l = (jl % TILE) * TILE + il % TILE;
 }
 
+// The 144 inverse entries of this thread's node rows, loaded into registers
+// at the top of a kernel, before its reductions: they depend on nothing the
+// kernel computes, so their memory latency overlaps the reductions' (left
+// to the compiler after the barriers, one kernel issued them one at a
+// time, waiting for each).
+struct TileRows {
+  float4 t[3 * (TU / 4)];
+};
+
+__device__ __forceinline__ void tile_load(const float4 *T4, long long N, long long g, bool load,
+                                          TileRows &R) {
+  if (!load) return;
+#pragma unroll
+  for (int e = 0; e < 3 * (TU / 4); ++e) R.t[e] = T4[(long long)e * N + g];
+}
+
 // z = M^-1 x for this thread's node; every thread of the workgroup calls it
-// (it holds a barrier); tv is LDS of BS * 3 doubles.
-__device__ __forceinline__ void tile_apply(const float4 *T4, long long N, long long g, bool valid,
-                                           const double x[3], double z[3], double *tv) {
+// (it holds barriers); tv is LDS of BS * 3 doubles.
+__device__ __forceinline__ void tile_apply(const TileRows &R, bool valid, const double x[3],
+                                           double z[3], double *tv) {
   int tile, l;
   tile_slot(tile, l);
   double *v = tv + tile * TU;
@@ -197,7 +213,7 @@ __device__ __forceinline__ void tile_apply(const float4 *T4, long long N, long l
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < TU / 4; ++q) {
-      const float4 t = T4[(long long)(r * (TU / 4) + q) * N + g];
+      const float4 t = R.t[r * (TU / 4) + q];
       acc += (double)t.x * v[4 * q] + (double)t.y * v[4 * q + 1] + (double)t.z * v[4 * q + 2] +
              (double)t.w * v[4 * q + 3];
     }
@@ -365,11 +381,17 @@ __global__ void __launch_bounds__(BS) k_bicg_init(KArgs<T> K) {
   }
 }
 
+// The two kernels that apply the preconditioner hold the tile inverse rows
+// (144 registers) across their reductions: allow up to 2 waves per SIMD.
 template <typename T>
-__global__ void __launch_bounds__(BS) k_bicg_1(KArgs<T> K) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_bicg_1(KArgs<T> K) {
   KPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
+  const bool use_tile = K.tile && I.dim == 2;
+  TileRows TR;
+  tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
   double r[3] = {0, 0, 0}, p[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   if (H.valid) {
 #pragma unroll
@@ -399,17 +421,17 @@ __global__ void __launch_bounds__(BS) k_bicg_1(KArgs<T> K) {
       p[c] = r[c] + beta * (p[c] - omega * v[c]);
       K.v[VP][c * N + H.g] = p[c];
     }
-    if (!(K.tile && I.dim == 2)) {
+    if (!use_tile) {
       double z[3];
       prec3<T>(K.Dinv, N, H.g, p, z);
 #pragma unroll
       for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
     }
   }
-  if (K.tile && I.dim == 2) {
+  if (use_tile) {
     __shared__ double tv[BS * 3];
     double z[3];
-    tile_apply(K.T4, N, H.g, H.valid, p, z, tv);
+    tile_apply(TR, H.valid, p, z, tv);
     if (H.valid) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
@@ -451,10 +473,14 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv(KArgs<T> K, int src, int dst, 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BS) k_bicg_3(KArgs<T> K) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_bicg_3(KArgs<T> K) {
   KPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
+  const bool use_tile = K.tile && I.dim == 2;
+  TileRows TR;
+  tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
   double r[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   if (H.valid) {
 #pragma unroll
@@ -479,15 +505,15 @@ __global__ void __launch_bounds__(BS) k_bicg_3(KArgs<T> K) {
       s[c] = r[c] - alpha * v[c];
       K.v[VS][c * N + H.g] = s[c];
     }
-    if (!(K.tile && I.dim == 2)) {
+    if (!use_tile) {
       prec3<T>(K.Dinv, N, H.g, s, z);
 #pragma unroll
       for (int c = 0; c < 3; ++c) K.v[VSH][c * N + H.g] = z[c];
     }
   }
-  if (K.tile && I.dim == 2) {
+  if (use_tile) {
     __shared__ double tv[BS * 3];
-    tile_apply(K.T4, N, H.g, H.valid, s, z, tv);
+    tile_apply(TR, H.valid, s, z, tv);
     if (H.valid) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) K.v[VSH][c * N + H.g] = z[c];
@@ -674,6 +700,28 @@ int launch_spmv_only(Batch &b, bool fp32) {
 __global__ void k_clear_done(Scal *scal, int ndev) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d < ndev) scal[d].done = 0;
+}
+
+// Timing only: a BiCGStab state whose stop test never fires.
+__global__ void k_time_prep(Scal *scal, int ndev) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndev) return;
+  scal[d].done = 0;
+  scal[d].tol2 = 0.0;
+  scal[d].it = 1;
+  scal[d].maxit = 1 << 30;
+}
+
+int launch_vector_kernel_only(Batch &b, int which) {
+  hipLaunchKernelGGL(k_time_prep, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
+                     b.ndev);
+  KArgs<double> K = make_kargs<double>(b, false);
+  K.parity = 0;
+  if (which == 3) hipLaunchKernelGGL(k_bicg_1<double>, dim3(b.nblk), dim3(BS), 0, b.stream, K);
+  else hipLaunchKernelGGL(k_bicg_3<double>, dim3(b.nblk), dim3(BS), 0, b.stream, K);
+  b.launches += 2;
+  TCAD_HIP_CHECK(hipGetLastError());
+  return TCAD_OK;
 }
 
 int clear_done(Batch &b) {

@@ -222,6 +222,9 @@ int launch_tile_setup(Batch &b, bool only_active);
 int launch_unscale(Batch &b, const double *n, const double *p, double *du);
 int launch_unscale_state(Batch &b, const double *n, const double *p, double *du);
 int launch_spmv_only(Batch &b, bool fp32);
+// timing only: one k_bicg_1 (which 3) or k_bicg_3 (which 4), FP64, with a
+// one-thread reset launch before it
+int launch_vector_kernel_only(Batch &b, int which);
 int clear_done(Batch &b);
 
 // newton.hip
