@@ -283,7 +283,7 @@ class _DeviceND:
         self.dp_edge[1][0, :] = (2.0 * mu_p_s * self.mu_p0[1, :]
                                  / (mu_p_s + self.mu_p0[1, :]) * self.VT / ph.D0_REF)
 
-    def _residual_jacobian(self, psi, n, p, voltages):
+    def _residual_jacobian(self, psi, n, p, voltages, current_rows_for=None):
         """device2d.py:1402-1920 / device3d.py:1485-1978.  Returns
         (F, J, Jn_edges, Jp_edges, F_n, F_p): F interleaved (3N) after
         contact/gate BCs, J CSR, per-axis SG edge currents (scaled) and
@@ -380,6 +380,13 @@ class _DeviceND:
                 rows.append(3 * kk); cols.append(3 * kk); vals.append(-bc.kappa * w)
         rows = np.concatenate(rows); cols = np.concatenate(cols)
         vals = np.concatenate(vals)
+        if current_rows_for is not None:
+            # device2d.py:1751-1758: the raw continuity rows of the requested
+            # nodes (the exact derivative of F_n, F_p there), before the
+            # contact rows replace them
+            k_req = np.asarray(current_rows_for, dtype=int)
+            sel = np.isin(rows, np.concatenate([3 * k_req + 1, 3 * k_req + 2]))
+            self._raw_current_rows = (rows[sel].copy(), cols[sel].copy(), vals[sel].copy())
         strip = []
         for name, bc in self.bcs.items():
             if isinstance(bc, DirichletBC):
