@@ -235,6 +235,27 @@ int tcad_transient_step(tcad_batch *b, int32_t max_iter, double tol_update,
 int tcad_continuity_residuals(tcad_batch *b, const double *d_psi, const double *d_n,
                               const double *d_p, double *d_Fn, double *d_Fp);
 
+/* Small-signal AC N-port admittance at the DC operating point
+ * (psi, n, p) and the voltages set by tcad_set_bias, for each of nfreq
+ * frequencies [Hz].  Every contact and gate of a device is a port, in the
+ * order: its contacts, then its gates (creation order).  h_Y receives,
+ * per device d, frequency f, row i and column k, the complex
+ * Y[d][f][i][k] = dI_i / dV_k [A/V; per cm in 2D] as (re, im) at
+ * h_Y[2 * (((d * nfreq + f) * P + i) * P + k)], P the largest port count
+ * of the batch (entries beyond a device's ports are 0).  Each complex
+ * system is solved by FP64 complex BiCGStab with restarted sweeps until
+ * the true residual is below rtol relative to the forcing (maxiter
+ * iterations per sweep).  h_status[d]: TCAD_NEWTON_OK, or
+ * TCAD_NEWTON_LINEAR_FAIL when a solve stalled above 1e-6; h_iters[d]
+ * (may be NULL): Krylov iterations spent.  h_active as for
+ * tcad_solve_bias.  Replaces: ac2d.y_parameters
+ * (pytcad/pytcad/ac2d.py:265-342), ac3d.y_parameters
+ * (pytcad/pytcad/ac3d.py:187-277). */
+int tcad_y_parameters(tcad_batch *b, const int32_t *h_active, int32_t nfreq,
+                      const double *h_freqs, double rtol, int32_t maxiter,
+                      const double *d_psi, const double *d_n, const double *d_p,
+                      double *h_Y, int32_t *h_status, int64_t *h_iters);
+
 /* Terminal currents of every contact of every device at (psi, n, p):
  * h_I[c] = sum over the contact's nodes of the raw F_n + F_p times
  * J0*LD (2D, A/cm) or J0*LD^2 (3D, A).  Replaces:

@@ -9,6 +9,18 @@ port, several frequencies): Y must match bit for bit
 (tests/test_oracle_ac.py); the 3D restatement differs only in the node
 indexing and the area unit (J0 LD^2) and is pinned by z-invariance
 against the pinned 2D one.
+
+y_parameters(..., refine=True) is not a reference method: the same
+systems with the solution refined in extended precision (80-bit long
+double residuals and solution) and the observations summed in extended
+precision.  An ohmic entry at a heavily doped contact cancels by up to
+1e11, and the reference's FP64 direct solve resolves it only to a few
+1e-15 of its terms; the refined Y is the value those entries approach.
+Pinned by the exact identities of the discretisation that the FP64 path
+only meets to its rounding (tests/test_oracle_ac.py): with ohmic ports
+only, raising every contact by the same voltage shifts psi uniformly and
+moves no current, so each row of Y sums to zero at every frequency; and by
+agreement with the bit-pinned FP64 path on entries that do not cancel.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -65,12 +77,29 @@ class YParamResult:
         self.port_names = tuple(port_names)
 
 
-def y_parameters(device, freqs):
+def _refined_solve(A, lu, b, steps):
+    """Iterative refinement with extended-precision residuals; returns
+    the solution in long double."""
+    Acsr = A.tocsr()
+    rows = np.repeat(np.arange(Acsr.shape[0]), np.diff(Acsr.indptr))
+    data = Acsr.data.astype(np.clongdouble)
+    bl = b.astype(np.clongdouble)
+    x = lu.solve(b).astype(np.clongdouble)
+    for _ in range(steps):
+        Ax = np.zeros(Acsr.shape[0], dtype=np.clongdouble)
+        np.add.at(Ax, rows, data * x[Acsr.indices])
+        r = bl - Ax
+        x = x + lu.solve(r.astype(complex)).astype(np.clongdouble)
+    return x
+
+
+def y_parameters(device, freqs, refine=False, refine_steps=4):
     """ac2d.py:265-342 / ac3d.py:187-277: every bc is a port; ohmic
     forcing b[3m] = 1 on the contact's psi rows, gate forcing
     b[3m] = -kappa w; ohmic observation S_i . du_k, gate observation
     j w_s sum kappa w (delta_ik - du_k[3m]); one LU of J + j w_s C per
-    frequency; Y * J0 * LD^(dim-1) / VT."""
+    frequency; Y * J0 * LD^(dim-1) / VT.  refine=True: see the module
+    docstring."""
     if device.psi is None:
         raise RuntimeError("y_parameters needs a converged DC operating point")
     N = device.N
@@ -107,6 +136,26 @@ def y_parameters(device, freqs):
                                                            [node_sets[i] for i in ohmic])):
             S_ohmic[i] = S
     Y = np.empty((len(freqs), P, P), dtype=complex)
+    ldpow = device.LD if device.ndim == 2 else device.LD ** 2
+    if refine:
+        Yl = np.empty((len(freqs), P, P), dtype=np.clongdouble)
+        for kf, f in enumerate(freqs):
+            omega_s = 2.0 * np.pi * f * t0
+            A = (J0c + 1j * omega_s * Cmat).tocsc()
+            lu = splu(A)
+            du = [_refined_solve(A, lu, b_ports[k], refine_steps) for k in range(P)]
+            wl = np.longdouble(omega_s)
+            for k in range(P):
+                for i in range(P):
+                    if kinds[i] == "ohmic":
+                        Yl[kf, i, k] = np.sum(S_ohmic[i].astype(np.longdouble) * du[k])
+                    else:
+                        delta = 1.0 if i == k else 0.0
+                        s = np.sum(weights[i].astype(np.longdouble)
+                                   * (delta - du[k][3 * node_sets[i]]))
+                        Yl[kf, i, k] = 1j * wl * s
+        scale = np.longdouble(device.J0) * np.longdouble(ldpow) / np.longdouble(device.VT)
+        return YParamResult(freqs, (Yl * scale).astype(complex), names)
     for kf, f in enumerate(freqs):
         omega_s = 2.0 * np.pi * f * t0
         lu = splu((J0c + 1j * omega_s * Cmat).tocsc())
@@ -119,7 +168,6 @@ def y_parameters(device, freqs):
                     delta = 1.0 if i == k else 0.0
                     Y[kf, i, k] = 1j * omega_s * np.sum(
                         weights[i] * (delta - du[k][3 * node_sets[i]]))
-    ldpow = device.LD if device.ndim == 2 else device.LD ** 2
     return YParamResult(freqs, Y * device.J0 * ldpow / device.VT, names)
 
 

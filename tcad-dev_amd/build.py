@@ -14,7 +14,7 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "tcadhip", "libtcadhip.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["assemble.hip", "krylov.hip", "batch.hip"]
+SOURCES = ["assemble.hip", "krylov.hip", "batch.hip", "ac.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

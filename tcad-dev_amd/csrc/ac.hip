@@ -1,0 +1,841 @@
+// Small-signal AC admittance: the N-port Y(f) of a converged DC operating
+// point, pytcad/pytcad/ac2d.py:265-342 and ac3d.py:187-277.
+//
+// Every contact and gate is a port.  For port k and frequency f the
+// complex linear system (J + j w_s C) du = b_k is solved, J the Newton
+// Jacobian at the operating point (Dirichlet rows at ohmic contact nodes),
+// C the storage matrix (-dV on electron rows, +dV on hole rows, zero at
+// contact nodes; ac2d.py:107-124), w_s = 2 pi f Ns / R0 (ac2d.py:79-83).
+// An ohmic port drives du_psi = 1 on its nodes; a gate port forces
+// -kappa w on its nodes' potential rows (ac2d.py:296-313).  Row i of Y:
+// an ohmic contact observes the raw continuity rows of its nodes,
+// S_i . du (ac2d.py:168-194); a gate observes its displacement current
+// j w_s sum kappa w (delta_ik - du_psi) (ac2d.py:320-333).  Y is returned
+// as Y J0 LD^(dim-1) / VT (A/V; per cm in 2D).
+//
+// The reference factors J + j w_s C once per frequency (SuperLU) and
+// back-solves every port.  Here the system is the equilibrated stencil
+// matrix of the Newton solver (R J Cc, Cc = (1, n, p) per node) plus the
+// imaginary storage diagonal in the same scaled frame, solved by complex
+// BiCGStab with complex node block-Jacobi preconditioning; FP64
+// throughout.  Restarted sweeps recompute the true residual r = b - A x
+// from the current x (iterative refinement in working precision) until
+// ||r|| <= rtol ||b||, or the residual stops falling.  All devices of a
+// batch are solved together, one port index at a time; reductions are
+// per workgroup partials re-reduced in a fixed order per device, so Y is
+// bit-reproducible and identical for a device alone or in a batch.
+//
+// Roofline: HBM/L2 bound like the real solver.  Algorithmic bytes per node
+// and iteration, 2D: 2 SpMV x (29 x 8 matrix + 48 x + 48 y + 16 storage
+// diagonal) + 2 preconditioner applications x (18 x 8 + 48 + 48) + ~12
+// complex vector passes x 48.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "node_helpers.h"
+
+namespace tcad {
+namespace {
+
+enum { CX = 0, CR, CRH, CP, CV, CS, CT, CPH, CSH, NCV };
+
+struct Cx {
+  double re, im;
+};
+
+__device__ __forceinline__ Cx cmul(Cx a, Cx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+__device__ __forceinline__ Cx cdiv(Cx a, Cx b) {
+  // Smith's algorithm: no overflow for |b| near the double range
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, den = b.re + b.im * r;
+    return {(a.re + a.im * r) / den, (a.im - a.re * r) / den};
+  }
+  const double r = b.re / b.im, den = b.re * r + b.im;
+  return {(a.re * r + a.im) / den, (a.im * r - a.re) / den};
+}
+
+__device__ __forceinline__ Cx csub(Cx a, Cx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ Cx cadd(Cx a, Cx b) { return {a.re + b.re, a.im + b.im}; }
+
+// Per-device scalars of the complex solve, device resident.
+struct ACScal {
+  double rho[2][2];                 // (rhat, r) by iteration parity: re, im
+  double alpha[2], omega[2];        // complex
+  double bb;                        // ||r||^2 at the sweep start (true residual)
+  double b2;                        // ||b||^2
+  double rr;                        // last ||r||^2
+  double tol2;                      // sweep stop: rr <= tol2 * bb
+  double abs2;                      // overall stop: rr <= abs2 * b2
+  int it, maxit, done, status, sweep, pad;
+};
+
+struct CArgs {
+  Where W;
+  long long N;
+  int nblk;
+  const double *A;                  // scaled real Newton matrix, FP64
+  const double *Mi;                 // 18 planes: complex node block inverses (re 0..8, im 9..17)
+  const double *dn, *dp;            // scaled storage diagonal (n and p rows)
+  const double *omega;              // scaled angular frequency per device
+  double *v[NCV];                   // complex vectors: re c*N + g, im (3 + c)*N + g
+  const double *b;                  // scaled right-hand side (real), 3 planes
+  double *part;                     // 8 partial sums per workgroup
+  ACScal *acs;
+  int parity;
+  int first;
+};
+
+#define CPROLOGUE                                   \
+  __shared__ double sh[BS / 64];                    \
+  const int d = dev_index(K.W);                     \
+  const DevInfo &I = dev_info(K.W, d);              \
+  ACScal &S = K.acs[d];                             \
+  if (S.done) return;                               \
+  const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
+
+__device__ __forceinline__ void cload(const double *v, long long N, long long g, double x[6]) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) x[c] = v[c * N + g];
+}
+
+__device__ __forceinline__ void cstore(double *v, long long N, long long g, const double x[6]) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) v[c * N + g] = x[c];
+}
+
+// y = (A + j w diag(0, dn, dp)) x at one node; the real stencil is read
+// once for both halves of x.
+__device__ __forceinline__ void cspmv(const CArgs &K, const NodeHood &H, double w,
+                                      const double *x, double y[6]) {
+  const long long N = K.N, g = H.g;
+  double xr[6];
+  cload(x, N, g, xr);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double a0 = K.A[(3 * r) * N + g], a1 = K.A[(3 * r + 1) * N + g],
+                 a2 = K.A[(3 * r + 2) * N + g];
+    y[r] = a0 * xr[0] + a1 * xr[1] + a2 * xr[2];
+    y[3 + r] = a0 * xr[3] + a1 * xr[4] + a2 * xr[5];
+  }
+#pragma unroll
+  for (int dir = 0; dir < 6; ++dir) {
+    if (!H.nb[dir]) continue;
+    const long long gn = H.nbg[dir];
+    const double *o = K.A + (long long)(9 + 5 * dir) * N + g;
+    const double o0 = o[0], o1 = o[N], o2 = o[2 * N], o3 = o[3 * N], o4 = o[4 * N];
+    const double pr = x[gn], nr = x[N + gn], qr = x[2 * N + gn];
+    const double pi = x[3 * N + gn], ni = x[4 * N + gn], qi = x[5 * N + gn];
+    y[0] += o0 * pr;
+    y[3] += o0 * pi;
+    y[1] += o1 * pr + o2 * nr;
+    y[4] += o1 * pi + o2 * ni;
+    y[2] += o3 * pr + o4 * qr;
+    y[5] += o3 * pi + o4 * qi;
+  }
+  const double wn = w * K.dn[g], wp = w * K.dp[g];
+  y[1] -= wn * xr[4];
+  y[4] += wn * xr[1];
+  y[2] -= wp * xr[5];
+  y[5] += wp * xr[2];
+}
+
+// z = M^-1 x with the node's complex block inverse
+__device__ __forceinline__ void cprec(const CArgs &K, long long g, const double x[6],
+                                      double z[6]) {
+  const long long N = K.N;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double zr = 0.0, zi = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double mr = K.Mi[(3 * r + c) * N + g], mi = K.Mi[(9 + 3 * r + c) * N + g];
+      zr += mr * x[c] - mi * x[3 + c];
+      zi += mr * x[3 + c] + mi * x[c];
+    }
+    z[r] = zr;
+    z[3 + r] = zi;
+  }
+}
+
+// Complex inverse of each node block of A + j w diag(0, dn, dp), by
+// cofactors (the complex counterpart of the node block-Jacobi).
+__global__ void __launch_bounds__(BS) k_ac_prec(CArgs K, double *Mi, int *flag) {
+  const int d = dev_index(K.W);
+  const DevInfo &I = dev_info(K.W, d);
+  const NodeHood H = hood(I, blockIdx.x);
+  if (!H.valid) return;
+  const long long N = K.N, g = H.g;
+  const double w = K.omega[d];
+  Cx a[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) a[e] = {K.A[e * N + g], 0.0};
+  a[4].im = w * K.dn[g];
+  a[8].im = w * K.dp[g];
+  const Cx c0 = csub(cmul(a[4], a[8]), cmul(a[5], a[7]));
+  const Cx c1 = csub(cmul(a[5], a[6]), cmul(a[3], a[8]));
+  const Cx c2 = csub(cmul(a[3], a[7]), cmul(a[4], a[6]));
+  const Cx det = cadd(cadd(cmul(a[0], c0), cmul(a[1], c1)), cmul(a[2], c2));
+  const Cx adj[9] = {c0, csub(cmul(a[2], a[7]), cmul(a[1], a[8])),
+                     csub(cmul(a[1], a[5]), cmul(a[2], a[4])),
+                     c1, csub(cmul(a[0], a[8]), cmul(a[2], a[6])),
+                     csub(cmul(a[2], a[3]), cmul(a[0], a[5])),
+                     c2, csub(cmul(a[1], a[6]), cmul(a[0], a[7])),
+                     csub(cmul(a[0], a[4]), cmul(a[1], a[3]))};
+  const double dm = hypot(det.re, det.im);
+  if (!(dm > 1e-300) || !isfinite(dm)) {
+    atomicOr(flag, 1);
+    atomicOr(flag + 1 + d, 1);
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    const Cx m = cdiv(adj[e], det);
+    Mi[e * N + g] = m.re;
+    Mi[(9 + e) * N + g] = m.im;
+  }
+}
+
+// Port k's forcing on every node of every device: 1 on the nodes of an
+// ohmic port, -kappa w on the nodes of a gate port, 0 elsewhere (and on
+// every node of a device with fewer than k + 1 ports).
+__global__ void __launch_bounds__(BS) k_ac_forcing(Where W, const int *cidx,
+                                                   const int *cn_contact, const int *gidx,
+                                                   const int *gn_gate, const double *gn_kw,
+                                                   int k, double *acg) {
+  const int d = dev_index(W);
+  const DevInfo &I = dev_info(W, d);
+  const NodeHood H = hood(I, blockIdx.x);
+  if (!H.valid) return;
+  const long long g = H.g;
+  double v = 0.0;
+  if (k < I.nc) {
+    const int ct = cidx[g];
+    if (ct >= 0 && cn_contact[ct] == I.c0 + k) v = 1.0;
+  } else if (k < I.nc + I.ng) {
+    const int gt = gidx[g];
+    if (gt >= 0 && gn_gate[gt] == I.g0 + (k - I.nc)) v = -gn_kw[gt];
+  }
+  acg[g] = v;
+}
+
+__global__ void k_ac_reset(ACScal *acs, const int *act, int ndev, double tol2, double abs2,
+                           int maxit, int sweep) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndev) return;
+  ACScal &S = acs[d];
+  S.rho[0][0] = S.rho[1][0] = 1.0;
+  S.rho[0][1] = S.rho[1][1] = 0.0;
+  S.alpha[0] = 1.0;
+  S.alpha[1] = 0.0;
+  S.omega[0] = 1.0;
+  S.omega[1] = 0.0;
+  S.it = 0;
+  S.maxit = maxit;
+  S.tol2 = tol2;
+  S.abs2 = abs2;
+  S.sweep = sweep;
+  if (sweep == 0) S.b2 = 0.0;
+  if (act[d]) {
+    S.status = 0;
+    S.rr = 0.0;
+    S.done = 0;
+  } else {
+    S.done = 1;
+  }
+}
+
+// r = b - A x (x = 0 on the first sweep), rhat = r, p = v = 0; partials
+// (rhat, r) = ||r||^2 and ||r||^2
+__global__ void __launch_bounds__(BS) k_cb_init(CArgs K) {
+  CPROLOGUE
+  (void)lead;
+  const NodeHood H = hood(I, blockIdx.x);
+  const long long N = K.N;
+  double rr = 0.0;
+  if (H.valid) {
+    double r[6];
+    if (K.first) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        r[c] = K.b[c * N + H.g];
+        r[3 + c] = 0.0;
+      }
+      const double z[6] = {0, 0, 0, 0, 0, 0};
+      cstore(K.v[CX], N, H.g, z);
+    } else {
+      double ax[6];
+      cspmv(K, H, K.omega[d], K.v[CX], ax);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        r[c] = K.b[c * N + H.g] - ax[c];
+        r[3 + c] = -ax[3 + c];
+      }
+    }
+    cstore(K.v[CR], N, H.g, r);
+    cstore(K.v[CRH], N, H.g, r);
+    const double z[6] = {0, 0, 0, 0, 0, 0};
+    cstore(K.v[CP], N, H.g, z);
+    cstore(K.v[CV], N, H.g, z);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) rr += r[c] * r[c];
+  }
+  const double s = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    K.part[blockIdx.x] = s;
+    K.part[(long long)K.nblk + blockIdx.x] = 0.0;
+    K.part[2LL * K.nblk + blockIdx.x] = s;
+  }
+}
+
+// reduce rho = (rhat, r), ||r||^2; stop test; p = r + beta (p - omega v);
+// phat = M^-1 p
+__global__ void __launch_bounds__(BS) k_cb_1(CArgs K) {
+  CPROLOGUE
+  const NodeHood H = hood(I, blockIdx.x);
+  const long long N = K.N;
+  double r[6] = {0, 0, 0, 0, 0, 0}, p[6] = {0, 0, 0, 0, 0, 0}, v[6] = {0, 0, 0, 0, 0, 0};
+  if (H.valid) {
+    cload(K.v[CR], N, H.g, r);
+    cload(K.v[CP], N, H.g, p);
+    cload(K.v[CV], N, H.g, v);
+  }
+  const Cx rho = {device_sum(K.part, K.nblk, 0, I, sh), device_sum(K.part, K.nblk, 1, I, sh)};
+  const double rr = device_sum(K.part, K.nblk, 2, I, sh);
+  const bool first_it = S.it == 0;
+  const double bb = first_it ? rr : S.bb;
+  const double b2 = (first_it && S.sweep == 0) ? rr : S.b2;
+  const bool conv = rr <= S.tol2 * bb || rr <= S.abs2 * b2;
+  if (conv || S.it >= S.maxit || !isfinite(rr) || (rho.re == 0.0 && rho.im == 0.0)) {
+    if (lead) {
+      S.rr = rr;
+      if (first_it) {
+        S.bb = bb;
+        S.b2 = b2;
+      }
+      if (!conv) S.status = 1;
+      S.done = 1;
+    }
+    return;
+  }
+  const Cx omega = {S.omega[0], S.omega[1]}, alpha = {S.alpha[0], S.alpha[1]};
+  const Cx rho_old = {S.rho[K.parity ^ 1][0], S.rho[K.parity ^ 1][1]};
+  const Cx beta = cmul(cdiv(rho, rho_old), cdiv(alpha, omega));
+  if (H.valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const Cx pv = {p[c] - (omega.re * v[c] - omega.im * v[3 + c]),
+                     p[3 + c] - (omega.re * v[3 + c] + omega.im * v[c])};
+      const Cx bp = cmul(beta, pv);
+      p[c] = r[c] + bp.re;
+      p[3 + c] = r[3 + c] + bp.im;
+    }
+    cstore(K.v[CP], N, H.g, p);
+    double z[6];
+    cprec(K, H.g, p, z);
+    cstore(K.v[CPH], N, H.g, z);
+  }
+  if (lead) {
+    S.rho[K.parity][0] = rho.re;
+    S.rho[K.parity][1] = rho.im;
+    S.rr = rr;
+    if (first_it) {
+      S.bb = bb;
+      S.b2 = b2;
+    }
+  }
+}
+
+// dst = (A + j w D) src; partial sum conj(a) dst (conj_dst = 0) or
+// conj(dst) a (conj_dst = 1) at q0, q0 + 1, and ||dst||^2 at q0 + 2
+__global__ void __launch_bounds__(BS) k_cb_spmv(CArgs K, int src, int dst, int dot_a,
+                                                int conj_dst, int want_sq, int q0) {
+  CPROLOGUE
+  (void)lead;
+  const NodeHood H = hood(I, blockIdx.x);
+  const long long N = K.N;
+  double sre = 0.0, sim = 0.0, sq = 0.0;
+  if (H.valid) {
+    double y[6], a[6];
+    cspmv(K, H, K.omega[d], K.v[src], y);
+    cstore(K.v[dst], N, H.g, y);
+    cload(K.v[dot_a], N, H.g, a);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      sre += a[c] * y[c] + a[3 + c] * y[3 + c];
+      const double im = a[c] * y[3 + c] - a[3 + c] * y[c];
+      sim += conj_dst ? -im : im;
+      sq += y[c] * y[c] + y[3 + c] * y[3 + c];
+    }
+  }
+  const double a0 = block_sum(sre, sh);
+  const double a1 = block_sum(sim, sh);
+  if (threadIdx.x == 0) {
+    K.part[(long long)q0 * K.nblk + blockIdx.x] = a0;
+    K.part[(long long)(q0 + 1) * K.nblk + blockIdx.x] = a1;
+  }
+  if (want_sq) {
+    const double a2 = block_sum(sq, sh);
+    if (threadIdx.x == 0) K.part[(long long)(q0 + 2) * K.nblk + blockIdx.x] = a2;
+  }
+}
+
+// alpha = rho / (rhat, v); s = r - alpha v; shat = M^-1 s
+__global__ void __launch_bounds__(BS) k_cb_3(CArgs K) {
+  CPROLOGUE
+  const NodeHood H = hood(I, blockIdx.x);
+  const long long N = K.N;
+  double r[6] = {0, 0, 0, 0, 0, 0}, v[6] = {0, 0, 0, 0, 0, 0};
+  if (H.valid) {
+    cload(K.v[CR], N, H.g, r);
+    cload(K.v[CV], N, H.g, v);
+  }
+  const Cx rv = {device_sum(K.part, K.nblk, 3, I, sh), device_sum(K.part, K.nblk, 4, I, sh)};
+  const Cx rho = {S.rho[K.parity][0], S.rho[K.parity][1]};
+  const Cx alpha = cdiv(rho, rv);
+  if ((rv.re == 0.0 && rv.im == 0.0) || !isfinite(alpha.re) || !isfinite(alpha.im)) {
+    if (lead) {
+      S.status = 1;
+      S.done = 1;
+    }
+    return;
+  }
+  if (H.valid) {
+    double s[6], z[6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      s[c] = r[c] - (alpha.re * v[c] - alpha.im * v[3 + c]);
+      s[3 + c] = r[3 + c] - (alpha.re * v[3 + c] + alpha.im * v[c]);
+    }
+    cstore(K.v[CS], N, H.g, s);
+    cprec(K, H.g, s, z);
+    cstore(K.v[CSH], N, H.g, z);
+  }
+  if (lead) {
+    S.alpha[0] = alpha.re;
+    S.alpha[1] = alpha.im;
+  }
+}
+
+// omega = (t, s) / (t, t); x += alpha phat + omega shat; r = s - omega t;
+// partials (rhat, r), ||r||^2
+__global__ void __launch_bounds__(BS) k_cb_5(CArgs K) {
+  CPROLOGUE
+  const NodeHood H = hood(I, blockIdx.x);
+  const long long N = K.N;
+  double x[6] = {0, 0, 0, 0, 0, 0}, ph[6] = {0, 0, 0, 0, 0, 0}, shh[6] = {0, 0, 0, 0, 0, 0},
+         s[6] = {0, 0, 0, 0, 0, 0}, t[6] = {0, 0, 0, 0, 0, 0}, rh[6] = {0, 0, 0, 0, 0, 0};
+  if (H.valid) {
+    cload(K.v[CX], N, H.g, x);
+    cload(K.v[CPH], N, H.g, ph);
+    cload(K.v[CSH], N, H.g, shh);
+    cload(K.v[CS], N, H.g, s);
+    cload(K.v[CT], N, H.g, t);
+    cload(K.v[CRH], N, H.g, rh);
+  }
+  const Cx ts = {device_sum(K.part, K.nblk, 5, I, sh), device_sum(K.part, K.nblk, 6, I, sh)};
+  const double tt = device_sum(K.part, K.nblk, 7, I, sh);
+  const Cx omega = tt > 0.0 ? Cx{ts.re / tt, ts.im / tt} : Cx{0.0, 0.0};
+  const Cx alpha = {S.alpha[0], S.alpha[1]};
+  double srh = 0.0, srhi = 0.0, srr = 0.0;
+  if (H.valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double xr = x[c] + (alpha.re * ph[c] - alpha.im * ph[3 + c]) +
+                        (omega.re * shh[c] - omega.im * shh[3 + c]);
+      const double xi = x[3 + c] + (alpha.re * ph[3 + c] + alpha.im * ph[c]) +
+                        (omega.re * shh[3 + c] + omega.im * shh[c]);
+      x[c] = xr;
+      x[3 + c] = xi;
+      const double rr_ = s[c] - (omega.re * t[c] - omega.im * t[3 + c]);
+      const double ri_ = s[3 + c] - (omega.re * t[3 + c] + omega.im * t[c]);
+      s[c] = rr_;
+      s[3 + c] = ri_;
+      srh += rh[c] * rr_ + rh[3 + c] * ri_;
+      srhi += rh[c] * ri_ - rh[3 + c] * rr_;
+      srr += rr_ * rr_ + ri_ * ri_;
+    }
+    cstore(K.v[CX], N, H.g, x);
+    cstore(K.v[CR], N, H.g, s);
+  }
+  const double a0 = block_sum(srh, sh);
+  const double a1 = block_sum(srhi, sh);
+  const double a2 = block_sum(srr, sh);
+  if (threadIdx.x == 0) {
+    K.part[blockIdx.x] = a0;
+    K.part[(long long)K.nblk + blockIdx.x] = a1;
+    K.part[2LL * K.nblk + blockIdx.x] = a2;
+  }
+  if (lead) {
+    // omega == 0: stagnation; a tiny value keeps the next beta finite and
+    // the next stop test (or the next sweep) decides
+    const bool zero = omega.re == 0.0 && omega.im == 0.0;
+    S.omega[0] = zero ? 1e-300 : omega.re;
+    S.omega[1] = zero ? 0.0 : omega.im;
+    S.it = S.it + 1;
+  }
+}
+
+// Column k of Y for every device: one workgroup per row (contacts, then
+// gates, batch-global ids); y is the scaled solution (du = Cc y).
+struct ObsArgs {
+  const DevInfo *info;
+  const int *row_dev;               // row -> device
+  const int *cptr, *cnodes;         // contact node lists (global nodes)
+  const double *cscale;             // per contact: J0, LD^(dim-1)
+  const int *gptr, *gnode;          // gate table rows and their nodes
+  const double *gn_kw;
+  const double *raw;                // 21 planes: raw contact current rows
+  const double *y;                  // complex, 6 planes
+  const double *n, *p;
+  const double *omega;              // per device
+  long long N;
+  int ncontact;
+  int k;
+  double *ycol;                     // 2 per row: re, im
+};
+
+__global__ void __launch_bounds__(BS) k_ac_observe(ObsArgs O) {
+  __shared__ double sh[BS / 64];
+  const int row = blockIdx.x;
+  const int d = O.row_dev[row];
+  const DevInfo &I = O.info[d];
+  const long long N = O.N;
+  double sre = 0.0, sim = 0.0;
+  if (row < O.ncontact) {
+    const long long sy = I.nx, sz = (long long)I.nx * I.ny;
+    for (int t = O.cptr[row] + threadIdx.x; t < O.cptr[row + 1]; t += blockDim.x) {
+      const long long g = O.cnodes[t];
+      const long long kl = g - I.off;
+      const int i = (int)(kl % I.nx), j = (int)((kl / I.nx) % I.ny), kz = (int)(kl / sz);
+      const bool nb[6] = {i > 0, i < I.nx - 1, j > 0, j < I.ny - 1, I.dim == 3 && kz > 0,
+                          I.dim == 3 && kz < I.nz - 1};
+      const long long gn[6] = {g - 1, g + 1, g - sy, g + sy, g - sz, g + sz};
+      // du at the node: (y_psi, n y_n, p y_p)
+      const double nn = O.n[g], pp = O.p[g];
+      double re = O.raw[0 * N + g] * O.y[g] + O.raw[1 * N + g] * (nn * O.y[N + g]) +
+                  O.raw[2 * N + g] * (pp * O.y[2 * N + g]);
+      double im = O.raw[0 * N + g] * O.y[3 * N + g] +
+                  O.raw[1 * N + g] * (nn * O.y[4 * N + g]) +
+                  O.raw[2 * N + g] * (pp * O.y[5 * N + g]);
+#pragma unroll
+      for (int dir = 0; dir < 6; ++dir) {
+        if (!nb[dir]) continue;
+        const long long q = gn[dir];
+        const double c0 = O.raw[(3 + 3 * dir) * N + g], c1 = O.raw[(4 + 3 * dir) * N + g],
+                     c2 = O.raw[(5 + 3 * dir) * N + g];
+        const double nq = O.n[q], pq = O.p[q];
+        re += c0 * O.y[q] + c1 * (nq * O.y[N + q]) + c2 * (pq * O.y[2 * N + q]);
+        im += c0 * O.y[3 * N + q] + c1 * (nq * O.y[4 * N + q]) + c2 * (pq * O.y[5 * N + q]);
+      }
+      sre += re;
+      sim += im;
+    }
+    const double a = block_sum(sre, sh);
+    const double b = block_sum(sim, sh);
+    if (threadIdx.x == 0) {
+      const double J0 = O.cscale[2 * row], ld = O.cscale[2 * row + 1];
+      O.ycol[2 * row] = a * J0 * ld / I.VT;
+      O.ycol[2 * row + 1] = b * J0 * ld / I.VT;
+    }
+  } else {
+    const int gate = row - O.ncontact;
+    const bool self = O.k >= I.nc && O.k < I.nc + I.ng && gate == I.g0 + (O.k - I.nc);
+    const double delta = self ? 1.0 : 0.0;
+    for (int t = O.gptr[gate] + threadIdx.x; t < O.gptr[gate + 1]; t += blockDim.x) {
+      const long long g = O.gnode[t];
+      const double kw = O.gn_kw[t];
+      sre += kw * (delta - O.y[g]);
+      sim += kw * (0.0 - O.y[3 * N + g]);
+    }
+    const double a = block_sum(sre, sh);
+    const double b = block_sum(sim, sh);
+    if (threadIdx.x == 0) {
+      // j w_s (a + j b)
+      const double w = O.omega[d];
+      const double ld = I.dim == 3 ? I.LD * I.LD : I.LD;
+      O.ycol[2 * row] = (-w * b) * I.J0 * ld / I.VT;
+      O.ycol[2 * row + 1] = (w * a) * I.J0 * ld / I.VT;
+    }
+  }
+}
+
+// Device workspace of one tcad_y_parameters call.
+struct AcWork {
+  double *base = nullptr;
+  double *v[NCV] = {};
+  double *Mi = nullptr, *dn = nullptr, *dp = nullptr, *raw = nullptr, *acg = nullptr;
+  double *part = nullptr, *omega = nullptr, *ycol = nullptr;
+  ACScal *acs = nullptr;
+  int *act = nullptr, *row_dev = nullptr;
+  ~AcWork() {
+    if (base) (void)hipFree(base);
+  }
+};
+
+int alloc_work(Batch &b, int nrows, AcWork &w) {
+  const size_t N = (size_t)b.ntot;
+  const size_t nd = NCV * 6 * N + 18 * N + 2 * N + 21 * N + N + 8 * (size_t)b.nblk + b.ndev +
+                    2 * (size_t)nrows;
+  const size_t bytes = nd * sizeof(double) + sizeof(ACScal) * b.ndev +
+                       sizeof(int) * ((size_t)b.ndev + nrows) + 256;
+  TCAD_HIP_CHECK(hipMalloc((void **)&w.base, bytes));
+  double *q = w.base;
+  for (int i = 0; i < NCV; ++i, q += 6 * N) w.v[i] = q;
+  w.Mi = q; q += 18 * N;
+  w.dn = q; q += N;
+  w.dp = q; q += N;
+  w.raw = q; q += 21 * N;
+  w.acg = q; q += N;
+  w.part = q; q += 8 * (size_t)b.nblk;
+  w.omega = q; q += b.ndev;
+  w.ycol = q; q += 2 * (size_t)nrows;
+  w.acs = reinterpret_cast<ACScal *>(q);
+  int *ip = reinterpret_cast<int *>(w.acs + b.ndev);
+  w.act = ip;
+  w.row_dev = ip + b.ndev;
+  return TCAD_OK;
+}
+
+CArgs make_cargs(Batch &b, const AcWork &w) {
+  CArgs K{};
+  K.W = make_where(b);
+  K.N = b.ntot;
+  K.nblk = b.nblk;
+  K.A = b.A64;
+  K.Mi = w.Mi;
+  K.dn = w.dn;
+  K.dp = w.dp;
+  K.omega = w.omega;
+  for (int i = 0; i < NCV; ++i) K.v[i] = w.v[i];
+  K.b = b.bs;
+  K.part = w.part;
+  K.acs = w.acs;
+  return K;
+}
+
+void launch_citeration(Batch &b, CArgs K, int parity) {
+  const dim3 g(b.nblk), t(BS);
+  K.parity = parity;
+  hipLaunchKernelGGL(k_cb_1, g, t, 0, b.stream, K);
+  hipLaunchKernelGGL(k_cb_spmv, g, t, 0, b.stream, K, (int)CPH, (int)CV, (int)CRH, 0, 0, 3);
+  hipLaunchKernelGGL(k_cb_3, g, t, 0, b.stream, K);
+  hipLaunchKernelGGL(k_cb_spmv, g, t, 0, b.stream, K, (int)CSH, (int)CT, (int)CS, 1, 1, 5);
+  hipLaunchKernelGGL(k_cb_5, g, t, 0, b.stream, K);
+  b.launches += 5;
+}
+
+// Refinement sweeps of the complex solve of the current right-hand side
+// (b.bs) at the current frequency, for the devices in act.  Per device:
+// worst outcome so far in status (0 ok, TCAD_NEWTON_LINEAR_FAIL), Krylov
+// iterations added to iters.
+constexpr int kMaxSweeps = 12;
+constexpr double kSweepRtol = 1e-8;     // residual reduction per sweep
+constexpr double kAcceptRel = 1e-6;     // stalled above this: a failed solve
+
+int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int maxit,
+                  std::vector<int> &status, std::vector<long long> &iters) {
+  CArgs K = make_cargs(b, w);
+  std::vector<ACScal> hs(b.ndev);
+  std::vector<double> prev(b.ndev, INFINITY);
+  std::vector<int> conv(b.ndev, 0), stalled_ok(b.ndev, 0);
+  const std::vector<int> act0 = act;
+  for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
+    int open = 0;
+    for (int d = 0; d < b.ndev; ++d) open += act[d];
+    if (!open) break;
+    TCAD_HIP_CHECK(hipMemcpyAsync(w.act, act.data(), sizeof(int) * b.ndev,
+                                  hipMemcpyHostToDevice, b.stream));
+    hipLaunchKernelGGL(k_ac_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, w.acs,
+                       (const int *)w.act, b.ndev, kSweepRtol * kSweepRtol, rtol * rtol, maxit,
+                       sweep);
+    K.first = sweep == 0 ? 1 : 0;
+    hipLaunchKernelGGL(k_cb_init, dim3(b.nblk), dim3(BS), 0, b.stream, K);
+    b.launches += 2;
+    int it = 0, chunk = 8;
+    while (true) {
+      for (int c = 0; c < chunk && it < maxit; ++c, ++it) launch_citeration(b, K, it & 1);
+      if (it >= maxit) {
+        CArgs K1 = K;
+        K1.parity = it & 1;
+        hipLaunchKernelGGL(k_cb_1, dim3(b.nblk), dim3(BS), 0, b.stream, K1);
+        b.launches++;
+      }
+      TCAD_HIP_CHECK(hipGetLastError());
+      TCAD_HIP_CHECK(hipMemcpyAsync(hs.data(), w.acs, sizeof(ACScal) * b.ndev,
+                                    hipMemcpyDeviceToHost, b.stream));
+      TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+      int left = 0;
+      for (int d = 0; d < b.ndev; ++d) left += hs[d].done ? 0 : 1;
+      if (left == 0 || it >= maxit) break;
+      if (chunk < 64) chunk *= 2;
+    }
+    for (int d = 0; d < b.ndev; ++d) {
+      if (!act[d]) continue;
+      const ACScal &S = hs[d];
+      iters[d] += S.it;
+      // S.bb: true residual at this sweep's start; converged there if the
+      // sweep stopped before its first iteration
+      const bool done_at_start = S.it == 0 && S.bb <= rtol * rtol * S.b2;
+      if (done_at_start || !(S.b2 > 0.0)) {
+        conv[d] = 1;
+        act[d] = 0;
+      } else if (!std::isfinite(S.bb) || (sweep > 0 && S.bb > 0.25 * prev[d])) {
+        stalled_ok[d] = std::isfinite(S.bb) && S.bb <= kAcceptRel * kAcceptRel * S.b2;
+        act[d] = 0;
+      }
+      prev[d] = S.bb;
+    }
+  }
+  for (int d = 0; d < b.ndev; ++d) {
+    if (!act0[d]) continue;
+    bool ok = conv[d] || stalled_ok[d];
+    if (!conv[d] && !stalled_ok[d] && act[d])
+      ok = std::isfinite(prev[d]) && prev[d] <= kAcceptRel * kAcceptRel * hs[d].b2;
+    if (!ok) status[d] = TCAD_NEWTON_LINEAR_FAIL;
+  }
+  return TCAD_OK;
+}
+
+int y_parameters_impl(Batch &b, const int32_t *h_active, int nfreq, const double *h_freqs,
+                      double rtol, int maxit, const double *psi, const double *n,
+                      const double *p, double *h_Y, int32_t *h_status, int64_t *h_iters) {
+  const int nrows = b.ncontact + b.ngate;
+  int P = 0;
+  for (const DevInfo &I : b.info) P = std::max(P, I.nc + I.ng);
+  std::vector<int> act(b.ndev, 1);
+  if (h_active)
+    for (int d = 0; d < b.ndev; ++d) act[d] = h_active[d] ? 1 : 0;
+  std::vector<int> status(b.ndev, TCAD_NEWTON_OK);
+  std::vector<long long> iters(b.ndev, 0);
+  std::memset(h_Y, 0, sizeof(double) * 2 * (size_t)b.ndev * nfreq * P * P);
+  if (nrows == 0 || P == 0) {
+    for (int d = 0; d < b.ndev; ++d) {
+      h_status[d] = TCAD_NEWTON_OK;
+      if (h_iters) h_iters[d] = 0;
+    }
+    return TCAD_OK;
+  }
+  AcWork w;
+  int rc = alloc_work(b, nrows, w);
+  if (rc) return rc;
+  std::vector<int> row_dev(nrows, 0);
+  for (int d = 0; d < b.ndev; ++d) {
+    const DevInfo &I = b.info[d];
+    for (int c = 0; c < I.nc; ++c) row_dev[I.c0 + c] = d;
+    for (int g = 0; g < I.ng; ++g) row_dev[b.ncontact + I.g0 + g] = d;
+  }
+  TCAD_HIP_CHECK(hipMemcpyAsync(w.row_dev, row_dev.data(), sizeof(int) * nrows,
+                                hipMemcpyHostToDevice, b.stream));
+  TCAD_HIP_CHECK(hipMemsetAsync(w.raw, 0, sizeof(double) * 21 * (size_t)b.ntot, b.stream));
+  if ((rc = launch_set_active(b, h_active))) return rc;
+
+  // operating-point matrix, storage diagonal and raw contact rows
+  AcAssembly prep{};
+  prep.mode = 1;
+  prep.dn = w.dn;
+  prep.dp = w.dp;
+  prep.raw = w.raw;
+  if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, false, true, 0, nullptr, &prep)))
+    return rc;
+  if ((rc = check_flag(b))) return rc;
+
+  ObsArgs O{};
+  O.info = b.d_info;
+  O.row_dev = w.row_dev;
+  O.cptr = b.d_cptr;
+  O.cnodes = b.d_cnodes;
+  O.cscale = b.d_cscale;
+  O.gptr = b.d_gptr;
+  O.gnode = b.d_gnode;
+  O.gn_kw = b.d_gn_kw;
+  O.raw = w.raw;
+  O.y = w.v[CX];
+  O.n = n;
+  O.p = p;
+  O.omega = w.omega;
+  O.N = b.ntot;
+  O.ncontact = b.ncontact;
+  O.ycol = w.ycol;
+  std::vector<double> ycol(2 * (size_t)nrows), om(b.ndev);
+  CArgs K = make_cargs(b, w);
+  for (int k = 0; k < P; ++k) {
+    hipLaunchKernelGGL(k_ac_forcing, dim3(b.nblk), dim3(BS), 0, b.stream, K.W,
+                       (const int *)b.d_cidx, (const int *)b.d_cn_contact, (const int *)b.d_gidx,
+                       (const int *)b.d_gn_gate, (const double *)b.d_gn_kw, k, w.acg);
+    b.launches++;
+    AcAssembly rhs{};
+    rhs.mode = 2;
+    rhs.acg = w.acg;
+    if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, false, true, 0, nullptr, &rhs)))
+      return rc;
+    std::vector<int> act_k(b.ndev);
+    for (int d = 0; d < b.ndev; ++d)
+      act_k[d] = act[d] && k < b.info[d].nc + b.info[d].ng ? 1 : 0;
+    for (int f = 0; f < nfreq; ++f) {
+      for (int d = 0; d < b.ndev; ++d)
+        om[d] = 2.0 * M_PI * h_freqs[f] * (b.info[d].Ns / b.info[d].R0);
+      TCAD_HIP_CHECK(hipMemcpyAsync(w.omega, om.data(), sizeof(double) * b.ndev,
+                                    hipMemcpyHostToDevice, b.stream));
+      hipLaunchKernelGGL(k_ac_prec, dim3(b.nblk), dim3(BS), 0, b.stream, K, w.Mi, b.d_flag);
+      b.launches++;
+      TCAD_HIP_CHECK(hipGetLastError());
+      if ((rc = check_flag(b))) return rc;
+      if ((rc = complex_solve(b, w, act_k, rtol, maxit, status, iters))) return rc;
+      O.k = k;
+      hipLaunchKernelGGL(k_ac_observe, dim3(nrows), dim3(BS), 0, b.stream, O);
+      b.launches++;
+      TCAD_HIP_CHECK(hipGetLastError());
+      TCAD_HIP_CHECK(hipMemcpyAsync(ycol.data(), w.ycol, sizeof(double) * 2 * nrows,
+                                    hipMemcpyDeviceToHost, b.stream));
+      TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+      for (int d = 0; d < b.ndev; ++d) {
+        if (!act_k[d]) continue;
+        const DevInfo &I = b.info[d];
+        double *Yd = h_Y + 2 * (((size_t)d * nfreq + f) * P * P);
+        for (int i = 0; i < I.nc + I.ng; ++i) {
+          const int row = i < I.nc ? I.c0 + i : b.ncontact + I.g0 + (i - I.nc);
+          Yd[2 * ((size_t)i * P + k)] = ycol[2 * row];
+          Yd[2 * ((size_t)i * P + k) + 1] = ycol[2 * row + 1];
+        }
+      }
+    }
+  }
+  for (int d = 0; d < b.ndev; ++d) {
+    h_status[d] = status[d];
+    if (h_iters) h_iters[d] = iters[d];
+  }
+  return TCAD_OK;
+}
+
+}  // namespace
+}  // namespace tcad
+
+extern "C" {
+
+int tcad_y_parameters(tcad_batch *bp, const int32_t *h_active, int32_t nfreq,
+                      const double *h_freqs, double rtol, int32_t maxiter, const double *psi,
+                      const double *n, const double *p, double *h_Y, int32_t *h_status,
+                      int64_t *h_iters) {
+  GUARD(bp);
+  tcad::Batch &b = *bp->impl;
+  if (!psi || !n || !p || !h_Y || !h_status || (nfreq > 0 && !h_freqs)) {
+    tcad::set_error("tcad_y_parameters: null argument");
+    return TCAD_E_INVALID;
+  }
+  if (nfreq < 0 || maxiter <= 0 || !(rtol > 0.0) || !(rtol < 1.0)) {
+    tcad::set_error("tcad_y_parameters: need nfreq >= 0, maxiter > 0 and 0 < rtol < 1");
+    return TCAD_E_INVALID;
+  }
+  for (int f = 0; f < nfreq; ++f)
+    if (!(h_freqs[f] >= 0.0) || !std::isfinite(h_freqs[f])) {
+      tcad::set_error("tcad_y_parameters: frequencies must be finite and >= 0");
+      return TCAD_E_INVALID;
+    }
+  return tcad::y_parameters_impl(b, h_active, nfreq, h_freqs, rtol, maxiter, psi, n, p, h_Y,
+                                 h_status, h_iters);
+}
+
+}  // extern "C"

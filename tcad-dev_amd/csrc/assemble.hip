@@ -45,6 +45,14 @@ struct AsmArgs {
   const double *n_old, *p_old, *Fn_old, *Fp_old;
   const double *dt_s;               // per device, scaled time step
   double theta;
+  // small-signal AC (ac2d.py:265-342): 1 = prepare (the matrix as usual,
+  // plus the scaled storage diagonal and the raw current rows of the
+  // contact nodes); 2 = only the scaled right-hand side of a port's
+  // forcing acg (row scales from the prepare call)
+  int ac;
+  const double *acg;
+  double *ac_dn, *ac_dp;            // R * (-dV) * n, R * dV * p (0 at contacts)
+  double *ac_raw;                   // 21 planes: raw (n + p)-row sums, node then directions
   double *F, *Fnp, *A64, *Dinv64, *bs, *rowscale;
   float *A32, *Dinv32;
   int *flag;                        // [0] any, [1 + d] per device
@@ -231,9 +239,47 @@ __device__ void assemble_node(const AsmArgs &A, const DevInfo &I, long long k, i
     D[0] = D[0] + -kw;
   }
 
+  const int ct = A.cidx[g];
+  if (A.ac >= 2) {
+    // AC forcing b (ac2d.py:296-313) on the psi rows: an ohmic port's 1 on
+    // its nodes is du_psi there (Dirichlet rows; the interior sees
+    // -J[:, contact] b after elimination), a gate port's -kappa w forces
+    // its nodes' potential rows
+    double r3[3] = {0.0, 0.0, 0.0};
+    r3[0] = A.acg[g];
+    if (ct < 0) {
+#pragma unroll
+      for (int dir = 0; dir < 6; ++dir) {
+        if (dir / 2 >= DIM || !has_nb[dir]) continue;
+        const long long gn = g + ((dir & 1) ? str[dir / 2] : -str[dir / 2]);
+        if (A.cidx[gn] < 0) continue;
+        const double gv = A.acg[gn];
+        r3[0] = r3[0] - off[dir][0] * gv;
+        r3[1] = r3[1] - off[dir][1] * gv;
+        r3[2] = r3[2] - off[dir][3] * gv;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) A.bs[r * N + g] = r3[r] * A.rowscale[r * N + g];
+    return;
+  }
+  if (A.ac == 1 && ct >= 0) {
+    // d(F_n + F_p)/du of a contact node, before its rows become Dirichlet
+    // rows (device2d.py:1751-1758): node columns, then per direction
+    A.ac_raw[0 * N + g] = D[3] + D[6];
+    A.ac_raw[1 * N + g] = D[4] + D[7];
+    A.ac_raw[2 * N + g] = D[5] + D[8];
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      const bool on = dir / 2 < DIM && has_nb[dir];
+      A.ac_raw[(3 + 3 * dir) * N + g] = on ? off[dir][1] + off[dir][3] : 0.0;
+      A.ac_raw[(4 + 3 * dir) * N + g] = on ? off[dir][2] : 0.0;
+      A.ac_raw[(5 + 3 * dir) * N + g] = on ? off[dir][4] : 0.0;
+    }
+  }
+
   double F3[3] = {Fpsi, Fn, Fp};
   double rhs[3];
-  const int ct = A.cidx[g];
   if (A.Fnp) A.Fnp[g] = Fn + Fp;      // raw, pre-Dirichlet (terminal current)
   if (A.Fn_raw) {
     A.Fn_raw[g] = Fn;
@@ -387,6 +433,11 @@ __device__ void assemble_node(const AsmArgs &A, const DevInfo &I, long long k, i
     A.bs[r * N + g] = rhs[r] * rs[r];
     A.rowscale[r * N + g] = rs[r];
   }
+  if (A.ac == 1) {
+    // storage term of the AC system (ac2d.py:107-124) in the scaled frame
+    A.ac_dn[g] = ct < 0 ? rs[1] * (-dV) * nn : 0.0;
+    A.ac_dp[g] = ct < 0 ? rs[2] * dV * pp : 0.0;
+  }
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
     A.A64[e * N + g] = Ds[e];
@@ -443,8 +494,15 @@ static AsmArgs make_args(Batch &b, const double *psi, const double *n, const dou
 
 int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
                     double *F, double *Fnp, bool want_fp32, bool only_active, int res_only,
-                    const TransientArgs *tr) {
+                    const TransientArgs *tr, const AcAssembly *ac) {
   AsmArgs A = make_args(b, psi, n, p);
+  if (ac) {
+    A.ac = ac->mode;
+    A.acg = ac->acg;
+    A.ac_dn = ac->dn;
+    A.ac_dp = ac->dp;
+    A.ac_raw = ac->raw;
+  }
   A.scal = only_active ? b.d_scal : nullptr;
   A.res_only = res_only;
   if (tr) {
@@ -462,7 +520,7 @@ int launch_assemble(Batch &b, const double *psi, const double *n, const double *
   A.F = F; A.Fnp = Fnp;
   A.A64 = b.A64; A.A32 = b.A32; A.Dinv64 = b.Dinv64; A.Dinv32 = b.Dinv32;
   A.bs = b.bs; A.rowscale = b.rowscale;
-  if (!res_only) {
+  if (!res_only && !(ac && ac->mode >= 2)) {
     b.last_n = n;
     b.last_p = p;
     b.last_psi = psi;

@@ -518,7 +518,7 @@ static int launch_flags_to_status(Batch &b) {
   return TCAD_OK;
 }
 
-static int check_flag(Batch &b) {
+int check_flag(Batch &b) {
   int flag = 0;
   TCAD_HIP_CHECK(hipMemcpyAsync(&flag, b.d_flag, sizeof(int), hipMemcpyDeviceToHost, b.stream));
   TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
@@ -761,7 +761,7 @@ static void free_batch(Batch *b) {
                   b->d_dv[1], b->d_dv[2], b->d_cidx, b->d_gidx, b->d_cn_lnr, b->d_cn_n0,
                   b->d_cn_p0, b->d_cn_contact, b->d_gn_kw, b->d_gn_psib, b->d_gn_gate,
                   b->d_contact_V, b->d_gate_Vg, b->d_gate_Vfb, b->d_cptr, b->d_cnodes,
-                  b->d_cscale, b->d_c2dev, b->d_surf[0], b->d_surf[1], b->d_surf[2],
+                  b->d_cscale, b->d_c2dev, b->d_gptr, b->d_gnode, b->d_surf[0], b->d_surf[1], b->d_surf[2],
                   b->d_surf[3], b->A64, b->A32, b->Dinv64, b->Dinv32, b->bs, b->rowscale,
                   b->colscale, b->y64, b->F, b->Fnp, b->part, b->d_scal, b->Aeq, b->beq,
                   b->d_flag, b->d_mask, b->cand, b->d_Icand, b->T4, b->d_dts};
@@ -798,7 +798,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   b->f = *fields;
   std::vector<double> hx[3], dvw[3];
   std::vector<int> blk2dev;
-  std::vector<int> cidx, gidx, cn_contact, gn_gate, cptr{0}, cnodes;
+  std::vector<int> cidx, gidx, cn_contact, gn_gate, cptr{0}, cnodes, gptr{0}, gate_node;
   std::vector<double> cn_lnr, cn_n0, cn_p0, gn_kw, gn_psib, cscale;
   std::vector<double> surf[4];
   long long off = 0, end_max = 0;
@@ -920,7 +920,9 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
         gn_kw.push_back(D.h_gate_kw[t]);
         gn_psib.push_back(D.h_gate_psib[t]);
         gn_gate.push_back(gglobal + gg);
+        gate_node.push_back((int)gnode);
       }
+      gptr.push_back((int)gn_kw.size());
     }
     gglobal += D.ngates;
     b->dim_max = std::max(b->dim_max, D.dim);
@@ -966,6 +968,8 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   TRY(upload(&b->d_gn_gate, gn_gate, s));
   TRY(upload(&b->d_cptr, cptr, s));
   TRY(upload(&b->d_cnodes, cnodes, s));
+  TRY(upload(&b->d_gptr, gptr, s));
+  TRY(upload(&b->d_gnode, gate_node.empty() ? std::vector<int>{0} : gate_node, s));
   TRY(upload(&b->d_cscale, cscale, s));
   for (int i = 0; i < 4; ++i) TRY(upload(&b->d_surf[i], surf[i], s));
   TRY(upload(&b->d_contact_V, std::vector<double>(std::max(1, cglobal), 0.0), s));
@@ -1015,16 +1019,6 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
 // extern "C" boundary
 // ---------------------------------------------------------------------------
 using tcad::Batch;
-
-struct tcad_batch {
-  Batch *impl;
-};
-
-#define GUARD(bp)                                 \
-  if (!(bp) || !(bp)->impl) {                     \
-    tcad::set_error("null tcad_batch handle");    \
-    return TCAD_E_INVALID;                        \
-  }
 
 extern "C" {
 

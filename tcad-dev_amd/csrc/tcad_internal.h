@@ -135,6 +135,8 @@ struct Batch {
   double *d_contact_V = nullptr, *d_gate_Vg = nullptr, *d_gate_Vfb = nullptr;
   int *d_cptr = nullptr;                         // contact node lists (global nodes)
   int *d_cnodes = nullptr;
+  int *d_gptr = nullptr;                         // gate g: table rows gptr[g]..gptr[g+1]
+  int *d_gnode = nullptr;                        // gate table row -> global node
   double *d_cscale = nullptr;                    // per contact J0*LD^(dim-1)
   int *d_c2dev = nullptr;
   double *d_surf[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -196,12 +198,26 @@ struct TransientArgs {
   double *Fn_raw, *Fp_raw;          // if set: raw F_n, F_p per node
 };
 
+// Small-signal AC terms of the assembly (ac2d.py:107-194, 296-313).
+// mode 1: assemble the matrix as usual and also write the scaled storage
+//         diagonal (dn, dp) and the raw contact current rows (raw, 21
+//         planes); mode 2: write only the scaled right-hand side R b of
+//         the port forcing acg (du_psi = acg on contact nodes, psi-row
+//         forcing acg elsewhere), using the row scales of mode 1.
+struct AcAssembly {
+  int mode;
+  const double *acg;
+  double *dn, *dp;
+  double *raw;
+};
+
 // only_active: skip devices whose Newton solve is not running (Scal.active);
 // res_only: 1 = write only the raw F_n + F_p per node (Fnp); 2 = only F
 // (all rows, boundary and transient terms included), no matrix
 int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
                     double *F, double *Fnp, bool want_fp32, bool only_active = false,
-                    int res_only = 0, const TransientArgs *tr = nullptr);
+                    int res_only = 0, const TransientArgs *tr = nullptr,
+                    const AcAssembly *ac = nullptr);
 int launch_surface_mobility(Batch &b, const double *psi, bool only_active = false);
 int launch_assemble_poisson(Batch &b, const double *psi, double *F);
 int launch_edge_currents(Batch &b, const double *psi, const double *n, const double *p,
@@ -238,5 +254,18 @@ int launch_slaved_densities(Batch &b, const double *psi, double *n, double *p);
 // the forcing term of each device's first Newton solve
 int launch_set_active(Batch &b, const int32_t *h_mask, double eta0 = 0.0);
 int sync_scalars(Batch &b);
+// TCAD_E_NONFINITE (and clears the flags) if an assembly flagged a
+// non-finite or singular node block
+int check_flag(Batch &b);
 
 }  // namespace tcad
+
+struct tcad_batch {
+  tcad::Batch *impl;
+};
+
+#define GUARD(bp)                                 \
+  if (!(bp) || !(bp)->impl) {                     \
+    tcad::set_error("null tcad_batch handle");    \
+    return TCAD_E_INVALID;                        \
+  }

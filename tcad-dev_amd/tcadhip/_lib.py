@@ -109,6 +109,9 @@ def lib():
                                  c_double_p, ctypes.c_double, vp, vp, vp, vp, vp, vp, vp,
                                  c_int32_p, c_int32_p, c_int32_p], ctypes.c_int),
         "tcad_continuity_residuals": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
+        "tcad_y_parameters": ([vp, c_int32_p, ctypes.c_int32, c_double_p, ctypes.c_double,
+                               ctypes.c_int32, vp, vp, vp, c_double_p, c_int32_p,
+                               ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "tcad_terminal_currents": ([vp, vp, vp, vp, c_double_p], ctypes.c_int),
         "tcad_edge_currents": ([vp, vp, vp, vp, vp], ctypes.c_int),
         "tcad_surface_mobility": ([vp, vp], ctypes.c_int),
@@ -160,5 +163,5 @@ def exported_symbols():
             "tcad_assemble", "tcad_export_jacobian", "tcad_solve_linear",
             "tcad_solve_equilibrium", "tcad_solve_bias", "tcad_terminal_currents",
             "tcad_edge_currents", "tcad_surface_mobility", "tcad_time_kernel",
-            "tcad_transient_step", "tcad_continuity_residuals",
+            "tcad_transient_step", "tcad_continuity_residuals", "tcad_y_parameters",
             "tcad_launch_count", "tcad_last_error", "tcad_version"]

@@ -396,6 +396,25 @@ class GPUBatch:
             _lib.dptr(out)), "tcad_terminal_currents")
         return out[:len(self.contacts)]
 
+    def y_parameters(self, freqs, rtol, maxiter, active=None):
+        """Small-signal Y(f) of the selected devices at their current
+        state and bias: (Y [ndev, nfreq, P, P] complex in each device's
+        port order -- its contacts, then its gates -- with P the largest
+        port count, per-device outcome codes, Krylov iterations)."""
+        f = np.ascontiguousarray(freqs, dtype=np.float64)
+        P = max(len(s.contacts()) + len(s.gates()) for s in self.setups)
+        Y = np.zeros((self.ndev, len(f), P, P, 2), dtype=np.float64)
+        status = np.zeros(self.ndev, dtype=np.int32)
+        iters = np.zeros(self.ndev, dtype=np.int64)
+        m = self._mask(active)
+        self.set_bias()
+        _lib.check(_lib.lib().tcad_y_parameters(
+            self._handle, _lib.iptr(m), len(f), _lib.dptr(f), float(rtol), int(maxiter),
+            self.psi.data_ptr(), self.n.data_ptr(), self.p.data_ptr(), _lib.dptr(Y),
+            _lib.iptr(status), iters.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
+            "tcad_y_parameters")
+        return Y[..., 0] + 1j * Y[..., 1], status, iters
+
     def device_slice(self, d):
         o = int(self.offsets[d])
         return slice(o, o + self.setups[d].N)

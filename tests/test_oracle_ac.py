@@ -80,3 +80,27 @@ def test_ac_3d_z_invariant_equals_2d():
     Y2 = oac.y_parameters(d2, f).Y
     Y3 = oac.y_parameters(d3, f).Y
     assert np.allclose(Y3, Y2 * z[-1], rtol=1e-6, atol=1e-9 * np.abs(Y2).max() * z[-1])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_refined_oracle_pinned(name):
+    """The extended-precision Y (refine=True) is within the reference's own
+    rounding of the fixture, and -- for the ohmic-only diodes, where a
+    common voltage on every contact moves no current -- meets the row-sum
+    identity sum_k Y[i, k] = 0 to ~1e-17 of the rows' rounding floor,
+    which the FP64 direct solve (the reference's) misses by up to 50x."""
+    from ac_floor import observation_floor
+    spec, d, dev = dc_device(name)
+    freqs = d["freqs"]
+    Yr = oac.y_parameters(dev, freqs, refine=True).Y
+    Yfix = d["Y_re"] + 1j * d["Y_im"]
+    floor = observation_floor(dev, freqs)
+    for f in range(len(freqs)):
+        tol = 1e-6 * np.abs(Yfix[f]).max() + 1e-12 * floor[f]
+        assert np.all(np.abs(Yr[f] - Yfix[f]) <= tol), f
+    if "gate" in [str(p) for p in d["port_names"]]:
+        return
+    rowfloor = floor.sum(axis=2)
+    assert np.all(np.abs(Yr.sum(axis=2)) <= 3e-17 * rowfloor)
+    if name == "ac_diode_rev.npz":
+        assert np.any(np.abs(Yfix.sum(axis=2)) > 3e-17 * rowfloor)
