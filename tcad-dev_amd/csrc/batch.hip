@@ -926,6 +926,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
     }
     gglobal += D.ngates;
     b->dim_max = std::max(b->dim_max, D.dim);
+    if (D.dim == 2) b->any2d = 1;
     off += I.n;
     blk += I.nblk;
   }

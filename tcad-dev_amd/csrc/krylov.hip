@@ -35,6 +35,9 @@
 // Roofline: HBM/L2 bound.  Algorithmic bytes per node and BiCGStab
 // iteration, 2D (29 entries), matrix width w (8 or 4): 2 SpMV x
 // (29 w + 24 + 24 + 24) + 2 precond x (9 w + 48) + ~12 vector passes x 24.
+#include <cstdlib>
+#include <cstring>
+
 #include "node_helpers.h"
 
 namespace tcad {
@@ -133,9 +136,21 @@ __device__ __forceinline__ void tile_load(const float4 *T4, long long N, long lo
   for (int e = 0; e < 3 * (TU / 4); ++e) R.t[e] = T4[(long long)e * N + g];
 }
 
+// The same rows read from memory at the point of use.
+struct TileMem {
+  const float4 *T4;
+  long long N, g;
+};
+
+__device__ __forceinline__ float4 tile_row(const TileRows &R, int e) { return R.t[e]; }
+__device__ __forceinline__ float4 tile_row(const TileMem &M, int e) {
+  return M.T4[(long long)e * M.N + M.g];
+}
+
 // z = M^-1 x for this thread's node; every thread of the workgroup calls it
 // (it holds barriers); tv is LDS of BS * 3 doubles.
-__device__ __forceinline__ void tile_apply(const TileRows &R, bool valid, const double x[3],
+template <typename Rows>
+__device__ __forceinline__ void tile_apply(const Rows &R, bool valid, const double x[3],
                                            double z[3], double *tv) {
   int tile, l;
   tile_slot(tile, l);
@@ -150,7 +165,7 @@ __device__ __forceinline__ void tile_apply(const TileRows &R, bool valid, const 
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < TU / 4; ++q) {
-      const float4 t = R.t[r * (TU / 4) + q];
+      const float4 t = tile_row(R, r * (TU / 4) + q);
       acc += (double)t.x * v[4 * q] + (double)t.y * v[4 * q + 1] + (double)t.z * v[4 * q + 2] +
              (double)t.w * v[4 * q + 3];
     }
@@ -318,17 +333,24 @@ __global__ void __launch_bounds__(BS) k_bicg_init(KArgs<T> K) {
   }
 }
 
-// The two kernels that apply the preconditioner hold the tile inverse rows
-// (144 registers) across their reductions: allow up to 2 waves per SIMD.
-template <typename T>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_bicg_1(KArgs<T> K) {
+// The two kernels that apply the preconditioner come in three variants
+// (PV): node block-Jacobi only (no tile code: full occupancy, for batches
+// without tile-preconditioned devices); tiles with the 144 inverse entries
+// loaded into registers before the reductions (their latency overlaps
+// the reductions', at 2 waves per SIMD: best when the grid is small and
+// latency-bound); tiles read at the point of use (full occupancy: large
+// grids, where more waves in flight hide the latency instead).
+enum { PV_NODE = 0, PV_TILE_REG = 1, PV_TILE_MEM = 2 };
+
+template <typename T, int PV>
+__device__ __forceinline__ void bicg_1_body(const KArgs<T> &K) {
   KPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
-  const bool use_tile = K.tile && I.dim == 2;
+  const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
   TileRows TR;
-  tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
+  if (PV == PV_TILE_REG) tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
+  const TileMem TM{K.T4, N, H.g};
   double r[3] = {0, 0, 0}, p[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   if (H.valid) {
 #pragma unroll
@@ -368,7 +390,8 @@ k_bicg_1(KArgs<T> K) {
   if (use_tile) {
     __shared__ double tv[BS * 3];
     double z[3];
-    tile_apply(TR, H.valid, p, z, tv);
+    if (PV == PV_TILE_REG) tile_apply(TR, H.valid, p, z, tv);
+    else tile_apply(TM, H.valid, p, z, tv);
     if (H.valid) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
@@ -409,15 +432,15 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv(KArgs<T> K, int src, int dst, 
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_bicg_3(KArgs<T> K) {
+template <typename T, int PV>
+__device__ __forceinline__ void bicg_3_body(const KArgs<T> &K) {
   KPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
-  const bool use_tile = K.tile && I.dim == 2;
+  const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
   TileRows TR;
-  tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
+  if (PV == PV_TILE_REG) tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
+  const TileMem TM{K.T4, N, H.g};
   double r[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   if (H.valid) {
 #pragma unroll
@@ -450,13 +473,67 @@ k_bicg_3(KArgs<T> K) {
   }
   if (use_tile) {
     __shared__ double tv[BS * 3];
-    tile_apply(TR, H.valid, s, z, tv);
+    if (PV == PV_TILE_REG) tile_apply(TR, H.valid, s, z, tv);
+    else tile_apply(TM, H.valid, s, z, tv);
     if (H.valid) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) K.v[VSH][c * N + H.g] = z[c];
     }
   }
   if (lead) S.alpha = alpha;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_bicg_1_node(KArgs<T> K) { bicg_1_body<T, PV_NODE>(K); }
+template <typename T>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_bicg_1_treg(KArgs<T> K) { bicg_1_body<T, PV_TILE_REG>(K); }
+template <typename T>
+__global__ void __launch_bounds__(BS) k_bicg_1_tmem(KArgs<T> K) { bicg_1_body<T, PV_TILE_MEM>(K); }
+template <typename T>
+__global__ void __launch_bounds__(BS) k_bicg_3_node(KArgs<T> K) { bicg_3_body<T, PV_NODE>(K); }
+template <typename T>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_bicg_3_treg(KArgs<T> K) { bicg_3_body<T, PV_TILE_REG>(K); }
+template <typename T>
+__global__ void __launch_bounds__(BS) k_bicg_3_tmem(KArgs<T> K) { bicg_3_body<T, PV_TILE_MEM>(K); }
+
+// Variant of the preconditioner kernels for this batch: node-only when no
+// device applies tiles; else registers up to kTileRegBlocks workgroups
+// (about two resident per CU at 2 waves per SIMD), memory beyond.  The
+// environment variable TCADHIP_TILE_ROWS = reg | mem overrides the choice.
+constexpr int kTileRegBlocks = 512;
+
+static int precond_variant(const Batch &b) {
+  if (!b.tile || !b.any2d) return PV_NODE;
+  static const int forced = []() -> int {
+    const char *e = std::getenv("TCADHIP_TILE_ROWS");
+    if (e && std::strcmp(e, "reg") == 0) return PV_TILE_REG;
+    if (e && std::strcmp(e, "mem") == 0) return PV_TILE_MEM;
+    return -1;
+  }();
+  if (forced >= 0) return forced;
+  return b.nblk <= kTileRegBlocks ? PV_TILE_REG : PV_TILE_MEM;
+}
+
+template <typename T>
+static void launch_k1(Batch &b, const KArgs<T> &K) {
+  const dim3 g(b.nblk), t(BS);
+  switch (precond_variant(b)) {
+    case PV_NODE: hipLaunchKernelGGL(k_bicg_1_node<T>, g, t, 0, b.stream, K); break;
+    case PV_TILE_REG: hipLaunchKernelGGL(k_bicg_1_treg<T>, g, t, 0, b.stream, K); break;
+    default: hipLaunchKernelGGL(k_bicg_1_tmem<T>, g, t, 0, b.stream, K); break;
+  }
+}
+
+template <typename T>
+static void launch_k3(Batch &b, const KArgs<T> &K) {
+  const dim3 g(b.nblk), t(BS);
+  switch (precond_variant(b)) {
+    case PV_NODE: hipLaunchKernelGGL(k_bicg_3_node<T>, g, t, 0, b.stream, K); break;
+    case PV_TILE_REG: hipLaunchKernelGGL(k_bicg_3_treg<T>, g, t, 0, b.stream, K); break;
+    default: hipLaunchKernelGGL(k_bicg_3_tmem<T>, g, t, 0, b.stream, K); break;
+  }
 }
 
 template <typename T>
@@ -565,9 +642,9 @@ template <typename T>
 static void launch_iteration(Batch &b, KArgs<T> K, int parity) {
   const dim3 g(b.nblk), t(BS);
   K.parity = parity;
-  hipLaunchKernelGGL(k_bicg_1<T>, g, t, 0, b.stream, K);
+  launch_k1<T>(b, K);
   hipLaunchKernelGGL(k_bicg_spmv<T>, g, t, 0, b.stream, K, (int)VPH, (int)VV, (int)VRH, 0, 2);
-  hipLaunchKernelGGL(k_bicg_3<T>, g, t, 0, b.stream, K);
+  launch_k3<T>(b, K);
   hipLaunchKernelGGL(k_bicg_spmv<T>, g, t, 0, b.stream, K, (int)VSH, (int)VT, (int)VS, 1, 2);
   hipLaunchKernelGGL(k_bicg_5<T>, g, t, 0, b.stream, K);
   b.launches += 5;
@@ -589,7 +666,7 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
     if (it >= maxit) {
       KArgs<T> K1 = K;
       K1.parity = it & 1;
-      hipLaunchKernelGGL(k_bicg_1<T>, dim3(b.nblk), dim3(BS), 0, b.stream, K1);
+      launch_k1<T>(b, K1);
       b.launches++;
     }
     TCAD_HIP_CHECK(hipGetLastError());
@@ -646,8 +723,8 @@ int launch_vector_kernel_only(Batch &b, int which) {
                      b.ndev);
   KArgs<double> K = make_kargs<double>(b, false);
   K.parity = 0;
-  if (which == 3) hipLaunchKernelGGL(k_bicg_1<double>, dim3(b.nblk), dim3(BS), 0, b.stream, K);
-  else hipLaunchKernelGGL(k_bicg_3<double>, dim3(b.nblk), dim3(BS), 0, b.stream, K);
+  if (which == 3) launch_k1<double>(b, K);
+  else launch_k3<double>(b, K);
   b.launches += 2;
   TCAD_HIP_CHECK(hipGetLastError());
   return TCAD_OK;

@@ -116,6 +116,7 @@ struct Batch {
   long long ntot = 0;
   int nblk = 0;
   int dim_max = 2;
+  int any2d = 0;                    // some device is 2D
   int ne = 0;                       // stencil entries per node
   hipStream_t stream = nullptr;
   long long launches = 0;
