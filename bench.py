@@ -227,6 +227,46 @@ def bench_serve(count=30, rate=2.0, check=3, lanes=2):
             "plain_fp64_points_per_s_on_those": round(alone_pts / dt_alone, 4)}
 
 
+def bench_ac(dev, spec, freqs=tuple(np.logspace(6, 12, 7)), cpu_freqs=2):
+    """Small-signal Y(f) of the MOSFET at its last sweep point (Vg = 1.0 V,
+    Vd = 0.05 V): 4 ports x len(freqs) complex solves on the GPU, f_T;
+    the FP64 oracle (SciPy SuperLU per frequency) on the first `cpu_freqs`
+    frequencies from the same operating point, for time and agreement."""
+    import torch
+    from oracle import ac as oac
+    from oracle_build import oracle_device
+    from tcadhip import ac
+    ac.y_parameters(dev, [freqs[0]])                   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = ac.y_parameters(dev, list(freqs))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    odev = oracle_device(spec)
+    odev.psi, odev.n, odev.p = dev.psi, dev.n, dev.p
+    for nm, bc in dev.bcs.items():
+        obc = odev.bcs[nm]
+        if hasattr(bc, "Vg"):
+            obc.Vg = bc.Vg
+        else:
+            obc.V = bc.V
+    odev._update_surface_mobility(odev.psi)
+    t1 = time.perf_counter()
+    ores = oac.y_parameters(odev, list(freqs[:cpu_freqs]))
+    dt_cpu = time.perf_counter() - t1
+    err = max(float(np.abs(res.Y[f] - ores.Y[f]).max() / np.abs(ores.Y[f]).max())
+              for f in range(cpu_freqs))
+    return {"operating_point": {"gate": 1.0, "drain": 0.05}, "ports": list(res.port_names),
+            "frequencies": len(freqs), "seconds": round(dt, 3),
+            "frequencies_per_s": round(len(freqs) / dt, 3),
+            "krylov_iterations": res.iterations,
+            "f_T_Hz": ac.cutoff_frequency(res, "gate", "drain"),
+            "cpu_oracle_frequencies_per_s": round(cpu_freqs / dt_cpu, 3),
+            "cpu_sample": f"FP64 oracle, SciPy SuperLU, first {cpu_freqs} frequencies, "
+                          f"{dt_cpu:.1f} s",
+            "max_err_vs_oracle_rel_to_max_Y": err}
+
+
 def cpu_baseline(spec, npoints, budget_s=30.0):
     """The FP64 oracle (NumPy + SciPy SuperLU, single thread) on the
     first `npoints` bias points of the same sweep, equilibrium included."""
@@ -261,7 +301,9 @@ def main():
     ap.add_argument("--forcing", action="store_true",
                     help="Newton forcing terms (inexact early solves)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the 3D tri-gate, batched-diode and serving workloads")
+                    help="skip the AC, 3D tri-gate, batched-diode and serving workloads")
+    ap.add_argument("--extras", default="ac,trigate,batch,serve",
+                    help="which of the extra workloads to run (comma-separated)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (the reference "
                          "kernel for scaling HBM traffic counters) and exit")
@@ -376,11 +418,17 @@ def main():
                       f"{cpu['points']} of {npts} Vg points of the same 200x120 sweep, "
                       f"{cpu['seconds']:.1f} s"},
     }
-    if not args.no_extras:
+    extras = [] if args.no_extras else args.extras.split(",")
+    if "ac" in extras:
+        print("[bench] small-signal AC", file=sys.stderr, flush=True)
+        line["ac_mosfet_200x120"] = bench_ac(dev, spec)
+    if "trigate" in extras:
         print("[bench] 3D tri-gate 64x48x48", file=sys.stderr, flush=True)
         line["trigate_3d_64x48x48"] = bench_trigate()
+    if "batch" in extras:
         print("[bench] 256 batched diodes", file=sys.stderr, flush=True)
         line["batched_diodes_256"] = bench_batch()
+    if "serve" in extras:
         print("[bench] serving", file=sys.stderr, flush=True)
         line["serving"] = bench_serve()
     print(json.dumps(line), flush=True)

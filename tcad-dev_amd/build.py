@@ -3,6 +3,10 @@
 Every HIP translation unit is compiled with -ffp-contract=off (no fused
 multiply-add, so the residual rounds like the reference's NumPy
 expressions) and with packed-FP32 instructions disabled.
+
+TCADHIP_BS=<threads> builds another node-workgroup size (default 256) and
+TCADHIP_OUT=<path> writes the library elsewhere (tcadhip loads such a build
+through TCADHIP_LIB).
 """
 import os
 import subprocess
@@ -12,13 +16,16 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-OUT = os.path.join(HERE, "tcadhip", "libtcadhip.so")
-OBJ = os.path.join(HERE, "_build")
+OUT = os.environ.get("TCADHIP_OUT") or os.path.join(HERE, "tcadhip", "libtcadhip.so")
+BS = os.environ.get("TCADHIP_BS")
+OBJ = os.path.join(HERE, "_build" + (f"_bs{BS}" if BS else ""))
 SOURCES = ["assemble.hip", "krylov.hip", "batch.hip", "ac.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(REPO, "include"), "-I", CSRC, "-Wall", "-Wno-unused-result"]
+if BS:
+    FLAGS.append(f"-DTCAD_BS={int(BS)}")
 
 
 def _compile(src):

@@ -90,7 +90,7 @@ struct CArgs {
 };
 
 #define CPROLOGUE                                   \
-  __shared__ double sh[BS / 64];                    \
+  __shared__ double sh[64];                         \
   const int d = dev_index(K.W);                     \
   const DevInfo &I = dev_info(K.W, d);              \
   ACScal &S = K.acs[d];                             \
@@ -302,8 +302,10 @@ __global__ void __launch_bounds__(BS) k_cb_1(CArgs K) {
     cload(K.v[CP], N, H.g, p);
     cload(K.v[CV], N, H.g, v);
   }
-  const Cx rho = {device_sum(K.part, K.nblk, 0, I, sh), device_sum(K.part, K.nblk, 1, I, sh)};
-  const double rr = device_sum(K.part, K.nblk, 2, I, sh);
+  double red[3];
+  device_sums<3>(K.part, K.nblk, 0, I, sh, red);
+  const Cx rho = {red[0], red[1]};
+  const double rr = red[2];
   const bool first_it = S.it == 0;
   const double bb = first_it ? rr : S.bb;
   const double b2 = (first_it && S.sweep == 0) ? rr : S.b2;
@@ -392,7 +394,9 @@ __global__ void __launch_bounds__(BS) k_cb_3(CArgs K) {
     cload(K.v[CR], N, H.g, r);
     cload(K.v[CV], N, H.g, v);
   }
-  const Cx rv = {device_sum(K.part, K.nblk, 3, I, sh), device_sum(K.part, K.nblk, 4, I, sh)};
+  double red[2];
+  device_sums<2>(K.part, K.nblk, 3, I, sh, red);
+  const Cx rv = {red[0], red[1]};
   const Cx rho = {S.rho[K.parity][0], S.rho[K.parity][1]};
   const Cx alpha = cdiv(rho, rv);
   if ((rv.re == 0.0 && rv.im == 0.0) || !isfinite(alpha.re) || !isfinite(alpha.im)) {
@@ -435,8 +439,10 @@ __global__ void __launch_bounds__(BS) k_cb_5(CArgs K) {
     cload(K.v[CT], N, H.g, t);
     cload(K.v[CRH], N, H.g, rh);
   }
-  const Cx ts = {device_sum(K.part, K.nblk, 5, I, sh), device_sum(K.part, K.nblk, 6, I, sh)};
-  const double tt = device_sum(K.part, K.nblk, 7, I, sh);
+  double red[3];
+  device_sums<3>(K.part, K.nblk, 5, I, sh, red);
+  const Cx ts = {red[0], red[1]};
+  const double tt = red[2];
   const Cx omega = tt > 0.0 ? Cx{ts.re / tt, ts.im / tt} : Cx{0.0, 0.0};
   const Cx alpha = {S.alpha[0], S.alpha[1]};
   double srh = 0.0, srhi = 0.0, srr = 0.0;

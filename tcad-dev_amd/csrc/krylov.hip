@@ -112,6 +112,8 @@ constexpr int TILE = 4;
 constexpr int TU = 3 * TILE * TILE;         // 48 unknowns per tile
 constexpr int TNS = 3 * TU + 1;             // LDS doubles per node row set (+1: banks)
 static_assert(PY % TILE == 0 && PX % TILE == 0, "patches must hold whole tiles");
+constexpr int QPP = PX / 16;                // tile-setup quarters (16 columns) per patch
+static_assert(PX % 16 == 0, "tile setup works on 16-column quarters");
 
 // tile slot of this thread inside its patch, and node index inside the tile
 __device__ __forceinline__ void tile_slot(int &tile, int &l) {
@@ -173,14 +175,14 @@ __device__ __forceinline__ void tile_apply(const Rows &R, bool valid, const doub
   }
 }
 
-// Setup: one workgroup of 64 threads per quarter patch (four tiles, 16 x 4
-// nodes); block Gauss-Jordan in FP64 with the 3x3 node blocks as pivots
+// Setup: one workgroup of 64 threads per 16-column quarter of a patch (four
+// tiles side by side, 16 x 4 nodes per tile row); block Gauss-Jordan in FP64 with the 3x3 node blocks as pivots
 // (no row exchange, like the node block-Jacobi inverse), in place in LDS.
 __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const double *A,
                                                    const Scal *scal, float4 *T4, int *flag) {
   __shared__ double M[4 * 16 * TNS];
   __shared__ double Pv[4 * 9];
-  const int patch = blockIdx.x >> 2, quarter = blockIdx.x & 3;
+  const int patch = blockIdx.x / QPP, quarter = blockIdx.x % QPP;
   const int d = W.single ? 0 : W.blk2dev[patch];
   if (scal && !scal[d].active) return;
   const DevInfo &I = dev_info(W, d);
@@ -297,7 +299,7 @@ __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const d
 }
 
 #define KPROLOGUE                                   \
-  __shared__ double sh[BS / 64];                    \
+  __shared__ double sh[64];                         \
   const int d = dev_index(K.W);                     \
   const DevInfo &I = dev_info(K.W, d);              \
   Scal &S = K.scal[d];                              \
@@ -360,8 +362,9 @@ __device__ __forceinline__ void bicg_1_body(const KArgs<T> &K) {
       v[c] = K.v[VV][c * N + H.g];
     }
   }
-  const double rho = device_sum(K.part, K.nblk, 0, I, sh);
-  const double rr = device_sum(K.part, K.nblk, 1, I, sh);
+  double red[2];
+  device_sums<2>(K.part, K.nblk, 0, I, sh, red);
+  const double rho = red[0], rr = red[1];
   const double bb = (S.it == 0) ? rr : S.bb;
   if (rr <= S.tol2 * bb || S.it >= S.maxit || !isfinite(rr) || rho == 0.0) {
     if (lead) {
@@ -555,8 +558,9 @@ __global__ void __launch_bounds__(BS) k_bicg_5(KArgs<T> K) {
       rh[c] = K.v[VRH][ix];
     }
   }
-  const double ts = device_sum(K.part, K.nblk, 2, I, sh);
-  const double tt = device_sum(K.part, K.nblk, 3, I, sh);
+  double red[2];
+  device_sums<2>(K.part, K.nblk, 2, I, sh, red);
+  const double ts = red[0], tt = red[1];
   const double omega = (tt > 0.0) ? ts / tt : 0.0;
   const double alpha = S.alpha;
   double srh = 0.0, srr = 0.0;
@@ -631,7 +635,7 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
 }
 
 int launch_tile_setup(Batch &b, bool only_active) {
-  hipLaunchKernelGGL(k_tile_setup, dim3(4 * b.nblk), dim3(64), 0, b.stream, make_where(b), b.ntot,
+  hipLaunchKernelGGL(k_tile_setup, dim3(QPP * b.nblk), dim3(64), 0, b.stream, make_where(b), b.ntot,
                      b.A64, only_active ? b.d_scal : nullptr, b.T4, b.d_flag);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
@@ -756,7 +760,7 @@ struct IRArgs {
 
 // r64 = bs - A64 y64 (y64 = 0 when `first`), partials (r,r), (b,b)
 __global__ void __launch_bounds__(BS) k_ir_residual(IRArgs R, int first) {
-  __shared__ double sh[BS / 64];
+  __shared__ double sh[64];
   const int d = dev_index(R.W);
   const DevInfo &I = dev_info(R.W, d);
   Scal &S = R.scal[d];
@@ -788,14 +792,15 @@ __global__ void __launch_bounds__(BS) k_ir_residual(IRArgs R, int first) {
 
 // One workgroup per device: refinement stop test; nrm for the inner rhs.
 __global__ void k_ir_check(IRArgs R, int ndev, double tol2, int per_dev) {
-  __shared__ double sh[BS / 64];
+  __shared__ double sh[64];
   const int d = blockIdx.x;
   if (d >= ndev) return;
   const DevInfo &I = dev_info(R.W, d);
   Scal &S = R.scal[d];
   if (!S.active) return;
-  const double rr = device_sum(R.part, R.nblk, 0, I, sh);
-  const double bb = device_sum(R.part, R.nblk, 1, I, sh);
+  double red[2];
+  device_sums<2>(R.part, R.nblk, 0, I, sh, red);
+  const double rr = red[0], bb = red[1];
   if (threadIdx.x == 0) {
     S.nrm = sqrt(rr);
     S.rr = rr;
@@ -928,7 +933,7 @@ __device__ __forceinline__ double pspmv(const double *A, long long N, const Node
 }
 
 #define PPROLOGUE                                   \
-  __shared__ double sh[BS / 64];                    \
+  __shared__ double sh[64];                    \
   const int d = dev_index(P.W);                     \
   const DevInfo &I = dev_info(P.W, d);              \
   Scal &S = P.scal[d];                              \
@@ -966,8 +971,9 @@ __global__ void __launch_bounds__(BS) k_pcg_a(PArgs P) {
     z = P.z[H.g];
     pv = P.p[H.g];
   }
-  const double rz = device_sum(P.part, P.nblk, 0, I, sh);
-  const double rr = device_sum(P.part, P.nblk, 1, I, sh);
+  double red[2];
+  device_sums<2>(P.part, P.nblk, 0, I, sh, red);
+  const double rz = red[0], rr = red[1];
   const double bb = (S.it == 0) ? rr : S.bb;
   if (rr <= S.tol2 * bb || S.it >= S.maxit || !isfinite(rr)) {
     if (lead) {

@@ -38,13 +38,72 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return s;
 }
 
-// Sum of a device's per-workgroup partials of quantity q, fixed order.
+// Block-wide FP64 sums of NQ values with one pair of barriers, fixed tree;
+// sh holds at least 16 * NQ doubles.  Results valid in every thread.
+template <int NQ>
+__device__ __forceinline__ void block_sums(double v[NQ], double *sh, double out[NQ]) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sh[16 * q + wid] = v[q];
+  }
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += sh[16 * q + w];
+    out[q] = s;
+  }
+}
+
+// Sums of a device's per-workgroup partials of quantities q0 .. q0+NQ-1,
+// fixed order: four independent accumulators per quantity, so the loads
+// of one round are in flight together (a device spans up to ~1000
+// workgroups, and one dependent load per partial was a latency chain in
+// every consumer workgroup), then one block tree.  sh: 16 * NQ doubles.
+template <int NQ>
+__device__ __forceinline__ void device_sums(const double *part, int nblk_total, int q0,
+                                            const DevInfo &I, double *sh, double out[NQ]) {
+  double a[NQ][4];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[q][k] = 0.0;
+  const int n = I.nblk, st = blockDim.x;
+  const double *p = part + I.blk0;
+  int t = threadIdx.x;
+  for (; t + 3 * st < n; t += 4 * st) {
+    double v[NQ][4];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[q][k] = p[(long long)(q0 + q) * nblk_total + t + k * st];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a[q][k] += v[q][k];
+  }
+  for (; t < n; t += st) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) a[q][0] += p[(long long)(q0 + q) * nblk_total + t];
+  }
+  double s[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) s[q] = (a[q][0] + a[q][1]) + (a[q][2] + a[q][3]);
+  block_sums<NQ>(s, sh, out);
+}
+
+// Sum of a device's per-workgroup partials of quantity q (sh: 16 doubles).
 __device__ __forceinline__ double device_sum(const double *part, int nblk_total, int q,
                                              const DevInfo &I, double *sh) {
-  double s = 0.0;
-  for (int t = threadIdx.x; t < I.nblk; t += blockDim.x)
-    s += part[(long long)q * nblk_total + I.blk0 + t];
-  return block_sum(s, sh);
+  double o[1];
+  device_sums<1>(part, nblk_total, q, I, sh, o);
+  return o[0];
 }
 
 struct NodeHood {

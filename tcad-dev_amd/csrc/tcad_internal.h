@@ -34,10 +34,12 @@
 namespace tcad {
 
 #ifndef TCAD_BS
-#define TCAD_BS 256
+#define TCAD_BS 128
 #endif
 constexpr int BS = TCAD_BS;         // threads per node workgroup (multiple of 64)
-constexpr int PX = 64;              // patch width: one wavefront per patch row
+// patch width: one wavefront per patch row from 256 threads up; smaller
+// workgroups keep 4 rows (whole 4 x 4 tiles) and narrow the patch
+constexpr int PX = BS >= 256 ? 64 : BS / 4;
 constexpr int PY = BS / PX;         // patch height
 static_assert(BS % PX == 0, "workgroup must hold whole patch rows");
 constexpr int NDIAG = 9;
