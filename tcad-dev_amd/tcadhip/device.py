@@ -437,7 +437,8 @@ class _GPUDevice:
                  models=None):
         self._s = DeviceSetup(coords, doping, Ntotal, T, material, models)
         for k in ("dim", "shape", "N", "VT", "Ns", "LD", "J0", "R0", "C", "nie_s", "nie",
-                  "mu_n0", "mu_p0", "tau_n", "tau_p", "dV", "eps", "models"):
+                  "mu_n0", "mu_p0", "tau_n", "tau_p", "dV", "eps", "models", "doping",
+                  "Ntot"):
             setattr(self, k, getattr(self._s, k))
         self.bcs = self._s.bcs
         self._batch = None
@@ -542,6 +543,31 @@ class _GPUDevice:
     @property
     def p_cm3(self):
         return self.p * self.Ns
+
+    def current_density_edges(self):
+        """Electron and hole current densities [A/cm^2] on the mesh edges
+        at the current state and bias, per axis: {"x": (Jn, Jp), ...}
+        with the reference's edge shapes, e.g. (Ny, Nx-1) along x in 2D
+        (the Jn_x/Jp_x ... that solve_bias leaves on the device,
+        device2d.py:2188-2190, device3d.py:2266-2270)."""
+        torch = _torch()
+        if not self._solved:
+            raise RuntimeError("current_density_edges: solve the device first")
+        B = self._gpu()
+        B.set_bias()
+        N = self.N
+        J = torch.zeros(2 * self.dim * N, dtype=torch.float64, device="cuda")
+        _lib.check(_lib.lib().tcad_edge_currents(B.handle, B.psi.data_ptr(), B.n.data_ptr(),
+                                                 B.p.data_ptr(), J.data_ptr()),
+                   "tcad_edge_currents")
+        Jh = J.cpu().numpy().reshape(self.dim, 2, N) * self.J0
+        out = {}
+        for a, name in enumerate("xyz"[:self.dim]):
+            ax = self.dim - 1 - a                    # array axis of this mesh axis
+            sl = [slice(None)] * self.dim
+            sl[ax] = slice(None, -1)
+            out[name] = tuple(Jh[a, c].reshape(self.shape)[tuple(sl)] for c in (0, 1))
+        return out
 
     # -- residual / Jacobian boundary (tests and inspection) -----------------
     def _update_surface_mobility(self, psi):
