@@ -247,14 +247,15 @@ __global__ void k_ac_reset(ACScal *acs, const int *act, int ndev, double tol2, d
   }
 }
 
-// r = b - A x (x = 0 on the first sweep), rhat = r, p = v = 0; partials
-// (rhat, r) = ||r||^2 and ||r||^2
+// r = b - A x (x = 0 when K.first), rhat = r, p = v = 0; partials
+// (rhat, r) = ||r||^2, ||r||^2 and ||b||^2 (quantity 3, read by the first
+// k_cb_1 of a solve: the reference of its relative residual)
 __global__ void __launch_bounds__(BS) k_cb_init(CArgs K) {
   CPROLOGUE
   (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
-  double rr = 0.0;
+  double rr = 0.0, bb = 0.0;
   if (H.valid) {
     double r[6];
     if (K.first) {
@@ -281,12 +282,16 @@ __global__ void __launch_bounds__(BS) k_cb_init(CArgs K) {
     cstore(K.v[CV], N, H.g, z);
 #pragma unroll
     for (int c = 0; c < 6; ++c) rr += r[c] * r[c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) bb += K.b[c * N + H.g] * K.b[c * N + H.g];
   }
-  const double s = block_sum(rr, sh);
+  double in[2] = {rr, bb}, s[2];
+  block_sums<2>(in, sh, s);
   if (threadIdx.x == 0) {
-    K.part[blockIdx.x] = s;
+    K.part[blockIdx.x] = s[0];
     K.part[(long long)K.nblk + blockIdx.x] = 0.0;
-    K.part[2LL * K.nblk + blockIdx.x] = s;
+    K.part[2LL * K.nblk + blockIdx.x] = s[0];
+    K.part[3LL * K.nblk + blockIdx.x] = s[1];
   }
 }
 
@@ -302,13 +307,14 @@ __global__ void __launch_bounds__(BS) k_cb_1(CArgs K) {
     cload(K.v[CP], N, H.g, p);
     cload(K.v[CV], N, H.g, v);
   }
-  double red[3];
-  device_sums<3>(K.part, K.nblk, 0, I, sh, red);
+  const bool first_it = S.it == 0;
+  double red[4];
+  if (first_it) device_sums<4>(K.part, K.nblk, 0, I, sh, red);
+  else device_sums<3>(K.part, K.nblk, 0, I, sh, red);
   const Cx rho = {red[0], red[1]};
   const double rr = red[2];
-  const bool first_it = S.it == 0;
   const double bb = first_it ? rr : S.bb;
-  const double b2 = (first_it && S.sweep == 0) ? rr : S.b2;
+  const double b2 = (first_it && S.sweep == 0) ? red[3] : S.b2;
   const bool conv = rr <= S.tol2 * bb || rr <= S.abs2 * b2;
   if (conv || S.it >= S.maxit || !isfinite(rr) || (rho.re == 0.0 && rho.im == 0.0)) {
     if (lead) {
@@ -641,8 +647,10 @@ constexpr int kMaxSweeps = 12;
 constexpr double kSweepRtol = 1e-8;     // residual reduction per sweep
 constexpr double kAcceptRel = 1e-6;     // stalled above this: a failed solve
 
+// warm: start from the solution left in x (the previous frequency's, same
+// port) instead of zero.
 int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int maxit,
-                  std::vector<int> &status, std::vector<long long> &iters) {
+                  bool warm, std::vector<int> &status, std::vector<long long> &iters) {
   CArgs K = make_cargs(b, w);
   std::vector<ACScal> hs(b.ndev);
   std::vector<double> prev(b.ndev, INFINITY);
@@ -657,7 +665,7 @@ int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int ma
     hipLaunchKernelGGL(k_ac_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, w.acs,
                        (const int *)w.act, b.ndev, kSweepRtol * kSweepRtol, rtol * rtol, maxit,
                        sweep);
-    K.first = sweep == 0 ? 1 : 0;
+    K.first = sweep == 0 && !warm ? 1 : 0;
     hipLaunchKernelGGL(k_cb_init, dim3(b.nblk), dim3(BS), 0, b.stream, K);
     b.launches += 2;
     int it = 0, chunk = 8;
@@ -789,7 +797,7 @@ int y_parameters_impl(Batch &b, const int32_t *h_active, int nfreq, const double
       b.launches++;
       TCAD_HIP_CHECK(hipGetLastError());
       if ((rc = check_flag(b))) return rc;
-      if ((rc = complex_solve(b, w, act_k, rtol, maxit, status, iters))) return rc;
+      if ((rc = complex_solve(b, w, act_k, rtol, maxit, f > 0, status, iters))) return rc;
       O.k = k;
       hipLaunchKernelGGL(k_ac_observe, dim3(nrows), dim3(BS), 0, b.stream, O);
       b.launches++;

@@ -70,6 +70,7 @@ class DeviceSpec:
     gates: List[GateSpec]
     sweep: List[Dict[str, float]] = field(default_factory=list)
     name: str = ""
+    T: float = 300.0
 
     @property
     def dim(self):

@@ -715,14 +715,14 @@ def device_from_spec(spec):
     mat = ph.SILICON.with_overrides(spec.material)
     models = ph.Models(**spec.models)
     if spec.dim == 2:
-        dev = Device2D(Mesh2D(*spec.coords), spec.doping, spec.Ntotal, material=mat,
+        dev = Device2D(Mesh2D(*spec.coords), spec.doping, spec.Ntotal, T=spec.T, material=mat,
                        models=models)
         for c in spec.contacts:
             dev.add_contact(c.name, c.idx[1], c.idx[0], V=c.V)
         for g in spec.gates:
             dev.add_gate(g.name, g.idx[1], g.idx[0], g.tox_cm, g.Vfb, g.Vg)
     else:
-        dev = Device3D(Mesh3D(*spec.coords), spec.doping, spec.Ntotal, material=mat,
+        dev = Device3D(Mesh3D(*spec.coords), spec.doping, spec.Ntotal, T=spec.T, material=mat,
                        models=models)
         for c in spec.contacts:
             dev.add_contact(c.name, c.idx[2], c.idx[1], c.idx[0], V=c.V)

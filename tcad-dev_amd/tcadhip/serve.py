@@ -113,6 +113,7 @@ class Request:
     spec: object
     arrival: float                   # seconds after the start of the run
     kind: str = ""
+    keep_state: bool = False         # hand back the final (psi, n, p)
 
 
 @dataclass
@@ -126,7 +127,7 @@ class Served:
     admitted: float = 0.0
     done: float = 0.0
     result: SweepResult = field(default_factory=SweepResult)
-
+    state: tuple = None              # final scaled (psi, n, p) on the host, if asked
     @property
     def latency(self):
         return self.done - self.arrival
@@ -206,6 +207,12 @@ class _Lane:
                 keep.append(m)
                 continue
             out[m["req"].rid].done = now()
+            if m["req"].keep_state:
+                sl = slice(m["off"], m["off"] + m["setup"].N)
+                with self._ctx():
+                    out[m["req"].rid].state = tuple(
+                        self.pool.planes[k][sl].cpu().numpy().reshape(m["setup"].shape)
+                        for k in ("psi", "n", "p"))
             self.pool.release(m["off"], m["setup"].N)
             with guard:
                 self.load -= m["req"].spec.N * len(m["req"].spec.sweep)

@@ -17,7 +17,9 @@ refused with a ValueError naming what is missing, never approximated.
 Entry points: DeviceSpec.from_json / to_json, build_device(spec),
 solve_job(spec) -> result dict, write_result(result, path) (atomic),
 run_job(job_path, out_path), validate_result(path_or_mapping),
-ResultFile(path) (reader).
+ResultFile(path) (reader); for the serving loop, request_from_job(spec)
+-> a tcadhip.serve.Request and served_result(spec, served) -> the same
+result dict from what the server handed back.
 """
 import json
 import math
@@ -640,6 +642,12 @@ def solve_job(spec, opts=None):
         result = extract_fields(dev, spec, solved)
         if spec.ac is not None:
             result.update(run_ac(dev, spec))
+    return _stamp(result, spec, opts)
+
+
+def _stamp(result, spec, opts):
+    """The grammar's stamps, geometry keys and run record
+    (solver_runner.py:1140-1226)."""
     result["result__schema"] = np.array(RESULT_SCHEMA_VERSION)
     result["geom__kind"] = np.array(GEOM_STRUCTURED)
     result["mesh__shape"] = np.array(spec.mesh.shape())
@@ -670,6 +678,83 @@ def solve_job(spec, opts=None):
         "T": spec.T, "models": spec.models, "numerics": numerics,
         "sweep": sweep_meta, "transient": transient_meta}))
     return result
+
+
+def config_spec(spec, points, name="job"):
+    """The job as a tcadhip.configs.DeviceSpec with the given full bias
+    points, the input of the batched sweep and serving drivers."""
+    from . import configs
+    flags = check_supported(spec)
+    dim = spec.mesh.dimensionality
+    shape = spec.mesh.shape()
+    coords = [np.asarray(spec.mesh.axes[a], dtype=float) for a in "xyz"[:dim]]
+    doping = np.asarray(spec.doping.values, dtype=float).reshape(shape)
+    ntot = None if spec.doping.ntotal is None else \
+        np.asarray(spec.doping.ntotal, dtype=float).reshape(shape)
+
+    def idx(c):                                   # array order (j, i) / (k, j, i)
+        return tuple(np.asarray(c.nodes[k], dtype=int) for k in "kji"[3 - dim:])
+
+    contacts = [configs.ContactSpec(c.name, idx(c), float(c.V))
+                for c in spec.contacts if c.kind == "ohmic"]
+    gates = [configs.GateSpec(c.name, idx(c), float(c.tox_cm), float(c.Vfb),
+                              "y" if dim == 2 else c.normal_axis, float(c.V))
+             for c in spec.contacts if c.kind == "gate"]
+    return configs.DeviceSpec(coords, doping, ntot, flags, {}, contacts, gates,
+                              sweep=[dict(p) for p in points], name=name, T=float(spec.T))
+
+
+def request_from_job(spec, rid, arrival=0.0):
+    """A serving request for a sweep job (every point a full bias) or a
+    single-bias job; the server hands back the final state with it."""
+    from .serve import Request
+    if spec.transient is not None or spec.ac is not None:
+        raise ValueError("the serving loop runs sweep and bias jobs; transient and AC jobs "
+                         "run through solve_job")
+    if spec.sweep is not None:
+        points = [merge_bias(spec, {spec.sweep.contact: V}) for V in spec.sweep.voltages()]
+    elif spec.bias is not None:
+        points = [merge_bias(spec)]
+    else:
+        raise ValueError("an equilibrium-only job has no bias point to serve")
+    return Request(rid, config_spec(spec, points, name=f"job{rid}"), float(arrival), "job",
+                   keep_state=True)
+
+
+def served_result(spec, served, opts=None):
+    """The result dict of a job the server solved (request_from_job):
+    fields at the last converged point (the lagged surface mobility
+    re-evaluated at that state), the terminal currents, and for a sweep
+    job the sweep block -- points past a failure are reported not
+    converged, with NaN currents."""
+    from .device import NewtonOptions
+    if served.state is None:
+        raise ValueError("served_result needs a request made by request_from_job")
+    res = served.result
+    dev = build_device(spec)
+    dev.set_state(*served.state)
+    reached = len(res.currents)
+    if reached:
+        dev._set_voltages(res.points[-1])
+        if dev.models.surface_mobility:
+            dev._update_surface_mobility(served.state[0])
+    fields = extract_fields(dev, spec, reached > 0)
+    if spec.sweep is not None:
+        sw = spec.sweep
+        voltages = sw.voltages()
+        conv = np.arange(len(voltages)) < reached
+        fields.update({
+            "sweep__voltage": np.asarray(voltages, dtype=float),
+            "sweep__converged": conv,
+            "unit__sweep_current": np.array({2: "A/cm", 3: "A"}[spec.mesh.dimensionality]),
+            "sweep__meta": np.array(json.dumps({"contact": sw.contact, "start": sw.start,
+                                                "stop": sw.stop, "step": sw.step,
+                                                "dimensionality": spec.mesh.dimensionality}))})
+        for nm in _ohmic(spec):
+            vals = np.full(len(voltages), np.nan)
+            vals[:reached] = [float(c[nm]) for c in res.currents]
+            fields[f"sweep__current__{nm}"] = vals
+    return _stamp(fields, spec, opts or NewtonOptions())
 
 
 def write_result(result, out_path):

@@ -143,3 +143,31 @@ def test_3d_job_grammar_and_values(tmp_path):
     # an ohmic resistor conducts: I = V / R with R > 0, and |sum I| ~ 0
     assert chans["left"][2] > chans["left"][1] > 0.0
     assert abs(I["left"] + I["right"]) <= 1e-6 * abs(I["left"])
+
+
+def test_served_jobs_emit_the_same_results():
+    """Sweep jobs through the continuous-batching server (request_from_job /
+    served_result): valid results whose sweep currents are the bits the
+    same job gives through solve_job (the server's per-device reductions
+    do not depend on what else is resident), fields at the same state."""
+    print("[wire] served jobs", flush=True)
+    wire = _wire()
+    from tcadhip import serve
+    names = ["wire_resistor_2d_sweep", "wire_mosfet_2d_gate_sweep"]
+    specs = [wire.DeviceSpec.from_json(os.path.join(GOLDEN, f"{n}.json")) for n in names]
+    reqs = [wire.request_from_job(s, rid, arrival=0.0) for rid, s in enumerate(specs)]
+    S = serve.Server(20_000)
+    served = S.run(reqs, realtime=False)
+    for spec, sv in zip(specs, served):
+        got = wire.served_result(spec, sv)
+        assert wire.validate_result(got) == 3
+        ref = wire.solve_job(spec)
+        assert sorted(got) == sorted(ref)
+        for k in ref:
+            if k.startswith("sweep__current__") or k in ("sweep__converged", "sweep__voltage",
+                                                         "field__potential",
+                                                         "field__electron_density"):
+                assert np.array_equal(got[k], ref[k]), k
+        terms = sorted(k for k in ref if k.startswith("terminal__") and k.endswith("__value"))
+        _close_currents(np.array([float(got[k]) for k in terms]),
+                        np.array([float(ref[k]) for k in terms]), "served terminals")
