@@ -988,7 +988,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   for (int i = 0; i < 9; ++i) TRY(dalloc(&b->v64[i], 3 * N));
   TRY(dalloc(&b->ir_r, 3 * N));
   TRY(dalloc(&b->ir_rhs, 3 * N));
-  TRY(dalloc(&b->part, 4 * (size_t)b->nblk));
+  TRY(dalloc(&b->part, 8 * (size_t)b->nblk));
   TRY(dalloc(&b->d_scal, (size_t)ndev));
   TRY(dalloc(&b->Aeq, 7 * N));
   TRY(dalloc(&b->beq, N));

@@ -1,25 +1,29 @@
 // Krylov solvers on the stencil-block Newton matrix.
 //
-// * BiCGStab (right node-block-Jacobi preconditioning), templated on the
-//   storage precision T of the matrix and block inverses (double or
-//   float); vectors and dot products are FP64.  One iteration is five
-//   kernels:
-//     K1  reduce (rhat,r),(r,r) -> rho, stop test; p = r + beta(p - w v);
-//         phat = Dinv p
-//     K2  v = A phat, partial (rhat, v)
-//     K3  reduce -> alpha; s = r - alpha v; shat = Dinv s
-//     K4  t = A shat, partials (t,s), (t,t)
-//     K5  reduce -> omega; x += alpha phat + omega shat; r = s - omega t;
-//         partials (rhat,r), (r,r)
-//   Each workgroup writes one partial sum per quantity; every workgroup of
-//   a device re-reduces that device's partials in a fixed order at the top
-//   of the next kernel, so the scalars are identical in all workgroups,
-//   bit-reproducible run to run and identical for a device alone or in a
-//   batch, with no atomics and no cross-workgroup synchronisation inside a
-//   kernel.  (A last-arriving-workgroup finalize was measured 1.5x slower
-//   on gfx950: its device-scope release/acquire writes back and
-//   invalidates the XCD's L2 per workgroup.)  The vector loads of K1/K3/K5
-//   are issued before the reduction so the two latencies overlap.
+// * BiCGStab (right block-Jacobi preconditioning: 3x3 node blocks, or the
+//   4x4-node tiles of 2D devices), templated on the storage precision T of
+//   the matrix and block inverses (double or float); vectors and dot
+//   products are FP64.  One iteration is four kernels:
+//     k_bicg_spmv    v = A phat, partial (rhat, v)
+//     k_bicg_s       reduce (r, r) -> stop test; reduce -> alpha;
+//                    s = r - alpha v; shat = M^-1 s
+//     k_bicg_spmv_t  t = A shat, partials (t,s), (t,t), (rhat,s), (rhat,t)
+//     k_bicg_u       reduce -> omega; x += alpha phat + omega shat;
+//                    r = s - omega t; rho' = (rhat,s) - omega (rhat,t);
+//                    p = r + beta (p - omega v); phat = M^-1 p; partial (r,r)
+//   The update of one iteration and the direction of the next share a
+//   kernel: rho' comes from the dot products of t, so the textbook's fifth
+//   kernel and its reduction point disappear (the stop test on the new r
+//   moves to k_bicg_s).  Each workgroup writes one partial sum per
+//   quantity; every workgroup of a device re-reduces that device's
+//   partials in a fixed order at the top of the next kernel, so the
+//   scalars are identical in all workgroups, bit-reproducible run to run
+//   and identical for a device alone or in a batch, with no atomics and no
+//   cross-workgroup synchronisation inside a kernel.  (A last-arriving-
+//   workgroup finalize was measured 1.5x slower on gfx950: its
+//   device-scope release/acquire writes back and invalidates the XCD's L2
+//   per workgroup.)  The vector loads are issued before the reductions so
+//   the two latencies overlap.
 // * Every Newton linear solve is FP64 iterative refinement: the true
 //   residual r = b - A64 y is formed in FP64 and BiCGStab solves
 //   A d = r / ||r|| with the matrix and block inverses stored in FP64
@@ -306,37 +310,8 @@ __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const d
   if (S.done) return;                               \
   const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
 
-// x = 0, r = rhat = b (optionally b / nrm), p = v = 0; partials (r,r)
-template <typename T>
-__global__ void __launch_bounds__(BS) k_bicg_init(KArgs<T> K) {
-  KPROLOGUE
-  (void)lead;
-  const NodeHood H = hood(I, blockIdx.x);
-  const long long N = K.N;
-  const double sc = K.scale_b ? 1.0 / S.nrm : 1.0;
-  double rr = 0.0;
-  if (H.valid) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const long long ix = c * N + H.g;
-      const double bv = K.scale_b ? K.b[ix] * sc : K.b[ix];
-      K.v[VX][ix] = 0.0;
-      K.v[VR][ix] = bv;
-      K.v[VRH][ix] = bv;
-      K.v[VP][ix] = 0.0;
-      K.v[VV][ix] = 0.0;
-      rr += bv * bv;
-    }
-  }
-  const double s = block_sum(rr, sh);
-  if (threadIdx.x == 0) {
-    K.part[blockIdx.x] = s;
-    K.part[(long long)K.nblk + blockIdx.x] = s;
-  }
-}
-
-// The two kernels that apply the preconditioner come in three variants
-// (PV): node block-Jacobi only (no tile code: full occupancy, for batches
+// The kernels that apply the preconditioner come in three variants (PV):
+// node block-Jacobi only (no tile code: full occupancy, for batches
 // without tile-preconditioned devices); tiles with the 144 inverse entries
 // loaded into registers before the reductions (their latency overlaps
 // the reductions', at 2 waves per SIMD: best when the grid is small and
@@ -344,67 +319,58 @@ __global__ void __launch_bounds__(BS) k_bicg_init(KArgs<T> K) {
 // grids, where more waves in flight hide the latency instead).
 enum { PV_NODE = 0, PV_TILE_REG = 1, PV_TILE_MEM = 2 };
 
+// z = M^-1 x at this thread's node (every thread of the workgroup calls
+// it: the tile variant holds barriers)
 template <typename T, int PV>
-__device__ __forceinline__ void bicg_1_body(const KArgs<T> &K) {
+__device__ __forceinline__ void apply_prec(const KArgs<T> &K, const NodeHood &H, bool use_tile,
+                                           const TileRows &TR, const double x[3], double z[3]) {
+  if (!use_tile) {
+    if (H.valid) prec3<T>(K.Dinv, K.N, H.g, x, z);
+    return;
+  }
+  __shared__ double tv[BS * 3];
+  if (PV == PV_TILE_REG) {
+    tile_apply(TR, H.valid, x, z, tv);
+  } else {
+    const TileMem TM{K.T4, K.N, H.g};
+    tile_apply(TM, H.valid, x, z, tv);
+  }
+}
+
+// x = 0, r = rhat = p = b (optionally b / nrm), phat = M^-1 p; partial
+// (r, r) at quantity 1 (rho_0 = (rhat, r) is the same sum)
+template <typename T, int PV>
+__device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
   KPROLOGUE
+  (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
   const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
   TileRows TR;
   if (PV == PV_TILE_REG) tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
-  const TileMem TM{K.T4, N, H.g};
-  double r[3] = {0, 0, 0}, p[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+  const double sc = K.scale_b ? 1.0 / S.nrm : 1.0;
+  double b3[3] = {0, 0, 0}, rr = 0.0;
   if (H.valid) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      r[c] = K.v[VR][c * N + H.g];
-      p[c] = K.v[VP][c * N + H.g];
-      v[c] = K.v[VV][c * N + H.g];
+      const long long ix = c * N + H.g;
+      const double bv = K.scale_b ? K.b[ix] * sc : K.b[ix];
+      b3[c] = bv;
+      K.v[VX][ix] = 0.0;
+      K.v[VR][ix] = bv;
+      K.v[VRH][ix] = bv;
+      K.v[VP][ix] = bv;
+      rr += bv * bv;
     }
   }
-  double red[2];
-  device_sums<2>(K.part, K.nblk, 0, I, sh, red);
-  const double rho = red[0], rr = red[1];
-  const double bb = (S.it == 0) ? rr : S.bb;
-  if (rr <= S.tol2 * bb || S.it >= S.maxit || !isfinite(rr) || rho == 0.0) {
-    if (lead) {
-      S.rr = rr;
-      if (S.it == 0) S.bb = bb;
-      if (!(rr <= S.tol2 * bb)) S.status = 1;
-      S.done = 1;
-    }
-    return;
-  }
-  const double omega = S.omega;
-  const double beta = (rho / S.rho2[K.parity ^ 1]) * (S.alpha / omega);
+  double z[3];
+  apply_prec<T, PV>(K, H, use_tile, TR, b3, z);
   if (H.valid) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      p[c] = r[c] + beta * (p[c] - omega * v[c]);
-      K.v[VP][c * N + H.g] = p[c];
-    }
-    if (!use_tile) {
-      double z[3];
-      prec3<T>(K.Dinv, N, H.g, p, z);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
-    }
+    for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
   }
-  if (use_tile) {
-    __shared__ double tv[BS * 3];
-    double z[3];
-    if (PV == PV_TILE_REG) tile_apply(TR, H.valid, p, z, tv);
-    else tile_apply(TM, H.valid, p, z, tv);
-    if (H.valid) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
-    }
-  }
-  if (lead) {
-    S.rho2[K.parity] = rho;
-    S.rr = rr;
-    if (S.it == 0) S.bb = bb;
-  }
+  const double s = block_sum(rr, sh);
+  if (threadIdx.x == 0) K.part[(long long)K.nblk + blockIdx.x] = s;
 }
 
 // dst = A src; partial (dot_a, dst) [and (dst, dst)] at quantities q0[, q0+1]
@@ -435,15 +401,48 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv(KArgs<T> K, int src, int dst, 
   }
 }
 
+// t = A shat; partials (t, s), (t, t), (rhat, s), (rhat, t) at quantities
+// 3..6: with them the update kernel forms omega and the next rho
+// (rhat, s - omega t) without another reduction
+template <typename T>
+__global__ void __launch_bounds__(BS) k_bicg_spmv_t(KArgs<T> K) {
+  KPROLOGUE
+  (void)lead;
+  const NodeHood H = hood(I, blockIdx.x);
+  const long long N = K.N;
+  double q[4] = {0.0, 0.0, 0.0, 0.0};
+  if (H.valid) {
+    double y[3];
+    spmv_node<T>(K.A, N, H, K.v[VSH], y);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long ix = c * N + H.g;
+      K.v[VT][ix] = y[c];
+      const double sv = K.v[VS][ix], rh = K.v[VRH][ix];
+      q[0] += y[c] * sv;
+      q[1] += y[c] * y[c];
+      q[2] += rh * sv;
+      q[3] += rh * y[c];
+    }
+  }
+  double o[4];
+  block_sums<4>(q, sh, o);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) K.part[(long long)(3 + k) * K.nblk + blockIdx.x] = o[k];
+  }
+}
+
+// Stop test on (r, r) (quantity 1); alpha = rho / (rhat, v) (quantity 2);
+// s = r - alpha v; shat = M^-1 s
 template <typename T, int PV>
-__device__ __forceinline__ void bicg_3_body(const KArgs<T> &K) {
+__device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
   KPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
   const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
   TileRows TR;
   if (PV == PV_TILE_REG) tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
-  const TileMem TM{K.T4, N, H.g};
   double r[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   if (H.valid) {
 #pragma unroll
@@ -452,8 +451,22 @@ __device__ __forceinline__ void bicg_3_body(const KArgs<T> &K) {
       v[c] = K.v[VV][c * N + H.g];
     }
   }
-  const double rv = device_sum(K.part, K.nblk, 2, I, sh);
-  const double alpha = S.rho2[K.parity] / rv;
+  double red[2];
+  device_sums<2>(K.part, K.nblk, 1, I, sh, red);
+  const double rr = red[0], rv = red[1];
+  const bool first = S.it == 0;
+  const double bb = first ? rr : S.bb;
+  const double rho = first ? rr : S.rho2[K.parity];
+  if (rr <= S.tol2 * bb || S.it >= S.maxit || !isfinite(rr) || rho == 0.0) {
+    if (lead) {
+      S.rr = rr;
+      if (first) S.bb = bb;
+      if (!(rr <= S.tol2 * bb)) S.status = 1;
+      S.done = 1;
+    }
+    return;
+  }
+  const double alpha = rho / rv;
   if (!isfinite(alpha) || rv == 0.0) {
     if (lead) {
       S.status = 1;
@@ -461,45 +474,107 @@ __device__ __forceinline__ void bicg_3_body(const KArgs<T> &K) {
     }
     return;
   }
-  double s[3] = {0, 0, 0}, z[3];
+  double s3[3] = {0, 0, 0}, z[3];
   if (H.valid) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      s[c] = r[c] - alpha * v[c];
-      K.v[VS][c * N + H.g] = s[c];
-    }
-    if (!use_tile) {
-      prec3<T>(K.Dinv, N, H.g, s, z);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) K.v[VSH][c * N + H.g] = z[c];
+      s3[c] = r[c] - alpha * v[c];
+      K.v[VS][c * N + H.g] = s3[c];
     }
   }
-  if (use_tile) {
-    __shared__ double tv[BS * 3];
-    if (PV == PV_TILE_REG) tile_apply(TR, H.valid, s, z, tv);
-    else tile_apply(TM, H.valid, s, z, tv);
-    if (H.valid) {
+  apply_prec<T, PV>(K, H, use_tile, TR, s3, z);
+  if (H.valid) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) K.v[VSH][c * N + H.g] = z[c];
+    for (int c = 0; c < 3; ++c) K.v[VSH][c * N + H.g] = z[c];
+  }
+  if (lead) {
+    S.alpha = alpha;
+    S.rr = rr;
+    if (first) {
+      S.bb = bb;
+      S.rho2[K.parity] = rho;
     }
   }
-  if (lead) S.alpha = alpha;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(BS) k_bicg_1_node(KArgs<T> K) { bicg_1_body<T, PV_NODE>(K); }
-template <typename T>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_bicg_1_treg(KArgs<T> K) { bicg_1_body<T, PV_TILE_REG>(K); }
-template <typename T>
-__global__ void __launch_bounds__(BS) k_bicg_1_tmem(KArgs<T> K) { bicg_1_body<T, PV_TILE_MEM>(K); }
-template <typename T>
-__global__ void __launch_bounds__(BS) k_bicg_3_node(KArgs<T> K) { bicg_3_body<T, PV_NODE>(K); }
-template <typename T>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 2)))
-k_bicg_3_treg(KArgs<T> K) { bicg_3_body<T, PV_TILE_REG>(K); }
-template <typename T>
-__global__ void __launch_bounds__(BS) k_bicg_3_tmem(KArgs<T> K) { bicg_3_body<T, PV_TILE_MEM>(K); }
+// omega = (t, s) / (t, t); x += alpha phat + omega shat; r = s - omega t;
+// the next rho = (rhat, s) - omega (rhat, t), beta, p = r + beta (p -
+// omega v), phat = M^-1 p: the end of one iteration fused with the start of
+// the next (one launch and one reduction point fewer than the textbook
+// order); partial (r, r) at quantity 1 for the next stop test.
+template <typename T, int PV>
+__device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
+  KPROLOGUE
+  const NodeHood H = hood(I, blockIdx.x);
+  const long long N = K.N;
+  const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
+  TileRows TR;
+  if (PV == PV_TILE_REG) tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
+  double x[3] = {0, 0, 0}, ph[3] = {0, 0, 0}, shh[3] = {0, 0, 0}, s[3] = {0, 0, 0},
+         t[3] = {0, 0, 0}, p[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+  if (H.valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long ix = c * N + H.g;
+      x[c] = K.v[VX][ix];
+      ph[c] = K.v[VPH][ix];
+      shh[c] = K.v[VSH][ix];
+      s[c] = K.v[VS][ix];
+      t[c] = K.v[VT][ix];
+      p[c] = K.v[VP][ix];
+      v[c] = K.v[VV][ix];
+    }
+  }
+  double red[4];
+  device_sums<4>(K.part, K.nblk, 3, I, sh, red);
+  const double ts = red[0], tt = red[1], rs = red[2], rt = red[3];
+  const double omega = (tt > 0.0) ? ts / tt : 0.0;
+  const double alpha = S.alpha;
+  // omega == 0: stagnation; the tiny value keeps beta finite and the next
+  // stop test decides between converged and failed
+  const double om_b = omega != 0.0 ? omega : 1e-300;
+  const double rho_new = rs - omega * rt;
+  const double beta = (rho_new / S.rho2[K.parity]) * (alpha / om_b);
+  double rr = 0.0, pn[3] = {0, 0, 0};
+  if (H.valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long ix = c * N + H.g;
+      K.v[VX][ix] = x[c] + alpha * ph[c] + omega * shh[c];
+      const double r = s[c] - omega * t[c];
+      K.v[VR][ix] = r;
+      rr += r * r;
+      pn[c] = r + beta * (p[c] - om_b * v[c]);
+      K.v[VP][ix] = pn[c];
+    }
+  }
+  double z[3];
+  apply_prec<T, PV>(K, H, use_tile, TR, pn, z);
+  if (H.valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
+  }
+  const double a = block_sum(rr, sh);
+  if (threadIdx.x == 0) K.part[(long long)K.nblk + blockIdx.x] = a;
+  if (lead) {
+    S.rho2[K.parity ^ 1] = rho_new;
+    S.omega = om_b;
+    S.it = S.it + 1;
+  }
+}
+
+#define TCAD_PV_KERNELS(name, body)                                                        \
+  template <typename T>                                                                    \
+  __global__ void __launch_bounds__(BS) name##_node(KArgs<T> K) { body<T, PV_NODE>(K); }   \
+  template <typename T>                                                                    \
+  __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 2)))         \
+  name##_treg(KArgs<T> K) { body<T, PV_TILE_REG>(K); }                                     \
+  template <typename T>                                                                    \
+  __global__ void __launch_bounds__(BS) name##_tmem(KArgs<T> K) { body<T, PV_TILE_MEM>(K); }
+
+TCAD_PV_KERNELS(k_bicg_init, bicg_init_body)
+TCAD_PV_KERNELS(k_bicg_s, bicg_s_body)
+TCAD_PV_KERNELS(k_bicg_u, bicg_u_body)
 
 // Variant of the preconditioner kernels for this batch: node-only when no
 // device applies tiles; else registers up to kTileRegBlocks workgroups
@@ -519,75 +594,21 @@ static int precond_variant(const Batch &b) {
   return b.nblk <= kTileRegBlocks ? PV_TILE_REG : PV_TILE_MEM;
 }
 
-template <typename T>
-static void launch_k1(Batch &b, const KArgs<T> &K) {
-  const dim3 g(b.nblk), t(BS);
-  switch (precond_variant(b)) {
-    case PV_NODE: hipLaunchKernelGGL(k_bicg_1_node<T>, g, t, 0, b.stream, K); break;
-    case PV_TILE_REG: hipLaunchKernelGGL(k_bicg_1_treg<T>, g, t, 0, b.stream, K); break;
-    default: hipLaunchKernelGGL(k_bicg_1_tmem<T>, g, t, 0, b.stream, K); break;
+#define TCAD_PV_LAUNCH(name)                                                               \
+  template <typename T>                                                                    \
+  static void launch_##name(Batch &b, const KArgs<T> &K) {                                 \
+    const dim3 g(b.nblk), t(BS);                                                           \
+    switch (precond_variant(b)) {                                                          \
+      case PV_NODE: hipLaunchKernelGGL(name##_node<T>, g, t, 0, b.stream, K); break;       \
+      case PV_TILE_REG: hipLaunchKernelGGL(name##_treg<T>, g, t, 0, b.stream, K); break;   \
+      default: hipLaunchKernelGGL(name##_tmem<T>, g, t, 0, b.stream, K); break;            \
+    }                                                                                      \
+    b.launches++;                                                                          \
   }
-}
 
-template <typename T>
-static void launch_k3(Batch &b, const KArgs<T> &K) {
-  const dim3 g(b.nblk), t(BS);
-  switch (precond_variant(b)) {
-    case PV_NODE: hipLaunchKernelGGL(k_bicg_3_node<T>, g, t, 0, b.stream, K); break;
-    case PV_TILE_REG: hipLaunchKernelGGL(k_bicg_3_treg<T>, g, t, 0, b.stream, K); break;
-    default: hipLaunchKernelGGL(k_bicg_3_tmem<T>, g, t, 0, b.stream, K); break;
-  }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(BS) k_bicg_5(KArgs<T> K) {
-  KPROLOGUE
-  const NodeHood H = hood(I, blockIdx.x);
-  const long long N = K.N;
-  double x[3] = {0, 0, 0}, ph[3] = {0, 0, 0}, sh3[3] = {0, 0, 0}, s[3] = {0, 0, 0},
-         t[3] = {0, 0, 0}, rh[3] = {0, 0, 0};
-  if (H.valid) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const long long ix = c * N + H.g;
-      x[c] = K.v[VX][ix];
-      ph[c] = K.v[VPH][ix];
-      sh3[c] = K.v[VSH][ix];
-      s[c] = K.v[VS][ix];
-      t[c] = K.v[VT][ix];
-      rh[c] = K.v[VRH][ix];
-    }
-  }
-  double red[2];
-  device_sums<2>(K.part, K.nblk, 2, I, sh, red);
-  const double ts = red[0], tt = red[1];
-  const double omega = (tt > 0.0) ? ts / tt : 0.0;
-  const double alpha = S.alpha;
-  double srh = 0.0, srr = 0.0;
-  if (H.valid) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const long long ix = c * N + H.g;
-      K.v[VX][ix] = x[c] + alpha * ph[c] + omega * sh3[c];
-      const double r = s[c] - omega * t[c];
-      K.v[VR][ix] = r;
-      srh += rh[c] * r;
-      srr += r * r;
-    }
-  }
-  const double a = block_sum(srh, sh);
-  const double b2 = block_sum(srr, sh);
-  if (threadIdx.x == 0) {
-    K.part[blockIdx.x] = a;
-    K.part[(long long)K.nblk + blockIdx.x] = b2;
-  }
-  if (lead) {
-    // omega == 0: stagnation; the tiny value keeps the next beta finite and
-    // the next stop test decides between converged and failed
-    S.omega = omega != 0.0 ? omega : 1e-300;
-    S.it = S.it + 1;
-  }
-}
+TCAD_PV_LAUNCH(k_bicg_init)
+TCAD_PV_LAUNCH(k_bicg_s)
+TCAD_PV_LAUNCH(k_bicg_u)
 
 // Reset the per-device Krylov state for a new solve; devices not active
 // are marked done.  per_dev: each device's own target S.eta2 (not below
@@ -646,32 +667,30 @@ template <typename T>
 static void launch_iteration(Batch &b, KArgs<T> K, int parity) {
   const dim3 g(b.nblk), t(BS);
   K.parity = parity;
-  launch_k1<T>(b, K);
   hipLaunchKernelGGL(k_bicg_spmv<T>, g, t, 0, b.stream, K, (int)VPH, (int)VV, (int)VRH, 0, 2);
-  launch_k3<T>(b, K);
-  hipLaunchKernelGGL(k_bicg_spmv<T>, g, t, 0, b.stream, K, (int)VSH, (int)VT, (int)VS, 1, 2);
-  hipLaunchKernelGGL(k_bicg_5<T>, g, t, 0, b.stream, K);
-  b.launches += 5;
+  launch_k_bicg_s<T>(b, K);
+  hipLaunchKernelGGL(k_bicg_spmv_t<T>, g, t, 0, b.stream, K);
+  launch_k_bicg_u<T>(b, K);
+  b.launches += 2;
 }
 
 // Host driver: reset, init, then chunks of iterations with one scalar
 // read-back per chunk until every device is done or maxit is reached
-// (one extra K1 then records the final stop test).
+// (one extra stop-test kernel then records the final state).
 template <typename T>
 static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int chunk,
                         bool per_dev = false, double floor = 0.0) {
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
                      b.ndev, rtol * rtol, maxit, per_dev ? 1 : 0, floor * floor);
-  hipLaunchKernelGGL(k_bicg_init<T>, dim3(b.nblk), dim3(BS), 0, b.stream, K);
-  b.launches += 2;
+  b.launches++;
+  launch_k_bicg_init<T>(b, K);
   int it = 0;
   while (true) {
     for (int c = 0; c < chunk && it < maxit; ++c, ++it) launch_iteration<T>(b, K, it & 1);
     if (it >= maxit) {
       KArgs<T> K1 = K;
       K1.parity = it & 1;
-      launch_k1<T>(b, K1);
-      b.launches++;
+      launch_k_bicg_s<T>(b, K1);
     }
     TCAD_HIP_CHECK(hipGetLastError());
     int rc = sync_scalars(b);
@@ -727,9 +746,9 @@ int launch_vector_kernel_only(Batch &b, int which) {
                      b.ndev);
   KArgs<double> K = make_kargs<double>(b, false);
   K.parity = 0;
-  if (which == 3) launch_k1<double>(b, K);
-  else launch_k3<double>(b, K);
-  b.launches += 2;
+  if (which == 3) launch_k_bicg_u<double>(b, K);
+  else launch_k_bicg_s<double>(b, K);
+  b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
   return TCAD_OK;
 }

@@ -163,7 +163,7 @@ struct Batch {
   // Krylov work vectors (FP64) and the refinement buffers
   double *v64[9] = {};
   double *ir_r = nullptr, *ir_rhs = nullptr;
-  double *part = nullptr;                        // nblk * 4 partial sums
+  double *part = nullptr;                        // nblk * 8 partial sums
   Scal *d_scal = nullptr;
   Scal *h_scal = nullptr;                        // pinned host mirror
   // equilibrium Poisson system
