@@ -90,7 +90,7 @@ struct CArgs {
 };
 
 #define CPROLOGUE                                   \
-  __shared__ double sh[64];                         \
+  __shared__ double sh[128];                        \
   const int d = dev_index(K.W);                     \
   const DevInfo &I = dev_info(K.W, d);              \
   ACScal &S = K.acs[d];                             \
@@ -247,9 +247,10 @@ __global__ void k_ac_reset(ACScal *acs, const int *act, int ndev, double tol2, d
   }
 }
 
-// r = b - A x (x = 0 when K.first), rhat = r, p = v = 0; partials
-// (rhat, r) = ||r||^2, ||r||^2 and ||b||^2 (quantity 3, read by the first
-// k_cb_1 of a solve: the reference of its relative residual)
+// r = b - A x (x = 0 when K.first), rhat = p = r, phat = M^-1 p; partials
+// ||r||^2 (quantity 0; rho_0 = (rhat, r) is the same sum) and ||b||^2
+// (quantity 1, read by the first k_cb_s of a solve: the reference of its
+// relative residual)
 __global__ void __launch_bounds__(BS) k_cb_init(CArgs K) {
   CPROLOGUE
   (void)lead;
@@ -277,9 +278,10 @@ __global__ void __launch_bounds__(BS) k_cb_init(CArgs K) {
     }
     cstore(K.v[CR], N, H.g, r);
     cstore(K.v[CRH], N, H.g, r);
-    const double z[6] = {0, 0, 0, 0, 0, 0};
-    cstore(K.v[CP], N, H.g, z);
-    cstore(K.v[CV], N, H.g, z);
+    cstore(K.v[CP], N, H.g, r);
+    double z[6];
+    cprec(K, H.g, r, z);
+    cstore(K.v[CPH], N, H.g, z);
 #pragma unroll
     for (int c = 0; c < 6; ++c) rr += r[c] * r[c];
 #pragma unroll
@@ -289,32 +291,91 @@ __global__ void __launch_bounds__(BS) k_cb_init(CArgs K) {
   block_sums<2>(in, sh, s);
   if (threadIdx.x == 0) {
     K.part[blockIdx.x] = s[0];
-    K.part[(long long)K.nblk + blockIdx.x] = 0.0;
-    K.part[2LL * K.nblk + blockIdx.x] = s[0];
-    K.part[3LL * K.nblk + blockIdx.x] = s[1];
+    K.part[(long long)K.nblk + blockIdx.x] = s[1];
   }
 }
 
-// reduce rho = (rhat, r), ||r||^2; stop test; p = r + beta (p - omega v);
-// phat = M^-1 p
-__global__ void __launch_bounds__(BS) k_cb_1(CArgs K) {
+// dst = (A + j w D) src; partial sum conj(a) dst at q0, q0 + 1
+__global__ void __launch_bounds__(BS) k_cb_spmv(CArgs K, int src, int dst, int dot_a, int q0) {
+  CPROLOGUE
+  (void)lead;
+  const NodeHood H = hood(I, blockIdx.x);
+  const long long N = K.N;
+  double q[2] = {0.0, 0.0};
+  if (H.valid) {
+    double y[6], a[6];
+    cspmv(K, H, K.omega[d], K.v[src], y);
+    cstore(K.v[dst], N, H.g, y);
+    cload(K.v[dot_a], N, H.g, a);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      q[0] += a[c] * y[c] + a[3 + c] * y[3 + c];
+      q[1] += a[c] * y[3 + c] - a[3 + c] * y[c];
+    }
+  }
+  double o[2];
+  block_sums<2>(q, sh, o);
+  if (threadIdx.x == 0) {
+    K.part[(long long)q0 * K.nblk + blockIdx.x] = o[0];
+    K.part[(long long)(q0 + 1) * K.nblk + blockIdx.x] = o[1];
+  }
+}
+
+// t = (A + j w D) shat; partials t^H s (1, 2), t^H t (3), rhat^H s (4, 5),
+// rhat^H t (6, 7): omega and the next rho = rhat^H (s - omega t) without
+// another reduction
+__global__ void __launch_bounds__(BS) k_cb_spmv_t(CArgs K) {
+  CPROLOGUE
+  (void)lead;
+  const NodeHood H = hood(I, blockIdx.x);
+  const long long N = K.N;
+  double q[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (H.valid) {
+    double t[6], s6[6], rh[6];
+    cspmv(K, H, K.omega[d], K.v[CSH], t);
+    cstore(K.v[CT], N, H.g, t);
+    cload(K.v[CS], N, H.g, s6);
+    cload(K.v[CRH], N, H.g, rh);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double tr = t[c], ti = t[3 + c], sr = s6[c], si = s6[3 + c];
+      const double hr = rh[c], hi = rh[3 + c];
+      q[0] += tr * sr + ti * si;
+      q[1] += tr * si - ti * sr;
+      q[2] += tr * tr + ti * ti;
+      q[3] += hr * sr + hi * si;
+      q[4] += hr * si - hi * sr;
+      q[5] += hr * tr + hi * ti;
+      q[6] += hr * ti - hi * tr;
+    }
+  }
+  double o[7];
+  block_sums<7>(q, sh, o);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) K.part[(long long)(1 + k) * K.nblk + blockIdx.x] = o[k];
+  }
+}
+
+// Stop test on ||r||^2 (quantity 0); alpha = rho / rhat^H v (2, 3);
+// s = r - alpha v; shat = M^-1 s
+__global__ void __launch_bounds__(BS) k_cb_s(CArgs K) {
   CPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
-  double r[6] = {0, 0, 0, 0, 0, 0}, p[6] = {0, 0, 0, 0, 0, 0}, v[6] = {0, 0, 0, 0, 0, 0};
+  double r[6] = {0, 0, 0, 0, 0, 0}, v[6] = {0, 0, 0, 0, 0, 0};
   if (H.valid) {
     cload(K.v[CR], N, H.g, r);
-    cload(K.v[CP], N, H.g, p);
     cload(K.v[CV], N, H.g, v);
   }
-  const bool first_it = S.it == 0;
   double red[4];
-  if (first_it) device_sums<4>(K.part, K.nblk, 0, I, sh, red);
-  else device_sums<3>(K.part, K.nblk, 0, I, sh, red);
-  const Cx rho = {red[0], red[1]};
-  const double rr = red[2];
+  device_sums<4>(K.part, K.nblk, 0, I, sh, red);
+  const bool first_it = S.it == 0;
+  const double rr = red[0];
   const double bb = first_it ? rr : S.bb;
-  const double b2 = (first_it && S.sweep == 0) ? red[3] : S.b2;
+  const double b2 = (first_it && S.sweep == 0) ? red[1] : S.b2;
+  const Cx rv = {red[2], red[3]};
+  const Cx rho = first_it ? Cx{rr, 0.0} : Cx{S.rho[K.parity][0], S.rho[K.parity][1]};
   const bool conv = rr <= S.tol2 * bb || rr <= S.abs2 * b2;
   if (conv || S.it >= S.maxit || !isfinite(rr) || (rho.re == 0.0 && rho.im == 0.0)) {
     if (lead) {
@@ -328,82 +389,6 @@ __global__ void __launch_bounds__(BS) k_cb_1(CArgs K) {
     }
     return;
   }
-  const Cx omega = {S.omega[0], S.omega[1]}, alpha = {S.alpha[0], S.alpha[1]};
-  const Cx rho_old = {S.rho[K.parity ^ 1][0], S.rho[K.parity ^ 1][1]};
-  const Cx beta = cmul(cdiv(rho, rho_old), cdiv(alpha, omega));
-  if (H.valid) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const Cx pv = {p[c] - (omega.re * v[c] - omega.im * v[3 + c]),
-                     p[3 + c] - (omega.re * v[3 + c] + omega.im * v[c])};
-      const Cx bp = cmul(beta, pv);
-      p[c] = r[c] + bp.re;
-      p[3 + c] = r[3 + c] + bp.im;
-    }
-    cstore(K.v[CP], N, H.g, p);
-    double z[6];
-    cprec(K, H.g, p, z);
-    cstore(K.v[CPH], N, H.g, z);
-  }
-  if (lead) {
-    S.rho[K.parity][0] = rho.re;
-    S.rho[K.parity][1] = rho.im;
-    S.rr = rr;
-    if (first_it) {
-      S.bb = bb;
-      S.b2 = b2;
-    }
-  }
-}
-
-// dst = (A + j w D) src; partial sum conj(a) dst (conj_dst = 0) or
-// conj(dst) a (conj_dst = 1) at q0, q0 + 1, and ||dst||^2 at q0 + 2
-__global__ void __launch_bounds__(BS) k_cb_spmv(CArgs K, int src, int dst, int dot_a,
-                                                int conj_dst, int want_sq, int q0) {
-  CPROLOGUE
-  (void)lead;
-  const NodeHood H = hood(I, blockIdx.x);
-  const long long N = K.N;
-  double sre = 0.0, sim = 0.0, sq = 0.0;
-  if (H.valid) {
-    double y[6], a[6];
-    cspmv(K, H, K.omega[d], K.v[src], y);
-    cstore(K.v[dst], N, H.g, y);
-    cload(K.v[dot_a], N, H.g, a);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      sre += a[c] * y[c] + a[3 + c] * y[3 + c];
-      const double im = a[c] * y[3 + c] - a[3 + c] * y[c];
-      sim += conj_dst ? -im : im;
-      sq += y[c] * y[c] + y[3 + c] * y[3 + c];
-    }
-  }
-  const double a0 = block_sum(sre, sh);
-  const double a1 = block_sum(sim, sh);
-  if (threadIdx.x == 0) {
-    K.part[(long long)q0 * K.nblk + blockIdx.x] = a0;
-    K.part[(long long)(q0 + 1) * K.nblk + blockIdx.x] = a1;
-  }
-  if (want_sq) {
-    const double a2 = block_sum(sq, sh);
-    if (threadIdx.x == 0) K.part[(long long)(q0 + 2) * K.nblk + blockIdx.x] = a2;
-  }
-}
-
-// alpha = rho / (rhat, v); s = r - alpha v; shat = M^-1 s
-__global__ void __launch_bounds__(BS) k_cb_3(CArgs K) {
-  CPROLOGUE
-  const NodeHood H = hood(I, blockIdx.x);
-  const long long N = K.N;
-  double r[6] = {0, 0, 0, 0, 0, 0}, v[6] = {0, 0, 0, 0, 0, 0};
-  if (H.valid) {
-    cload(K.v[CR], N, H.g, r);
-    cload(K.v[CV], N, H.g, v);
-  }
-  double red[2];
-  device_sums<2>(K.part, K.nblk, 3, I, sh, red);
-  const Cx rv = {red[0], red[1]};
-  const Cx rho = {S.rho[K.parity][0], S.rho[K.parity][1]};
   const Cx alpha = cdiv(rho, rv);
   if ((rv.re == 0.0 && rv.im == 0.0) || !isfinite(alpha.re) || !isfinite(alpha.im)) {
     if (lead) {
@@ -413,79 +398,96 @@ __global__ void __launch_bounds__(BS) k_cb_3(CArgs K) {
     return;
   }
   if (H.valid) {
-    double s[6], z[6];
+    double s6[6], z[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      s[c] = r[c] - (alpha.re * v[c] - alpha.im * v[3 + c]);
-      s[3 + c] = r[3 + c] - (alpha.re * v[3 + c] + alpha.im * v[c]);
+      s6[c] = r[c] - (alpha.re * v[c] - alpha.im * v[3 + c]);
+      s6[3 + c] = r[3 + c] - (alpha.re * v[3 + c] + alpha.im * v[c]);
     }
-    cstore(K.v[CS], N, H.g, s);
-    cprec(K, H.g, s, z);
+    cstore(K.v[CS], N, H.g, s6);
+    cprec(K, H.g, s6, z);
     cstore(K.v[CSH], N, H.g, z);
   }
   if (lead) {
     S.alpha[0] = alpha.re;
     S.alpha[1] = alpha.im;
+    S.rr = rr;
+    if (first_it) {
+      S.bb = bb;
+      S.b2 = b2;
+      S.rho[K.parity][0] = rho.re;
+      S.rho[K.parity][1] = rho.im;
+    }
   }
 }
 
-// omega = (t, s) / (t, t); x += alpha phat + omega shat; r = s - omega t;
-// partials (rhat, r), ||r||^2
-__global__ void __launch_bounds__(BS) k_cb_5(CArgs K) {
+// omega = t^H s / t^H t; x += alpha phat + omega shat; r = s - omega t;
+// rho' = rhat^H s - omega rhat^H t; p = r + beta (p - omega v);
+// phat = M^-1 p (this iteration's end fused with the next one's start);
+// partial ||r||^2 at quantity 0
+__global__ void __launch_bounds__(BS) k_cb_u(CArgs K) {
   CPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
   double x[6] = {0, 0, 0, 0, 0, 0}, ph[6] = {0, 0, 0, 0, 0, 0}, shh[6] = {0, 0, 0, 0, 0, 0},
-         s[6] = {0, 0, 0, 0, 0, 0}, t[6] = {0, 0, 0, 0, 0, 0}, rh[6] = {0, 0, 0, 0, 0, 0};
+         s6[6] = {0, 0, 0, 0, 0, 0}, t[6] = {0, 0, 0, 0, 0, 0}, p[6] = {0, 0, 0, 0, 0, 0},
+         v[6] = {0, 0, 0, 0, 0, 0};
   if (H.valid) {
     cload(K.v[CX], N, H.g, x);
     cload(K.v[CPH], N, H.g, ph);
     cload(K.v[CSH], N, H.g, shh);
-    cload(K.v[CS], N, H.g, s);
+    cload(K.v[CS], N, H.g, s6);
     cload(K.v[CT], N, H.g, t);
-    cload(K.v[CRH], N, H.g, rh);
+    cload(K.v[CP], N, H.g, p);
+    cload(K.v[CV], N, H.g, v);
   }
-  double red[3];
-  device_sums<3>(K.part, K.nblk, 5, I, sh, red);
+  double red[7];
+  device_sums<7>(K.part, K.nblk, 1, I, sh, red);
   const Cx ts = {red[0], red[1]};
   const double tt = red[2];
+  const Cx rs = {red[3], red[4]}, rt = {red[5], red[6]};
   const Cx omega = tt > 0.0 ? Cx{ts.re / tt, ts.im / tt} : Cx{0.0, 0.0};
+  // omega == 0: stagnation; a tiny value keeps beta finite and the next
+  // stop test (or the next sweep) decides
+  const bool zero = omega.re == 0.0 && omega.im == 0.0;
+  const Cx om_b = zero ? Cx{1e-300, 0.0} : omega;
   const Cx alpha = {S.alpha[0], S.alpha[1]};
-  double srh = 0.0, srhi = 0.0, srr = 0.0;
+  const Cx rho_new = csub(rs, cmul(omega, rt));
+  const Cx rho_old = {S.rho[K.parity][0], S.rho[K.parity][1]};
+  const Cx beta = cmul(cdiv(rho_new, rho_old), cdiv(alpha, om_b));
+  double rr = 0.0;
   if (H.valid) {
+    double pn[6], z[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double xr = x[c] + (alpha.re * ph[c] - alpha.im * ph[3 + c]) +
-                        (omega.re * shh[c] - omega.im * shh[3 + c]);
-      const double xi = x[3 + c] + (alpha.re * ph[3 + c] + alpha.im * ph[c]) +
-                        (omega.re * shh[3 + c] + omega.im * shh[c]);
-      x[c] = xr;
-      x[3 + c] = xi;
-      const double rr_ = s[c] - (omega.re * t[c] - omega.im * t[3 + c]);
-      const double ri_ = s[3 + c] - (omega.re * t[3 + c] + omega.im * t[c]);
-      s[c] = rr_;
-      s[3 + c] = ri_;
-      srh += rh[c] * rr_ + rh[3 + c] * ri_;
-      srhi += rh[c] * ri_ - rh[3 + c] * rr_;
-      srr += rr_ * rr_ + ri_ * ri_;
+      x[c] = x[c] + (alpha.re * ph[c] - alpha.im * ph[3 + c]) +
+             (omega.re * shh[c] - omega.im * shh[3 + c]);
+      x[3 + c] = x[3 + c] + (alpha.re * ph[3 + c] + alpha.im * ph[c]) +
+                 (omega.re * shh[3 + c] + omega.im * shh[c]);
+      const double rre = s6[c] - (omega.re * t[c] - omega.im * t[3 + c]);
+      const double rim = s6[3 + c] - (omega.re * t[3 + c] + omega.im * t[c]);
+      s6[c] = rre;
+      s6[3 + c] = rim;
+      rr += rre * rre + rim * rim;
+      const Cx pv = {p[c] - (om_b.re * v[c] - om_b.im * v[3 + c]),
+                     p[3 + c] - (om_b.re * v[3 + c] + om_b.im * v[c])};
+      const Cx bp = cmul(beta, pv);
+      pn[c] = rre + bp.re;
+      pn[3 + c] = rim + bp.im;
     }
     cstore(K.v[CX], N, H.g, x);
-    cstore(K.v[CR], N, H.g, s);
+    cstore(K.v[CR], N, H.g, s6);
+    cstore(K.v[CP], N, H.g, pn);
+    cprec(K, H.g, pn, z);
+    cstore(K.v[CPH], N, H.g, z);
   }
-  const double a0 = block_sum(srh, sh);
-  const double a1 = block_sum(srhi, sh);
-  const double a2 = block_sum(srr, sh);
-  if (threadIdx.x == 0) {
-    K.part[blockIdx.x] = a0;
-    K.part[(long long)K.nblk + blockIdx.x] = a1;
-    K.part[2LL * K.nblk + blockIdx.x] = a2;
-  }
+  const double a = block_sum(rr, sh);
+  if (threadIdx.x == 0) K.part[blockIdx.x] = a;
   if (lead) {
-    // omega == 0: stagnation; a tiny value keeps the next beta finite and
-    // the next stop test (or the next sweep) decides
-    const bool zero = omega.re == 0.0 && omega.im == 0.0;
-    S.omega[0] = zero ? 1e-300 : omega.re;
-    S.omega[1] = zero ? 0.0 : omega.im;
+    S.rho[K.parity ^ 1][0] = rho_new.re;
+    S.rho[K.parity ^ 1][1] = rho_new.im;
+    S.omega[0] = om_b.re;
+    S.omega[1] = om_b.im;
     S.it = S.it + 1;
   }
 }
@@ -631,12 +633,11 @@ CArgs make_cargs(Batch &b, const AcWork &w) {
 void launch_citeration(Batch &b, CArgs K, int parity) {
   const dim3 g(b.nblk), t(BS);
   K.parity = parity;
-  hipLaunchKernelGGL(k_cb_1, g, t, 0, b.stream, K);
-  hipLaunchKernelGGL(k_cb_spmv, g, t, 0, b.stream, K, (int)CPH, (int)CV, (int)CRH, 0, 0, 3);
-  hipLaunchKernelGGL(k_cb_3, g, t, 0, b.stream, K);
-  hipLaunchKernelGGL(k_cb_spmv, g, t, 0, b.stream, K, (int)CSH, (int)CT, (int)CS, 1, 1, 5);
-  hipLaunchKernelGGL(k_cb_5, g, t, 0, b.stream, K);
-  b.launches += 5;
+  hipLaunchKernelGGL(k_cb_spmv, g, t, 0, b.stream, K, (int)CPH, (int)CV, (int)CRH, 2);
+  hipLaunchKernelGGL(k_cb_s, g, t, 0, b.stream, K);
+  hipLaunchKernelGGL(k_cb_spmv_t, g, t, 0, b.stream, K);
+  hipLaunchKernelGGL(k_cb_u, g, t, 0, b.stream, K);
+  b.launches += 4;
 }
 
 // Refinement sweeps of the complex solve of the current right-hand side
@@ -674,7 +675,7 @@ int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int ma
       if (it >= maxit) {
         CArgs K1 = K;
         K1.parity = it & 1;
-        hipLaunchKernelGGL(k_cb_1, dim3(b.nblk), dim3(BS), 0, b.stream, K1);
+        hipLaunchKernelGGL(k_cb_s, dim3(b.nblk), dim3(BS), 0, b.stream, K1);
         b.launches++;
       }
       TCAD_HIP_CHECK(hipGetLastError());

@@ -992,7 +992,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   TRY(dalloc(&b->d_scal, (size_t)ndev));
   TRY(dalloc(&b->Aeq, 7 * N));
   TRY(dalloc(&b->beq, N));
-  for (int i = 0; i < 5; ++i) TRY(dalloc(&b->veq[i], N));
+  for (int i = 0; i < 6; ++i) TRY(dalloc(&b->veq[i], N));
   TRY(dalloc(&b->d_flag, (size_t)ndev + 1));
   TRY(dalloc(&b->d_mask, (size_t)ndev));
   TRY(dalloc(&b->d_dts, (size_t)ndev));

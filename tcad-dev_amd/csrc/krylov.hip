@@ -31,7 +31,8 @@
 //   arithmetic -- FP32 vector arithmetic was measured unstable on these
 //   Jacobians, residual growth by 1e20+ in the second sweep).
 // * PCG (Jacobi) for the SPD equilibrium Poisson system, FP64, same
-//   reduction scheme (three kernels per iteration).
+//   reduction scheme (two kernels per iteration: the direction update is
+//   recomputed at the neighbours inside the product kernel).
 //
 // Replaces linsolve.solve_linear's node-block-Jacobi Krylov path
 // (pytcad/pytcad/linsolve.py:363-427, 725-985).
@@ -937,6 +938,7 @@ struct PArgs {
   const double *A;
   const double *b;
   double *x, *r, *z, *p, *q;
+  double *p2;                       // second buffer of p (by iteration parity)
   double *part;
   Scal *scal;
   int parity;
@@ -982,13 +984,27 @@ __global__ void __launch_bounds__(BS) k_pcg_init(PArgs P) {
   }
 }
 
-__global__ void __launch_bounds__(BS) k_pcg_a(PArgs P) {
+// Reduce (r, z), (r, r) -> stop test, beta; p = z + beta p_old at this
+// node and, on the fly, at its neighbours (p is double-buffered by
+// parity, so no workgroup overwrites a value another still reads);
+// q = A p; partial (p, q).  The direction update and the product share a
+// kernel: two launches per iteration.
+__global__ void __launch_bounds__(BS) k_pcg_ab(PArgs P) {
   PPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
-  double z = 0.0, pv = 0.0;
+  const long long N = P.N;
+  const double *pin = P.parity ? P.p2 : P.p;
+  double *pout = P.parity ? P.p : P.p2;
+  double zc = 0.0, pc = 0.0, zn[6] = {0, 0, 0, 0, 0, 0}, pn[6] = {0, 0, 0, 0, 0, 0};
   if (H.valid) {
-    z = P.z[H.g];
-    pv = P.p[H.g];
+    zc = P.z[H.g];
+    pc = pin[H.g];
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      if (!H.nb[dir]) continue;
+      zn[dir] = P.z[H.nbg[dir]];
+      pn[dir] = pin[H.nbg[dir]];
+    }
   }
   double red[2];
   device_sums<2>(P.part, P.nblk, 0, I, sh, red);
@@ -1004,7 +1020,19 @@ __global__ void __launch_bounds__(BS) k_pcg_a(PArgs P) {
     return;
   }
   const double beta = rz / S.rho2[P.parity ^ 1];
-  if (H.valid) P.p[H.g] = z + beta * pv;
+  double pq = 0.0;
+  if (H.valid) {
+    const double p0 = zc + beta * pc;
+    double q = P.A[H.g] * p0;
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir)
+      if (H.nb[dir]) q += P.A[(1 + dir) * N + H.g] * (zn[dir] + beta * pn[dir]);
+    pout[H.g] = p0;
+    P.q[H.g] = q;
+    pq = p0 * q;
+  }
+  const double a = block_sum(pq, sh);
+  if (threadIdx.x == 0) P.part[2 * (long long)P.nblk + blockIdx.x] = a;
   if (lead) {
     S.rho2[P.parity] = rz;
     S.rr = rr;
@@ -1012,48 +1040,37 @@ __global__ void __launch_bounds__(BS) k_pcg_a(PArgs P) {
   }
 }
 
-__global__ void __launch_bounds__(BS) k_pcg_b(PArgs P) {
-  PPROLOGUE
-  (void)lead;
-  const NodeHood H = hood(I, blockIdx.x);
-  double pq = 0.0;
-  if (H.valid) {
-    const double qv = pspmv(P.A, P.N, H, P.p);
-    P.q[H.g] = qv;
-    pq = P.p[H.g] * qv;
-  }
-  const double a = block_sum(pq, sh);
-  if (threadIdx.x == 0) P.part[2 * (long long)P.nblk + blockIdx.x] = a;
-}
-
+// Reduce (p, q) -> alpha; x += alpha p; r -= alpha q; z = r / diag;
+// partials (r, z), (r, r)
 __global__ void __launch_bounds__(BS) k_pcg_c(PArgs P) {
   PPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
+  const double *pcur = P.parity ? P.p : P.p2;
   double x = 0, pv = 0, r = 0, q = 0, Ad = 1;
   if (H.valid) {
     x = P.x[H.g];
-    pv = P.p[H.g];
+    pv = pcur[H.g];
     r = P.r[H.g];
     q = P.q[H.g];
     Ad = P.A[H.g];
   }
   const double pq = device_sum(P.part, P.nblk, 2, I, sh);
   const double alpha = S.rho2[P.parity] / pq;
-  double rz = 0.0, rr = 0.0;
+  double in[2] = {0.0, 0.0};
   if (H.valid) {
     P.x[H.g] = x + alpha * pv;
     const double rn = r - alpha * q;
     P.r[H.g] = rn;
     const double zv = rn / Ad;
     P.z[H.g] = zv;
-    rz = rn * zv;
-    rr = rn * rn;
+    in[0] = rn * zv;
+    in[1] = rn * rn;
   }
-  const double a = block_sum(rz, sh);
-  const double c = block_sum(rr, sh);
+  double o[2];
+  block_sums<2>(in, sh, o);
   if (threadIdx.x == 0) {
-    P.part[blockIdx.x] = a;
-    P.part[(long long)P.nblk + blockIdx.x] = c;
+    P.part[blockIdx.x] = o[0];
+    P.part[(long long)P.nblk + blockIdx.x] = o[1];
   }
   if (lead) S.it = S.it + 1;
 }
@@ -1064,6 +1081,7 @@ int pcg_poisson(Batch &b, double rtol, int maxit) {
   P.N = b.ntot; P.nblk = b.nblk;
   P.A = b.Aeq; P.b = b.beq;
   P.x = b.veq[0]; P.r = b.veq[1]; P.z = b.veq[2]; P.p = b.veq[3]; P.q = b.veq[4];
+  P.p2 = b.veq[5];
   P.part = b.part; P.scal = b.d_scal;
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
                      b.ndev, rtol * rtol, maxit, 0, 0.0);
@@ -1074,14 +1092,13 @@ int pcg_poisson(Batch &b, double rtol, int maxit) {
   while (true) {
     for (int c = 0; c < chunk && it < maxit; ++c, ++it) {
       P.parity = it & 1;
-      hipLaunchKernelGGL(k_pcg_a, g, t, 0, b.stream, P);
-      hipLaunchKernelGGL(k_pcg_b, g, t, 0, b.stream, P);
+      hipLaunchKernelGGL(k_pcg_ab, g, t, 0, b.stream, P);
       hipLaunchKernelGGL(k_pcg_c, g, t, 0, b.stream, P);
-      b.launches += 3;
+      b.launches += 2;
     }
     if (it >= maxit) {
       P.parity = it & 1;
-      hipLaunchKernelGGL(k_pcg_a, g, t, 0, b.stream, P);
+      hipLaunchKernelGGL(k_pcg_ab, g, t, 0, b.stream, P);
       b.launches++;
     }
     TCAD_HIP_CHECK(hipGetLastError());

@@ -43,7 +43,7 @@ from .sweep import SweepResult, Track, record_currents, solve_round
 # library workspace per node of a batch (upper bound over 2D/3D): the
 # stencil-block matrix in FP64 and FP32 (39 entries in 3D), its block
 # inverse in both, 15 three-plane FP64 vectors, the equilibrium system
-WORKSPACE_BYTES_PER_NODE = 39 * 12 + 9 * 12 + 15 * 3 * 8 + 8 + 7 * 8 + 8 + 5 * 8
+WORKSPACE_BYTES_PER_NODE = 39 * 12 + 9 * 12 + 15 * 3 * 8 + 8 + 7 * 8 + 8 + 6 * 8
 # pool planes per node: 15 static fields + psi, n, p (FP64)
 POOL_BYTES_PER_NODE = 18 * 8
 
