@@ -87,7 +87,7 @@ def measured_traffic(kernel):
     return None
 
 
-def kernel_timing(dev, which, reps=200):
+def kernel_timing(dev, which, reps=1000):
     import ctypes
     from tcadhip import _lib
     out = ctypes.c_double()
@@ -267,6 +267,27 @@ def bench_ac(dev, spec, freqs=tuple(np.logspace(6, 12, 7)), cpu_freqs=2):
             "max_err_vs_oracle_rel_to_max_Y": err}
 
 
+def copy_ceiling(mib=1024, reps=10):
+    """On-box bandwidth ceiling: a plain device-to-device copy of `mib` MiB,
+    bytes read + written per second (GB/s), timed with HIP events."""
+    import torch
+    n = mib << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    a.fill_(1)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    gbps = 2 * n * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return round(gbps, 1)
+
+
 def cpu_baseline(spec, npoints, budget_s=30.0):
     """The FP64 oracle (NumPy + SciPy SuperLU, single thread) on the
     first `npoints` bias points of the same sweep, equilibrium included."""
@@ -410,7 +431,8 @@ def main():
                      "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 5),
                      "traffic": measured_traffic("k_bicg_spmv<double>") if w == 8 else None,
-                     "bytes_per_launch": spmv_bytes, "avg_launch_us": round(us, 3)},
+                     "bytes_per_launch": spmv_bytes, "avg_launch_us": round(us, 3),
+                     "copy_ceiling_GBps": copy_ceiling()},
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["points"] / cpu["seconds"], 5), "unit": "bias points/s",
             "cores": 1, "kind": "port",

@@ -278,12 +278,14 @@ int tcad_edge_currents(tcad_batch *b, const double *d_psi, const double *d_n,
 int tcad_surface_mobility(tcad_batch *b, const double *d_psi);
 
 /* Bench instrumentation: average duration [microseconds] of `reps`
- * back-to-back launches of one kernel on the last assembled system,
- * timed with HIP events on the library stream.  which: 0 FP64 stencil
- * SpMV, 1 FP32 stencil SpMV, 2 coupled assembly (FP32 copy on), 3 and 4
- * the FP64 BiCGStab vector kernels that apply the preconditioner (each
- * launch preceded by a one-thread state reset, included in the time).
- * Has no reference counterpart. */
+ * back-to-back launches of one kernel on the last assembled system, timed
+ * with HIP events on the library stream after an untimed burst of the
+ * same length (as inside the solver, each launch waits for the previous
+ * one; the time per launch includes that dependency).  which: 0 FP64 stencil SpMV,
+ * 1 FP32 stencil SpMV, 2 coupled assembly (FP32 copy on), 3 the fused
+ * BiCGStab update kernel and 4 the alpha / s kernel, both FP64 and
+ * applying the preconditioner (each launch preceded by a one-thread state
+ * reset, included in the time).  Has no reference counterpart. */
 int tcad_time_kernel(tcad_batch *b, int32_t which, int32_t reps, double *h_avg_us);
 
 /* Wall-clock-free counters of the last call: kernel launches issued. */

@@ -764,7 +764,7 @@ static void free_batch(Batch *b) {
                   b->d_cscale, b->d_c2dev, b->d_gptr, b->d_gnode, b->d_surf[0], b->d_surf[1], b->d_surf[2],
                   b->d_surf[3], b->A64, b->A32, b->Dinv64, b->Dinv32, b->bs, b->rowscale,
                   b->colscale, b->y64, b->F, b->Fnp, b->part, b->d_scal, b->Aeq, b->beq,
-                  b->d_flag, b->d_mask, b->cand, b->d_Icand, b->T4, b->d_dts};
+                  b->d_flag, b->d_mask, b->cand, b->d_Icand, b->T4, b->T4s, b->d_dts};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (int i = 0; i < 9; ++i)
@@ -997,6 +997,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   TRY(dalloc(&b->d_mask, (size_t)ndev));
   TRY(dalloc(&b->d_dts, (size_t)ndev));
   TRY(dalloc(&b->T4, 36 * N));
+  TRY(dalloc(&b->T4s, 3 * N));
   TRY(dalloc(&b->cand, 3 * N));
   TRY(dalloc(&b->d_Icand, (size_t)std::max(1, cglobal)));
 #undef TRY
@@ -1300,7 +1301,10 @@ int tcad_time_kernel(tcad_batch *bp, int32_t which, int32_t reps, double *h_avg_
     if (which >= 3) return tcad::launch_vector_kernel_only(b, which);
     return tcad::launch_spmv_only(b, which == 1);
   };
-  if ((rc = launch())) return rc;
+  // an untimed burst first (clocks and caches settle), then the timed one:
+  // back-to-back dependent launches, as inside the solver
+  for (int r = 0; r < reps; ++r)
+    if ((rc = launch())) return rc;
   hipEvent_t e0, e1;
   TCAD_HIP_CHECK(hipEventCreate(&e0));
   TCAD_HIP_CHECK(hipEventCreate(&e1));

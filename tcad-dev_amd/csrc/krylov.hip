@@ -59,7 +59,8 @@ struct KArgs {
   int nblk;
   const T *A;                       // matrix storage precision
   const T *Dinv;                    // preconditioner storage precision
-  const float4 *T4;                 // 4x4-tile inverses (FP32), tile != 0
+  const tile4 *T4;                  // 4x4-tile inverses (FP16 rows), tile != 0
+  const float *T4s;                 // their per-row scales
   int tile;                         // 1: tile block-Jacobi, 0: node block-Jacobi
   double *v[9];                     // Krylov vectors, always FP64
   const double *b;
@@ -106,9 +107,11 @@ __device__ __forceinline__ void prec3(const T *Di, long long N, long long g, con
 // exactly.  3D devices keep the node block-Jacobi: on the 64x48x48
 // tri-gate, x-y tiles cut BiCGStab iterations only 2700 -> 1800 per point
 // (the z couplings stay outside the tiles) for twice the bytes per
-// application, 6.7 vs 9.6 points/s.  The inverse rows of node k are 36 float4 planes
-// T4[(r * 12 + q) * N + g] (row r of the node, columns 4q..4q+3 of the
-// tile, tile unknown u = 3 l + c with l = 4 (j % 4) + i % 4).  Measured
+// application, 6.7 vs 9.6 points/s.  The inverse rows of node k are 36
+// planes of four FP16 entries, T4[(r * 12 + q) * N + g] (row r of the
+// node, columns 4q..4q+3 of the tile, tile unknown u = 3 l + c with
+// l = 4 (j % 4) + i % 4), each row scaled by its largest magnitude, kept
+// in T4s[r * N + g] (288 + 12 B per node instead of 576 in FP32).  Measured
 // BiCGStab iterations to 1e-10 on the equilibrated systems: 200x120 MOSFET
 // 573 -> 270, a 159x100 reverse-biased 1e19 diode 564 -> 196 (node
 // block-Jacobi -> tiles).
@@ -133,25 +136,33 @@ __device__ __forceinline__ void tile_slot(int &tile, int &l) {
 // to the compiler after the barriers, one kernel issued them one at a
 // time, waiting for each).
 struct TileRows {
-  float4 t[3 * (TU / 4)];
+  tile4 t[3 * (TU / 4)];
+  float sc[3];
 };
 
-__device__ __forceinline__ void tile_load(const float4 *T4, long long N, long long g, bool load,
-                                          TileRows &R) {
+__device__ __forceinline__ void tile_load(const tile4 *T4, const float *T4s, long long N,
+                                          long long g, bool load, TileRows &R) {
   if (!load) return;
 #pragma unroll
   for (int e = 0; e < 3 * (TU / 4); ++e) R.t[e] = T4[(long long)e * N + g];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) R.sc[r] = T4s[(long long)r * N + g];
 }
 
 // The same rows read from memory at the point of use.
 struct TileMem {
-  const float4 *T4;
+  const tile4 *T4;
+  const float *T4s;
   long long N, g;
 };
 
-__device__ __forceinline__ float4 tile_row(const TileRows &R, int e) { return R.t[e]; }
-__device__ __forceinline__ float4 tile_row(const TileMem &M, int e) {
+__device__ __forceinline__ tile4 tile_row(const TileRows &R, int e) { return R.t[e]; }
+__device__ __forceinline__ tile4 tile_row(const TileMem &M, int e) {
   return M.T4[(long long)e * M.N + M.g];
+}
+__device__ __forceinline__ float tile_scale(const TileRows &R, int r) { return R.sc[r]; }
+__device__ __forceinline__ float tile_scale(const TileMem &M, int r) {
+  return M.T4s[(long long)r * M.N + M.g];
 }
 
 // z = M^-1 x for this thread's node; every thread of the workgroup calls it
@@ -172,11 +183,11 @@ __device__ __forceinline__ void tile_apply(const Rows &R, bool valid, const doub
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < TU / 4; ++q) {
-      const float4 t = tile_row(R, r * (TU / 4) + q);
+      const tile4 t = tile_row(R, r * (TU / 4) + q);
       acc += (double)t.x * v[4 * q] + (double)t.y * v[4 * q + 1] + (double)t.z * v[4 * q + 2] +
              (double)t.w * v[4 * q + 3];
     }
-    z[r] = acc;
+    z[r] = acc * (double)tile_scale(R, r);
   }
 }
 
@@ -184,7 +195,8 @@ __device__ __forceinline__ void tile_apply(const Rows &R, bool valid, const doub
 // tiles side by side, 16 x 4 nodes per tile row); block Gauss-Jordan in FP64 with the 3x3 node blocks as pivots
 // (no row exchange, like the node block-Jacobi inverse), in place in LDS.
 __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const double *A,
-                                                   const Scal *scal, float4 *T4, int *flag) {
+                                                   const Scal *scal, tile4 *T4, float *T4s,
+                                                   int *flag) {
   __shared__ double M[4 * 16 * TNS];
   __shared__ double Pv[4 * 9];
   const int patch = blockIdx.x / QPP, quarter = blockIdx.x % QPP;
@@ -289,12 +301,24 @@ __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const d
     __syncthreads();
     if (valid) {
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r) {
+        // FP16 entries scaled by the row's largest magnitude (entries below
+        // ~6e-8 of it, far under the preconditioner's accuracy, flush)
+        double amax = 0.0;
+        for (int e = 0; e < TU; ++e) amax = fmax(amax, fabs(Mr[r * TU + e]));
+        const float sc = amax > 0.0 && isfinite(amax) ? (float)amax : 1.0f;
+        const double inv = 1.0 / (double)sc;
+        T4s[(long long)r * N + g] = sc;
         for (int q = 0; q < TU / 4; ++q) {
           const double *m = Mr + r * TU + 4 * q;
-          T4[(long long)(r * (TU / 4) + q) * N + g] =
-              make_float4((float)m[0], (float)m[1], (float)m[2], (float)m[3]);
+          tile4 h;
+          h.x = (_Float16)(m[0] * inv);
+          h.y = (_Float16)(m[1] * inv);
+          h.z = (_Float16)(m[2] * inv);
+          h.w = (_Float16)(m[3] * inv);
+          T4[(long long)(r * (TU / 4) + q) * N + g] = h;
         }
+      }
     }
     if (bad) {                          // error flags only; never part of a sum
       atomicOr(flag, 1);
@@ -333,7 +357,7 @@ __device__ __forceinline__ void apply_prec(const KArgs<T> &K, const NodeHood &H,
   if (PV == PV_TILE_REG) {
     tile_apply(TR, H.valid, x, z, tv);
   } else {
-    const TileMem TM{K.T4, K.N, H.g};
+    const TileMem TM{K.T4, K.T4s, K.N, H.g};
     tile_apply(TM, H.valid, x, z, tv);
   }
 }
@@ -348,7 +372,7 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
   const long long N = K.N;
   const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
   TileRows TR;
-  if (PV == PV_TILE_REG) tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
+  if (PV == PV_TILE_REG) tile_load(K.T4, K.T4s, N, H.g, use_tile && H.valid, TR);
   const double sc = K.scale_b ? 1.0 / S.nrm : 1.0;
   double b3[3] = {0, 0, 0}, rr = 0.0;
   if (H.valid) {
@@ -443,7 +467,7 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
   const long long N = K.N;
   const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
   TileRows TR;
-  if (PV == PV_TILE_REG) tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
+  if (PV == PV_TILE_REG) tile_load(K.T4, K.T4s, N, H.g, use_tile && H.valid, TR);
   double r[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   if (H.valid) {
 #pragma unroll
@@ -510,7 +534,7 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
   const long long N = K.N;
   const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
   TileRows TR;
-  if (PV == PV_TILE_REG) tile_load(K.T4, N, H.g, use_tile && H.valid, TR);
+  if (PV == PV_TILE_REG) tile_load(K.T4, K.T4s, N, H.g, use_tile && H.valid, TR);
   double x[3] = {0, 0, 0}, ph[3] = {0, 0, 0}, shh[3] = {0, 0, 0}, s[3] = {0, 0, 0},
          t[3] = {0, 0, 0}, p[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   if (H.valid) {
@@ -650,6 +674,7 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
   K.A = fp32 ? (const T *)b.A32 : (const T *)b.A64;
   K.Dinv = fp32 ? (const T *)b.Dinv32 : (const T *)b.Dinv64;
   K.T4 = b.T4;
+  K.T4s = b.T4s;
   K.tile = b.tile;
   K.b = b.ir_r;
   K.scale_b = 1;
@@ -658,7 +683,7 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
 
 int launch_tile_setup(Batch &b, bool only_active) {
   hipLaunchKernelGGL(k_tile_setup, dim3(QPP * b.nblk), dim3(64), 0, b.stream, make_where(b), b.ntot,
-                     b.A64, only_active ? b.d_scal : nullptr, b.T4, b.d_flag);
+                     b.A64, only_active ? b.d_scal : nullptr, b.T4, b.T4s, b.d_flag);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
   return TCAD_OK;

@@ -47,6 +47,10 @@ constexpr int NOFF = 5;
 
 inline int nentries(int dim) { return NDIAG + NOFF * 2 * dim; }
 
+// Four stored entries of a 4x4-tile inverse row, FP16 (each row of a node
+// carries its own FP32 scale).
+typedef _Float16 tile4 __attribute__((ext_vector_type(4)));
+
 // Per-device metadata, device resident.
 struct DevInfo {
   int dim, nx, ny, nz;
@@ -149,7 +153,8 @@ struct Batch {
   float *A32 = nullptr;
   double *Dinv64 = nullptr;
   float *Dinv32 = nullptr;
-  float4 *T4 = nullptr;                          // 4x4-tile inverses, 36 planes
+  tile4 *T4 = nullptr;                           // 4x4-tile inverses, 36 planes (FP16)
+  float *T4s = nullptr;                          // their per-row scales, 3 planes
   int tile = 1;                                  // preconditioner of the next solves
   double *bs = nullptr;                          // scaled rhs
   double *rowscale = nullptr;
