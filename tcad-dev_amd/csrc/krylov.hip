@@ -178,17 +178,21 @@ __device__ __forceinline__ void tile_apply(const Rows &R, bool valid, const doub
   for (int c = 0; c < 3; ++c) v[3 * l + c] = valid ? x[c] : 0.0;
   __syncthreads();
   if (!valid) return;
+  // each tile value is read from LDS once and used for all three rows (the
+  // LDS traffic of the row-by-row order was 3x, and bound the kernels on
+  // large batches); every row still sums in the same order
+  double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    double acc = 0.0;
+  for (int q = 0; q < TU / 4; ++q) {
+    const double v0 = v[4 * q], v1 = v[4 * q + 1], v2 = v[4 * q + 2], v3 = v[4 * q + 3];
 #pragma unroll
-    for (int q = 0; q < TU / 4; ++q) {
+    for (int r = 0; r < 3; ++r) {
       const tile4 t = tile_row(R, r * (TU / 4) + q);
-      acc += (double)t.x * v[4 * q] + (double)t.y * v[4 * q + 1] + (double)t.z * v[4 * q + 2] +
-             (double)t.w * v[4 * q + 3];
+      acc[r] += (double)t.x * v0 + (double)t.y * v1 + (double)t.z * v2 + (double)t.w * v3;
     }
-    z[r] = acc * (double)tile_scale(R, r);
   }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) z[r] = acc[r] * (double)tile_scale(R, r);
 }
 
 // Setup: one workgroup of 64 threads per 16-column quarter of a patch (four
