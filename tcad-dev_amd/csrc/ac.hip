@@ -198,11 +198,12 @@ __global__ void __launch_bounds__(BS) k_ac_prec(CArgs K, double *Mi, int *flag) 
   }
 }
 
-// Port k's forcing on every node of every device: 1 on the nodes of an
-// ohmic port, -kappa w on the nodes of a gate port, 0 elsewhere (and on
-// every node of a device with fewer than k + 1 ports).
-__global__ void __launch_bounds__(BS) k_ac_forcing(Where W, const int *cidx,
-                                                   const int *cn_contact, const int *gidx,
+// Port k's forcing on every node of every device: -kappa w on the nodes of
+// a gate port, 0 elsewhere; an ohmic port's 1 on its nodes is scattered
+// by k_ac_contact_forcing afterwards, from the contact's own node list
+// (contacts may share nodes, as a tri-gate's body and source do, and the
+// reference drives every node of the port's list, ac2d.py:296-304).
+__global__ void __launch_bounds__(BS) k_ac_forcing(Where W, const int *gidx,
                                                    const int *gn_gate, const double *gn_kw,
                                                    int k, double *acg) {
   const int d = dev_index(W);
@@ -211,14 +212,21 @@ __global__ void __launch_bounds__(BS) k_ac_forcing(Where W, const int *cidx,
   if (!H.valid) return;
   const long long g = H.g;
   double v = 0.0;
-  if (k < I.nc) {
-    const int ct = cidx[g];
-    if (ct >= 0 && cn_contact[ct] == I.c0 + k) v = 1.0;
-  } else if (k < I.nc + I.ng) {
+  if (k >= I.nc && k < I.nc + I.ng) {
     const int gt = gidx[g];
     if (gt >= 0 && gn_gate[gt] == I.g0 + (k - I.nc)) v = -gn_kw[gt];
   }
   acg[g] = v;
+}
+
+// One workgroup per device: 1 on every node of its ohmic port k's list.
+__global__ void __launch_bounds__(BS) k_ac_contact_forcing(const DevInfo *info, const int *cptr,
+                                                           const int *cnodes, int k,
+                                                           double *acg) {
+  const DevInfo &I = info[blockIdx.x];
+  if (k >= I.nc) return;
+  const int c = I.c0 + k;
+  for (int t = cptr[c] + threadIdx.x; t < cptr[c + 1]; t += blockDim.x) acg[cnodes[t]] = 1.0;
 }
 
 __global__ void k_ac_reset(ACScal *acs, const int *act, int ndev, double tol2, double abs2,
@@ -778,9 +786,12 @@ int y_parameters_impl(Batch &b, const int32_t *h_active, int nfreq, const double
   CArgs K = make_cargs(b, w);
   for (int k = 0; k < P; ++k) {
     hipLaunchKernelGGL(k_ac_forcing, dim3(b.nblk), dim3(BS), 0, b.stream, K.W,
-                       (const int *)b.d_cidx, (const int *)b.d_cn_contact, (const int *)b.d_gidx,
-                       (const int *)b.d_gn_gate, (const double *)b.d_gn_kw, k, w.acg);
-    b.launches++;
+                       (const int *)b.d_gidx, (const int *)b.d_gn_gate,
+                       (const double *)b.d_gn_kw, k, w.acg);
+    hipLaunchKernelGGL(k_ac_contact_forcing, dim3(b.ndev), dim3(BS), 0, b.stream,
+                       (const DevInfo *)b.d_info, (const int *)b.d_cptr, (const int *)b.d_cnodes,
+                       k, w.acg);
+    b.launches += 2;
     AcAssembly rhs{};
     rhs.mode = 2;
     rhs.acg = w.acg;

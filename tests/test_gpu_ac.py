@@ -197,3 +197,35 @@ def test_cutoff_frequency_matches_oracle():
     assert (fg is None) == (fo is None)
     if fg is not None:
         assert abs(fg - fo) <= 1e-5 * fo
+
+
+def test_ac_3d_trigate_with_gates_vs_oracle():
+    """A small tri-gate (top and side gates, 12x10x8): the 3D gate ports'
+    displacement rows and the ohmic rows against the oracle's refined 3D Y
+    at the same bias."""
+    print("[ac] small 3D tri-gate with gate ports vs oracle", flush=True)
+    ac, NewtonOptions, device_from_spec = _mods()
+    from ac_floor import observation_floor
+    from oracle import ac as oac
+    from oracle import physics as oph
+    from oracle_build import oracle_device
+    from tcadhip import configs
+    spec = configs.trigate_3d(nx=12, ny=10, nz=8, vg_list=[0.0, 0.25, 0.5], Vd=0.05,
+                              name="ac_trigate_small")
+    gpu = device_from_spec(spec)
+    gpu.solve_equilibrium(NewtonOptions(max_iter=400))
+    ref = oracle_device(spec)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ref.solve_equilibrium(oph.NewtonOptions(max_iter=400))
+        for bias in spec.sweep:
+            ref.solve_bias(bias, oph.NewtonOptions())
+    for bias in spec.sweep:
+        gpu.solve_bias(bias, NewtonOptions())
+    f = [1e9, 1e11]
+    res = ac.y_parameters(gpu, f)
+    ores = oac.y_parameters(ref, f, refine=True)
+    assert list(res.port_names) == list(ores.port_names)
+    assert sum(p.startswith("gate") for p in res.port_names) == 3
+    worst, wf = _check_close(res.Y, ores.Y, observation_floor(ref, f), "3d tri-gate")
+    print(f"  ports {list(res.port_names)}, worst error / tolerance {worst:.2e}", flush=True)

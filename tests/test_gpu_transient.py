@@ -110,3 +110,46 @@ def test_transient_vs_reference(name):
     assert np.abs(res.psi_hist[-1] - d["psi_final"]).max() <= 1e-8
     for got, want in ((res.n_hist[-1], d["n_final"]), (res.p_hist[-1], d["p_final"])):
         assert (np.abs(got - want) / np.maximum(want, 1e-10)).max() <= 1e-6
+
+
+def test_transient_3d_vs_oracle():
+    """A z-invariant 3D diode stepped 0 -> 0.3 V (backward Euler, adaptive
+    dt) on the GPU and in the oracle's 3D restatement (transient3d.py, with
+    its exit when the line search rejects every step): the same time grid,
+    Newton counts within +-1 of the oracle with the equilibrated direct
+    solve, currents within the bars above."""
+    print("[transient] 3D diode step", flush=True)
+    tr, NewtonOptions, _ = _mods()
+    import tcadhip
+    from oracle import dd
+    from oracle import physics as oph
+    from oracle import transient as otr
+    x = np.linspace(0, 1e-4, 14)
+    y = np.linspace(0, 0.3e-4, 6)
+    z = np.linspace(0, 0.2e-4, 5)
+    dop = np.tile(np.where(x < 0.45e-4, -2e17, 5e17), (5, 6, 1))
+    J, K = np.meshgrid(np.arange(6), np.arange(5), indexing="ij")
+    flags = dict(doping_mobility=True, srh=True, auger=True, bgn=True)
+    ref = dd.Device3D(x, y, z, dop, models=oph.Models(**flags))
+    gpu = tcadhip.Device3D(tcadhip.Mesh3D(x, y, z), dop, models=tcadhip.Models(**flags))
+    for dv in (ref, gpu):
+        dv.add_contact("a", 0, J.ravel(), K.ravel())
+        dv.add_contact("c", 13, J.ravel(), K.ravel())
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ref.solve_equilibrium(oph.NewtonOptions())
+        rres = otr.solve_transient(ref, {"a": otr.StepWaveform(0.0, 0.3)}, 5e-11, 1e-12,
+                                   opts=oph.NewtonOptions(), equilibrated=True)
+    gpu.solve_equilibrium(NewtonOptions())
+    gres = tr.solve_transient(gpu, {"a": tr.StepWaveform(0.0, 0.3)}, 5e-11, 1e-12,
+                              opts=NewtonOptions(), keep_history=False)
+    assert np.array_equal(gres.times, rres.times), (gres.times, rres.times)
+    assert np.all(np.abs(gres.iterations - rres.iterations) <= 1), (gres.iterations,
+                                                                     rres.iterations)
+    cur = np.array([gres.terminal_current[c] for c in ("a", "c")]).T
+    ref_cur = np.array([rres.terminal_current[c] for c in ("a", "c")]).T
+    big = np.abs(ref_cur).max()
+    for s in range(ref_cur.shape[0]):
+        tol = 1e-6 * np.abs(ref_cur[s]).max() + 1e-9 * big
+        assert np.abs(cur[s] - ref_cur[s]).max() <= tol, (s, cur[s], ref_cur[s])
+    print(f"  steps {len(gres.times) - 1}", flush=True)
