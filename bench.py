@@ -228,8 +228,9 @@ def bench_serve(count=30, rate=2.0, check=3, lanes=2):
 
 
 def bench_ac(dev, spec, freqs=tuple(np.logspace(6, 12, 7)), cpu_freqs=2):
-    """Small-signal Y(f) of the MOSFET at its last sweep point (Vg = 1.0 V,
-    Vd = 0.05 V): 4 ports x len(freqs) complex solves on the GPU, f_T;
+    """Small-signal Y(f) of the MOSFET at the last point of its sweep
+    (Vg = 1.0 V, Vd = 0.05 V for the full 21-point sweep): 4 ports x
+    len(freqs) complex solves on the GPU, f_T;
     the FP64 oracle (SciPy SuperLU per frequency) on the first `cpu_freqs`
     frequencies from the same operating point, for time and agreement."""
     import torch
@@ -256,7 +257,7 @@ def bench_ac(dev, spec, freqs=tuple(np.logspace(6, 12, 7)), cpu_freqs=2):
     dt_cpu = time.perf_counter() - t1
     err = max(float(np.abs(res.Y[f] - ores.Y[f]).max() / np.abs(ores.Y[f]).max())
               for f in range(cpu_freqs))
-    return {"operating_point": {"gate": 1.0, "drain": 0.05}, "ports": list(res.port_names),
+    return {"operating_point": dict(spec.sweep[-1]), "ports": list(res.port_names),
             "frequencies": len(freqs), "seconds": round(dt, 3),
             "frequencies_per_s": round(len(freqs) / dt, 3),
             "krylov_iterations": res.iterations,

@@ -17,8 +17,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.environ.get("TCADHIP_OUT") or os.path.join(HERE, "tcadhip", "libtcadhip.so")
-BS = os.environ.get("TCADHIP_BS")
-OBJ = os.path.join(HERE, "_build" + (f"_bs{BS}" if BS else ""))
+BS = os.environ.get("TCADHIP_BS")          # node workgroup size (default 128)
+PY = os.environ.get("TCADHIP_PY")          # patch height (default 4)
+TY = os.environ.get("TCADHIP_TY")          # preconditioner tile height (default 4)
+OBJ = os.path.join(HERE, "_build" + (f"_bs{BS}" if BS else "") + (f"_py{PY}" if PY else "")
+                   + (f"_ty{TY}" if TY else ""))
 SOURCES = ["assemble.hip", "krylov.hip", "batch.hip", "ac.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -26,6 +29,10 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
          "-I", os.path.join(REPO, "include"), "-I", CSRC, "-Wall", "-Wno-unused-result"]
 if BS:
     FLAGS.append(f"-DTCAD_BS={int(BS)}")
+if PY:
+    FLAGS.append(f"-DTCAD_PY={int(PY)}")
+if TY:
+    FLAGS.append(f"-DTCAD_TY={int(TY)}")
 
 
 def _compile(src):

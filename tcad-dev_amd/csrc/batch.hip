@@ -996,7 +996,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   TRY(dalloc(&b->d_flag, (size_t)ndev + 1));
   TRY(dalloc(&b->d_mask, (size_t)ndev));
   TRY(dalloc(&b->d_dts, (size_t)ndev));
-  TRY(dalloc(&b->T4, 36 * N));
+  TRY(dalloc(&b->T4, (size_t)TILE_PLANES * N));
   TRY(dalloc(&b->T4s, 3 * N));
   TRY(dalloc(&b->cand, 3 * N));
   TRY(dalloc(&b->d_Icand, (size_t)std::max(1, cglobal)));

@@ -101,42 +101,44 @@ __device__ __forceinline__ void prec3(const T *Di, long long N, long long g, con
 }
 
 // ---------------------------------------------------------------------------
-// 4x4-tile block-Jacobi (2D devices).  Every 4 x 4 node tile (48 unknowns)
-// is one diagonal block of the preconditioner: the scaled Newton matrix
-// restricted to the tile (couplings leaving the tile dropped), inverted
-// exactly.  3D devices keep the node block-Jacobi: on the 64x48x48
-// tri-gate, x-y tiles cut BiCGStab iterations only 2700 -> 1800 per point
-// (the z couplings stay outside the tiles) for twice the bytes per
-// application, 6.7 vs 9.6 points/s.  The inverse rows of node k are 36
-// planes of four FP16 entries, T4[(r * 12 + q) * N + g] (row r of the
-// node, columns 4q..4q+3 of the tile, tile unknown u = 3 l + c with
-// l = 4 (j % 4) + i % 4), each row scaled by its largest magnitude, kept
-// in T4s[r * N + g] (288 + 12 B per node instead of 576 in FP32).  Measured
-// BiCGStab iterations to 1e-10 on the equilibrated systems: 200x120 MOSFET
-// 573 -> 270, a 159x100 reverse-biased 1e19 diode 564 -> 196 (node
-// block-Jacobi -> tiles).
+// Tile block-Jacobi (2D devices).  Every TILE_X x TILE_Y node tile is one
+// diagonal block of the preconditioner: the scaled Newton matrix restricted
+// to the tile (couplings leaving the tile dropped), inverted exactly.  3D
+// devices keep the node block-Jacobi: on the 64x48x48 tri-gate, x-y tiles
+// cut BiCGStab iterations only 2700 -> 1800 per point (the z couplings stay
+// outside the tiles), and with their bytes per application the sweep ran
+// 8.0 instead of 10.0 points/s.  The inverse rows of node k are
+// TILE_PLANES planes of four FP16 entries, T4[(r * TILE_U / 4 + q) * N + g]
+// (row r of the node, columns 4q..4q+3 of the tile, tile unknown
+// u = 3 l + c with l = TILE_X (j % TILE_Y) + i % TILE_X), each row scaled
+// by its largest magnitude, kept in T4s[r * N + g].  Measured BiCGStab
+// iterations to 1e-10 on the equilibrated systems of the 200x120 MOSFET
+// (Vg = 0.5 V): node block-Jacobi ~630, 4 x 4 tiles ~255, 4 x 8 tiles
+// ~185 (the surface mesh is finest in y, so y-long tiles hold the strong
+// couplings).
 // ---------------------------------------------------------------------------
-constexpr int TILE = 4;
-constexpr int TU = 3 * TILE * TILE;         // 48 unknowns per tile
+constexpr int TU = TILE_U;
 constexpr int TNS = 3 * TU + 1;             // LDS doubles per node row set (+1: banks)
-static_assert(PY % TILE == 0 && PX % TILE == 0, "patches must hold whole tiles");
-constexpr int QPP = PX / 16;                // tile-setup quarters (16 columns) per patch
-static_assert(PX % 16 == 0, "tile setup works on 16-column quarters");
+// tile setup: a 64-thread workgroup inverts STPW tiles side by side
+constexpr int STPW = 64 / TILE_NODES;
+constexpr int SCOLS = STPW * TILE_X;        // node columns of one setup workgroup
+static_assert(64 % TILE_NODES == 0 && PX % SCOLS == 0, "setup groups must tile the patch");
+constexpr int SPP = (PX / SCOLS) * (PY / TILE_Y);   // setup workgroups per patch
 
 // tile slot of this thread inside its patch, and node index inside the tile
 __device__ __forceinline__ void tile_slot(int &tile, int &l) {
   const int il = threadIdx.x % PX, jl = threadIdx.x / PX;
-  tile = (jl / TILE) * (PX / TILE) + il / TILE;
-  l = (jl % TILE) * TILE + il % TILE;
+  tile = (jl / TILE_Y) * (PX / TILE_X) + il / TILE_X;
+  l = (jl % TILE_Y) * TILE_X + il % TILE_X;
 }
 
-// The 144 inverse entries of this thread's node rows, loaded into registers
-// at the top of a kernel, before its reductions: they depend on nothing the
-// kernel computes, so their memory latency overlaps the reductions' (left
-// to the compiler after the barriers, one kernel issued them one at a
-// time, waiting for each).
+// The inverse rows of this thread's node, loaded into registers at the top
+// of a kernel, before its reductions: they depend on nothing the kernel
+// computes, so their memory latency overlaps the reductions' (left to the
+// compiler after the barriers, one kernel issued them one at a time,
+// waiting for each).
 struct TileRows {
-  tile4 t[3 * (TU / 4)];
+  tile4 t[TILE_PLANES];
   float sc[3];
 };
 
@@ -144,7 +146,7 @@ __device__ __forceinline__ void tile_load(const tile4 *T4, const float *T4s, lon
                                           long long g, bool load, TileRows &R) {
   if (!load) return;
 #pragma unroll
-  for (int e = 0; e < 3 * (TU / 4); ++e) R.t[e] = T4[(long long)e * N + g];
+  for (int e = 0; e < TILE_PLANES; ++e) R.t[e] = T4[(long long)e * N + g];
 #pragma unroll
   for (int r = 0; r < 3; ++r) R.sc[r] = T4s[(long long)r * N + g];
 }
@@ -195,15 +197,18 @@ __device__ __forceinline__ void tile_apply(const Rows &R, bool valid, const doub
   for (int r = 0; r < 3; ++r) z[r] = acc[r] * (double)tile_scale(R, r);
 }
 
-// Setup: one workgroup of 64 threads per 16-column quarter of a patch (four
-// tiles side by side, 16 x 4 nodes per tile row); block Gauss-Jordan in FP64 with the 3x3 node blocks as pivots
-// (no row exchange, like the node block-Jacobi inverse), in place in LDS.
+// Setup: one workgroup of 64 threads per STPW tiles side by side (SCOLS x
+// TILE_Y nodes), one thread per node holding its 3 rows of the tile matrix
+// in LDS; block Gauss-Jordan in FP64 with the 3x3 node blocks as pivots (no
+// row exchange, like the node block-Jacobi inverse), in place.  Each pivot
+// step: every thread inverts the pivot block (broadcast LDS reads), the
+// tile's threads scale the pivot rows column by column in parallel, then
+// every other row eliminates the pivot columns.
 __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const double *A,
                                                    const Scal *scal, tile4 *T4, float *T4s,
                                                    int *flag) {
-  __shared__ double M[4 * 16 * TNS];
-  __shared__ double Pv[4 * 9];
-  const int patch = blockIdx.x / QPP, quarter = blockIdx.x % QPP;
+  __shared__ double M[64 * TNS];
+  const int patch = blockIdx.x / SPP, sub = blockIdx.x % SPP;
   const int d = W.single ? 0 : W.blk2dev[patch];
   if (scal && !scal[d].active) return;
   const DevInfo &I = dev_info(W, d);
@@ -211,123 +216,122 @@ __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const d
   const unsigned local = (unsigned)(patch - I.blk0);
   const unsigned bx = local % (unsigned)I.px, t2 = local / (unsigned)I.px;
   const unsigned by = t2 % (unsigned)I.py, bz = t2 / (unsigned)I.py;
-  const int lc = threadIdx.x % 16, lr = threadIdx.x / 16;          // 16 x 4 quarter
-  const int i = (int)bx * PX + quarter * 16 + lc;
-  const int jq = (int)by * PY;                                      // PY rows per patch
-  const int tq = lc / TILE;                                         // tile in quarter
-  const int l = lr * TILE + lc % TILE;
-  // the quarter covers PY rows as PY / 4 tile rows; handle each tile row
-  for (int trow = 0; trow < PY / TILE; ++trow) {
-    const int j = jq + trow * TILE + lr;
-    const bool valid = i < I.nx && j < I.ny;
-    const long long g = I.off + ((long long)bz * I.ny + j) * I.nx + i;
-    double *Mr = M + (tq * 16 + l) * TNS;                           // this node's 3 rows
+  const int lc = threadIdx.x % SCOLS, lr = threadIdx.x / SCOLS;
+  const int tq = lc / TILE_X;                                       // tile in the group
+  const int l = lr * TILE_X + lc % TILE_X;                          // node in the tile
+  const int i = (int)bx * PX + (sub % (PX / SCOLS)) * SCOLS + lc;
+  const int j = (int)by * PY + (sub / (PX / SCOLS)) * TILE_Y + lr;
+  const bool valid = i < I.nx && j < I.ny;
+  const long long g = I.off + ((long long)bz * I.ny + j) * I.nx + i;
+  double *Mt = M + tq * TILE_NODES * TNS;                           // this tile's rows
+  double *Mr = Mt + l * TNS;                                        // this node's 3 rows
+  for (int e = 0; e < 3 * TU; ++e) Mr[e] = 0.0;
+  if (valid) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Mr[r * TU + 3 * l + c] = A[(3 * r + c) * N + g];
+    const int il = lc % TILE_X, jl = lr;
+    const bool has[4] = {il > 0 && i > 0, il < TILE_X - 1 && i < I.nx - 1,
+                         jl > 0 && j > 0, jl < TILE_Y - 1 && j < I.ny - 1};
+    const int ln[4] = {l - 1, l + 1, l - TILE_X, l + TILE_X};
+#pragma unroll
+    for (int dir = 0; dir < 4; ++dir) {
+      if (!has[dir]) continue;
+      const double *o = A + (long long)(9 + 5 * dir) * N + g;
+      const int cb = 3 * ln[dir];
+      Mr[0 * TU + cb] = o[0];
+      Mr[1 * TU + cb] = o[N];
+      Mr[1 * TU + cb + 1] = o[2 * N];
+      Mr[2 * TU + cb] = o[3 * N];
+      Mr[2 * TU + cb + 2] = o[4 * N];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Mr[c * TU + 3 * l + c] = 1.0;
+  }
+  bool bad = false;
+  for (int sp = 0; sp < TILE_NODES; ++sp) {
+    const int pc = 3 * sp;
+    double *Ms = Mt + sp * TNS;                                     // pivot rows
     __syncthreads();
-    for (int e = 0; e < 3 * TU; ++e) Mr[e] = 0.0;
-    if (valid) {
+    const double a0 = Ms[pc], a1 = Ms[pc + 1], a2 = Ms[pc + 2];
+    const double a3 = Ms[TU + pc], a4 = Ms[TU + pc + 1], a5 = Ms[TU + pc + 2];
+    const double a6 = Ms[2 * TU + pc], a7 = Ms[2 * TU + pc + 1], a8 = Ms[2 * TU + pc + 2];
+    const double c0 = a4 * a8 - a5 * a7, c1 = a5 * a6 - a3 * a8, c2 = a3 * a7 - a4 * a6;
+    const double det = a0 * c0 + a1 * c1 + a2 * c2;
+    if (l == sp && (!(fabs(det) > 1e-300) || !isfinite(det))) bad = true;
+    const double id = 1.0 / det;
+    const double P[9] = {c0 * id, (a2 * a7 - a1 * a8) * id, (a1 * a5 - a2 * a4) * id,
+                         c1 * id, (a0 * a8 - a2 * a6) * id, (a2 * a3 - a0 * a5) * id,
+                         c2 * id, (a1 * a6 - a0 * a7) * id, (a0 * a4 - a1 * a3) * id};
+    double F[9];                                // this row's pivot-column block
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[3 * r + c] = Mr[r * TU + pc + c];
+    __syncthreads();                            // pivot block read by all
+    // pivot rows := P * pivot rows (pivot block := P), columns split over
+    // the tile's threads
+#pragma unroll 2
+    for (int col = l; col < TU; col += TILE_NODES) {
+      if (col / 3 == sp) {
+        const int c = col - pc;
+        Ms[col] = P[c];
+        Ms[TU + col] = P[3 + c];
+        Ms[2 * TU + col] = P[6 + c];
+      } else {
+        const double m0 = Ms[col], m1 = Ms[TU + col], m2 = Ms[2 * TU + col];
+        Ms[col] = P[0] * m0 + P[1] * m1 + P[2] * m2;
+        Ms[TU + col] = P[3] * m0 + P[4] * m1 + P[5] * m2;
+        Ms[2 * TU + col] = P[6] * m0 + P[7] * m1 + P[8] * m2;
+      }
+    }
+    __syncthreads();
+    if (l != sp) {
+      // row -= F * pivot row over every column outside the pivot block;
+      // the pivot block becomes -F P
+#pragma unroll 8
+      for (int col = 0; col < TU; ++col) {
+        if (col / 3 == sp) continue;
+        const double s0 = Ms[col], s1 = Ms[TU + col], s2 = Ms[2 * TU + col];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          Mr[r * TU + col] -= F[3 * r] * s0 + F[3 * r + 1] * s1 + F[3 * r + 2] * s2;
+      }
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) Mr[r * TU + 3 * l + c] = A[(3 * r + c) * N + g];
-      const int il = lc % TILE, jl = lr;
-      const bool has[4] = {il > 0 && i > 0, il < TILE - 1 && i < I.nx - 1,
-                           jl > 0 && j > 0, jl < TILE - 1 && j < I.ny - 1};
-      const int ln[4] = {l - 1, l + 1, l - TILE, l + TILE};
-#pragma unroll
-      for (int dir = 0; dir < 4; ++dir) {
-        if (!has[dir]) continue;
-        const double *o = A + (long long)(9 + 5 * dir) * N + g;
-        const int cb = 3 * ln[dir];
-        Mr[0 * TU + cb] = o[0];
-        Mr[1 * TU + cb] = o[N];
-        Mr[1 * TU + cb + 1] = o[2 * N];
-        Mr[2 * TU + cb] = o[3 * N];
-        Mr[2 * TU + cb + 2] = o[4 * N];
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) Mr[c * TU + 3 * l + c] = 1.0;
+        for (int c = 0; c < 3; ++c)
+          Mr[r * TU + pc + c] = -(F[3 * r] * P[c] + F[3 * r + 1] * P[3 + c] +
+                                  F[3 * r + 2] * P[6 + c]);
     }
-    bool bad = false;
-    for (int sp = 0; sp < 16; ++sp) {
-      __syncthreads();
-      const int pc = 3 * sp;
-      if (l == sp) {
-        const double a0 = Mr[pc], a1 = Mr[pc + 1], a2 = Mr[pc + 2];
-        const double a3 = Mr[TU + pc], a4 = Mr[TU + pc + 1], a5 = Mr[TU + pc + 2];
-        const double a6 = Mr[2 * TU + pc], a7 = Mr[2 * TU + pc + 1], a8 = Mr[2 * TU + pc + 2];
-        const double c0 = a4 * a8 - a5 * a7, c1 = a5 * a6 - a3 * a8, c2 = a3 * a7 - a4 * a6;
-        const double det = a0 * c0 + a1 * c1 + a2 * c2;
-        if (!(fabs(det) > 1e-300) || !isfinite(det)) bad = true;
-        const double id = 1.0 / det;
-        const double P[9] = {c0 * id, (a2 * a7 - a1 * a8) * id, (a1 * a5 - a2 * a4) * id,
-                             c1 * id, (a0 * a8 - a2 * a6) * id, (a2 * a3 - a0 * a5) * id,
-                             c2 * id, (a1 * a6 - a0 * a7) * id, (a0 * a4 - a1 * a3) * id};
-        for (int col = 0; col < TU; ++col) {
-          if (col / 3 == sp) continue;
-          const double m0 = Mr[col], m1 = Mr[TU + col], m2 = Mr[2 * TU + col];
-          Mr[col] = P[0] * m0 + P[1] * m1 + P[2] * m2;
-          Mr[TU + col] = P[3] * m0 + P[4] * m1 + P[5] * m2;
-          Mr[2 * TU + col] = P[6] * m0 + P[7] * m1 + P[8] * m2;
-        }
+  }
+  __syncthreads();
+  if (valid) {
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            Mr[r * TU + pc + c] = P[3 * r + c];
-            Pv[tq * 9 + 3 * r + c] = P[3 * r + c];
-          }
-      }
-      __syncthreads();
-      if (l != sp) {
-        const double *Ms = M + (tq * 16 + sp) * TNS;                // pivot rows (updated)
-        double F[9];
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) F[3 * r + c] = Mr[r * TU + pc + c];
-        for (int col = 0; col < TU; ++col) {
-          if (col / 3 == sp) continue;
-          const double s0 = Ms[col], s1 = Ms[TU + col], s2 = Ms[2 * TU + col];
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-            Mr[r * TU + col] -= F[3 * r] * s0 + F[3 * r + 1] * s1 + F[3 * r + 2] * s2;
-        }
-        const double *P = Pv + tq * 9;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            Mr[r * TU + pc + c] = -(F[3 * r] * P[c] + F[3 * r + 1] * P[3 + c] +
-                                    F[3 * r + 2] * P[6 + c]);
+    for (int r = 0; r < 3; ++r) {
+      // FP16 entries scaled by the row's largest magnitude (entries below
+      // ~6e-8 of it, far under the preconditioner's accuracy, flush)
+      double amax = 0.0;
+      for (int e = 0; e < TU; ++e) amax = fmax(amax, fabs(Mr[r * TU + e]));
+      const float sc = amax > 0.0 && isfinite(amax) ? (float)amax : 1.0f;
+      const double inv = 1.0 / (double)sc;
+      T4s[(long long)r * N + g] = sc;
+      for (int q = 0; q < TU / 4; ++q) {
+        const double *m = Mr + r * TU + 4 * q;
+        tile4 h;
+        h.x = (_Float16)(m[0] * inv);
+        h.y = (_Float16)(m[1] * inv);
+        h.z = (_Float16)(m[2] * inv);
+        h.w = (_Float16)(m[3] * inv);
+        T4[(long long)(r * (TU / 4) + q) * N + g] = h;
       }
     }
-    __syncthreads();
-    if (valid) {
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        // FP16 entries scaled by the row's largest magnitude (entries below
-        // ~6e-8 of it, far under the preconditioner's accuracy, flush)
-        double amax = 0.0;
-        for (int e = 0; e < TU; ++e) amax = fmax(amax, fabs(Mr[r * TU + e]));
-        const float sc = amax > 0.0 && isfinite(amax) ? (float)amax : 1.0f;
-        const double inv = 1.0 / (double)sc;
-        T4s[(long long)r * N + g] = sc;
-        for (int q = 0; q < TU / 4; ++q) {
-          const double *m = Mr + r * TU + 4 * q;
-          tile4 h;
-          h.x = (_Float16)(m[0] * inv);
-          h.y = (_Float16)(m[1] * inv);
-          h.z = (_Float16)(m[2] * inv);
-          h.w = (_Float16)(m[3] * inv);
-          T4[(long long)(r * (TU / 4) + q) * N + g] = h;
-        }
-      }
-    }
-    if (bad) {                          // error flags only; never part of a sum
-      atomicOr(flag, 1);
-      atomicOr(flag + 1 + d, 1);
-    }
+  }
+  if (bad) {                              // error flags only; never part of a sum
+    atomicOr(flag, 1);
+    atomicOr(flag + 1 + d, 1);
   }
 }
 
@@ -686,7 +690,7 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
 }
 
 int launch_tile_setup(Batch &b, bool only_active) {
-  hipLaunchKernelGGL(k_tile_setup, dim3(QPP * b.nblk), dim3(64), 0, b.stream, make_where(b), b.ntot,
+  hipLaunchKernelGGL(k_tile_setup, dim3(SPP * b.nblk), dim3(64), 0, b.stream, make_where(b), b.ntot,
                      b.A64, only_active ? b.d_scal : nullptr, b.T4, b.T4s, b.d_flag);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());

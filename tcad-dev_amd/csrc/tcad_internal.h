@@ -36,19 +36,31 @@ namespace tcad {
 #ifndef TCAD_BS
 #define TCAD_BS 128
 #endif
+#ifndef TCAD_PY
+#define TCAD_PY 4
+#endif
 constexpr int BS = TCAD_BS;         // threads per node workgroup (multiple of 64)
-// patch width: one wavefront per patch row from 256 threads up; smaller
-// workgroups keep 4 rows (whole 4 x 4 tiles) and narrow the patch
-constexpr int PX = BS >= 256 ? 64 : BS / 4;
-constexpr int PY = BS / PX;         // patch height
+constexpr int PY = TCAD_PY;         // patch height (whole tiles)
+constexpr int PX = BS / PY;         // patch width
 static_assert(BS % PX == 0, "workgroup must hold whole patch rows");
 constexpr int NDIAG = 9;
 constexpr int NOFF = 5;
 
 inline int nentries(int dim) { return NDIAG + NOFF * 2 * dim; }
 
-// Four stored entries of a 4x4-tile inverse row, FP16 (each row of a node
-// carries its own FP32 scale).
+// Tile block-Jacobi of 2D devices: TILE_X x TILE_Y node tiles, each
+// inverted exactly; node k keeps its 3 rows of the tile inverse as
+// TILE_PLANES planes of four FP16 entries (tile4), each row with its own
+// FP32 scale.
+#ifndef TCAD_TY
+#define TCAD_TY 4
+#endif
+constexpr int TILE_X = 4;
+constexpr int TILE_Y = TCAD_TY;
+constexpr int TILE_NODES = TILE_X * TILE_Y;
+constexpr int TILE_U = 3 * TILE_NODES;          // unknowns per tile
+constexpr int TILE_PLANES = 3 * TILE_U / 4;     // tile4 planes per node
+static_assert(PX % TILE_X == 0 && PY % TILE_Y == 0, "patches must hold whole tiles");
 typedef _Float16 tile4 __attribute__((ext_vector_type(4)));
 
 // Per-device metadata, device resident.
@@ -153,7 +165,7 @@ struct Batch {
   float *A32 = nullptr;
   double *Dinv64 = nullptr;
   float *Dinv32 = nullptr;
-  tile4 *T4 = nullptr;                           // 4x4-tile inverses, 36 planes (FP16)
+  tile4 *T4 = nullptr;                           // tile inverses, TILE_PLANES planes (FP16)
   float *T4s = nullptr;                          // their per-row scales, 3 planes
   int tile = 1;                                  // preconditioner of the next solves
   double *bs = nullptr;                          // scaled rhs
