@@ -253,12 +253,12 @@ int bicgstab(Batch &b, bool fp32, double rtol, int maxit, bool zero_guess, int c
 int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int inner_maxit,
              double inner_rtol, bool per_dev = false);
 int pcg_poisson(Batch &b, double rtol, int maxit);
-// inverses of the 4x4-tile diagonal blocks of the last assembled matrix
+// inverses of the tile diagonal blocks of the last assembled matrix
 int launch_tile_setup(Batch &b, bool only_active);
 int launch_unscale(Batch &b, const double *n, const double *p, double *du);
 int launch_unscale_state(Batch &b, const double *n, const double *p, double *du);
 int launch_spmv_only(Batch &b, bool fp32);
-// timing only: one k_bicg_1 (which 3) or k_bicg_3 (which 4), FP64, with a
+// timing only: one k_bicg_u (which 3) or k_bicg_s (which 4), FP64, with a
 // one-thread reset launch before it
 int launch_vector_kernel_only(Batch &b, int which);
 int clear_done(Batch &b);

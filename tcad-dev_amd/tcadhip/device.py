@@ -32,19 +32,19 @@ _MODEL_FLAGS = ("doping_mobility", "srh", "auger", "bgn", "surface_mobility")
 class NewtonOptions:
     """pytcad/pytcad/device.py:539-628 for the fields this path reads.
 
-    linsolve: "auto" | "direct" | "bicgstab" -> FP64 BiCGStab with node
-    block-Jacobi (no direct factorisation runs on the GPU);
-    "bicgstab_mixed" -> FP32 BiCGStab + block-Jacobi inside FP64
-    iterative refinement.  linsolve_rtol is the relative residual of the
+    linsolve: "auto" | "direct" | "bicgstab" -> FP64 BiCGStab with the
+    block-Jacobi preconditioner `precond` names (no direct factorisation
+    runs on the GPU); "bicgstab_mixed" -> BiCGStab on the FP32-stored
+    matrix inside FP64 iterative refinement.  linsolve_rtol is the relative residual of the
     (row-equilibrated) Newton system.  forcing: inexact early Newton
     solves (target clamp(0.01 * previous update, linsolve_rtol, 1e-4));
     the converging step is refined to the FP64 rounding floor either way.
+    Forcing is off by default: on the 200x120 MOSFET sweep it added a
+    Newton iteration per point and cost more Krylov work than it saved.
     precond: "tile" (default; 2D devices get the exact inverse of every
-    4 x 4 node tile's 48 x 48 diagonal block, FP32, 3D devices the node
-    block-Jacobi) or "node" (the reference's 3x3 node block-Jacobi for
-    every device).  Off by default:
-    on the 200x120 MOSFET sweep it added a Newton iteration per point and
-    cost more Krylov work than it saved."""
+    4 x 4 node tile's 48 x 48 diagonal block, stored as FP16 rows with a
+    FP32 scale per row; 3D devices the node block-Jacobi) or "node" (the
+    reference's 3x3 node block-Jacobi for every device)."""
     max_iter: int = 100
     tol_update: float = 1e-8
     max_dpsi: float = 5.0
