@@ -60,9 +60,10 @@ extern "C" {
 #define TCAD_LIN_FORCING 0x100
 /* Flag OR'd into lin_mode (tcad_solve_bias, tcad_solve_linear): precondition
  * every device with the reference's node block-Jacobi (3x3 block per node,
- * linsolve.py:363-427).  Without it, 2D devices use 4x4-tile block-Jacobi
- * (the exact inverse of the matrix restricted to each 4 x 4 node tile, 48
- * unknowns, stored in FP32) and 3D devices node block-Jacobi. */
+ * linsolve.py:363-427).  Without it, every device uses tile block-Jacobi:
+ * the exact inverse of the matrix restricted to each 16-node tile (4 x 4
+ * in 2D, 2 x 2 x 4 in 3D; 48 unknowns), stored as FP16 rows with an FP32
+ * scale per row. */
 #define TCAD_PRECOND_NODE 0x200
 
 /* Per-device outcome of a Newton solve (h_status). */

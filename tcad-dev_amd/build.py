@@ -19,9 +19,9 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.environ.get("TCADHIP_OUT") or os.path.join(HERE, "tcadhip", "libtcadhip.so")
 BS = os.environ.get("TCADHIP_BS")          # node workgroup size (default 128)
 PY = os.environ.get("TCADHIP_PY")          # patch height (default 4)
-TY = os.environ.get("TCADHIP_TY")          # preconditioner tile height (default 4)
+T3 = os.environ.get("TCADHIP_T3")          # 3D tile x and y extent, e.g. "2x2" (default)
 OBJ = os.path.join(HERE, "_build" + (f"_bs{BS}" if BS else "") + (f"_py{PY}" if PY else "")
-                   + (f"_ty{TY}" if TY else ""))
+                   + (f"_t3{T3}" if T3 else ""))
 SOURCES = ["assemble.hip", "krylov.hip", "batch.hip", "ac.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -31,8 +31,9 @@ if BS:
     FLAGS.append(f"-DTCAD_BS={int(BS)}")
 if PY:
     FLAGS.append(f"-DTCAD_PY={int(PY)}")
-if TY:
-    FLAGS.append(f"-DTCAD_TY={int(TY)}")
+if T3:
+    t3x, t3y = (int(v) for v in T3.split("x"))
+    FLAGS += [f"-DTCAD_T3X={t3x}", f"-DTCAD_T3Y={t3y}"]
 
 
 def _compile(src):

@@ -829,9 +829,15 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
     ranges.emplace_back(I.off, I.off + I.n);
     end_max = std::max(end_max, I.off + I.n);
     I.blk0 = blk;
-    I.px = (D.nx + PX - 1) / PX;
-    I.py = (D.ny + PY - 1) / PY;
-    I.nblk = I.px * I.py * nz;
+    if (D.dim == 3) {
+      I.px = (D.nx + PX3 - 1) / PX3;
+      I.py = (D.ny + PY3 - 1) / PY3;
+      I.nblk = I.px * I.py * ((nz + PZ3 - 1) / PZ3);
+    } else {
+      I.px = (D.nx + PX - 1) / PX;
+      I.py = (D.ny + PY - 1) / PY;
+      I.nblk = I.px * I.py * nz;
+    }
     const int n_ax[3] = {D.nx, D.ny, nz};
     const double *hs[3] = {D.h_hx, D.h_hy, D.h_hz};
     const double *dvs[3] = {D.h_dvx, D.h_dvy, D.h_dvz};
@@ -926,7 +932,6 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
     }
     gglobal += D.ngates;
     b->dim_max = std::max(b->dim_max, D.dim);
-    if (D.dim == 2) b->any2d = 1;
     off += I.n;
     blk += I.nblk;
   }

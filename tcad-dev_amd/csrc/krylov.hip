@@ -101,35 +101,46 @@ __device__ __forceinline__ void prec3(const T *Di, long long N, long long g, con
 }
 
 // ---------------------------------------------------------------------------
-// Tile block-Jacobi (2D devices).  Every TILE_X x TILE_Y node tile is one
-// diagonal block of the preconditioner: the scaled Newton matrix restricted
-// to the tile (couplings leaving the tile dropped), inverted exactly.  3D
-// devices keep the node block-Jacobi: on the 64x48x48 tri-gate, x-y tiles
-// cut BiCGStab iterations only 2700 -> 1800 per point (the z couplings stay
-// outside the tiles), and with their bytes per application the sweep ran
-// 8.0 instead of 10.0 points/s.  The inverse rows of node k are
-// TILE_PLANES planes of four FP16 entries, T4[(r * TILE_U / 4 + q) * N + g]
-// (row r of the node, columns 4q..4q+3 of the tile, tile unknown
-// u = 3 l + c with l = TILE_X (j % TILE_Y) + i % TILE_X), each row scaled
-// by its largest magnitude, kept in T4s[r * N + g].  Measured BiCGStab
-// iterations to 1e-10 on the equilibrated systems of the 200x120 MOSFET
-// (Vg = 0.5 V): node block-Jacobi ~630, 4 x 4 tiles ~255, 4 x 8 tiles
-// ~185 (the surface mesh is finest in y, so y-long tiles hold the strong
-// couplings).
+// Tile block-Jacobi.  Every 16-node tile (4 x 4 in 2D, T3X x T3Y x T3Z in
+// 3D) is one diagonal block of the preconditioner: the scaled Newton matrix
+// restricted to the tile (couplings leaving the tile dropped), inverted
+// exactly.  The inverse rows of node k are TILE_PLANES planes of four FP16
+// entries, T4[(r * TILE_U / 4 + q) * N + g] (row r of the node, columns
+// 4q..4q+3 of the tile, tile unknown u = 3 l + c with l the node's place in
+// the tile, x fastest), each row scaled by its largest magnitude, kept in
+// T4s[r * N + g].  Measured BiCGStab iterations to 1e-10 on the
+// equilibrated systems of the 200x120 MOSFET (Vg = 0.5 V): node
+// block-Jacobi ~630, 4 x 4 tiles ~255 (4 x 8 tiles ~185 for twice the
+// bytes: slower on the GPU).  On a 32x24x24 tri-gate: node block-Jacobi
+// 145-217, 4 x 4 x 1 x-y tiles 114-190 (the z couplings of the finely
+// meshed fin stay outside them), 2 x 2 x 4 tiles 72-86.  On the 64x48x48
+// tri-gate sweep (GPU, 34 points): 74,304 -> 25,969 Krylov iterations,
+// 9.3 -> 18.5 points/s (1 x 2 x 8 tiles: 19,540 iterations but 8-node x
+// rows, 16.0 points/s).
 // ---------------------------------------------------------------------------
 constexpr int TU = TILE_U;
 constexpr int TNS = 3 * TU + 1;             // LDS doubles per node row set (+1: banks)
-// tile setup: a 64-thread workgroup inverts STPW tiles side by side
+// tile setup: a 64-thread workgroup inverts STPW tiles side by side in x
 constexpr int STPW = 64 / TILE_NODES;
-constexpr int SCOLS = STPW * TILE_X;        // node columns of one setup workgroup
-static_assert(64 % TILE_NODES == 0 && PX % SCOLS == 0, "setup groups must tile the patch");
+constexpr int SCOLS = STPW * TILE_X;        // node columns of one 2D setup workgroup
+constexpr int SCOLS3 = STPW * T3X;          // node columns of one 3D setup workgroup
+static_assert(64 % TILE_NODES == 0 && PX % SCOLS == 0 && PX3 % SCOLS3 == 0,
+              "setup groups must tile the patch");
 constexpr int SPP = (PX / SCOLS) * (PY / TILE_Y);   // setup workgroups per patch
+static_assert(PX3 / SCOLS3 == SPP, "2D and 3D patches need the same setup grid");
 
 // tile slot of this thread inside its patch, and node index inside the tile
-__device__ __forceinline__ void tile_slot(int &tile, int &l) {
-  const int il = threadIdx.x % PX, jl = threadIdx.x / PX;
-  tile = (jl / TILE_Y) * (PX / TILE_X) + il / TILE_X;
-  l = (jl % TILE_Y) * TILE_X + il % TILE_X;
+__device__ __forceinline__ void tile_slot(int dim, int &tile, int &l) {
+  if (dim == 3) {
+    const int il = threadIdx.x % PX3, jl = (threadIdx.x / PX3) % PY3,
+              kl = threadIdx.x / (PX3 * PY3);
+    tile = il / T3X;                        // PY3 = T3Y, PZ3 = T3Z: one tile row
+    l = (kl * T3Y + jl) * T3X + il % T3X;
+  } else {
+    const int il = threadIdx.x % PX, jl = threadIdx.x / PX;
+    tile = (jl / TILE_Y) * (PX / TILE_X) + il / TILE_X;
+    l = (jl % TILE_Y) * TILE_X + il % TILE_X;
+  }
 }
 
 // The inverse rows of this thread's node, loaded into registers at the top
@@ -170,10 +181,10 @@ __device__ __forceinline__ float tile_scale(const TileMem &M, int r) {
 // z = M^-1 x for this thread's node; every thread of the workgroup calls it
 // (it holds barriers); tv is LDS of BS * 3 doubles.
 template <typename Rows>
-__device__ __forceinline__ void tile_apply(const Rows &R, bool valid, const double x[3],
+__device__ __forceinline__ void tile_apply(const Rows &R, int dim, bool valid, const double x[3],
                                            double z[3], double *tv) {
   int tile, l;
-  tile_slot(tile, l);
+  tile_slot(dim, tile, l);
   double *v = tv + tile * TU;
   __syncthreads();                          // tv free (a previous use is done)
 #pragma unroll
@@ -197,13 +208,13 @@ __device__ __forceinline__ void tile_apply(const Rows &R, bool valid, const doub
   for (int r = 0; r < 3; ++r) z[r] = acc[r] * (double)tile_scale(R, r);
 }
 
-// Setup: one workgroup of 64 threads per STPW tiles side by side (SCOLS x
-// TILE_Y nodes), one thread per node holding its 3 rows of the tile matrix
-// in LDS; block Gauss-Jordan in FP64 with the 3x3 node blocks as pivots (no
-// row exchange, like the node block-Jacobi inverse), in place.  Each pivot
-// step: every thread inverts the pivot block (broadcast LDS reads), the
-// tile's threads scale the pivot rows column by column in parallel, then
-// every other row eliminates the pivot columns.
+// Setup: one workgroup of 64 threads per STPW tiles side by side in x, one
+// thread per node holding its 3 rows of the tile matrix in LDS; block
+// Gauss-Jordan in FP64 with the 3x3 node blocks as pivots (no row exchange,
+// like the node block-Jacobi inverse), in place.  Each pivot step: every
+// thread inverts the pivot block (broadcast LDS reads), the tile's threads
+// scale the pivot rows column by column in parallel, then every other row
+// eliminates the pivot columns.
 __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const double *A,
                                                    const Scal *scal, tile4 *T4, float *T4s,
                                                    int *flag) {
@@ -212,17 +223,51 @@ __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const d
   const int d = W.single ? 0 : W.blk2dev[patch];
   if (scal && !scal[d].active) return;
   const DevInfo &I = dev_info(W, d);
-  if (I.dim != 2) return;                   // 3D devices keep node block-Jacobi
   const unsigned local = (unsigned)(patch - I.blk0);
   const unsigned bx = local % (unsigned)I.px, t2 = local / (unsigned)I.px;
   const unsigned by = t2 % (unsigned)I.py, bz = t2 / (unsigned)I.py;
-  const int lc = threadIdx.x % SCOLS, lr = threadIdx.x / SCOLS;
-  const int tq = lc / TILE_X;                                       // tile in the group
-  const int l = lr * TILE_X + lc % TILE_X;                          // node in the tile
-  const int i = (int)bx * PX + (sub % (PX / SCOLS)) * SCOLS + lc;
-  const int j = (int)by * PY + (sub / (PX / SCOLS)) * TILE_Y + lr;
-  const bool valid = i < I.nx && j < I.ny;
-  const long long g = I.off + ((long long)bz * I.ny + j) * I.nx + i;
+  // this thread's node (i, j, kk), its tile in the group (tq), its place in
+  // the tile (l), and which of its neighbours x-, x+, y-, y+, z-, z+ lie in
+  // the tile (ln: their places)
+  int tq, l, i, j, kk;
+  bool has[6];
+  int ln[6];
+  if (I.dim == 3) {
+    const int lc = threadIdx.x % SCOLS3, ly = (threadIdx.x / SCOLS3) % T3Y,
+              lz = threadIdx.x / (SCOLS3 * T3Y);
+    const int il = lc % T3X;
+    tq = lc / T3X;
+    l = (lz * T3Y + ly) * T3X + il;
+    i = (int)bx * PX3 + sub * SCOLS3 + lc;
+    j = (int)by * PY3 + ly;
+    kk = (int)bz * PZ3 + lz;
+    has[0] = il > 0 && i > 0;
+    has[1] = il < T3X - 1 && i < I.nx - 1;
+    has[2] = ly > 0 && j > 0;
+    has[3] = ly < T3Y - 1 && j < I.ny - 1;
+    has[4] = lz > 0 && kk > 0;
+    has[5] = lz < T3Z - 1 && kk < I.nz - 1;
+    ln[2] = l - T3X; ln[3] = l + T3X;
+    ln[4] = l - T3X * T3Y; ln[5] = l + T3X * T3Y;
+  } else {
+    const int lc = threadIdx.x % SCOLS, lr = threadIdx.x / SCOLS;
+    const int il = lc % TILE_X;
+    tq = lc / TILE_X;
+    l = lr * TILE_X + il;
+    i = (int)bx * PX + (sub % (PX / SCOLS)) * SCOLS + lc;
+    j = (int)by * PY + (sub / (PX / SCOLS)) * TILE_Y + lr;
+    kk = 0;
+    has[0] = il > 0 && i > 0;
+    has[1] = il < TILE_X - 1 && i < I.nx - 1;
+    has[2] = lr > 0 && j > 0;
+    has[3] = lr < TILE_Y - 1 && j < I.ny - 1;
+    has[4] = has[5] = false;
+    ln[2] = l - TILE_X; ln[3] = l + TILE_X;
+    ln[4] = ln[5] = l;
+  }
+  ln[0] = l - 1; ln[1] = l + 1;
+  const bool valid = i < I.nx && j < I.ny && kk < I.nz;
+  const long long g = I.off + ((long long)kk * I.ny + j) * I.nx + i;
   double *Mt = M + tq * TILE_NODES * TNS;                           // this tile's rows
   double *Mr = Mt + l * TNS;                                        // this node's 3 rows
   for (int e = 0; e < 3 * TU; ++e) Mr[e] = 0.0;
@@ -231,12 +276,8 @@ __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const d
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int c = 0; c < 3; ++c) Mr[r * TU + 3 * l + c] = A[(3 * r + c) * N + g];
-    const int il = lc % TILE_X, jl = lr;
-    const bool has[4] = {il > 0 && i > 0, il < TILE_X - 1 && i < I.nx - 1,
-                         jl > 0 && j > 0, jl < TILE_Y - 1 && j < I.ny - 1};
-    const int ln[4] = {l - 1, l + 1, l - TILE_X, l + TILE_X};
 #pragma unroll
-    for (int dir = 0; dir < 4; ++dir) {
+    for (int dir = 0; dir < 6; ++dir) {
       if (!has[dir]) continue;
       const double *o = A + (long long)(9 + 5 * dir) * N + g;
       const int cb = 3 * ln[dir];
@@ -355,18 +396,19 @@ enum { PV_NODE = 0, PV_TILE_REG = 1, PV_TILE_MEM = 2 };
 // z = M^-1 x at this thread's node (every thread of the workgroup calls
 // it: the tile variant holds barriers)
 template <typename T, int PV>
-__device__ __forceinline__ void apply_prec(const KArgs<T> &K, const NodeHood &H, bool use_tile,
-                                           const TileRows &TR, const double x[3], double z[3]) {
+__device__ __forceinline__ void apply_prec(const KArgs<T> &K, const NodeHood &H, int dim,
+                                           bool use_tile, const TileRows &TR, const double x[3],
+                                           double z[3]) {
   if (!use_tile) {
     if (H.valid) prec3<T>(K.Dinv, K.N, H.g, x, z);
     return;
   }
   __shared__ double tv[BS * 3];
   if (PV == PV_TILE_REG) {
-    tile_apply(TR, H.valid, x, z, tv);
+    tile_apply(TR, dim, H.valid, x, z, tv);
   } else {
     const TileMem TM{K.T4, K.T4s, K.N, H.g};
-    tile_apply(TM, H.valid, x, z, tv);
+    tile_apply(TM, dim, H.valid, x, z, tv);
   }
 }
 
@@ -378,7 +420,7 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
   (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
-  const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
+  const bool use_tile = PV != PV_NODE && K.tile;
   TileRows TR;
   if (PV == PV_TILE_REG) tile_load(K.T4, K.T4s, N, H.g, use_tile && H.valid, TR);
   const double sc = K.scale_b ? 1.0 / S.nrm : 1.0;
@@ -397,7 +439,7 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
     }
   }
   double z[3];
-  apply_prec<T, PV>(K, H, use_tile, TR, b3, z);
+  apply_prec<T, PV>(K, H, I.dim, use_tile, TR, b3, z);
   if (H.valid) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
@@ -473,7 +515,7 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
   KPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
-  const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
+  const bool use_tile = PV != PV_NODE && K.tile;
   TileRows TR;
   if (PV == PV_TILE_REG) tile_load(K.T4, K.T4s, N, H.g, use_tile && H.valid, TR);
   double r[3] = {0, 0, 0}, v[3] = {0, 0, 0};
@@ -515,7 +557,7 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
       K.v[VS][c * N + H.g] = s3[c];
     }
   }
-  apply_prec<T, PV>(K, H, use_tile, TR, s3, z);
+  apply_prec<T, PV>(K, H, I.dim, use_tile, TR, s3, z);
   if (H.valid) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) K.v[VSH][c * N + H.g] = z[c];
@@ -540,7 +582,7 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
   KPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
-  const bool use_tile = PV != PV_NODE && K.tile && I.dim == 2;
+  const bool use_tile = PV != PV_NODE && K.tile;
   TileRows TR;
   if (PV == PV_TILE_REG) tile_load(K.T4, K.T4s, N, H.g, use_tile && H.valid, TR);
   double x[3] = {0, 0, 0}, ph[3] = {0, 0, 0}, shh[3] = {0, 0, 0}, s[3] = {0, 0, 0},
@@ -582,7 +624,7 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
     }
   }
   double z[3];
-  apply_prec<T, PV>(K, H, use_tile, TR, pn, z);
+  apply_prec<T, PV>(K, H, I.dim, use_tile, TR, pn, z);
   if (H.valid) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
@@ -616,7 +658,7 @@ TCAD_PV_KERNELS(k_bicg_u, bicg_u_body)
 constexpr int kTileRegBlocks = 512;
 
 static int precond_variant(const Batch &b) {
-  if (!b.tile || !b.any2d) return PV_NODE;
+  if (!b.tile) return PV_NODE;
   static const int forced = []() -> int {
     const char *e = std::getenv("TCADHIP_TILE_ROWS");
     if (e && std::strcmp(e, "reg") == 0) return PV_TILE_REG;

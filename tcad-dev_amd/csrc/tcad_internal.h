@@ -14,11 +14,12 @@
 //   precond      Dinv64/Dinv32[e*N + k], e < 9: inverse of the scaled node
 //                block (node block-Jacobi)
 //   vectors      3 planes of N_total (v[c*N + k])
-// Kernels map one thread to one node.  Each device is cut into patches of
-// PX x PY nodes (x fastest) in every z-plane, one workgroup per patch:
-// thread t is node (i0 + t % PX, j0 + t / PX), so every wavefront reads one
-// contiguous row segment of PX nodes (coalesced), and a patch holds whole
-// 4 x 4 node tiles (the tile preconditioner inverts them in LDS).
+// Kernels map one thread to one node.  Each 2D device is cut into patches
+// of PX x PY nodes (x fastest), each 3D device into PX3 x PY3 x PZ3 boxes,
+// one workgroup per patch: thread t is node (i0 + t % PX, j0 + t / PX), or
+// (i0 + t % PX3, j0 + t / PX3 % PY3, k0 + t / (PX3 PY3)), so every
+// wavefront reads contiguous x-row segments (coalesced), and a patch holds
+// whole 16-node tiles (the tile preconditioner inverts them in LDS).
 // Workgroups never straddle devices, so per-device reductions run over a
 // contiguous block range in a fixed order and are deterministic.
 #pragma once
@@ -40,27 +41,35 @@ namespace tcad {
 #define TCAD_PY 4
 #endif
 constexpr int BS = TCAD_BS;         // threads per node workgroup (multiple of 64)
-constexpr int PY = TCAD_PY;         // patch height (whole tiles)
-constexpr int PX = BS / PY;         // patch width
+constexpr int PY = TCAD_PY;         // 2D patch height (whole tiles)
+constexpr int PX = BS / PY;         // 2D patch width
 static_assert(BS % PX == 0, "workgroup must hold whole patch rows");
 constexpr int NDIAG = 9;
 constexpr int NOFF = 5;
 
 inline int nentries(int dim) { return NDIAG + NOFF * 2 * dim; }
 
-// Tile block-Jacobi of 2D devices: TILE_X x TILE_Y node tiles, each
-// inverted exactly; node k keeps its 3 rows of the tile inverse as
+// Tile block-Jacobi: 16-node tiles, 4 x 4 in 2D and T3X x T3Y x T3Z in 3D,
+// each inverted exactly; node k keeps its 3 rows of the tile inverse as
 // TILE_PLANES planes of four FP16 entries (tile4), each row with its own
-// FP32 scale.
-#ifndef TCAD_TY
-#define TCAD_TY 4
+// FP32 scale.  3D devices are cut into PX3 x PY3 x PZ3 patches (x fastest)
+// that hold whole tiles: the z couplings of the finely meshed fin are the
+// strongest, so the 3D tiles extend along z.
+#ifndef TCAD_T3X
+#define TCAD_T3X 2
 #endif
-constexpr int TILE_X = 4;
-constexpr int TILE_Y = TCAD_TY;
+#ifndef TCAD_T3Y
+#define TCAD_T3Y 2
+#endif
+constexpr int TILE_X = 4, TILE_Y = 4;
 constexpr int TILE_NODES = TILE_X * TILE_Y;
+constexpr int T3X = TCAD_T3X, T3Y = TCAD_T3Y, T3Z = TILE_NODES / (T3X * T3Y);
+constexpr int PY3 = T3Y, PZ3 = T3Z, PX3 = BS / (PY3 * PZ3);
 constexpr int TILE_U = 3 * TILE_NODES;          // unknowns per tile
 constexpr int TILE_PLANES = 3 * TILE_U / 4;     // tile4 planes per node
-static_assert(PX % TILE_X == 0 && PY % TILE_Y == 0, "patches must hold whole tiles");
+static_assert(PX % TILE_X == 0 && PY % TILE_Y == 0, "2D patches must hold whole tiles");
+static_assert(T3X * T3Y * T3Z == TILE_NODES && PX3 * PY3 * PZ3 == BS && PX3 % T3X == 0,
+              "3D patches must hold whole 16-node tiles");
 typedef _Float16 tile4 __attribute__((ext_vector_type(4)));
 
 // Per-device metadata, device resident.
@@ -69,7 +78,7 @@ struct DevInfo {
   long long off;                    // first node in the batch planes
   long long n;                      // node count
   int blk0, nblk;                   // node-kernel block range (patches)
-  int px, py;                       // patches along x and y
+  int px, py;                       // patches along x and y (PX x PY in 2D, PX3 x PY3 in 3D)
   int h_off[3];                     // offsets into the concatenated spacing arrays
   int dv_off[3];                    // offsets into the concatenated width arrays
   double VT, Ns, LD, J0, R0, Cn, Cp;
@@ -121,10 +130,16 @@ __device__ __forceinline__ NodeAt node_at(const DevInfo &I, int blk) {
   const unsigned upx = (unsigned)I.px, upy = (unsigned)I.py;
   const unsigned bx = local % upx, t = local / upx;
   const unsigned by = t % upy, bz = t / upy;
-  P.i = (int)(bx * PX + threadIdx.x % PX);
-  P.j = (int)(by * PY + threadIdx.x / PX);
-  P.kk = (int)bz;
-  P.valid = P.i < I.nx && P.j < I.ny;
+  if (I.dim == 3) {
+    P.i = (int)(bx * PX3 + threadIdx.x % PX3);
+    P.j = (int)(by * PY3 + (threadIdx.x / PX3) % PY3);
+    P.kk = (int)(bz * PZ3 + threadIdx.x / (PX3 * PY3));
+  } else {
+    P.i = (int)(bx * PX + threadIdx.x % PX);
+    P.j = (int)(by * PY + threadIdx.x / PX);
+    P.kk = 0;
+  }
+  P.valid = P.i < I.nx && P.j < I.ny && P.kk < I.nz;
   P.k = ((long long)P.kk * I.ny + P.j) * I.nx + P.i;
   return P;
 }
@@ -134,7 +149,6 @@ struct Batch {
   long long ntot = 0;
   int nblk = 0;
   int dim_max = 2;
-  int any2d = 0;                    // some device is 2D
   int ne = 0;                       // stencil entries per node
   hipStream_t stream = nullptr;
   long long launches = 0;

@@ -41,10 +41,10 @@ class NewtonOptions:
     the converging step is refined to the FP64 rounding floor either way.
     Forcing is off by default: on the 200x120 MOSFET sweep it added a
     Newton iteration per point and cost more Krylov work than it saved.
-    precond: "tile" (default; 2D devices get the exact inverse of every
-    4 x 4 node tile's 48 x 48 diagonal block, stored as FP16 rows with a
-    FP32 scale per row; 3D devices the node block-Jacobi) or "node" (the
-    reference's 3x3 node block-Jacobi for every device)."""
+    precond: "tile" (default; every 16-node tile -- 4 x 4 in 2D, 2 x 2 x 4
+    in 3D -- gets the exact inverse of its 48 x 48 diagonal block, stored
+    as FP16 rows with a FP32 scale per row) or "node" (the reference's 3x3
+    node block-Jacobi for every device)."""
     max_iter: int = 100
     tol_update: float = 1e-8
     max_dpsi: float = 5.0
