@@ -87,13 +87,26 @@ def measured_traffic(kernel):
     return None
 
 
-def kernel_timing(dev, which, reps=1000):
+def kernel_timing(handle, which, reps=1000):
+    """Average launch time [us] of one library kernel on a batch handle
+    (tcad_time_kernel: an untimed burst, then `reps` timed launches)."""
     import ctypes
     from tcadhip import _lib
     out = ctypes.c_double()
-    _lib.check(_lib.lib().tcad_time_kernel(dev._gpu().handle, which, reps, ctypes.byref(out)),
+    _lib.check(_lib.lib().tcad_time_kernel(handle, which, reps, ctypes.byref(out)),
                "tcad_time_kernel")
     return out.value
+
+
+def spmv_bytes(spec, w=8):
+    """Algorithmic bytes of one stencil SpMV launch over a 2D device: the
+    matrix (9 node-block entries per node + 5 per directed neighbour link,
+    width w) + the source, product and dotted vectors (3 FP64 planes
+    each)."""
+    nx, ny = spec.coords[0].size, spec.coords[1].size
+    N = nx * ny
+    edges = 2 * ((nx - 1) * ny + nx * (ny - 1))
+    return (9 * N + 5 * edges) * w + 3 * N * 8 * 3
 
 
 def bench_trigate(vds=(0.05, 0.7)):
@@ -163,10 +176,24 @@ def bench_batch(lanes=1, sample=8):
     def final(r):
         return np.array([list(c.values()) for c in r.currents[-1:]]).ravel()
 
+    # the headline's roofline kernel over the whole batch (1.7 M nodes),
+    # where bandwidth rather than launch latency bounds it
+    from tcadhip.device import GPUBatch
+    B = GPUBatch([setup_from_spec(s) for s in specs])
+    B.solve_equilibrium(NewtonOptions())
+    B.solve_bias(NewtonOptions(max_iter=1))
+    us = kernel_timing(B.handle, 0)
+    nbytes = sum(spmv_bytes(s) for s in specs)
+    roof = {"kernel": "k_bicg_spmv<double>", "bytes_per_launch": int(nbytes),
+            "avg_launch_us": round(us, 3),
+            "achieved_GBps": round(nbytes / (us * 1e-6) / 1e9, 1),
+            "frac_of_8000": round(nbytes / (us * 1e-6) / 1e9 / 8000.0, 4)}
+    del B
     out = {"devices": len(specs), "nodes": int(sum(s.N for s in specs)),
            "continuation_solves": int(sum(len(s.sweep) for s in specs)), "lanes": lanes,
            "plain_fp64_devices_per_s": round(plain, 4),
-           "plain_sample": f"{len(idx)} evenly spaced devices solved one at a time"}
+           "plain_sample": f"{len(idx)} evenly spaced devices solved one at a time",
+           "spmv_roofline_whole_batch": roof}
     for mode, r in res.items():
         fails = sum(1 for x in r["results"] if x.failed)
         err, ok = 0.0, True
@@ -397,14 +424,12 @@ def main():
     # matrix entries + 3 reads of the source vector + 3 writes of the
     # product + 3 reads of the dotted vector (FP64 vectors), per node
     dev = r["dev"]
-    N = spec.N
-    nx, ny = spec.coords[0].size, spec.coords[1].size
-    edges = 2 * ((nx - 1) * ny + nx * (ny - 1))          # directed neighbour blocks
     w = 4 if chosen == "bicgstab_mixed" else 8
-    spmv_bytes = (9 * N + 5 * edges) * w + 3 * N * 8 * 3
+    nbytes = spmv_bytes(spec, w)
+    N = spec.N
     which = 1 if chosen == "bicgstab_mixed" else 0
-    us = kernel_timing(dev, which)
-    achieved = spmv_bytes / (us * 1e-6) / 1e9
+    us = kernel_timing(dev._gpu().handle, which)
+    achieved = nbytes / (us * 1e-6) / 1e9
     lat_ms = r["latencies"] * 1e3
     line = {
         "metric": METRIC,
@@ -432,7 +457,7 @@ def main():
                      "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 5),
                      "traffic": measured_traffic("k_bicg_spmv<double>") if w == 8 else None,
-                     "bytes_per_launch": spmv_bytes, "avg_launch_us": round(us, 3),
+                     "bytes_per_launch": nbytes, "avg_launch_us": round(us, 3),
                      "copy_ceiling_GBps": copy_ceiling()},
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["points"] / cpu["seconds"], 5), "unit": "bias points/s",

@@ -119,7 +119,6 @@ __device__ __forceinline__ void prec3(const T *Di, long long N, long long g, con
 // rows, 16.0 points/s).
 // ---------------------------------------------------------------------------
 constexpr int TU = TILE_U;
-constexpr int TNS = 3 * TU + 1;             // LDS doubles per node row set (+1: banks)
 // tile setup: a 64-thread workgroup inverts STPW tiles side by side in x
 constexpr int STPW = 64 / TILE_NODES;
 constexpr int SCOLS = STPW * TILE_X;        // node columns of one 2D setup workgroup
@@ -208,17 +207,42 @@ __device__ __forceinline__ void tile_apply(const Rows &R, int dim, bool valid, c
   for (int r = 0; r < 3; ++r) z[r] = acc[r] * (double)tile_scale(R, r);
 }
 
-// Setup: one workgroup of 64 threads per STPW tiles side by side in x, one
-// thread per node holding its 3 rows of the tile matrix in LDS; block
+// Setup: one workgroup of 128 threads per STPW tiles side by side in x (64
+// nodes), two threads per node, each owning half of the 48 columns; block
 // Gauss-Jordan in FP64 with the 3x3 node blocks as pivots (no row exchange,
-// like the node block-Jacobi inverse), in place.  Each pivot step: every
-// thread inverts the pivot block (broadcast LDS reads), the tile's threads
-// scale the pivot rows column by column in parallel, then every other row
-// eliminates the pivot columns.
-__global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const double *A,
-                                                   const Scal *scal, tile4 *T4, float *T4s,
-                                                   int *flag) {
-  __shared__ double M[64 * TNS];
+// like the node block-Jacobi inverse), in place in LDS.  The tile matrices
+// are stored column-major with the node's lane as the fastest index,
+// M[(r * 48 + col) * 64 + lane]: a wavefront's own-row accesses are 64
+// consecutive doubles (no bank conflicts; rows of a stride of 145 doubles
+// put the four tiles of a half-wavefront on the same banks), its pivot-row
+// reads are broadcasts.  Each pivot step: every thread inverts the pivot
+// block, the tile's threads scale the pivot rows column by column, then
+// every other row eliminates over its half of the columns without a branch
+// (the pivot block, updated like the rest, is then overwritten with -F P),
+// in two 12-column chunks whose reads all come before their writes (the
+// pivot rows and the own rows share one array, so a read after a write
+// could not be hoisted and each column waited a full LDS round trip).
+// Measured per call: 222 -> 73 us on the 200x120 MOSFET, 652 -> 310 us on
+// the 64x48x48 tri-gate (one thread per node with row-major rows).
+constexpr int HCOL = TU / 2;                // columns per thread (24)
+static_assert(HCOL % 3 == 0 && HCOL % 4 == 0, "a pivot block or a quad never straddles halves");
+#define TMAT(lane, r, col) M[((r) * TU + (col)) * 64 + (lane)]
+
+// lane (0..63) of the setup workgroup that holds node l of tile tq
+__device__ __forceinline__ int setup_lane(int dim, int tq, int l) {
+  if (dim == 3) {
+    const int il = l % T3X, ly = (l / T3X) % T3Y, lz = l / (T3X * T3Y);
+    return lz * (SCOLS3 * T3Y) + ly * SCOLS3 + tq * T3X + il;
+  }
+  return (l / TILE_X) * SCOLS + tq * TILE_X + l % TILE_X;
+}
+
+__global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const double *A,
+                                                    const Scal *scal, tile4 *T4, float *T4s,
+                                                    int *flag) {
+  __shared__ double M[3 * TU * 64];
+  __shared__ float amx[2 * 64 * 3];           // row maxima of each half
+  const int nt = threadIdx.x % 64, half = threadIdx.x / 64, c0 = half * HCOL;
   const int patch = blockIdx.x / SPP, sub = blockIdx.x % SPP;
   const int d = W.single ? 0 : W.blk2dev[patch];
   if (scal && !scal[d].active) return;
@@ -233,8 +257,7 @@ __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const d
   bool has[6];
   int ln[6];
   if (I.dim == 3) {
-    const int lc = threadIdx.x % SCOLS3, ly = (threadIdx.x / SCOLS3) % T3Y,
-              lz = threadIdx.x / (SCOLS3 * T3Y);
+    const int lc = nt % SCOLS3, ly = (nt / SCOLS3) % T3Y, lz = nt / (SCOLS3 * T3Y);
     const int il = lc % T3X;
     tq = lc / T3X;
     l = (lz * T3Y + ly) * T3X + il;
@@ -250,7 +273,7 @@ __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const d
     ln[2] = l - T3X; ln[3] = l + T3X;
     ln[4] = l - T3X * T3Y; ln[5] = l + T3X * T3Y;
   } else {
-    const int lc = threadIdx.x % SCOLS, lr = threadIdx.x / SCOLS;
+    const int lc = nt % SCOLS, lr = nt / SCOLS;
     const int il = lc % TILE_X;
     tq = lc / TILE_X;
     l = lr * TILE_X + il;
@@ -268,104 +291,132 @@ __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const d
   ln[0] = l - 1; ln[1] = l + 1;
   const bool valid = i < I.nx && j < I.ny && kk < I.nz;
   const long long g = I.off + ((long long)kk * I.ny + j) * I.nx + i;
-  double *Mt = M + tq * TILE_NODES * TNS;                           // this tile's rows
-  double *Mr = Mt + l * TNS;                                        // this node's 3 rows
-  for (int e = 0; e < 3 * TU; ++e) Mr[e] = 0.0;
-  if (valid) {
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+  for (int e = 0; e < HCOL; ++e)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) Mr[r * TU + 3 * l + c] = A[(3 * r + c) * N + g];
+    for (int r = 0; r < 3; ++r) TMAT(nt, r, c0 + e) = 0.0;
+  __syncthreads();
+  if (half == 0) {
+    if (valid) {
 #pragma unroll
-    for (int dir = 0; dir < 6; ++dir) {
-      if (!has[dir]) continue;
-      const double *o = A + (long long)(9 + 5 * dir) * N + g;
-      const int cb = 3 * ln[dir];
-      Mr[0 * TU + cb] = o[0];
-      Mr[1 * TU + cb] = o[N];
-      Mr[1 * TU + cb + 1] = o[2 * N];
-      Mr[2 * TU + cb] = o[3 * N];
-      Mr[2 * TU + cb + 2] = o[4 * N];
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) TMAT(nt, r, 3 * l + c) = A[(3 * r + c) * N + g];
+#pragma unroll
+      for (int dir = 0; dir < 6; ++dir) {
+        if (!has[dir]) continue;
+        const double *o = A + (long long)(9 + 5 * dir) * N + g;
+        const int cb = 3 * ln[dir];
+        TMAT(nt, 0, cb) = o[0];
+        TMAT(nt, 1, cb) = o[N];
+        TMAT(nt, 1, cb + 1) = o[2 * N];
+        TMAT(nt, 2, cb) = o[3 * N];
+        TMAT(nt, 2, cb + 2) = o[4 * N];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) TMAT(nt, c, 3 * l + c) = 1.0;
     }
-  } else {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) Mr[c * TU + 3 * l + c] = 1.0;
   }
   bool bad = false;
   for (int sp = 0; sp < TILE_NODES; ++sp) {
     const int pc = 3 * sp;
-    double *Ms = Mt + sp * TNS;                                     // pivot rows
+    const int pl = setup_lane(I.dim, tq, sp);                       // pivot node's lane
     __syncthreads();
-    const double a0 = Ms[pc], a1 = Ms[pc + 1], a2 = Ms[pc + 2];
-    const double a3 = Ms[TU + pc], a4 = Ms[TU + pc + 1], a5 = Ms[TU + pc + 2];
-    const double a6 = Ms[2 * TU + pc], a7 = Ms[2 * TU + pc + 1], a8 = Ms[2 * TU + pc + 2];
-    const double c0 = a4 * a8 - a5 * a7, c1 = a5 * a6 - a3 * a8, c2 = a3 * a7 - a4 * a6;
-    const double det = a0 * c0 + a1 * c1 + a2 * c2;
-    if (l == sp && (!(fabs(det) > 1e-300) || !isfinite(det))) bad = true;
+    const double a0 = TMAT(pl, 0, pc), a1 = TMAT(pl, 0, pc + 1), a2 = TMAT(pl, 0, pc + 2);
+    const double a3 = TMAT(pl, 1, pc), a4 = TMAT(pl, 1, pc + 1), a5 = TMAT(pl, 1, pc + 2);
+    const double a6 = TMAT(pl, 2, pc), a7 = TMAT(pl, 2, pc + 1), a8 = TMAT(pl, 2, pc + 2);
+    const double c0d = a4 * a8 - a5 * a7, c1d = a5 * a6 - a3 * a8, c2d = a3 * a7 - a4 * a6;
+    const double det = a0 * c0d + a1 * c1d + a2 * c2d;
+    if (l == sp && half == 0 && (!(fabs(det) > 1e-300) || !isfinite(det))) bad = true;
     const double id = 1.0 / det;
-    const double P[9] = {c0 * id, (a2 * a7 - a1 * a8) * id, (a1 * a5 - a2 * a4) * id,
-                         c1 * id, (a0 * a8 - a2 * a6) * id, (a2 * a3 - a0 * a5) * id,
-                         c2 * id, (a1 * a6 - a0 * a7) * id, (a0 * a4 - a1 * a3) * id};
+    const double P[9] = {c0d * id, (a2 * a7 - a1 * a8) * id, (a1 * a5 - a2 * a4) * id,
+                         c1d * id, (a0 * a8 - a2 * a6) * id, (a2 * a3 - a0 * a5) * id,
+                         c2d * id, (a1 * a6 - a0 * a7) * id, (a0 * a4 - a1 * a3) * id};
     double F[9];                                // this row's pivot-column block
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) F[3 * r + c] = Mr[r * TU + pc + c];
+      for (int c = 0; c < 3; ++c) F[3 * r + c] = TMAT(nt, r, pc + c);
     __syncthreads();                            // pivot block read by all
-    // pivot rows := P * pivot rows (pivot block := P), columns split over
-    // the tile's threads
-#pragma unroll 2
-    for (int col = l; col < TU; col += TILE_NODES) {
-      if (col / 3 == sp) {
+    // pivot rows := P * pivot rows (pivot block := P): the tile's 16 x 2
+    // threads take the 48 columns, 24 per half
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int cc = l + TILE_NODES * k;
+      if (cc < HCOL) {
+        const int col = c0 + cc;
+        const double m0 = TMAT(pl, 0, col), m1 = TMAT(pl, 1, col), m2 = TMAT(pl, 2, col);
+        double n0 = P[0] * m0 + P[1] * m1 + P[2] * m2;
+        double n1 = P[3] * m0 + P[4] * m1 + P[5] * m2;
+        double n2 = P[6] * m0 + P[7] * m1 + P[8] * m2;
         const int c = col - pc;
-        Ms[col] = P[c];
-        Ms[TU + col] = P[3 + c];
-        Ms[2 * TU + col] = P[6 + c];
-      } else {
-        const double m0 = Ms[col], m1 = Ms[TU + col], m2 = Ms[2 * TU + col];
-        Ms[col] = P[0] * m0 + P[1] * m1 + P[2] * m2;
-        Ms[TU + col] = P[3] * m0 + P[4] * m1 + P[5] * m2;
-        Ms[2 * TU + col] = P[6] * m0 + P[7] * m1 + P[8] * m2;
+        if (c >= 0 && c < 3) {
+          n0 = c == 0 ? P[0] : (c == 1 ? P[1] : P[2]);
+          n1 = c == 0 ? P[3] : (c == 1 ? P[4] : P[5]);
+          n2 = c == 0 ? P[6] : (c == 1 ? P[7] : P[8]);
+        }
+        TMAT(pl, 0, col) = n0;
+        TMAT(pl, 1, col) = n1;
+        TMAT(pl, 2, col) = n2;
       }
     }
     __syncthreads();
     if (l != sp) {
-      // row -= F * pivot row over every column outside the pivot block;
-      // the pivot block becomes -F P
-#pragma unroll 8
-      for (int col = 0; col < TU; ++col) {
-        if (col / 3 == sp) continue;
-        const double s0 = Ms[col], s1 = Ms[TU + col], s2 = Ms[2 * TU + col];
+      // two chunks of 12 columns, all reads of a chunk before its writes
+#pragma unroll
+      for (int ch = 0; ch < HCOL; ch += HCOL / 2) {
+        double sv[3][HCOL / 2], mv[3][HCOL / 2];
+#pragma unroll
+        for (int e = 0; e < HCOL / 2; ++e)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            sv[r][e] = TMAT(pl, r, c0 + ch + e);
+            mv[r][e] = TMAT(nt, r, c0 + ch + e);
+          }
+#pragma unroll
+        for (int e = 0; e < HCOL / 2; ++e)
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+            TMAT(nt, r, c0 + ch + e) = mv[r][e] - (F[3 * r] * sv[0][e] + F[3 * r + 1] * sv[1][e] +
+                                                   F[3 * r + 2] * sv[2][e]);
+      }
+      if (pc / HCOL == half) {
 #pragma unroll
         for (int r = 0; r < 3; ++r)
-          Mr[r * TU + col] -= F[3 * r] * s0 + F[3 * r + 1] * s1 + F[3 * r + 2] * s2;
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            TMAT(nt, r, pc + c) = -(F[3 * r] * P[c] + F[3 * r + 1] * P[3 + c] +
+                                    F[3 * r + 2] * P[6 + c]);
       }
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          Mr[r * TU + pc + c] = -(F[3 * r] * P[c] + F[3 * r + 1] * P[3 + c] +
-                                  F[3 * r + 2] * P[6 + c]);
     }
+  }
+  __syncthreads();
+  // FP16 entries scaled by the row's largest magnitude (entries below ~6e-8
+  // of it, far under the preconditioner's accuracy, flush); each half
+  // stores its own columns
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double m = 0.0;
+#pragma unroll
+    for (int e = 0; e < HCOL; ++e) m = fmax(m, fabs(TMAT(nt, r, c0 + e)));
+    amx[(half * 64 + nt) * 3 + r] = (float)m;
   }
   __syncthreads();
   if (valid) {
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      // FP16 entries scaled by the row's largest magnitude (entries below
-      // ~6e-8 of it, far under the preconditioner's accuracy, flush)
-      double amax = 0.0;
-      for (int e = 0; e < TU; ++e) amax = fmax(amax, fabs(Mr[r * TU + e]));
-      const float sc = amax > 0.0 && isfinite(amax) ? (float)amax : 1.0f;
+      const float amax = fmaxf(amx[nt * 3 + r], amx[(64 + nt) * 3 + r]);
+      const float sc = amax > 0.0f && isfinite(amax) ? amax : 1.0f;
       const double inv = 1.0 / (double)sc;
-      T4s[(long long)r * N + g] = sc;
-      for (int q = 0; q < TU / 4; ++q) {
-        const double *m = Mr + r * TU + 4 * q;
+      if (half == 0) T4s[(long long)r * N + g] = sc;
+#pragma unroll
+      for (int q = c0 / 4; q < (c0 + HCOL) / 4; ++q) {
         tile4 h;
-        h.x = (_Float16)(m[0] * inv);
-        h.y = (_Float16)(m[1] * inv);
-        h.z = (_Float16)(m[2] * inv);
-        h.w = (_Float16)(m[3] * inv);
+        h.x = (_Float16)(TMAT(nt, r, 4 * q) * inv);
+        h.y = (_Float16)(TMAT(nt, r, 4 * q + 1) * inv);
+        h.z = (_Float16)(TMAT(nt, r, 4 * q + 2) * inv);
+        h.w = (_Float16)(TMAT(nt, r, 4 * q + 3) * inv);
         T4[(long long)(r * (TU / 4) + q) * N + g] = h;
       }
     }
@@ -375,6 +426,7 @@ __global__ void __launch_bounds__(64) k_tile_setup(Where W, long long N, const d
     atomicOr(flag + 1 + d, 1);
   }
 }
+#undef TMAT
 
 #define KPROLOGUE                                   \
   __shared__ double sh[64];                         \
@@ -732,7 +784,7 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
 }
 
 int launch_tile_setup(Batch &b, bool only_active) {
-  hipLaunchKernelGGL(k_tile_setup, dim3(SPP * b.nblk), dim3(64), 0, b.stream, make_where(b), b.ntot,
+  hipLaunchKernelGGL(k_tile_setup, dim3(SPP * b.nblk), dim3(128), 0, b.stream, make_where(b), b.ntot,
                      b.A64, only_active ? b.d_scal : nullptr, b.T4, b.T4s, b.d_flag);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
