@@ -40,6 +40,7 @@
 // Roofline: HBM/L2 bound.  Algorithmic bytes per node and BiCGStab
 // iteration, 2D (29 entries), matrix width w (8 or 4): 2 SpMV x
 // (29 w + 24 + 24 + 24) + 2 precond x (9 w + 48) + ~12 vector passes x 24.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -812,9 +813,16 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
                      b.ndev, rtol * rtol, maxit, per_dev ? 1 : 0, floor * floor);
   b.launches++;
   launch_k_bicg_init<T>(b, K);
+  // Iterations are issued in chunks between host checks of the stop flags;
+  // a check costs a host round trip with the stream drained, an iteration
+  // issued after convergence four early-exit launches.  Consecutive Newton
+  // solves need similar counts, so the first chunk is 3/4 of the last
+  // solve's, then 16, 32, 64 (without a history: chunk, doubling to 64).
   int it = 0;
+  int next = b.kry_hint > 0 ? std::max(chunk, (3 * b.kry_hint) / 4) : chunk;
+  int step = b.kry_hint > 0 ? 16 : chunk;
   while (true) {
-    for (int c = 0; c < chunk && it < maxit; ++c, ++it) launch_iteration<T>(b, K, it & 1);
+    for (int c = 0; c < next && it < maxit; ++c, ++it) launch_iteration<T>(b, K, it & 1);
     if (it >= maxit) {
       KArgs<T> K1 = K;
       K1.parity = it & 1;
@@ -826,8 +834,12 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
     int open = 0;
     for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].done ? 0 : 1;
     if (open == 0 || it >= maxit) break;
-    if (chunk < 64) chunk *= 2;
+    next = step;
+    if (step < 64) step *= 2;
   }
+  int most = 0;
+  for (int d = 0; d < b.ndev; ++d) most = std::max(most, b.h_scal[d].it);
+  b.kry_hint = most;
   return TCAD_OK;
 }
 
@@ -1214,7 +1226,11 @@ int pcg_poisson(Batch &b, double rtol, int maxit) {
                      b.ndev, rtol * rtol, maxit, 0, 0.0);
   hipLaunchKernelGGL(k_pcg_init, dim3(b.nblk), dim3(BS), 0, b.stream, P);
   b.launches += 2;
-  int it = 0, chunk = 16;
+  // chunks between host checks as in BiCGStab: 3/4 of the last solve's
+  // iterations first, then 16, 32, 64, 128
+  int it = 0;
+  int chunk = b.pcg_hint > 0 ? std::max(16, (3 * b.pcg_hint) / 4) : 16;
+  int step = 16;
   const dim3 g(b.nblk), t(BS);
   while (true) {
     for (int c = 0; c < chunk && it < maxit; ++c, ++it) {
@@ -1234,8 +1250,12 @@ int pcg_poisson(Batch &b, double rtol, int maxit) {
     int open = 0;
     for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].done ? 0 : 1;
     if (open == 0 || it >= maxit) break;
-    if (chunk < 128) chunk *= 2;
+    chunk = step;
+    if (step < 128) step *= 2;
   }
+  int most = 0;
+  for (int d = 0; d < b.ndev; ++d) most = std::max(most, b.h_scal[d].it);
+  b.pcg_hint = most;
   return TCAD_OK;
 }
 

@@ -152,6 +152,9 @@ struct Batch {
   int ne = 0;                       // stencil entries per node
   hipStream_t stream = nullptr;
   long long launches = 0;
+  // iterations of the last BiCGStab / PCG solve (slowest device): the next
+  // solve's first chunk is sized from it
+  int kry_hint = 0, pcg_hint = 0;
   std::vector<DevInfo> info;        // host copy
   DevInfo *d_info = nullptr;
   int *d_blk2dev = nullptr;
