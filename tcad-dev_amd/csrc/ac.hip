@@ -30,6 +30,7 @@
 // diagonal) + 2 preconditioner applications x (18 x 8 + 48 + 48) + ~12
 // complex vector passes x 48.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -79,6 +80,9 @@ struct CArgs {
   int nblk;
   const double *A;                  // scaled real Newton matrix, FP64
   const double *Mi;                 // 18 planes: complex node block inverses (re 0..8, im 9..17)
+  const tile4 *Tr, *Ti;             // complex tile inverse rows (tile != 0)
+  const float *Ts;                  // their per-row scales
+  int tile;                         // 1: complex tile block-Jacobi, 0: node blocks
   const double *dn, *dp;            // scaled storage diagonal (n and p rows)
   const double *omega;              // scaled angular frequency per device
   double *v[NCV];                   // complex vectors: re c*N + g, im (3 + c)*N + g
@@ -198,6 +202,305 @@ __global__ void __launch_bounds__(BS) k_ac_prec(CArgs K, double *Mi, int *flag) 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Complex tile block-Jacobi: the exact inverse of (A + j w D) restricted to
+// every 16-node tile (the tiles of the real solver: 4 x 4 in 2D, T3X x T3Y
+// x T3Z in 3D), rebuilt per frequency.  Node k keeps its 3 inverse rows as
+// real and imaginary planes of four FP16 entries (Tr, Ti: TILE_PLANES
+// planes each) with one FP32 scale per row (Ts), the largest |re| or |im|
+// of the row.
+// ---------------------------------------------------------------------------
+constexpr int CTU = TILE_U;                         // 48 unknowns per tile
+constexpr int CHC = CTU / 2;                        // columns per setup thread
+constexpr int CTPW = 2;                             // tiles per setup workgroup
+constexpr int CNODES = CTPW * TILE_NODES;           // 32 nodes per setup workgroup
+constexpr int CSCOLS = CTPW * TILE_X, CSCOLS3 = CTPW * T3X;
+constexpr int CSPP = (PX / CSCOLS) * (PY / TILE_Y); // setup workgroups per patch
+constexpr int CCH = 6;                              // elimination chunk (columns)
+static_assert(PX % CSCOLS == 0 && PX3 % CSCOLS3 == 0 && PX3 / CSCOLS3 == CSPP,
+              "complex setup groups must tile 2D and 3D patches alike");
+static_assert(CHC % 3 == 0 && CHC % 4 == 0 && CHC % CCH == 0, "halves hold whole blocks");
+
+// lane (0..31) of the complex setup workgroup that holds node l of tile tq
+__device__ __forceinline__ int csetup_lane(int dim, int tq, int l) {
+  if (dim == 3) {
+    const int il = l % T3X, ly = (l / T3X) % T3Y, lz = l / (T3X * T3Y);
+    return lz * (CSCOLS3 * T3Y) + ly * CSCOLS3 + tq * T3X + il;
+  }
+  return (l / TILE_X) * CSCOLS + tq * TILE_X + l % TILE_X;
+}
+
+// Complex inverse of a 3x3 block (cofactors); returns |det|
+__device__ __forceinline__ double cinv3(const Cx a[9], Cx P[9]) {
+  const Cx c0 = csub(cmul(a[4], a[8]), cmul(a[5], a[7]));
+  const Cx c1 = csub(cmul(a[5], a[6]), cmul(a[3], a[8]));
+  const Cx c2 = csub(cmul(a[3], a[7]), cmul(a[4], a[6]));
+  const Cx det = cadd(cadd(cmul(a[0], c0), cmul(a[1], c1)), cmul(a[2], c2));
+  const Cx adj[9] = {c0, csub(cmul(a[2], a[7]), cmul(a[1], a[8])),
+                     csub(cmul(a[1], a[5]), cmul(a[2], a[4])),
+                     c1, csub(cmul(a[0], a[8]), cmul(a[2], a[6])),
+                     csub(cmul(a[2], a[3]), cmul(a[0], a[5])),
+                     c2, csub(cmul(a[1], a[6]), cmul(a[0], a[7])),
+                     csub(cmul(a[0], a[4]), cmul(a[1], a[3]))};
+#pragma unroll
+  for (int e = 0; e < 9; ++e) P[e] = cdiv(adj[e], det);
+  return hypot(det.re, det.im);
+}
+
+#define CMR(lane, r, col) Mre[((r) * CTU + (col)) * CNODES + (lane)]
+#define CMI(lane, r, col) Mim[((r) * CTU + (col)) * CNODES + (lane)]
+
+// Setup: 64 threads per CTPW tiles side by side in x (32 nodes), two
+// threads per node over the two halves of the 48 columns; complex block
+// Gauss-Jordan in FP64 with the 3x3 node blocks as pivots, in place in
+// LDS (column-major, lane fastest, as the real tile setup).
+__global__ void __launch_bounds__(64) k_ac_tile_setup(CArgs K, tile4 *Tr, tile4 *Ti, float *Ts,
+                                                      int *flag) {
+  __shared__ double Mre[3 * CTU * CNODES], Mim[3 * CTU * CNODES];
+  __shared__ float amx[2 * CNODES * 3];
+  const int nt = threadIdx.x % CNODES, half = threadIdx.x / CNODES, c0 = half * CHC;
+  const int patch = blockIdx.x / CSPP, sub = blockIdx.x % CSPP;
+  const int d = K.W.single ? 0 : K.W.blk2dev[patch];
+  const DevInfo &I = dev_info(K.W, d);
+  const long long N = K.N;
+  const double w = K.omega[d];
+  const unsigned local = (unsigned)(patch - I.blk0);
+  const unsigned bx = local % (unsigned)I.px, t2 = local / (unsigned)I.px;
+  const unsigned by = t2 % (unsigned)I.py, bz = t2 / (unsigned)I.py;
+  int tq, l, i, j, kk;
+  bool has[6];
+  int ln[6];
+  if (I.dim == 3) {
+    const int lc = nt % CSCOLS3, ly = (nt / CSCOLS3) % T3Y, lz = nt / (CSCOLS3 * T3Y);
+    const int il = lc % T3X;
+    tq = lc / T3X;
+    l = (lz * T3Y + ly) * T3X + il;
+    i = (int)bx * PX3 + sub * CSCOLS3 + lc;
+    j = (int)by * PY3 + ly;
+    kk = (int)bz * PZ3 + lz;
+    has[0] = il > 0 && i > 0;
+    has[1] = il < T3X - 1 && i < I.nx - 1;
+    has[2] = ly > 0 && j > 0;
+    has[3] = ly < T3Y - 1 && j < I.ny - 1;
+    has[4] = lz > 0 && kk > 0;
+    has[5] = lz < T3Z - 1 && kk < I.nz - 1;
+    ln[2] = l - T3X; ln[3] = l + T3X;
+    ln[4] = l - T3X * T3Y; ln[5] = l + T3X * T3Y;
+  } else {
+    const int lc = nt % CSCOLS, lr = nt / CSCOLS;
+    const int il = lc % TILE_X;
+    tq = lc / TILE_X;
+    l = lr * TILE_X + il;
+    i = (int)bx * PX + (sub % (PX / CSCOLS)) * CSCOLS + lc;
+    j = (int)by * PY + (sub / (PX / CSCOLS)) * TILE_Y + lr;
+    kk = 0;
+    has[0] = il > 0 && i > 0;
+    has[1] = il < TILE_X - 1 && i < I.nx - 1;
+    has[2] = lr > 0 && j > 0;
+    has[3] = lr < TILE_Y - 1 && j < I.ny - 1;
+    has[4] = has[5] = false;
+    ln[2] = l - TILE_X; ln[3] = l + TILE_X;
+    ln[4] = ln[5] = l;
+  }
+  ln[0] = l - 1; ln[1] = l + 1;
+  const bool valid = i < I.nx && j < I.ny && kk < I.nz;
+  const long long g = I.off + ((long long)kk * I.ny + j) * I.nx + i;
+#pragma unroll
+  for (int e = 0; e < CHC; ++e)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      CMR(nt, r, c0 + e) = 0.0;
+      CMI(nt, r, c0 + e) = 0.0;
+    }
+  __syncthreads();
+  if (half == 0) {
+    if (valid) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) CMR(nt, r, 3 * l + c) = K.A[(3 * r + c) * N + g];
+      CMI(nt, 1, 3 * l + 1) = w * K.dn[g];
+      CMI(nt, 2, 3 * l + 2) = w * K.dp[g];
+#pragma unroll
+      for (int dir = 0; dir < 6; ++dir) {
+        if (!has[dir]) continue;
+        const double *o = K.A + (long long)(9 + 5 * dir) * N + g;
+        const int cb = 3 * ln[dir];
+        CMR(nt, 0, cb) = o[0];
+        CMR(nt, 1, cb) = o[N];
+        CMR(nt, 1, cb + 1) = o[2 * N];
+        CMR(nt, 2, cb) = o[3 * N];
+        CMR(nt, 2, cb + 2) = o[4 * N];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) CMR(nt, c, 3 * l + c) = 1.0;
+    }
+  }
+  bool bad = false;
+  for (int sp = 0; sp < TILE_NODES; ++sp) {
+    const int pc = 3 * sp;
+    const int pl = csetup_lane(I.dim, tq, sp);
+    __syncthreads();
+    Cx a[9], P[9], F[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        a[3 * r + c] = {CMR(pl, r, pc + c), CMI(pl, r, pc + c)};
+        F[3 * r + c] = {CMR(nt, r, pc + c), CMI(nt, r, pc + c)};
+      }
+    const double dm = cinv3(a, P);
+    if (l == sp && half == 0 && (!(dm > 1e-300) || !isfinite(dm))) bad = true;
+    __syncthreads();                            // pivot block read by all
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int cc = l + TILE_NODES * k;
+      if (cc < CHC) {
+        const int col = c0 + cc;
+        const Cx m[3] = {{CMR(pl, 0, col), CMI(pl, 0, col)}, {CMR(pl, 1, col), CMI(pl, 1, col)},
+                         {CMR(pl, 2, col), CMI(pl, 2, col)}};
+        const int c = col - pc;
+        const bool pb = c >= 0 && c < 3;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          Cx nv = cadd(cadd(cmul(P[3 * r], m[0]), cmul(P[3 * r + 1], m[1])),
+                       cmul(P[3 * r + 2], m[2]));
+          if (pb) nv = c == 0 ? P[3 * r] : (c == 1 ? P[3 * r + 1] : P[3 * r + 2]);
+          CMR(pl, r, col) = nv.re;
+          CMI(pl, r, col) = nv.im;
+        }
+      }
+    }
+    __syncthreads();
+    if (l != sp) {
+#pragma unroll
+      for (int ch = 0; ch < CHC; ch += CCH) {
+        Cx sv[3][CCH], mv[3][CCH];
+#pragma unroll
+        for (int e = 0; e < CCH; ++e)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            sv[r][e] = {CMR(pl, r, c0 + ch + e), CMI(pl, r, c0 + ch + e)};
+            mv[r][e] = {CMR(nt, r, c0 + ch + e), CMI(nt, r, c0 + ch + e)};
+          }
+#pragma unroll
+        for (int e = 0; e < CCH; ++e)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const Cx v = csub(mv[r][e], cadd(cadd(cmul(F[3 * r], sv[0][e]),
+                                                  cmul(F[3 * r + 1], sv[1][e])),
+                                             cmul(F[3 * r + 2], sv[2][e])));
+            CMR(nt, r, c0 + ch + e) = v.re;
+            CMI(nt, r, c0 + ch + e) = v.im;
+          }
+      }
+      if (pc / CHC == half) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const Cx v = cadd(cadd(cmul(F[3 * r], P[c]), cmul(F[3 * r + 1], P[3 + c])),
+                              cmul(F[3 * r + 2], P[6 + c]));
+            CMR(nt, r, pc + c) = -v.re;
+            CMI(nt, r, pc + c) = -v.im;
+          }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double m = 0.0;
+#pragma unroll
+    for (int e = 0; e < CHC; ++e)
+      m = fmax(m, fmax(fabs(CMR(nt, r, c0 + e)), fabs(CMI(nt, r, c0 + e))));
+    amx[(half * CNODES + nt) * 3 + r] = (float)m;
+  }
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const float amax = fmaxf(amx[nt * 3 + r], amx[(CNODES + nt) * 3 + r]);
+      const float sc = amax > 0.0f && isfinite(amax) ? amax : 1.0f;
+      const double inv = 1.0 / (double)sc;
+      if (half == 0) Ts[(long long)r * N + g] = sc;
+#pragma unroll
+      for (int q = c0 / 4; q < (c0 + CHC) / 4; ++q) {
+        tile4 hr, hi;
+        hr.x = (_Float16)(CMR(nt, r, 4 * q) * inv);
+        hr.y = (_Float16)(CMR(nt, r, 4 * q + 1) * inv);
+        hr.z = (_Float16)(CMR(nt, r, 4 * q + 2) * inv);
+        hr.w = (_Float16)(CMR(nt, r, 4 * q + 3) * inv);
+        hi.x = (_Float16)(CMI(nt, r, 4 * q) * inv);
+        hi.y = (_Float16)(CMI(nt, r, 4 * q + 1) * inv);
+        hi.z = (_Float16)(CMI(nt, r, 4 * q + 2) * inv);
+        hi.w = (_Float16)(CMI(nt, r, 4 * q + 3) * inv);
+        Tr[(long long)(r * (CTU / 4) + q) * N + g] = hr;
+        Ti[(long long)(r * (CTU / 4) + q) * N + g] = hi;
+      }
+    }
+  }
+  if (bad) {                              // error flags only; never part of a sum
+    atomicOr(flag, 1);
+    atomicOr(flag + 1 + d, 1);
+  }
+}
+#undef CMR
+#undef CMI
+
+// z = M^-1 x at this thread's node with the complex tile inverse rows read
+// at the point of use (every thread of the workgroup calls it: it holds
+// barriers; tv is LDS of BS * 6 doubles)
+__device__ __forceinline__ void ctile_apply(const CArgs &K, int dim, bool valid, long long g,
+                                            const double x[6], double z[6], double *tv) {
+  int tile, l;
+  tile_slot(dim, tile, l);
+  double *vr = tv + tile * CTU, *vi = tv + 3 * BS + tile * CTU;
+  __syncthreads();                          // tv free (a previous use is done)
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    vr[3 * l + c] = valid ? x[c] : 0.0;
+    vi[3 * l + c] = valid ? x[3 + c] : 0.0;
+  }
+  __syncthreads();
+  if (!valid) return;
+  const long long N = K.N;
+  double ar[3] = {0.0, 0.0, 0.0}, ai[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < CTU / 4; ++q) {
+    const double r0 = vr[4 * q], r1 = vr[4 * q + 1], r2 = vr[4 * q + 2], r3 = vr[4 * q + 3];
+    const double i0 = vi[4 * q], i1 = vi[4 * q + 1], i2 = vi[4 * q + 2], i3 = vi[4 * q + 3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const tile4 tr = K.Tr[(long long)(r * (CTU / 4) + q) * N + g];
+      const tile4 ti = K.Ti[(long long)(r * (CTU / 4) + q) * N + g];
+      ar[r] += ((double)tr.x * r0 + (double)tr.y * r1 + (double)tr.z * r2 + (double)tr.w * r3) -
+               ((double)ti.x * i0 + (double)ti.y * i1 + (double)ti.z * i2 + (double)ti.w * i3);
+      ai[r] += ((double)tr.x * i0 + (double)tr.y * i1 + (double)tr.z * i2 + (double)tr.w * i3) +
+               ((double)ti.x * r0 + (double)ti.y * r1 + (double)ti.z * r2 + (double)ti.w * r3);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double sc = (double)K.Ts[(long long)r * N + g];
+    z[r] = ar[r] * sc;
+    z[3 + r] = ai[r] * sc;
+  }
+}
+
+// z = M^-1 x: complex tiles (every thread of the workgroup calls it) or the
+// complex node blocks
+__device__ __forceinline__ void capply(const CArgs &K, int dim, bool valid, long long g,
+                                       const double x[6], double z[6]) {
+  if (K.tile) {
+    __shared__ double tv[BS * 6];
+    ctile_apply(K, dim, valid, g, x, z, tv);
+  } else if (valid) {
+    cprec(K, g, x, z);
+  }
+}
+
 // Port k's forcing on every node of every device: -kappa w on the nodes of
 // a gate port, 0 elsewhere; an ohmic port's 1 on its nodes is scattered
 // by k_ac_contact_forcing afterwards, from the contact's own node list
@@ -265,8 +568,8 @@ __global__ void __launch_bounds__(BS) k_cb_init(CArgs K) {
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
   double rr = 0.0, bb = 0.0;
+  double r[6] = {0, 0, 0, 0, 0, 0}, z[6];
   if (H.valid) {
-    double r[6];
     if (K.first) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -287,14 +590,13 @@ __global__ void __launch_bounds__(BS) k_cb_init(CArgs K) {
     cstore(K.v[CR], N, H.g, r);
     cstore(K.v[CRH], N, H.g, r);
     cstore(K.v[CP], N, H.g, r);
-    double z[6];
-    cprec(K, H.g, r, z);
-    cstore(K.v[CPH], N, H.g, z);
 #pragma unroll
     for (int c = 0; c < 6; ++c) rr += r[c] * r[c];
 #pragma unroll
     for (int c = 0; c < 3; ++c) bb += K.b[c * N + H.g] * K.b[c * N + H.g];
   }
+  capply(K, I.dim, H.valid, H.g, r, z);
+  if (H.valid) cstore(K.v[CPH], N, H.g, z);
   double in[2] = {rr, bb}, s[2];
   block_sums<2>(in, sh, s);
   if (threadIdx.x == 0) {
@@ -405,17 +707,17 @@ __global__ void __launch_bounds__(BS) k_cb_s(CArgs K) {
     }
     return;
   }
+  double s6[6] = {0, 0, 0, 0, 0, 0}, z[6];
   if (H.valid) {
-    double s6[6], z[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       s6[c] = r[c] - (alpha.re * v[c] - alpha.im * v[3 + c]);
       s6[3 + c] = r[3 + c] - (alpha.re * v[3 + c] + alpha.im * v[c]);
     }
     cstore(K.v[CS], N, H.g, s6);
-    cprec(K, H.g, s6, z);
-    cstore(K.v[CSH], N, H.g, z);
   }
+  capply(K, I.dim, H.valid, H.g, s6, z);
+  if (H.valid) cstore(K.v[CSH], N, H.g, z);
   if (lead) {
     S.alpha[0] = alpha.re;
     S.alpha[1] = alpha.im;
@@ -464,8 +766,8 @@ __global__ void __launch_bounds__(BS) k_cb_u(CArgs K) {
   const Cx rho_old = {S.rho[K.parity][0], S.rho[K.parity][1]};
   const Cx beta = cmul(cdiv(rho_new, rho_old), cdiv(alpha, om_b));
   double rr = 0.0;
+  double pn[6] = {0, 0, 0, 0, 0, 0}, z[6];
   if (H.valid) {
-    double pn[6], z[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       x[c] = x[c] + (alpha.re * ph[c] - alpha.im * ph[3 + c]) +
@@ -486,9 +788,9 @@ __global__ void __launch_bounds__(BS) k_cb_u(CArgs K) {
     cstore(K.v[CX], N, H.g, x);
     cstore(K.v[CR], N, H.g, s6);
     cstore(K.v[CP], N, H.g, pn);
-    cprec(K, H.g, pn, z);
-    cstore(K.v[CPH], N, H.g, z);
   }
+  capply(K, I.dim, H.valid, H.g, pn, z);
+  if (H.valid) cstore(K.v[CPH], N, H.g, z);
   const double a = block_sum(rr, sh);
   if (threadIdx.x == 0) K.part[blockIdx.x] = a;
   if (lead) {
@@ -592,8 +894,13 @@ struct AcWork {
   double *part = nullptr, *omega = nullptr, *ycol = nullptr;
   ACScal *acs = nullptr;
   int *act = nullptr, *row_dev = nullptr;
+  void *tbase = nullptr;            // complex tile rows
+  tile4 *Tr = nullptr, *Ti = nullptr;
+  float *Ts = nullptr;
+  int tile = 1;
   ~AcWork() {
     if (base) (void)hipFree(base);
+    if (tbase) (void)hipFree(tbase);
   }
 };
 
@@ -618,6 +925,16 @@ int alloc_work(Batch &b, int nrows, AcWork &w) {
   int *ip = reinterpret_cast<int *>(w.acs + b.ndev);
   w.act = ip;
   w.row_dev = ip + b.ndev;
+  // complex tiles unless TCADHIP_AC_PRECOND=node (the complex node blocks)
+  const char *e = std::getenv("TCADHIP_AC_PRECOND");
+  w.tile = (e && std::strcmp(e, "node") == 0) ? 0 : 1;
+  if (w.tile) {
+    const size_t tb = 2 * (size_t)TILE_PLANES * N * sizeof(tile4) + 3 * N * sizeof(float);
+    TCAD_HIP_CHECK(hipMalloc(&w.tbase, tb));
+    w.Tr = reinterpret_cast<tile4 *>(w.tbase);
+    w.Ti = w.Tr + (size_t)TILE_PLANES * N;
+    w.Ts = reinterpret_cast<float *>(w.Ti + (size_t)TILE_PLANES * N);
+  }
   return TCAD_OK;
 }
 
@@ -628,6 +945,10 @@ CArgs make_cargs(Batch &b, const AcWork &w) {
   K.nblk = b.nblk;
   K.A = b.A64;
   K.Mi = w.Mi;
+  K.Tr = w.Tr;
+  K.Ti = w.Ti;
+  K.Ts = w.Ts;
+  K.tile = w.tile;
   K.dn = w.dn;
   K.dp = w.dp;
   K.omega = w.omega;
@@ -805,7 +1126,11 @@ int y_parameters_impl(Batch &b, const int32_t *h_active, int nfreq, const double
         om[d] = 2.0 * M_PI * h_freqs[f] * (b.info[d].Ns / b.info[d].R0);
       TCAD_HIP_CHECK(hipMemcpyAsync(w.omega, om.data(), sizeof(double) * b.ndev,
                                     hipMemcpyHostToDevice, b.stream));
-      hipLaunchKernelGGL(k_ac_prec, dim3(b.nblk), dim3(BS), 0, b.stream, K, w.Mi, b.d_flag);
+      if (w.tile)
+        hipLaunchKernelGGL(k_ac_tile_setup, dim3(CSPP * b.nblk), dim3(64), 0, b.stream, K, w.Tr,
+                           w.Ti, w.Ts, b.d_flag);
+      else
+        hipLaunchKernelGGL(k_ac_prec, dim3(b.nblk), dim3(BS), 0, b.stream, K, w.Mi, b.d_flag);
       b.launches++;
       TCAD_HIP_CHECK(hipGetLastError());
       if ((rc = check_flag(b))) return rc;

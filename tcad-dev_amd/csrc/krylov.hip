@@ -129,20 +129,6 @@ static_assert(64 % TILE_NODES == 0 && PX % SCOLS == 0 && PX3 % SCOLS3 == 0,
 constexpr int SPP = (PX / SCOLS) * (PY / TILE_Y);   // setup workgroups per patch
 static_assert(PX3 / SCOLS3 == SPP, "2D and 3D patches need the same setup grid");
 
-// tile slot of this thread inside its patch, and node index inside the tile
-__device__ __forceinline__ void tile_slot(int dim, int &tile, int &l) {
-  if (dim == 3) {
-    const int il = threadIdx.x % PX3, jl = (threadIdx.x / PX3) % PY3,
-              kl = threadIdx.x / (PX3 * PY3);
-    tile = il / T3X;                        // PY3 = T3Y, PZ3 = T3Z: one tile row
-    l = (kl * T3Y + jl) * T3X + il % T3X;
-  } else {
-    const int il = threadIdx.x % PX, jl = threadIdx.x / PX;
-    tile = (jl / TILE_Y) * (PX / TILE_X) + il / TILE_X;
-    l = (jl % TILE_Y) * TILE_X + il % TILE_X;
-  }
-}
-
 // The inverse rows of this thread's node, loaded into registers at the top
 // of a kernel, before its reductions: they depend on nothing the kernel
 // computes, so their memory latency overlaps the reductions' (left to the

@@ -130,6 +130,21 @@ __device__ __forceinline__ NodeHood hood(const DevInfo &I, int blk) {
 }
 
 
+// Tile slot of this thread inside its node patch (tile index within the
+// patch) and its node's place in the 16-node tile (x fastest).
+__device__ __forceinline__ void tile_slot(int dim, int &tile, int &l) {
+  if (dim == 3) {
+    const int il = threadIdx.x % PX3, jl = (threadIdx.x / PX3) % PY3,
+              kl = threadIdx.x / (PX3 * PY3);
+    tile = il / T3X;                        // PY3 = T3Y, PZ3 = T3Z: one tile row
+    l = (kl * T3Y + jl) * T3X + il % T3X;
+  } else {
+    const int il = threadIdx.x % PX, jl = threadIdx.x / PX;
+    tile = (jl / TILE_Y) * (PX / TILE_X) + il / TILE_X;
+    l = (jl % TILE_Y) * TILE_X + il % TILE_X;
+  }
+}
+
 inline Where make_where(Batch &b) {
   Where W{};
   W.info = b.d_info;
