@@ -29,6 +29,7 @@
 // and iteration, 2D: 2 SpMV x (29 x 8 matrix + 48 x + 48 y + 16 storage
 // diagonal) + 2 preconditioner applications x (18 x 8 + 48 + 48) + ~12
 // complex vector passes x 48.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -898,6 +899,7 @@ struct AcWork {
   tile4 *Tr = nullptr, *Ti = nullptr;
   float *Ts = nullptr;
   int tile = 1;
+  int hint[16] = {};                // iterations of the last sweep of each index (slowest device)
   ~AcWork() {
     if (base) (void)hipFree(base);
     if (tbase) (void)hipFree(tbase);
@@ -998,7 +1000,14 @@ int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int ma
     K.first = sweep == 0 && !warm ? 1 : 0;
     hipLaunchKernelGGL(k_cb_init, dim3(b.nblk), dim3(BS), 0, b.stream, K);
     b.launches += 2;
-    int it = 0, chunk = 8;
+    // chunks between host checks as in the real solver: 3/4 of the
+    // iterations of the last solve's sweep of the same index (a solve's
+    // first sweep is long, its restarts short), then 16, 32, 64
+    static_assert(kMaxSweeps <= 16, "one hint per sweep index");
+    const int hint = w.hint[sweep];
+    int it = 0;
+    int chunk = hint > 0 ? std::max(8, (3 * hint) / 4) : 8;
+    int step = hint > 0 ? 16 : 8;
     while (true) {
       for (int c = 0; c < chunk && it < maxit; ++c, ++it) launch_citeration(b, K, it & 1);
       if (it >= maxit) {
@@ -1014,8 +1023,12 @@ int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int ma
       int left = 0;
       for (int d = 0; d < b.ndev; ++d) left += hs[d].done ? 0 : 1;
       if (left == 0 || it >= maxit) break;
-      if (chunk < 64) chunk *= 2;
+      chunk = step;
+      if (step < 64) step *= 2;
     }
+    int most = 0;
+    for (int d = 0; d < b.ndev; ++d) most = std::max(most, hs[d].it);
+    w.hint[sweep] = most;
     for (int d = 0; d < b.ndev; ++d) {
       if (!act[d]) continue;
       const ACScal &S = hs[d];
