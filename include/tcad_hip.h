@@ -48,7 +48,7 @@ extern "C" {
 #define TCAD_E_NO_DEVICE (-6)
 
 /* Linear-solve modes (pytcad/pytcad/linsolve.py:338 `_METHODS` analogue). */
-#define TCAD_LIN_BICGSTAB_F64 0   /* FP64 BiCGStab + node block-Jacobi       */
+#define TCAD_LIN_BICGSTAB_F64 0   /* FP64 BiCGStab + block-Jacobi            */
 #define TCAD_LIN_BICGSTAB_MIXED 1 /* FP32 BiCGStab + block-Jacobi in FP64 IR */
 /* Flag OR'd into lin_mode of tcad_solve_bias: Newton forcing terms.  Each
  * Newton solve's relative residual target is clamp(0.01 * err, lin_rtol,
@@ -244,8 +244,9 @@ int tcad_continuity_residuals(tcad_batch *b, const double *d_psi, const double *
  * Y[d][f][i][k] = dI_i / dV_k [A/V; per cm in 2D] as (re, im) at
  * h_Y[2 * (((d * nfreq + f) * P + i) * P + k)], P the largest port count
  * of the batch (entries beyond a device's ports are 0).  Each complex
- * system is solved by FP64 complex BiCGStab with restarted sweeps until
- * the true residual is below rtol relative to the forcing (maxiter
+ * system is solved by FP64 complex BiCGStab, preconditioned with the
+ * exact inverse of J + j w C on every 16-node tile, with restarted sweeps
+ * until the true residual is below rtol relative to the forcing (maxiter
  * iterations per sweep).  h_status[d]: TCAD_NEWTON_OK, or
  * TCAD_NEWTON_LINEAR_FAIL when a solve stalled above 1e-6; h_iters[d]
  * (may be NULL): Krylov iterations spent.  h_active as for
