@@ -109,6 +109,20 @@ def spmv_bytes(spec, w=8):
     return (9 * N + 5 * edges) * w + 3 * N * 8 * 3
 
 
+def _cpu_diode(idx):
+    """One batch diode's continuation sweep on the FP64 CPU oracle (NumPy +
+    SciPy SuperLU), in a worker process of bench_batch's CPU baseline;
+    returns (final currents in contact order, seconds)."""
+    import warnings
+    warnings.simplefilter("ignore")
+    from tcadhip import configs
+    from oracle_build import oracle_sweep
+    spec = configs.batch_diodes()[idx]
+    t0 = time.perf_counter()
+    _, _, cur, _ = oracle_sweep(spec)
+    return np.asarray(cur[-1]).tolist(), time.perf_counter() - t0
+
+
 def bench_trigate(vds=(0.05, 0.7)):
     """3D tri-gate 64x48x48 (147,456 nodes, 442,368 unknowns): equilibrium
     + 17 Vg points at each Vd, FP64 path; Newton counts per point."""
@@ -138,6 +152,34 @@ def bench_trigate(vds=(0.05, 0.7)):
                           "id_first_last_A": [ids[0], ids[-1]]}
     out["points_per_s"] = round(total_pts / total_s, 4)
     out["nodes"] = 147456
+    # the CPU oracle's direct LU is out of reach at 442,368 unknowns: the
+    # GPU/CPU comparison runs on a reduced tri-gate (same builder, 24x18x14,
+    # Vd = 0.05 V, Vg 0 -> 0.8 V in 5 points), equilibrium included on both
+    from oracle_build import oracle_sweep
+    rs = configs.trigate_3d(nx=24, ny=18, nz=14, Vd=0.05, vg_list=[0.0, 0.2, 0.4, 0.6, 0.8])
+    gcur = None
+    for rep in range(2):                       # the first run warms up
+        dev = device_from_spec(rs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.solve_equilibrium(NewtonOptions(max_iter=400))
+        gcur = []
+        for bias in rs.sweep:
+            dev.solve_bias(bias, opts)
+            gcur.append([dev.terminal_current(c.name) for c in rs.contacts])
+        torch.cuda.synchronize()
+        dt_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, _, ocur, _ = oracle_sweep(rs)
+    dt_cpu = time.perf_counter() - t0
+    gcur = np.array(gcur)
+    err = float(max(np.abs(gcur[p] - ocur[p]).max() / np.abs(ocur[p]).max()
+                    for p in range(len(rs.sweep))))
+    out["reduced_24x18x14_vs_cpu_oracle"] = {
+        "points": len(rs.sweep), "gpu_points_per_s": round(len(rs.sweep) / dt_gpu, 3),
+        "cpu_oracle_points_per_s": round(len(rs.sweep) / dt_cpu, 4), "cpu_cores": 1,
+        "cpu": "FP64 oracle (NumPy + SciPy SuperLU), equilibrium + 5 points",
+        "max_current_err_vs_oracle_rel_to_max": err}
     return out
 
 
@@ -194,6 +236,29 @@ def bench_batch(lanes=1, sample=8):
            "plain_fp64_devices_per_s": round(plain, 4),
            "plain_sample": f"{len(idx)} evenly spaced devices solved one at a time",
            "spmv_roofline_whole_batch": roof}
+    # CPU baseline: the FP64 oracle, one device per worker process (one
+    # worker per host core, at most 16: one GPU's share of the host), on an
+    # evenly spaced sample; the rate is bounded by the sample's slowest
+    # device, so the mean per-device time is reported too
+    import multiprocessing as mp
+    workers = max(1, min(16, os.cpu_count() or 1))
+    cidx = list(range(0, len(specs), max(1, len(specs) // workers)))[:workers]
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(workers) as pool:
+        cres = pool.map(_cpu_diode, cidx, chunksize=1)
+    dt_cpu = time.perf_counter() - t0
+    cok = True
+    for d, (ccur, _) in zip(cidx, cres):
+        names = [c.name for c in specs[d].contacts]
+        got = np.array([res["bicgstab"]["results"][d].currents[-1][nm] for nm in names])
+        ref = np.array(ccur)
+        cok = cok and bool(np.all(np.abs(got - ref) <= 1e-6 * np.abs(ref).max()
+                                  + 10.0 * abs(ref.sum())))
+    out["cpu_oracle"] = {"devices_per_s": round(len(cidx) / dt_cpu, 4), "workers": workers,
+                         "mean_seconds_per_device": round(float(np.mean([t for _, t in cres])), 2),
+                         "sample": f"{len(cidx)} evenly spaced devices, FP64 oracle (NumPy + "
+                                   f"SciPy SuperLU), one per worker process",
+                         "gpu_fp64_batch_meets_1e-6_vs_oracle": cok}
     for mode, r in res.items():
         fails = sum(1 for x in r["results"] if x.failed)
         err, ok = 0.0, True
