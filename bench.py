@@ -197,11 +197,12 @@ def bench_batch(lanes=1, sample=8):
         setups = [setup_from_spec(s) for s in specs]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r, _ = sweep.run_lanes(setups, [s.sweep for s in specs], NewtonOptions(linsolve=mode),
-                               lanes=lanes)
+        r, bss = sweep.run_lanes(setups, [s.sweep for s in specs], NewtonOptions(linsolve=mode),
+                                 lanes=lanes)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        res[mode] = dict(seconds=dt, results=r)
+        res[mode] = dict(seconds=dt, results=r, rounds=sum(b.rounds for b in bss),
+                         rebuilds=sum(b.rebuilds for b in bss))
         print(f"[bench] batch {mode}: {len(specs) / dt:.3f} devices/s", file=sys.stderr,
               flush=True)
     idx = list(range(0, len(specs), len(specs) // sample))[:sample]
@@ -273,6 +274,7 @@ def bench_batch(lanes=1, sample=8):
                                     + 10.0 * abs(ref.sum())))
         out[mode] = {"devices_per_s": round(len(specs) / r["seconds"], 4),
                      "seconds": round(r["seconds"], 3), "failed": fails,
+                     "rounds": r["rounds"], "compactions": r["rebuilds"],
                      "vs_plain_fp64": round(len(specs) / r["seconds"] / plain, 2),
                      "max_rel_current_err_vs_fp64_on_sample": err,
                      "meets_1e-6_with_conservation_floor": ok}

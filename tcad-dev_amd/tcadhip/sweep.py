@@ -13,7 +13,13 @@ A step that succeeds grows by `growth` up to the full distance.
 
 The devices share every kernel launch: one round issues one masked
 tcad_solve_bias over all devices that still have points to reach; the
-others are masked out and keep their state untouched.
+others are masked out and keep their state untouched.  Masked devices
+still cost their share of every launch (their workgroups start and
+return), so once the open devices hold less than `compact` of the
+batch's nodes the sweep continues on a batch of the open devices alone,
+built over the same planes at the same offsets (no state moves; every
+per-device reduction runs over the device's own blocks, so the results
+are the bits the full batch would give).
 """
 import time
 from dataclasses import dataclass, field
@@ -182,8 +188,9 @@ class BatchedSweep:
     devices must be `setups` in order."""
 
     def __init__(self, setups, sweeps, opts=None, batch=None, max_step=None, min_step=None,
-                 growth=1.5, shrink=0.5, max_attempts=64):
+                 growth=1.5, shrink=0.5, max_attempts=64, compact=0.5):
         self.setups = list(setups)
+        self.compact = compact
         if len(sweeps) != len(self.setups):
             raise ValueError("one bias list per device")
         self.opts = opts or NewtonOptions()
@@ -204,11 +211,32 @@ class BatchedSweep:
             for d, t in enumerate(self.tracks):
                 if status[d] != _lib.NEWTON_OK or not conv[d]:
                     t.result.failed = "equilibrium did not converge"
+        ids = list(range(len(self.tracks)))
+        self.rebuilds = 0
         while any(t.open for t in self.tracks):
+            open_ids = [i for i in ids if self.tracks[i].open]
+            if (self.compact and len(open_ids) < len(ids)
+                    and self._nodes(open_ids) < self.compact * self._nodes(ids)):
+                B, ids = self._subbatch(open_ids), open_ids
+                self.rebuilds += 1
             self.rounds += 1
-            reached = solve_round(B, self.tracks, opts)
-            record_currents(B, self.tracks, reached, time.perf_counter() - t0)
+            tracks = [self.tracks[i] for i in ids]
+            reached = solve_round(B, tracks, opts)
+            record_currents(B, tracks, reached, time.perf_counter() - t0)
         return self.results
+
+    def _nodes(self, idx):
+        return sum(self.setups[i].N for i in idx)
+
+    def _subbatch(self, idx):
+        """A batch of the devices idx over the full batch's planes, at
+        their offsets there."""
+        full = self.B
+        planes = {k: full._t[k] for k in GPUBatch.FIELD_KEYS}
+        planes.update(psi=full.psi, n=full.n, p=full.p)
+        return GPUBatch([self.setups[i] for i in idx],
+                        offsets=[int(full.offsets[i]) for i in idx], planes=planes,
+                        stream=full.stream)
 
 
 def run_lanes(setups, sweeps, opts=None, lanes=1, max_step=None):
