@@ -407,7 +407,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-points", type=int, default=3)
+    ap.add_argument("--cpu-points", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--points", type=int, default=None,
                     help="first K bias points only (short profiling runs)")

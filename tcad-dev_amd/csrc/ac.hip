@@ -1030,9 +1030,12 @@ int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int ma
       chunk = step;
       if (step < 64) step *= 2;
     }
-    int most = 0;
-    for (int d = 0; d < b.ndev; ++d) most = std::max(most, hs[d].it);
-    w.hint[sweep] = most;
+    int most = 0, failed = it >= maxit ? 1 : 0;
+    for (int d = 0; d < b.ndev; ++d) {
+      most = std::max(most, hs[d].it);
+      if (act[d]) failed |= hs[d].status;
+    }
+    w.hint[sweep] = failed ? 0 : most;
     for (int d = 0; d < b.ndev; ++d) {
       if (!act[d]) continue;
       const ACScal &S = hs[d];

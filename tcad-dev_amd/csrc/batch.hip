@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -712,6 +713,7 @@ static int solve_equilibrium_impl(Batch &b, int max_iter, double tol, double max
   UArgs U = uargs(b);
   for (int it = 0; it < max_iter; ++it) {
     if ((rc = launch_assemble_poisson(b, psi, b.F))) return rc;
+    if (b.pcg_tile && (rc = launch_pcg_tile_setup(b))) return rc;
     if ((rc = launch_flags_to_status(b))) return rc;
     if ((rc = pcg_poisson(b, 1e-12, 20000))) return rc;
     hipLaunchKernelGGL(k_poisson_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.veq[0], psi,
@@ -764,7 +766,7 @@ static void free_batch(Batch *b) {
                   b->d_cscale, b->d_c2dev, b->d_gptr, b->d_gnode, b->d_surf[0], b->d_surf[1], b->d_surf[2],
                   b->d_surf[3], b->A64, b->A32, b->Dinv64, b->Dinv32, b->bs, b->rowscale,
                   b->colscale, b->y64, b->F, b->Fnp, b->part, b->d_scal, b->Aeq, b->beq,
-                  b->d_flag, b->d_mask, b->cand, b->d_Icand, b->T4, b->T4s, b->d_dts};
+                  b->d_flag, b->d_mask, b->cand, b->d_Icand, b->T4, b->T4s, b->P16, b->d_dts};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (int i = 0; i < 9; ++i)
@@ -1003,6 +1005,17 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   TRY(dalloc(&b->d_dts, (size_t)ndev));
   TRY(dalloc(&b->T4, (size_t)TILE_PLANES * N));
   TRY(dalloc(&b->T4s, 3 * N));
+  TRY(dalloc(&b->P16, (size_t)TILE_NODES * N));
+  {
+    // equilibrium PCG: Jacobi (default), or tile block-Jacobi with
+    // TCADHIP_PCG_PRECOND=tile (half the PCG iterations on the MOSFET and
+    // 4% on its sweep, but its equilibrium state differs from Jacobi's in the
+    // last bits, which moves the Newton path of a reverse-biased
+    // Crank-Nicolson diode ramp off the reference's: 5 instead of 8
+    // iterations at one step)
+    const char *e = std::getenv("TCADHIP_PCG_PRECOND");
+    b->pcg_tile = (e && std::strcmp(e, "tile") == 0) ? 1 : 0;
+  }
   TRY(dalloc(&b->cand, 3 * N));
   TRY(dalloc(&b->d_Icand, (size_t)std::max(1, cglobal)));
 #undef TRY

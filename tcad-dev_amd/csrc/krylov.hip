@@ -30,7 +30,8 @@
 //   (linsolve "bicgstab") or FP32 ("bicgstab_mixed"; FP64 vector
 //   arithmetic -- FP32 vector arithmetic was measured unstable on these
 //   Jacobians, residual growth by 1e20+ in the second sweep).
-// * PCG (Jacobi) for the SPD equilibrium Poisson system, FP64, same
+// * PCG (Jacobi; tile block-Jacobi as an option) for the SPD equilibrium
+//   Poisson system, FP64, same
 //   reduction scheme (two kernels per iteration: the direction update is
 //   recomputed at the neighbours inside the product kernel).
 //
@@ -194,6 +195,58 @@ __device__ __forceinline__ void tile_apply(const Rows &R, int dim, bool valid, c
   for (int r = 0; r < 3; ++r) z[r] = acc[r] * (double)tile_scale(R, r);
 }
 
+// Node of lane nt (0..63) of a tile-setup workgroup (setup workgroup sub of
+// patch (bx, by, bz) of device I): its coordinates (i, j, kk), its tile in
+// the group (tq), its place in the tile (l), and which of its neighbours
+// x-, x+, y-, y+, z-, z+ lie in the tile (has) at which places (ln).
+struct SetupNode {
+  int tq, l, i, j, kk;
+  bool has[6];
+  int ln[6];
+};
+
+__device__ __forceinline__ SetupNode setup_node(const DevInfo &I, unsigned bx, unsigned by,
+                                                unsigned bz, int sub, int nt) {
+  SetupNode S;
+  int &tq = S.tq, &l = S.l, &i = S.i, &j = S.j, &kk = S.kk;
+  bool *has = S.has;
+  int *ln = S.ln;
+  if (I.dim == 3) {
+    const int lc = nt % SCOLS3, ly = (nt / SCOLS3) % T3Y, lz = nt / (SCOLS3 * T3Y);
+    const int il = lc % T3X;
+    tq = lc / T3X;
+    l = (lz * T3Y + ly) * T3X + il;
+    i = (int)bx * PX3 + sub * SCOLS3 + lc;
+    j = (int)by * PY3 + ly;
+    kk = (int)bz * PZ3 + lz;
+    has[0] = il > 0 && i > 0;
+    has[1] = il < T3X - 1 && i < I.nx - 1;
+    has[2] = ly > 0 && j > 0;
+    has[3] = ly < T3Y - 1 && j < I.ny - 1;
+    has[4] = lz > 0 && kk > 0;
+    has[5] = lz < T3Z - 1 && kk < I.nz - 1;
+    ln[2] = l - T3X; ln[3] = l + T3X;
+    ln[4] = l - T3X * T3Y; ln[5] = l + T3X * T3Y;
+  } else {
+    const int lc = nt % SCOLS, lr = nt / SCOLS;
+    const int il = lc % TILE_X;
+    tq = lc / TILE_X;
+    l = lr * TILE_X + il;
+    i = (int)bx * PX + (sub % (PX / SCOLS)) * SCOLS + lc;
+    j = (int)by * PY + (sub / (PX / SCOLS)) * TILE_Y + lr;
+    kk = 0;
+    has[0] = il > 0 && i > 0;
+    has[1] = il < TILE_X - 1 && i < I.nx - 1;
+    has[2] = lr > 0 && j > 0;
+    has[3] = lr < TILE_Y - 1 && j < I.ny - 1;
+    has[4] = has[5] = false;
+    ln[2] = l - TILE_X; ln[3] = l + TILE_X;
+    ln[4] = ln[5] = l;
+  }
+  ln[0] = l - 1; ln[1] = l + 1;
+  return S;
+}
+
 // Setup: one workgroup of 128 threads per STPW tiles side by side in x (64
 // nodes), two threads per node, each owning half of the 48 columns; block
 // Gauss-Jordan in FP64 with the 3x3 node blocks as pivots (no row exchange,
@@ -237,45 +290,10 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
   const unsigned local = (unsigned)(patch - I.blk0);
   const unsigned bx = local % (unsigned)I.px, t2 = local / (unsigned)I.px;
   const unsigned by = t2 % (unsigned)I.py, bz = t2 / (unsigned)I.py;
-  // this thread's node (i, j, kk), its tile in the group (tq), its place in
-  // the tile (l), and which of its neighbours x-, x+, y-, y+, z-, z+ lie in
-  // the tile (ln: their places)
-  int tq, l, i, j, kk;
-  bool has[6];
-  int ln[6];
-  if (I.dim == 3) {
-    const int lc = nt % SCOLS3, ly = (nt / SCOLS3) % T3Y, lz = nt / (SCOLS3 * T3Y);
-    const int il = lc % T3X;
-    tq = lc / T3X;
-    l = (lz * T3Y + ly) * T3X + il;
-    i = (int)bx * PX3 + sub * SCOLS3 + lc;
-    j = (int)by * PY3 + ly;
-    kk = (int)bz * PZ3 + lz;
-    has[0] = il > 0 && i > 0;
-    has[1] = il < T3X - 1 && i < I.nx - 1;
-    has[2] = ly > 0 && j > 0;
-    has[3] = ly < T3Y - 1 && j < I.ny - 1;
-    has[4] = lz > 0 && kk > 0;
-    has[5] = lz < T3Z - 1 && kk < I.nz - 1;
-    ln[2] = l - T3X; ln[3] = l + T3X;
-    ln[4] = l - T3X * T3Y; ln[5] = l + T3X * T3Y;
-  } else {
-    const int lc = nt % SCOLS, lr = nt / SCOLS;
-    const int il = lc % TILE_X;
-    tq = lc / TILE_X;
-    l = lr * TILE_X + il;
-    i = (int)bx * PX + (sub % (PX / SCOLS)) * SCOLS + lc;
-    j = (int)by * PY + (sub / (PX / SCOLS)) * TILE_Y + lr;
-    kk = 0;
-    has[0] = il > 0 && i > 0;
-    has[1] = il < TILE_X - 1 && i < I.nx - 1;
-    has[2] = lr > 0 && j > 0;
-    has[3] = lr < TILE_Y - 1 && j < I.ny - 1;
-    has[4] = has[5] = false;
-    ln[2] = l - TILE_X; ln[3] = l + TILE_X;
-    ln[4] = ln[5] = l;
-  }
-  ln[0] = l - 1; ln[1] = l + 1;
+  const SetupNode SN = setup_node(I, bx, by, bz, sub, nt);
+  const int tq = SN.tq, l = SN.l, i = SN.i, j = SN.j, kk = SN.kk;
+  const bool *has = SN.has;
+  const int *ln = SN.ln;
   const bool valid = i < I.nx && j < I.ny && kk < I.nz;
   const long long g = I.off + ((long long)kk * I.ny + j) * I.nx + i;
 #pragma unroll
@@ -823,9 +841,14 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
     next = step;
     if (step < 64) step *= 2;
   }
-  int most = 0;
-  for (int d = 0; d < b.ndev; ++d) most = std::max(most, b.h_scal[d].it);
-  b.kry_hint = most;
+  // a solve stopped by its cap (or failed) says nothing about the next one's
+  // count: no hint then
+  int most = 0, failed = it >= maxit ? 1 : 0;
+  for (int d = 0; d < b.ndev; ++d) {
+    most = std::max(most, b.h_scal[d].it);
+    if (b.h_scal[d].active) failed |= b.h_scal[d].status;
+  }
+  b.kry_hint = failed ? 0 : most;
   return TCAD_OK;
 }
 
@@ -958,13 +981,18 @@ __global__ void k_ir_check(IRArgs R, int ndev, double tol2, int per_dev) {
   }
 }
 
+// y += ||r|| d for the devices whose inner solve did not diverge: a sweep
+// that ended with a larger residual than it started with (S.rr > S.bb, the
+// inner solve's own figures) would only move y away; the next sweep then
+// starts again from the same y
 __global__ void __launch_bounds__(BS) k_ir_update(IRArgs R) {
   const int d = dev_index(R.W);
   const DevInfo &I = dev_info(R.W, d);
   Scal &S = R.scal[d];
   if (!S.active) return;
   const NodeHood H = hood(I, blockIdx.x);
-  if (H.valid) {
+  const bool diverged = !(S.rr <= S.bb);
+  if (H.valid && !diverged) {
     const long long N = R.N;
     const double nrm = S.nrm;
 #pragma unroll
@@ -1000,6 +1028,12 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
   const dim3 gd((b.ndev + 63) / 64), td(64);
   hipLaunchKernelGGL(k_ir_setup, gd, td, 0, b.stream, b.d_scal, b.ndev, maxsweep);
   b.launches++;
+  const int tile_mode = b.tile;
+  struct Restore {
+    Batch &b;
+    int t;
+    ~Restore() { b.tile = t; }
+  } restore{b, tile_mode};
   for (int sweep = 0; sweep <= maxsweep; ++sweep) {
     hipLaunchKernelGGL(k_ir_residual, dim3(b.nblk), dim3(BS), 0, b.stream, R,
                        (restart && sweep == 0) ? 1 : 0);
@@ -1019,6 +1053,16 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
               : run_bicgstab<double>(b, make_kargs<double>(b, false), inner_rtol, inner_maxit, 8,
                                      per_dev, 0.0);
     if (rc) return rc;
+    // a tile-preconditioned sweep that broke down or diverged (FP16 tile
+    // inverses of a nearly singular tile, as some Crank-Nicolson steps at
+    // reverse bias assemble) is followed by sweeps with the FP64 node
+    // block-Jacobi, the reference's own preconditioner, for the rest of
+    // this solve
+    if (b.tile) {
+      bool broke = false;
+      for (int d = 0; d < b.ndev; ++d) broke |= b.h_scal[d].active && b.h_scal[d].status != 0;
+      if (broke) b.tile = 0;
+    }
     hipLaunchKernelGGL(k_ir_update, dim3(b.nblk), dim3(BS), 0, b.stream, R);
     b.launches++;
   }
@@ -1053,7 +1097,7 @@ int launch_unscale(Batch &b, const double *n, const double *p, double *du) {
 }
 
 // ---------------------------------------------------------------------------
-// PCG (Jacobi) for the SPD equilibrium Poisson system.
+// PCG (tile block-Jacobi, or Jacobi) for the SPD equilibrium Poisson system.
 //   vectors veq: 0 x, 1 r, 2 z, 3 p, 4 q
 // ---------------------------------------------------------------------------
 struct PArgs {
@@ -1066,8 +1110,106 @@ struct PArgs {
   double *p2;                       // second buffer of p (by iteration parity)
   double *part;
   Scal *scal;
+  const double *P16;                // tile-inverse rows (tile != 0)
+  int tile;                         // 1: tile block-Jacobi, 0: Jacobi
   int parity;
 };
+
+// The equilibrium Poisson matrix is SPD and scalar per node: its tile
+// block-Jacobi preconditioner is the exact inverse of the matrix restricted
+// to every 16-node tile (the coupled solver's tiles), FP64 (the inverse's
+// entries span the equilibrium's dynamic range), node k keeping its row,
+// P16[c * N + g] for the tile's node c.  Measured PCG iterations to 1e-12
+// on the equilibrium Newton systems (CPU probe): 200x120 MOSFET ~520
+// (Jacobi) -> ~260; 32x24x24 tri-gate 35-58 -> 22-34 (2 x 2 x 4 tiles);
+// the batch's uniformly meshed diodes 140 -> 160-180 (Jacobi suits them
+// slightly better; they spend under 3% of their time here).
+__global__ void __launch_bounds__(64) k_pcg_tile_setup(Where W, long long N, const double *A,
+                                                       const Scal *scal, double *P16, int *flag) {
+  __shared__ double M[TILE_NODES * 64];     // M[col * 64 + lane]
+  const int nt = threadIdx.x;
+  const int patch = blockIdx.x / SPP, sub = blockIdx.x % SPP;
+  const int d = W.single ? 0 : W.blk2dev[patch];
+  if (!scal[d].active) return;
+  const DevInfo &I = dev_info(W, d);
+  const unsigned local = (unsigned)(patch - I.blk0);
+  const unsigned bx = local % (unsigned)I.px, t2 = local / (unsigned)I.px;
+  const unsigned by = t2 % (unsigned)I.py, bz = t2 / (unsigned)I.py;
+  const SetupNode SN = setup_node(I, bx, by, bz, sub, nt);
+  const int tq = SN.tq, l = SN.l;
+  const bool valid = SN.i < I.nx && SN.j < I.ny && SN.kk < I.nz;
+  const long long g = I.off + ((long long)SN.kk * I.ny + SN.j) * I.nx + SN.i;
+#pragma unroll
+  for (int c = 0; c < TILE_NODES; ++c) M[c * 64 + nt] = 0.0;
+  if (valid) {
+    M[l * 64 + nt] = A[g];
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir)
+      if (SN.has[dir]) M[SN.ln[dir] * 64 + nt] = A[(1 + dir) * N + g];
+  } else {
+    M[l * 64 + nt] = 1.0;
+  }
+  bool bad = false;
+  for (int sp = 0; sp < TILE_NODES; ++sp) {
+    const int pl = setup_lane(I.dim, tq, sp);
+    __syncthreads();
+    const double piv = M[sp * 64 + pl];
+    const double f = M[sp * 64 + nt];
+    if (l == sp && (!(fabs(piv) > 1e-300) || !isfinite(piv))) bad = true;
+    const double pinv = 1.0 / piv;
+    __syncthreads();
+    // pivot row /= pivot (its pivot entry := 1 / pivot), one column per
+    // thread of the tile
+    M[l * 64 + pl] = l == sp ? pinv : M[l * 64 + pl] * pinv;
+    __syncthreads();
+    if (l != sp) {
+      double sv[TILE_NODES], mv[TILE_NODES];
+#pragma unroll
+      for (int c = 0; c < TILE_NODES; ++c) {
+        sv[c] = M[c * 64 + pl];
+        mv[c] = M[c * 64 + nt];
+      }
+#pragma unroll
+      for (int c = 0; c < TILE_NODES; ++c) M[c * 64 + nt] = mv[c] - f * sv[c];
+      M[sp * 64 + nt] = -f * pinv;
+    }
+  }
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < TILE_NODES; ++c) P16[(long long)c * N + g] = M[c * 64 + nt];
+  }
+  if (bad) {                                // error flags only; never part of a sum
+    atomicOr(flag, 1);
+    atomicOr(flag + 1 + d, 1);
+  }
+}
+
+int launch_pcg_tile_setup(Batch &b) {
+  hipLaunchKernelGGL(k_pcg_tile_setup, dim3(SPP * b.nblk), dim3(64), 0, b.stream, make_where(b),
+                     b.ntot, (const double *)b.Aeq, (const Scal *)b.d_scal, b.P16, b.d_flag);
+  b.launches++;
+  TCAD_HIP_CHECK(hipGetLastError());
+  return TCAD_OK;
+}
+
+// z = M^-1 x at this thread's node (every thread of the workgroup calls it
+// when P.tile: it holds barriers; tv is LDS of BS doubles)
+__device__ __forceinline__ double pcg_prec(const PArgs &P, int dim, bool valid, long long g,
+                                           double x, double *tv) {
+  if (!P.tile) return valid ? x / P.A[g] : 0.0;
+  int tile, l;
+  tile_slot(dim, tile, l);
+  double *v = tv + tile * TILE_NODES;
+  __syncthreads();                          // tv free (a previous use is done)
+  v[l] = valid ? x : 0.0;
+  __syncthreads();
+  if (!valid) return 0.0;
+  double z = 0.0;
+#pragma unroll
+  for (int c = 0; c < TILE_NODES; ++c) z += P.P16[(long long)c * P.N + g] * v[c];
+  return z;
+}
 
 __device__ __forceinline__ double pspmv(const double *A, long long N, const NodeHood &H,
                                         const double *x) {
@@ -1090,10 +1232,11 @@ __global__ void __launch_bounds__(BS) k_pcg_init(PArgs P) {
   PPROLOGUE
   (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
+  __shared__ double tv[BS];
   double rz = 0.0, rr = 0.0;
+  const double bv = H.valid ? P.b[H.g] : 0.0;
+  const double zv = pcg_prec(P, I.dim, H.valid, H.g, bv, tv);
   if (H.valid) {
-    const double bv = P.b[H.g];
-    const double zv = bv / P.A[H.g];
     P.x[H.g] = 0.0;
     P.r[H.g] = bv;
     P.z[H.g] = zv;
@@ -1171,22 +1314,22 @@ __global__ void __launch_bounds__(BS) k_pcg_c(PArgs P) {
   PPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const double *pcur = P.parity ? P.p : P.p2;
-  double x = 0, pv = 0, r = 0, q = 0, Ad = 1;
+  __shared__ double tv[BS];
+  double x = 0, pv = 0, r = 0, q = 0;
   if (H.valid) {
     x = P.x[H.g];
     pv = pcur[H.g];
     r = P.r[H.g];
     q = P.q[H.g];
-    Ad = P.A[H.g];
   }
   const double pq = device_sum(P.part, P.nblk, 2, I, sh);
   const double alpha = S.rho2[P.parity] / pq;
   double in[2] = {0.0, 0.0};
+  const double rn = r - alpha * q;
+  const double zv = pcg_prec(P, I.dim, H.valid, H.g, rn, tv);
   if (H.valid) {
     P.x[H.g] = x + alpha * pv;
-    const double rn = r - alpha * q;
     P.r[H.g] = rn;
-    const double zv = rn / Ad;
     P.z[H.g] = zv;
     in[0] = rn * zv;
     in[1] = rn * rn;
@@ -1208,6 +1351,7 @@ int pcg_poisson(Batch &b, double rtol, int maxit) {
   P.x = b.veq[0]; P.r = b.veq[1]; P.z = b.veq[2]; P.p = b.veq[3]; P.q = b.veq[4];
   P.p2 = b.veq[5];
   P.part = b.part; P.scal = b.d_scal;
+  P.P16 = b.P16; P.tile = b.pcg_tile;
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
                      b.ndev, rtol * rtol, maxit, 0, 0.0);
   hipLaunchKernelGGL(k_pcg_init, dim3(b.nblk), dim3(BS), 0, b.stream, P);
@@ -1239,9 +1383,12 @@ int pcg_poisson(Batch &b, double rtol, int maxit) {
     chunk = step;
     if (step < 128) step *= 2;
   }
-  int most = 0;
-  for (int d = 0; d < b.ndev; ++d) most = std::max(most, b.h_scal[d].it);
-  b.pcg_hint = most;
+  int most = 0, failed = it >= maxit ? 1 : 0;
+  for (int d = 0; d < b.ndev; ++d) {
+    most = std::max(most, b.h_scal[d].it);
+    if (b.h_scal[d].active) failed |= b.h_scal[d].status;
+  }
+  b.pcg_hint = failed ? 0 : most;
   return TCAD_OK;
 }
 

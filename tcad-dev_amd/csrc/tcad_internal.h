@@ -184,6 +184,8 @@ struct Batch {
   float *Dinv32 = nullptr;
   tile4 *T4 = nullptr;                           // tile inverses, TILE_PLANES planes (FP16)
   float *T4s = nullptr;                          // their per-row scales, 3 planes
+  double *P16 = nullptr;                         // equilibrium PCG tile-inverse rows, 16 planes
+  int pcg_tile = 0;                              // 1: tile block-Jacobi PCG, 0: Jacobi
   int tile = 1;                                  // preconditioner of the next solves
   double *bs = nullptr;                          // scaled rhs
   double *rowscale = nullptr;
@@ -272,6 +274,8 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
 int pcg_poisson(Batch &b, double rtol, int maxit);
 // inverses of the tile diagonal blocks of the last assembled matrix
 int launch_tile_setup(Batch &b, bool only_active);
+// inverses of the tile diagonal blocks of the equilibrium Poisson matrix
+int launch_pcg_tile_setup(Batch &b);
 int launch_unscale(Batch &b, const double *n, const double *p, double *du);
 int launch_unscale_state(Batch &b, const double *n, const double *p, double *du);
 int launch_spmv_only(Batch &b, bool fp32);

@@ -43,10 +43,10 @@ from .sweep import SweepResult, Track, record_currents, solve_round
 # library workspace per node of a batch (upper bound over 2D/3D): the
 # stencil-block matrix in FP64 and FP32 (39 entries in 3D), its node block
 # inverse in both, 16 three-plane FP64 vectors, the continuity residual
-# plane, the equilibrium system (7 entries, rhs, 6 vectors), the 4 x 4-tile
-# inverse rows (36 planes of 4 FP16 + 3 FP32 scales) and the per-node
-# contact and gate index tables
-WORKSPACE_BYTES_PER_NODE = (39 * 12 + 9 * 12 + 16 * 3 * 8 + 8 + 7 * 8 + 8 + 6 * 8
+# plane, the equilibrium system (7 entries, rhs, 6 vectors, 16 planes of
+# tile-inverse rows), the tile inverse rows of the coupled solve (36 planes
+# of 4 FP16 + 3 FP32 scales) and the per-node contact and gate index tables
+WORKSPACE_BYTES_PER_NODE = (39 * 12 + 9 * 12 + 16 * 3 * 8 + 8 + 7 * 8 + 8 + 6 * 8 + 16 * 8
                             + 36 * 8 + 3 * 4 + 2 * 4)
 # pool planes per node: 15 static fields + psi, n, p (FP64)
 POOL_BYTES_PER_NODE = 18 * 8

@@ -153,3 +153,32 @@ def test_transient_3d_vs_oracle():
         tol = 1e-6 * np.abs(ref_cur[s]).max() + 1e-9 * big
         assert np.abs(cur[s] - ref_cur[s]).max() <= tol, (s, cur[s], ref_cur[s])
     print(f"  steps {len(gres.times) - 1}", flush=True)
+
+
+def test_diverging_tile_solve_falls_back(monkeypatch):
+    """Regression: from the equilibrium the tile-preconditioned PCG reaches
+    (equal to the default's to ~2e-15), the reverse-biased Crank-Nicolson
+    diode ramp assembles a step matrix on which the FP16 tile-preconditioned
+    BiCGStab diverges (its residual grew 1e13-fold and every refinement
+    sweep failed).  The refinement now drops a diverged sweep's update and
+    finishes the solve with the FP64 node block-Jacobi: the transient
+    completes on the reference's time grid with the reference's currents
+    (Newton counts are not compared: this start moves one step's line-search
+    path, 5 instead of 8 iterations)."""
+    tr, NewtonOptions, device_from_spec = _mods()
+    print("[transient] transient_diode_cn.npz from the tile-PCG equilibrium", flush=True)
+    monkeypatch.setenv("TCADHIP_PCG_PRECOND", "tile")
+    spec, d = load("transient_diode_cn.npz")
+    dev = device_from_spec(spec)
+    opts = NewtonOptions(linsolve="bicgstab")
+    dev.solve_equilibrium(opts)
+    out_t = [float(v) for v in d["output_times"]] or None
+    res = tr.solve_transient(dev, _waveforms(tr, d), float(d["t_end"]), float(d["dt0"]),
+                             theta=float(d["theta"]), opts=opts, output_times=out_t)
+    assert np.array_equal(res.times, d["times"])
+    cur = np.array([res.terminal_current[c.name] for c in spec.contacts]).T
+    ref = d["currents"]
+    big = np.abs(ref).max()
+    for s in range(ref.shape[0]):
+        tol = 1e-6 * np.abs(ref[s]).max() + 1e-9 * big
+        assert np.abs(cur[s] - ref[s]).max() <= tol, (s, cur[s], ref[s], tol)
