@@ -563,6 +563,18 @@ __global__ void k_ac_reset(ACScal *acs, const int *act, int ndev, double tol2, d
   }
 }
 
+// x := xbak on the devices flagged in mask (their sweep diverged)
+__global__ void __launch_bounds__(BS) k_ac_restore(Where W, long long N, const int *mask,
+                                                   const double *xbak, double *x) {
+  const int d = dev_index(W);
+  if (!mask[d]) return;
+  const DevInfo &I = dev_info(W, d);
+  const NodeHood H = hood(I, blockIdx.x);
+  if (!H.valid) return;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) x[c * N + H.g] = xbak[c * N + H.g];
+}
+
 // r = b - A x (x = 0 when K.first), rhat = p = r, phat = M^-1 p; partials
 // ||r||^2 (quantity 0; rho_0 = (rhat, r) is the same sum) and ||b||^2
 // (quantity 1, read by the first k_cb_s of a solve: the reference of its
@@ -897,6 +909,7 @@ struct AcWork {
   double *v[NCV] = {};
   double *Mi = nullptr, *dn = nullptr, *dp = nullptr, *raw = nullptr, *acg = nullptr;
   double *part = nullptr, *omega = nullptr, *ycol = nullptr;
+  double *xbak = nullptr;           // x at the current sweep's start (6 planes)
   ACScal *acs = nullptr;
   int *act = nullptr, *row_dev = nullptr;
   void *tbase = nullptr;            // complex tile rows
@@ -912,13 +925,14 @@ struct AcWork {
 
 int alloc_work(Batch &b, int nrows, AcWork &w) {
   const size_t N = (size_t)b.ntot;
-  const size_t nd = NCV * 6 * N + 18 * N + 2 * N + 21 * N + N + 8 * (size_t)b.nblk + b.ndev +
+  const size_t nd = NCV * 6 * N + 6 * N + 18 * N + 2 * N + 21 * N + N + 8 * (size_t)b.nblk + b.ndev +
                     2 * (size_t)nrows;
   const size_t bytes = nd * sizeof(double) + sizeof(ACScal) * b.ndev +
                        sizeof(int) * ((size_t)b.ndev + nrows) + 256;
   TCAD_HIP_CHECK(hipMalloc((void **)&w.base, bytes));
   double *q = w.base;
   for (int i = 0; i < NCV; ++i, q += 6 * N) w.v[i] = q;
+  w.xbak = q; q += 6 * N;
   w.Mi = q; q += 18 * N;
   w.dn = q; q += N;
   w.dp = q; q += N;
@@ -990,7 +1004,7 @@ int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int ma
   CArgs K = make_cargs(b, w);
   std::vector<ACScal> hs(b.ndev);
   std::vector<double> prev(b.ndev, INFINITY);
-  std::vector<int> conv(b.ndev, 0), stalled_ok(b.ndev, 0);
+  std::vector<int> conv(b.ndev, 0), stalled_ok(b.ndev, 0), restored(b.ndev, 0);
   const std::vector<int> act0 = act;
   for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
     int open = 0;
@@ -1004,6 +1018,9 @@ int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int ma
     K.first = sweep == 0 && !warm ? 1 : 0;
     hipLaunchKernelGGL(k_cb_init, dim3(b.nblk), dim3(BS), 0, b.stream, K);
     b.launches += 2;
+    // x as this sweep starts (k_cb_init has zeroed it on a cold start)
+    TCAD_HIP_CHECK(hipMemcpyAsync(w.xbak, w.v[CX], sizeof(double) * 6 * (size_t)b.ntot,
+                                  hipMemcpyDeviceToDevice, b.stream));
     // chunks between host checks as in the real solver: 3/4 of the
     // iterations of the last solve's sweep of the same index (a solve's
     // first sweep is long, its restarts short), then 16, 32, 64
@@ -1036,17 +1053,60 @@ int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int ma
       if (act[d]) failed |= hs[d].status;
     }
     w.hint[sweep] = failed ? 0 : most;
+    // A sweep whose residual ended above where it started (FP16 complex tile
+    // inverses of a nearly singular tile can do that) leaves x as the sweep
+    // found it, and the rest of the solve uses the complex node blocks; a
+    // device that diverges with the node blocks too is a stalled solve.
+    std::vector<int> back(b.ndev, 0);
+    int nback = 0;
+    for (int d = 0; d < b.ndev; ++d) {
+      const ACScal &S = hs[d];
+      if (act[d] && S.status != 0 && !(S.rr <= S.bb)) {
+        back[d] = 1;
+        ++nback;
+      }
+    }
+    const bool tiles_failed = nback > 0 && K.tile;
+    if (nback) {
+      TCAD_HIP_CHECK(hipMemcpyAsync(w.act, back.data(), sizeof(int) * b.ndev,
+                                    hipMemcpyHostToDevice, b.stream));
+      hipLaunchKernelGGL(k_ac_restore, dim3(b.nblk), dim3(BS), 0, b.stream, K.W, K.N,
+                         (const int *)w.act, (const double *)w.xbak, w.v[CX]);
+      b.launches++;
+      if (tiles_failed) {
+        K.tile = 0;
+        hipLaunchKernelGGL(k_ac_prec, dim3(b.nblk), dim3(BS), 0, b.stream, K, w.Mi, b.d_flag);
+        b.launches++;
+        TCAD_HIP_CHECK(hipGetLastError());
+        int rc = check_flag(b);
+        if (rc) return rc;
+      }
+      TCAD_HIP_CHECK(hipGetLastError());
+    }
     for (int d = 0; d < b.ndev; ++d) {
       if (!act[d]) continue;
       const ACScal &S = hs[d];
       iters[d] += S.it;
+      if (back[d]) {
+        if (tiles_failed) {
+          restored[d] = 1;                  // retried from the same x: no stall test next
+          continue;
+        }
+        stalled_ok[d] = std::isfinite(prev[d]) && prev[d] <= kAcceptRel * kAcceptRel * S.b2;
+        act[d] = 0;
+        continue;
+      }
+      // a sweep retried from the restored x starts where the diverged one
+      // did: no stall test against it
+      const bool retried = restored[d] != 0;
+      restored[d] = 0;
       // S.bb: true residual at this sweep's start; converged there if the
       // sweep stopped before its first iteration
       const bool done_at_start = S.it == 0 && S.bb <= rtol * rtol * S.b2;
       if (done_at_start || !(S.b2 > 0.0)) {
         conv[d] = 1;
         act[d] = 0;
-      } else if (!std::isfinite(S.bb) || (sweep > 0 && S.bb > 0.25 * prev[d])) {
+      } else if (!std::isfinite(S.bb) || (!retried && sweep > 0 && S.bb > 0.25 * prev[d])) {
         stalled_ok[d] = std::isfinite(S.bb) && S.bb <= kAcceptRel * kAcceptRel * S.b2;
         act[d] = 0;
       }
