@@ -1,9 +1,10 @@
 // Krylov solvers on the stencil-block Newton matrix.
 //
 // * BiCGStab (right block-Jacobi preconditioning: 3x3 node blocks, or the
-//   4x4-node tiles of 2D devices), templated on the storage precision T of
-//   the matrix and block inverses (double or float); vectors and dot
-//   products are FP64.  One iteration is four kernels:
+//   16-node tiles, 4 x 4 in 2D and 2 x 2 x 4 in 3D), templated on the
+//   storage precision T of the matrix and block inverses (double or
+//   float); vectors and dot products are FP64.  One iteration is four
+//   kernels:
 //     k_bicg_spmv    v = A phat, partial (rhat, v)
 //     k_bicg_s       reduce (r, r) -> stop test; reduce -> alpha;
 //                    s = r - alpha v; shat = M^-1 s
@@ -61,7 +62,7 @@ struct KArgs {
   int nblk;
   const T *A;                       // matrix storage precision
   const T *Dinv;                    // preconditioner storage precision
-  const tile4 *T4;                  // 4x4-tile inverses (FP16 rows), tile != 0
+  const tile4 *T4;                  // tile inverses (FP16 rows), tile != 0
   const float *T4s;                 // their per-row scales
   int tile;                         // 1: tile block-Jacobi, 0: node block-Jacobi
   double *v[9];                     // Krylov vectors, always FP64
