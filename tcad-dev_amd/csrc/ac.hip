@@ -172,8 +172,10 @@ __device__ __forceinline__ void cprec(const CArgs &K, long long g, const double 
 
 // Complex inverse of each node block of A + j w diag(0, dn, dp), by
 // cofactors (the complex counterpart of the node block-Jacobi).
-__global__ void __launch_bounds__(BS) k_ac_prec(CArgs K, double *Mi, int *flag) {
+__global__ void __launch_bounds__(BS) k_ac_prec(CArgs K, const Scal *scal, double *Mi,
+                                                int *flag) {
   const int d = dev_index(K.W);
+  if (!scal[d].active) return;            // not assembled for this call
   const DevInfo &I = dev_info(K.W, d);
   const NodeHood H = hood(I, blockIdx.x);
   if (!H.valid) return;
@@ -259,13 +261,14 @@ __device__ __forceinline__ double cinv3(const Cx a[9], Cx P[9]) {
 // threads per node over the two halves of the 48 columns; complex block
 // Gauss-Jordan in FP64 with the 3x3 node blocks as pivots, in place in
 // LDS (column-major, lane fastest, as the real tile setup).
-__global__ void __launch_bounds__(64) k_ac_tile_setup(CArgs K, tile4 *Tr, tile4 *Ti, float *Ts,
-                                                      int *flag) {
+__global__ void __launch_bounds__(64) k_ac_tile_setup(CArgs K, const Scal *scal, tile4 *Tr,
+                                                      tile4 *Ti, float *Ts, int *flag) {
   __shared__ double Mre[3 * CTU * CNODES], Mim[3 * CTU * CNODES];
   __shared__ float amx[2 * CNODES * 3];
   const int nt = threadIdx.x % CNODES, half = threadIdx.x / CNODES, c0 = half * CHC;
   const int patch = blockIdx.x / CSPP, sub = blockIdx.x % CSPP;
   const int d = K.W.single ? 0 : K.W.blk2dev[patch];
+  if (!scal[d].active) return;            // not assembled for this call
   const DevInfo &I = dev_info(K.W, d);
   const long long N = K.N;
   const double w = K.omega[d];
@@ -1075,7 +1078,8 @@ int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int ma
       b.launches++;
       if (tiles_failed) {
         K.tile = 0;
-        hipLaunchKernelGGL(k_ac_prec, dim3(b.nblk), dim3(BS), 0, b.stream, K, w.Mi, b.d_flag);
+        hipLaunchKernelGGL(k_ac_prec, dim3(b.nblk), dim3(BS), 0, b.stream, K,
+                           (const Scal *)b.d_scal, w.Mi, b.d_flag);
         b.launches++;
         TCAD_HIP_CHECK(hipGetLastError());
         int rc = check_flag(b);
@@ -1207,10 +1211,11 @@ int y_parameters_impl(Batch &b, const int32_t *h_active, int nfreq, const double
       TCAD_HIP_CHECK(hipMemcpyAsync(w.omega, om.data(), sizeof(double) * b.ndev,
                                     hipMemcpyHostToDevice, b.stream));
       if (w.tile)
-        hipLaunchKernelGGL(k_ac_tile_setup, dim3(CSPP * b.nblk), dim3(64), 0, b.stream, K, w.Tr,
-                           w.Ti, w.Ts, b.d_flag);
+        hipLaunchKernelGGL(k_ac_tile_setup, dim3(CSPP * b.nblk), dim3(64), 0, b.stream, K,
+                           (const Scal *)b.d_scal, w.Tr, w.Ti, w.Ts, b.d_flag);
       else
-        hipLaunchKernelGGL(k_ac_prec, dim3(b.nblk), dim3(BS), 0, b.stream, K, w.Mi, b.d_flag);
+        hipLaunchKernelGGL(k_ac_prec, dim3(b.nblk), dim3(BS), 0, b.stream, K,
+                           (const Scal *)b.d_scal, w.Mi, b.d_flag);
       b.launches++;
       TCAD_HIP_CHECK(hipGetLastError());
       if ((rc = check_flag(b))) return rc;

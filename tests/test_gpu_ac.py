@@ -129,6 +129,32 @@ def test_ac_batch_matches_alone():
         assert not np.any(Yb[k][:, P:, :]) and not np.any(Yb[k][:, :, P:])
 
 
+def test_ac_active_mask_skips_unsolved_devices():
+    """Only the selected devices are assembled and preconditioned: a batch
+    whose other devices were never solved (their matrices still zero)
+    gives the selected device's Y bit for bit as alone, and zeros for the
+    others."""
+    print("[ac] active mask over devices never assembled", flush=True)
+    from tcadhip.device import GPUBatch
+    devs = [_operating_point(nm)[3] for nm in FIXTURES]
+    freqs = np.array([1e4, 1e9])
+    alone, status, _ = devs[0]._gpu().y_parameters(freqs, 1e-16, 20000)
+    assert status[0] == 0
+    batch = GPUBatch([dv._s for dv in devs])
+    sl = batch.device_slice(0)
+    src = devs[0]._gpu()
+    batch.psi[sl].copy_(src.psi)
+    batch.n[sl].copy_(src.n)
+    batch.p[sl].copy_(src.p)
+    for key in GPUBatch.FIELD_KEYS:
+        batch._t[key][sl].copy_(src._t[key])
+    Yb, status, _ = batch.y_parameters(freqs, 1e-16, 20000, active=[1, 0, 0])
+    assert status[0] == 0
+    P = alone.shape[-1]
+    assert np.array_equal(Yb[0][:, :P, :P], alone[0])
+    assert not np.any(Yb[1:])
+
+
 def test_ac_low_frequency_is_dc_conductance():
     print("[ac] 1 Hz conductance vs DC difference", flush=True)
     ac, NewtonOptions, _ = _mods()
