@@ -62,7 +62,7 @@ extern "C" {
  * every device with the reference's node block-Jacobi (3x3 block per node,
  * linsolve.py:363-427).  Without it, every device uses tile block-Jacobi:
  * the exact inverse of the matrix restricted to each 16-node tile (4 x 4
- * in 2D, 2 x 2 x 4 in 3D; 48 unknowns), stored as FP16 rows with an FP32
+ * in 2D, 2 x 1 x 8 in 3D; 48 unknowns), stored as FP16 rows with an FP32
  * scale per row. */
 #define TCAD_PRECOND_NODE 0x200
 

@@ -1,7 +1,7 @@
 // Krylov solvers on the stencil-block Newton matrix.
 //
 // * BiCGStab (right block-Jacobi preconditioning: 3x3 node blocks, or the
-//   16-node tiles, 4 x 4 in 2D and 2 x 2 x 4 in 3D), templated on the
+//   16-node tiles, 4 x 4 in 2D and 2 x 1 x 8 in 3D), templated on the
 //   storage precision T of the matrix and block inverses (double or
 //   float); vectors and dot products are FP64.  One iteration is four
 //   kernels:
@@ -116,10 +116,11 @@ __device__ __forceinline__ void prec3(const T *Di, long long N, long long g, con
 // block-Jacobi ~630, 4 x 4 tiles ~255 (4 x 8 tiles ~185 for twice the
 // bytes: slower on the GPU).  On a 32x24x24 tri-gate: node block-Jacobi
 // 145-217, 4 x 4 x 1 x-y tiles 114-190 (the z couplings of the finely
-// meshed fin stay outside them), 2 x 2 x 4 tiles 72-86.  On the 64x48x48
-// tri-gate sweep (GPU, 34 points): 74,304 -> 25,969 Krylov iterations,
-// 9.3 -> 18.5 points/s (1 x 2 x 8 tiles: 19,540 iterations but 8-node x
-// rows, 16.0 points/s).
+// meshed fin stay outside them), 2 x 2 x 4 tiles 72-86, 2 x 1 x 8 tiles
+// 68-89.  On the 64x48x48 tri-gate sweep (GPU, 34 points): node
+// block-Jacobi 74,304 Krylov iterations, 9.3 points/s; 2 x 2 x 4 tiles
+// 25,969, 18.1; 2 x 1 x 8 tiles (16 x 1 x 8 patches) 23,922, 20.3 (1 x 2 x 8
+// tiles: 19,540 iterations, but 8-node x rows, 16.0 points/s).
 // ---------------------------------------------------------------------------
 constexpr int TU = TILE_U;
 // tile setup: a 64-thread workgroup inverts STPW tiles side by side in x

@@ -59,7 +59,7 @@ inline int nentries(int dim) { return NDIAG + NOFF * 2 * dim; }
 #define TCAD_T3X 2
 #endif
 #ifndef TCAD_T3Y
-#define TCAD_T3Y 2
+#define TCAD_T3Y 1
 #endif
 constexpr int TILE_X = 4, TILE_Y = 4;
 constexpr int TILE_NODES = TILE_X * TILE_Y;

@@ -41,7 +41,7 @@ class NewtonOptions:
     the converging step is refined to the FP64 rounding floor either way.
     Forcing is off by default: on the 200x120 MOSFET sweep it added a
     Newton iteration per point and cost more Krylov work than it saved.
-    precond: "tile" (default; every 16-node tile -- 4 x 4 in 2D, 2 x 2 x 4
+    precond: "tile" (default; every 16-node tile -- 4 x 4 in 2D, 2 x 1 x 8
     in 3D -- gets the exact inverse of its 48 x 48 diagonal block, stored
     as FP16 rows with a FP32 scale per row) or "node" (the reference's 3x3
     node block-Jacobi for every device)."""

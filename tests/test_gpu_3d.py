@@ -84,9 +84,9 @@ def test_trigate_full_size_invariants():
 
 @pytest.mark.parametrize("shape", [("trigate", (19, 7, 9)), ("mosfet", (37, 11))])
 def test_ragged_patches_tile_and_node_precond(shape):
-    """Devices whose sizes leave every patch direction ragged (3D patches
-    are 16 x 2 x 4 nodes, 2D patches 32 x 4: 19 % 16, 7 % 2, 9 % 4, 37 % 32
-    and 11 % 4 are all nonzero), so the tile preconditioner's edge tiles
+    """Devices whose sizes leave the patches ragged (3D patches are
+    16 x 1 x 8 nodes, 2D patches 32 x 4: 19 % 16, 9 % 8, 37 % 32 and 11 % 4
+    are all nonzero), so the tile preconditioner's edge tiles
     hold nodes outside the device: the tile and node block-Jacobi
     preconditioners reach the same Newton counts and states, and both match
     the live oracle (Newton +-1, currents 1e-6 of the largest, psi 1e-9)."""
