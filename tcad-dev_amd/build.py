@@ -4,9 +4,11 @@ Every HIP translation unit is compiled with -ffp-contract=off (no fused
 multiply-add, so the residual rounds like the reference's NumPy
 expressions) and with packed-FP32 instructions disabled.
 
-TCADHIP_BS=<threads> builds another node-workgroup size (default 256) and
-TCADHIP_OUT=<path> writes the library elsewhere (tcadhip loads such a build
-through TCADHIP_LIB).
+TCADHIP_BS=<threads> builds another node-workgroup size (default 128),
+TCADHIP_PY=<rows> another 2D patch height (default 4), TCADHIP_T3=<x>x<y>
+another 3D tile extent in x and y (default 2x1; the z extent makes up the
+16 nodes of a tile), and TCADHIP_OUT=<path> writes the library elsewhere
+(tcadhip loads such a build through TCADHIP_LIB).
 """
 import os
 import subprocess
@@ -19,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.environ.get("TCADHIP_OUT") or os.path.join(HERE, "tcadhip", "libtcadhip.so")
 BS = os.environ.get("TCADHIP_BS")          # node workgroup size (default 128)
 PY = os.environ.get("TCADHIP_PY")          # patch height (default 4)
-T3 = os.environ.get("TCADHIP_T3")          # 3D tile x and y extent, e.g. "2x2" (default)
+T3 = os.environ.get("TCADHIP_T3")          # 3D tile x and y extent (default "2x1")
 OBJ = os.path.join(HERE, "_build" + (f"_bs{BS}" if BS else "") + (f"_py{PY}" if PY else "")
                    + (f"_t3{T3}" if T3 else ""))
 SOURCES = ["assemble.hip", "krylov.hip", "batch.hip", "ac.hip"]

@@ -101,7 +101,7 @@ struct CArgs {
 #define CPROLOGUE                                   \
   __shared__ double sh[128];                        \
   const int d = dev_index(K.W);                     \
-  const DevInfo &I = dev_info(K.W, d);              \
+  const DevInfo I = dev_info(K.W, d);               \
   ACScal &S = K.acs[d];                             \
   if (S.done) return;                               \
   const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(BS) k_ac_prec(CArgs K, const Scal *scal, doubl
                                                 int *flag) {
   const int d = dev_index(K.W);
   if (!scal[d].active) return;            // not assembled for this call
-  const DevInfo &I = dev_info(K.W, d);
+  const DevInfo I = dev_info(K.W, d);
   const NodeHood H = hood(I, blockIdx.x);
   if (!H.valid) return;
   const long long N = K.N, g = H.g;
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(64) k_ac_tile_setup(CArgs K, const Scal *scal,
   const int patch = blockIdx.x / CSPP, sub = blockIdx.x % CSPP;
   const int d = K.W.single ? 0 : K.W.blk2dev[patch];
   if (!scal[d].active) return;            // not assembled for this call
-  const DevInfo &I = dev_info(K.W, d);
+  const DevInfo I = dev_info(K.W, d);
   const long long N = K.N;
   const double w = K.omega[d];
   const unsigned local = (unsigned)(patch - I.blk0);
@@ -518,7 +518,7 @@ __global__ void __launch_bounds__(BS) k_ac_forcing(Where W, const int *gidx,
                                                    const int *gn_gate, const double *gn_kw,
                                                    int k, double *acg) {
   const int d = dev_index(W);
-  const DevInfo &I = dev_info(W, d);
+  const DevInfo I = dev_info(W, d);
   const NodeHood H = hood(I, blockIdx.x);
   if (!H.valid) return;
   const long long g = H.g;
@@ -571,7 +571,7 @@ __global__ void __launch_bounds__(BS) k_ac_restore(Where W, long long N, const i
                                                    const double *xbak, double *x) {
   const int d = dev_index(W);
   if (!mask[d]) return;
-  const DevInfo &I = dev_info(W, d);
+  const DevInfo I = dev_info(W, d);
   const NodeHood H = hood(I, blockIdx.x);
   if (!H.valid) return;
 #pragma unroll

@@ -71,7 +71,17 @@ struct KArgs {
   Scal *scal;
   int scale_b;                      // init: divide b by Scal::nrm (refinement rhs)
   int parity;
+  int late_stop;                    // 1: finished devices return at KSTOP (late_stop_ok)
 };
+
+// Where a finished device's workgroups return.  Early (before any load)
+// in a batch of several devices, most of whose workgroups may belong to
+// retired devices, and on large grids, where the iterations issued after
+// convergence inside a chunk would otherwise each read the whole system.
+// Late (KSTOP, after the vector loads are issued, so that the stop flag's
+// latency overlaps theirs) for one device on a small, latency-bound grid.
+constexpr int kLateStopBlocks = 512;
+inline int late_stop_ok(const Batch &b) { return b.ndev == 1 && b.nblk <= kLateStopBlocks; }
 
 template <typename T>
 __device__ __forceinline__ void spmv_node(const T *A, long long N, const NodeHood &H,
@@ -288,7 +298,7 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
   const int patch = blockIdx.x / SPP, sub = blockIdx.x % SPP;
   const int d = W.single ? 0 : W.blk2dev[patch];
   if (scal && !scal[d].active) return;
-  const DevInfo &I = dev_info(W, d);
+  const DevInfo I = dev_info(W, d);
   const unsigned local = (unsigned)(patch - I.blk0);
   const unsigned bx = local % (unsigned)I.px, t2 = local / (unsigned)I.px;
   const unsigned by = t2 % (unsigned)I.py, bz = t2 / (unsigned)I.py;
@@ -435,13 +445,17 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
 }
 #undef TMAT
 
+// KSTOP: the late return of a finished device's workgroups (late_stop_ok;
+// the loads before it stay in bounds).
 #define KPROLOGUE                                   \
   __shared__ double sh[64];                         \
   const int d = dev_index(K.W);                     \
-  const DevInfo &I = dev_info(K.W, d);              \
+  const DevInfo I = dev_info(K.W, d);               \
   Scal &S = K.scal[d];                              \
-  if (S.done) return;                               \
+  if (!K.late_stop && S.done) return;               \
   const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
+#define KSTOP \
+  if (S.done) return;
 
 // The kernels that apply the preconditioner come in three variants (PV):
 // node block-Jacobi only (no tile code: full occupancy, for batches
@@ -476,6 +490,7 @@ __device__ __forceinline__ void apply_prec(const KArgs<T> &K, const NodeHood &H,
 template <typename T, int PV>
 __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
   KPROLOGUE
+  KSTOP
   (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
@@ -515,15 +530,18 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv(KArgs<T> K, int src, int dst, 
   (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
-  double s0 = 0.0, s1 = 0.0;
+  double s0 = 0.0, s1 = 0.0, y[3] = {0, 0, 0}, a3[3] = {0, 0, 0};
   if (H.valid) {
-    double y[3];
     spmv_node<T>(K.A, N, H, K.v[src], y);
 #pragma unroll
+    for (int c = 0; c < 3; ++c) a3[c] = K.v[dot_a][c * N + H.g];
+  }
+  KSTOP
+  if (H.valid) {
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const long long ix = c * N + H.g;
-      K.v[dst][ix] = y[c];
-      s0 += K.v[dot_a][ix] * y[c];
+      K.v[dst][c * N + H.g] = y[c];
+      s0 += a3[c] * y[c];
       s1 += y[c] * y[c];
     }
   }
@@ -544,15 +562,22 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv_t(KArgs<T> K) {
   (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
-  double q[4] = {0.0, 0.0, 0.0, 0.0};
+  double q[4] = {0.0, 0.0, 0.0, 0.0}, y[3] = {0, 0, 0}, s3[3] = {0, 0, 0}, r3[3] = {0, 0, 0};
   if (H.valid) {
-    double y[3];
     spmv_node<T>(K.A, N, H, K.v[VSH], y);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      s3[c] = K.v[VS][c * N + H.g];
+      r3[c] = K.v[VRH][c * N + H.g];
+    }
+  }
+  KSTOP
+  if (H.valid) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long ix = c * N + H.g;
       K.v[VT][ix] = y[c];
-      const double sv = K.v[VS][ix], rh = K.v[VRH][ix];
+      const double sv = s3[c], rh = r3[c];
       q[0] += y[c] * sv;
       q[1] += y[c] * y[c];
       q[2] += rh * sv;
@@ -585,6 +610,7 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
       v[c] = K.v[VV][c * N + H.g];
     }
   }
+  KSTOP
   double red[2];
   device_sums<2>(K.part, K.nblk, 1, I, sh, red);
   const double rr = red[0], rv = red[1];
@@ -659,6 +685,7 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
       v[c] = K.v[VV][ix];
     }
   }
+  KSTOP
   double red[4];
   device_sums<4>(K.part, K.nblk, 3, I, sh, red);
   const double ts = red[0], tt = red[1], rs = red[2], rt = red[3];
@@ -787,6 +814,7 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
   K.tile = b.tile;
   K.b = b.ir_r;
   K.scale_b = 1;
+  K.late_stop = late_stop_ok(b);
   return K;
 }
 
@@ -932,7 +960,7 @@ struct IRArgs {
 __global__ void __launch_bounds__(BS) k_ir_residual(IRArgs R, int first) {
   __shared__ double sh[64];
   const int d = dev_index(R.W);
-  const DevInfo &I = dev_info(R.W, d);
+  const DevInfo I = dev_info(R.W, d);
   Scal &S = R.scal[d];
   if (!S.active) return;
   const NodeHood H = hood(I, blockIdx.x);
@@ -965,7 +993,7 @@ __global__ void k_ir_check(IRArgs R, int ndev, double tol2, int per_dev) {
   __shared__ double sh[64];
   const int d = blockIdx.x;
   if (d >= ndev) return;
-  const DevInfo &I = dev_info(R.W, d);
+  const DevInfo I = dev_info(R.W, d);
   Scal &S = R.scal[d];
   if (!S.active) return;
   double red[2];
@@ -989,7 +1017,7 @@ __global__ void k_ir_check(IRArgs R, int ndev, double tol2, int per_dev) {
 // starts again from the same y
 __global__ void __launch_bounds__(BS) k_ir_update(IRArgs R) {
   const int d = dev_index(R.W);
-  const DevInfo &I = dev_info(R.W, d);
+  const DevInfo I = dev_info(R.W, d);
   Scal &S = R.scal[d];
   if (!S.active) return;
   const NodeHood H = hood(I, blockIdx.x);
@@ -1078,7 +1106,7 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
 __global__ void __launch_bounds__(BS) k_unscale(Where W, long long N, const double *y,
                                                  const double *n, const double *p, double *du) {
   const int d = dev_index(W);
-  const DevInfo &I = dev_info(W, d);
+  const DevInfo I = dev_info(W, d);
   const NodeHood H = hood(I, blockIdx.x);
   if (!H.valid) return;
   du[H.g] = y[H.g];
@@ -1115,6 +1143,7 @@ struct PArgs {
   const double *P16;                // tile-inverse rows (tile != 0)
   int tile;                         // 1: tile block-Jacobi, 0: Jacobi
   int parity;
+  int late_stop;                    // as KArgs::late_stop
 };
 
 // The equilibrium Poisson matrix is SPD and scalar per node: its tile
@@ -1133,7 +1162,7 @@ __global__ void __launch_bounds__(64) k_pcg_tile_setup(Where W, long long N, con
   const int patch = blockIdx.x / SPP, sub = blockIdx.x % SPP;
   const int d = W.single ? 0 : W.blk2dev[patch];
   if (!scal[d].active) return;
-  const DevInfo &I = dev_info(W, d);
+  const DevInfo I = dev_info(W, d);
   const unsigned local = (unsigned)(patch - I.blk0);
   const unsigned bx = local % (unsigned)I.px, t2 = local / (unsigned)I.px;
   const unsigned by = t2 % (unsigned)I.py, bz = t2 / (unsigned)I.py;
@@ -1225,13 +1254,14 @@ __device__ __forceinline__ double pspmv(const double *A, long long N, const Node
 #define PPROLOGUE                                   \
   __shared__ double sh[64];                    \
   const int d = dev_index(P.W);                     \
-  const DevInfo &I = dev_info(P.W, d);              \
+  const DevInfo I = dev_info(P.W, d);               \
   Scal &S = P.scal[d];                              \
-  if (S.done) return;                               \
+  if (!P.late_stop && S.done) return;               \
   const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
 
 __global__ void __launch_bounds__(BS) k_pcg_init(PArgs P) {
   PPROLOGUE
+  KSTOP
   (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
   __shared__ double tv[BS];
@@ -1276,6 +1306,7 @@ __global__ void __launch_bounds__(BS) k_pcg_ab(PArgs P) {
       pn[dir] = pin[H.nbg[dir]];
     }
   }
+  KSTOP
   double red[2];
   device_sums<2>(P.part, P.nblk, 0, I, sh, red);
   const double rz = red[0], rr = red[1];
@@ -1324,6 +1355,7 @@ __global__ void __launch_bounds__(BS) k_pcg_c(PArgs P) {
     r = P.r[H.g];
     q = P.q[H.g];
   }
+  KSTOP
   const double pq = device_sum(P.part, P.nblk, 2, I, sh);
   const double alpha = S.rho2[P.parity] / pq;
   double in[2] = {0.0, 0.0};
@@ -1354,6 +1386,7 @@ int pcg_poisson(Batch &b, double rtol, int maxit) {
   P.p2 = b.veq[5];
   P.part = b.part; P.scal = b.d_scal;
   P.P16 = b.P16; P.tile = b.pcg_tile;
+  P.late_stop = late_stop_ok(b);
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
                      b.ndev, rtol * rtol, maxit, 0, 0.0);
   hipLaunchKernelGGL(k_pcg_init, dim3(b.nblk), dim3(BS), 0, b.stream, P);

@@ -13,7 +13,6 @@ namespace tcad {
 struct Where {
   const DevInfo *info;              // batch table
   const int *blk2dev;
-  DevInfo one;                      // the device when the batch holds one
   int single;
 };
 
@@ -21,9 +20,12 @@ __device__ __forceinline__ int dev_index(const Where &W) {
   return W.single ? 0 : W.blk2dev[blockIdx.x];
 }
 
-__device__ __forceinline__ const DevInfo &dev_info(const Where &W, int d) {
-  return W.single ? W.one : W.info[d];
-}
+// By value from the table, copied where the kernel starts: the workgroup's
+// device index is uniform and nothing is stored yet, so the fields arrive
+// in one round of batched scalar loads.  (A reference that selected a
+// kernel-argument copy for single-device batches made every field access
+// a conditional load of its own, a chain of dependent scalar loads.)
+__device__ __forceinline__ DevInfo dev_info(const Where &W, int d) { return W.info[d]; }
 
 // Block-wide FP64 sum, fixed tree; result valid in every thread.
 __device__ __forceinline__ double block_sum(double v, double *sh) {
@@ -150,7 +152,6 @@ inline Where make_where(Batch &b) {
   W.info = b.d_info;
   W.blk2dev = b.d_blk2dev;
   W.single = b.ndev == 1 ? 1 : 0;
-  W.one = b.info[0];
   return W;
 }
 
