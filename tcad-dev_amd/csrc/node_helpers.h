@@ -90,10 +90,20 @@ __device__ __forceinline__ void device_sums(const double *part, int nblk_total, 
 #pragma unroll
       for (int k = 0; k < 4; ++k) a[q][k] += v[q][k];
   }
-  for (; t < n; t += st) {
+  // the rest, at most three partials per thread (t + 3 st >= n here): all
+  // loads issued first, then added in the same order as one at a time
+  double r[3][NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) a[q][0] += p[(long long)(q0 + q) * nblk_total + t];
-  }
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      r[m][q] = t + m * st < n ? p[(long long)(q0 + q) * nblk_total + t + m * st] : 0.0;
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+    if (t + m * st < n) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) a[q][0] += r[m][q];
+    }
   double s[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) s[q] = (a[q][0] + a[q][1]) + (a[q][2] + a[q][3]);
