@@ -450,12 +450,27 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
 #define KPROLOGUE                                   \
   __shared__ double sh[64];                         \
   const int d = dev_index(K.W);                     \
-  const DevInfo I = dev_info(K.W, d);               \
   Scal &S = K.scal[d];                              \
-  if (!K.late_stop && S.done) return;               \
+  const int done0 = S.done;                         \
+  const DevInfo I = dev_info(K.W, d);               \
+  KARGS_EARLY(I.off, I.nx, I.ny, I.px, I.blk0)      \
+  KARGS_EARLY(done0, I.dim, I.py)                   \
+  if (!K.late_stop && done0) return;                \
   const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
 #define KSTOP \
-  if (S.done) return;
+  if (done0) return;
+// Kernel arguments a kernel needs after its first branch, pinned to the
+// top: their scalar loads join the first round instead of each waiting
+// behind the branch that precedes its use (the empty asm only consumes
+// the registers).
+#define KARGS_EARLY(...) asm volatile("" ::KARGS_S(__VA_ARGS__));
+#define KARGS_S1(a) "s"(a)
+#define KARGS_S(...) KARGS_SN(__VA_ARGS__, KARGS_S5, KARGS_S4, KARGS_S3, KARGS_S2, KARGS_S1)(__VA_ARGS__)
+#define KARGS_SN(_1, _2, _3, _4, _5, NAME, ...) NAME
+#define KARGS_S2(a, b) KARGS_S1(a), KARGS_S1(b)
+#define KARGS_S3(a, b, c) KARGS_S1(a), KARGS_S2(b, c)
+#define KARGS_S4(a, b, c, d) KARGS_S1(a), KARGS_S3(b, c, d)
+#define KARGS_S5(a, b, c, d, e) KARGS_S1(a), KARGS_S4(b, c, d, e)
 
 // The kernels that apply the preconditioner come in three variants (PV):
 // node block-Jacobi only (no tile code: full occupancy, for batches
@@ -522,35 +537,31 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
   if (threadIdx.x == 0) K.part[(long long)K.nblk + blockIdx.x] = s;
 }
 
-// dst = A src; partial (dot_a, dst) [and (dst, dst)] at quantities q0[, q0+1]
+// v = A phat; partial (rhat, v) at quantity 2 (the vectors are fixed, so
+// their addresses arrive with the first kernel-argument loads)
 template <typename T>
-__global__ void __launch_bounds__(BS) k_bicg_spmv(KArgs<T> K, int src, int dst, int dot_a,
-                                                  int want_sq, int q0) {
+__global__ void __launch_bounds__(BS) k_bicg_spmv(KArgs<T> K) {
+  KARGS_EARLY(K.A, K.v[VPH], K.v[VRH], K.v[VV], K.N)
   KPROLOGUE
   (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
-  double s0 = 0.0, s1 = 0.0, y[3] = {0, 0, 0}, a3[3] = {0, 0, 0};
+  double s0 = 0.0, y[3] = {0, 0, 0}, a3[3] = {0, 0, 0};
   if (H.valid) {
-    spmv_node<T>(K.A, N, H, K.v[src], y);
+    spmv_node<T>(K.A, N, H, K.v[VPH], y);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) a3[c] = K.v[dot_a][c * N + H.g];
+    for (int c = 0; c < 3; ++c) a3[c] = K.v[VRH][c * N + H.g];
   }
   KSTOP
   if (H.valid) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      K.v[dst][c * N + H.g] = y[c];
+      K.v[VV][c * N + H.g] = y[c];
       s0 += a3[c] * y[c];
-      s1 += y[c] * y[c];
     }
   }
   const double a = block_sum(s0, sh);
-  if (threadIdx.x == 0) K.part[(long long)q0 * K.nblk + blockIdx.x] = a;
-  if (want_sq) {
-    const double b2 = block_sum(s1, sh);
-    if (threadIdx.x == 0) K.part[(long long)(q0 + 1) * K.nblk + blockIdx.x] = b2;
-  }
+  if (threadIdx.x == 0) K.part[2 * (long long)K.nblk + blockIdx.x] = a;
 }
 
 // t = A shat; partials (t, s), (t, t), (rhat, s), (rhat, t) at quantities
@@ -558,6 +569,7 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv(KArgs<T> K, int src, int dst, 
 // (rhat, s - omega t) without another reduction
 template <typename T>
 __global__ void __launch_bounds__(BS) k_bicg_spmv_t(KArgs<T> K) {
+  KARGS_EARLY(K.A, K.v[VSH], K.v[VS], K.v[VRH], K.N)
   KPROLOGUE
   (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
@@ -596,6 +608,7 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv_t(KArgs<T> K) {
 // s = r - alpha v; shat = M^-1 s
 template <typename T, int PV>
 __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
+  KARGS_EARLY(K.v[VR], K.v[VV], K.T4, K.T4s, K.N)
   KPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
@@ -610,18 +623,22 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
       v[c] = K.v[VV][c * N + H.g];
     }
   }
+  // the device scalars this kernel reads are written by earlier launches
+  // only: read with the vectors, before the reduction
+  const int s_it = S.it, s_maxit = S.maxit;
+  const double s_bb = S.bb, s_rho = S.rho2[K.parity], s_tol2 = S.tol2;
   KSTOP
   double red[2];
   device_sums<2>(K.part, K.nblk, 1, I, sh, red);
   const double rr = red[0], rv = red[1];
-  const bool first = S.it == 0;
-  const double bb = first ? rr : S.bb;
-  const double rho = first ? rr : S.rho2[K.parity];
-  if (rr <= S.tol2 * bb || S.it >= S.maxit || !isfinite(rr) || rho == 0.0) {
+  const bool first = s_it == 0;
+  const double bb = first ? rr : s_bb;
+  const double rho = first ? rr : s_rho;
+  if (rr <= s_tol2 * bb || s_it >= s_maxit || !isfinite(rr) || rho == 0.0) {
     if (lead) {
       S.rr = rr;
       if (first) S.bb = bb;
-      if (!(rr <= S.tol2 * bb)) S.status = 1;
+      if (!(rr <= s_tol2 * bb)) S.status = 1;
       S.done = 1;
     }
     return;
@@ -664,6 +681,8 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
 // order); partial (r, r) at quantity 1 for the next stop test.
 template <typename T, int PV>
 __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
+  KARGS_EARLY(K.v[VX], K.v[VPH], K.v[VSH], K.v[VS], K.v[VT])
+  KARGS_EARLY(K.v[VP], K.v[VV], K.T4, K.T4s, K.N)
   KPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
@@ -685,17 +704,19 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
       v[c] = K.v[VV][ix];
     }
   }
+  // written by earlier launches only: read with the vectors
+  const double alpha = S.alpha, rho_old = S.rho2[K.parity];
+  const int s_it = S.it;
   KSTOP
   double red[4];
   device_sums<4>(K.part, K.nblk, 3, I, sh, red);
   const double ts = red[0], tt = red[1], rs = red[2], rt = red[3];
   const double omega = (tt > 0.0) ? ts / tt : 0.0;
-  const double alpha = S.alpha;
   // omega == 0: stagnation; the tiny value keeps beta finite and the next
   // stop test decides between converged and failed
   const double om_b = omega != 0.0 ? omega : 1e-300;
   const double rho_new = rs - omega * rt;
-  const double beta = (rho_new / S.rho2[K.parity]) * (alpha / om_b);
+  const double beta = (rho_new / rho_old) * (alpha / om_b);
   double rr = 0.0, pn[3] = {0, 0, 0};
   if (H.valid) {
 #pragma unroll
@@ -720,7 +741,7 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
   if (lead) {
     S.rho2[K.parity ^ 1] = rho_new;
     S.omega = om_b;
-    S.it = S.it + 1;
+    S.it = s_it + 1;
   }
 }
 
@@ -830,7 +851,7 @@ template <typename T>
 static void launch_iteration(Batch &b, KArgs<T> K, int parity) {
   const dim3 g(b.nblk), t(BS);
   K.parity = parity;
-  hipLaunchKernelGGL(k_bicg_spmv<T>, g, t, 0, b.stream, K, (int)VPH, (int)VV, (int)VRH, 0, 2);
+  hipLaunchKernelGGL(k_bicg_spmv<T>, g, t, 0, b.stream, K);
   launch_k_bicg_s<T>(b, K);
   hipLaunchKernelGGL(k_bicg_spmv_t<T>, g, t, 0, b.stream, K);
   launch_k_bicg_u<T>(b, K);
@@ -893,12 +914,10 @@ int bicgstab(Batch &b, bool fp32, double rtol, int maxit, bool, int chunk) {
 int launch_spmv_only(Batch &b, bool fp32) {
   if (fp32) {
     KArgs<float> K = make_kargs<float>(b, true);
-    hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(b.nblk), dim3(BS), 0, b.stream, K, (int)VPH,
-                       (int)VV, (int)VRH, 0, 2);
+    hipLaunchKernelGGL(k_bicg_spmv<float>, dim3(b.nblk), dim3(BS), 0, b.stream, K);
   } else {
     KArgs<double> K = make_kargs<double>(b, false);
-    hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(b.nblk), dim3(BS), 0, b.stream, K, (int)VPH,
-                       (int)VV, (int)VRH, 0, 2);
+    hipLaunchKernelGGL(k_bicg_spmv<double>, dim3(b.nblk), dim3(BS), 0, b.stream, K);
   }
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
@@ -1254,9 +1273,10 @@ __device__ __forceinline__ double pspmv(const double *A, long long N, const Node
 #define PPROLOGUE                                   \
   __shared__ double sh[64];                    \
   const int d = dev_index(P.W);                     \
-  const DevInfo I = dev_info(P.W, d);               \
   Scal &S = P.scal[d];                              \
-  if (!P.late_stop && S.done) return;               \
+  const int done0 = S.done;                         \
+  const DevInfo I = dev_info(P.W, d);               \
+  if (!P.late_stop && done0) return;                \
   const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
 
 __global__ void __launch_bounds__(BS) k_pcg_init(PArgs P) {
