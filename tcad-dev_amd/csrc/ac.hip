@@ -101,9 +101,12 @@ struct CArgs {
 #define CPROLOGUE                                   \
   __shared__ double sh[128];                        \
   const int d = dev_index(K.W);                     \
-  const DevInfo I = dev_info(K.W, d);               \
   ACScal &S = K.acs[d];                             \
-  if (S.done) return;                               \
+  const int done0 = S.done;                         \
+  const DevInfo I = dev_info(K.W, d);               \
+  KARGS_EARLY(I.off, I.nx, I.ny, I.px, I.blk0)      \
+  KARGS_EARLY(done0, I.dim, I.py)                   \
+  if (done0) return;                                \
   const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
 
 __device__ __forceinline__ void cload(const double *v, long long N, long long g, double x[6]) {

@@ -459,18 +459,6 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
   const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
 #define KSTOP \
   if (done0) return;
-// Kernel arguments a kernel needs after its first branch, pinned to the
-// top: their scalar loads join the first round instead of each waiting
-// behind the branch that precedes its use (the empty asm only consumes
-// the registers).
-#define KARGS_EARLY(...) asm volatile("" ::KARGS_S(__VA_ARGS__));
-#define KARGS_S1(a) "s"(a)
-#define KARGS_S(...) KARGS_SN(__VA_ARGS__, KARGS_S5, KARGS_S4, KARGS_S3, KARGS_S2, KARGS_S1)(__VA_ARGS__)
-#define KARGS_SN(_1, _2, _3, _4, _5, NAME, ...) NAME
-#define KARGS_S2(a, b) KARGS_S1(a), KARGS_S1(b)
-#define KARGS_S3(a, b, c) KARGS_S1(a), KARGS_S2(b, c)
-#define KARGS_S4(a, b, c, d) KARGS_S1(a), KARGS_S3(b, c, d)
-#define KARGS_S5(a, b, c, d, e) KARGS_S1(a), KARGS_S4(b, c, d, e)
 
 // The kernels that apply the preconditioner come in three variants (PV):
 // node block-Jacobi only (no tile code: full occupancy, for batches
@@ -1276,6 +1264,8 @@ __device__ __forceinline__ double pspmv(const double *A, long long N, const Node
   Scal &S = P.scal[d];                              \
   const int done0 = S.done;                         \
   const DevInfo I = dev_info(P.W, d);               \
+  KARGS_EARLY(I.off, I.nx, I.ny, I.px, I.blk0)      \
+  KARGS_EARLY(done0, I.dim, I.py)                   \
   if (!P.late_stop && done0) return;                \
   const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
 
@@ -1310,6 +1300,7 @@ __global__ void __launch_bounds__(BS) k_pcg_init(PArgs P) {
 // q = A p; partial (p, q).  The direction update and the product share a
 // kernel: two launches per iteration.
 __global__ void __launch_bounds__(BS) k_pcg_ab(PArgs P) {
+  KARGS_EARLY(P.z, P.p, P.p2, P.A, P.N)
   PPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = P.N;
@@ -1326,21 +1317,24 @@ __global__ void __launch_bounds__(BS) k_pcg_ab(PArgs P) {
       pn[dir] = pin[H.nbg[dir]];
     }
   }
+  // written by earlier launches only: read with the vectors
+  const int s_it = S.it, s_maxit = S.maxit;
+  const double s_bb = S.bb, s_tol2 = S.tol2, rho_old = S.rho2[P.parity ^ 1];
   KSTOP
   double red[2];
   device_sums<2>(P.part, P.nblk, 0, I, sh, red);
   const double rz = red[0], rr = red[1];
-  const double bb = (S.it == 0) ? rr : S.bb;
-  if (rr <= S.tol2 * bb || S.it >= S.maxit || !isfinite(rr)) {
+  const double bb = (s_it == 0) ? rr : s_bb;
+  if (rr <= s_tol2 * bb || s_it >= s_maxit || !isfinite(rr)) {
     if (lead) {
       S.rr = rr;
-      if (S.it == 0) S.bb = bb;
-      if (!(rr <= S.tol2 * bb)) S.status = 1;
+      if (s_it == 0) S.bb = bb;
+      if (!(rr <= s_tol2 * bb)) S.status = 1;
       S.done = 1;
     }
     return;
   }
-  const double beta = rz / S.rho2[P.parity ^ 1];
+  const double beta = rz / rho_old;
   double pq = 0.0;
   if (H.valid) {
     const double p0 = zc + beta * pc;
@@ -1357,13 +1351,14 @@ __global__ void __launch_bounds__(BS) k_pcg_ab(PArgs P) {
   if (lead) {
     S.rho2[P.parity] = rz;
     S.rr = rr;
-    if (S.it == 0) S.bb = bb;
+    if (s_it == 0) S.bb = bb;
   }
 }
 
 // Reduce (p, q) -> alpha; x += alpha p; r -= alpha q; z = r / diag;
 // partials (r, z), (r, r)
 __global__ void __launch_bounds__(BS) k_pcg_c(PArgs P) {
+  KARGS_EARLY(P.x, P.r, P.q, P.p, P.p2)
   PPROLOGUE
   const NodeHood H = hood(I, blockIdx.x);
   const double *pcur = P.parity ? P.p : P.p2;
@@ -1375,9 +1370,11 @@ __global__ void __launch_bounds__(BS) k_pcg_c(PArgs P) {
     r = P.r[H.g];
     q = P.q[H.g];
   }
+  const double rho = S.rho2[P.parity];            // written by k_pcg_ab
+  const int s_it = S.it;
   KSTOP
   const double pq = device_sum(P.part, P.nblk, 2, I, sh);
-  const double alpha = S.rho2[P.parity] / pq;
+  const double alpha = rho / pq;
   double in[2] = {0.0, 0.0};
   const double rn = r - alpha * q;
   const double zv = pcg_prec(P, I.dim, H.valid, H.g, rn, tv);
@@ -1394,7 +1391,7 @@ __global__ void __launch_bounds__(BS) k_pcg_c(PArgs P) {
     P.part[blockIdx.x] = o[0];
     P.part[(long long)P.nblk + blockIdx.x] = o[1];
   }
-  if (lead) S.it = S.it + 1;
+  if (lead) S.it = s_it + 1;
 }
 
 int pcg_poisson(Batch &b, double rtol, int maxit) {

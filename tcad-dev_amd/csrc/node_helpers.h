@@ -10,6 +10,20 @@ namespace tcad {
 // ---------------------------------------------------------------------------
 // shared device helpers
 // ---------------------------------------------------------------------------
+// Values a kernel needs after its first branches, pinned to the top of the
+// kernel: an empty asm that consumes them (uniform values, in scalar
+// registers) makes their loads join one early round instead of each
+// waiting behind the branch that precedes its first use.  Loads after it
+// are issued as vector loads (the asm counts as a possible store).
+#define KARGS_EARLY(...) asm volatile("" ::KARGS_S(__VA_ARGS__));
+#define KARGS_S1(a) "s"(a)
+#define KARGS_S(...) KARGS_SN(__VA_ARGS__, KARGS_S5, KARGS_S4, KARGS_S3, KARGS_S2, KARGS_S1)(__VA_ARGS__)
+#define KARGS_SN(_1, _2, _3, _4, _5, NAME, ...) NAME
+#define KARGS_S2(a, b) KARGS_S1(a), KARGS_S1(b)
+#define KARGS_S3(a, b, c) KARGS_S1(a), KARGS_S2(b, c)
+#define KARGS_S4(a, b, c, d) KARGS_S1(a), KARGS_S3(b, c, d)
+#define KARGS_S5(a, b, c, d, e) KARGS_S1(a), KARGS_S4(b, c, d, e)
+
 struct Where {
   const DevInfo *info;              // batch table
   const int *blk2dev;
