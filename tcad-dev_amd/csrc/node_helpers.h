@@ -41,9 +41,32 @@ __device__ __forceinline__ int dev_index(const Where &W) {
 // a conditional load of its own, a chain of dependent scalar loads.)
 __device__ __forceinline__ DevInfo dev_info(const Where &W, int d) { return W.info[d]; }
 
+// v of lane i + O within its row of 16 lanes (O = 1, 2, 4, 8), by DPP row
+// shifts of the two halves (lanes whose source leaves the row read 0).
+template <int O>
+__device__ __forceinline__ double row_down(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x100 + O, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x100 + O, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Wavefront sum into lane 0 by the fixed tree v += v[lane + o] for
+// o = 32, 16, 8, 4, 2, 1.  The last four steps only need lanes inside
+// lane 0's row of 16, so they are DPP moves instead of LDS permutes.
+__device__ __forceinline__ double wave_sum0(double v) {
+  v += __shfl_down(v, 32, 64);
+  v += __shfl_down(v, 16, 64);
+  v += row_down<8>(v);
+  v += row_down<4>(v);
+  v += row_down<2>(v);
+  v += row_down<1>(v);
+  return v;
+}
+
 // Block-wide FP64 sum, fixed tree; result valid in every thread.
 __device__ __forceinline__ double block_sum(double v, double *sh) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  v = wave_sum0(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) sh[wid] = v;
@@ -59,8 +82,7 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 template <int NQ>
 __device__ __forceinline__ void block_sums(double v[NQ], double *sh, double out[NQ]) {
 #pragma unroll
-  for (int q = 0; q < NQ; ++q)
-    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+  for (int q = 0; q < NQ; ++q) v[q] = wave_sum0(v[q]);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) {
