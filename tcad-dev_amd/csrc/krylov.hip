@@ -24,7 +24,11 @@
 //   workgroup finalize was measured 1.5x slower on gfx950: its
 //   device-scope release/acquire writes back and invalidates the XCD's L2
 //   per workgroup.)  The vector loads are issued before the reductions so
-//   the two latencies overlap.
+//   the two latencies overlap.  At this size every launch is latency: a
+//   kernel reaches its first vector load after two rounds of loads (its
+//   arguments; then its device's metadata with the stop flag, see
+//   KPROLOGUE), reads the device scalars with the vectors, and sums its
+//   partials with DPP row shifts where the tree allows (wave_sum0).
 // * Every Newton linear solve is FP64 iterative refinement: the true
 //   residual r = b - A64 y is formed in FP64 and BiCGStab solves
 //   A d = r / ||r|| with the matrix and block inverses stored in FP64
