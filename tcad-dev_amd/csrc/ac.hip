@@ -701,16 +701,21 @@ __global__ void __launch_bounds__(BS) k_cb_s(CArgs K) {
     cload(K.v[CR], N, H.g, r);
     cload(K.v[CV], N, H.g, v);
   }
+  // device scalars (written by earlier launches, or rewritten here with the
+  // same values): read with the vectors, before the reduction
+  const int s_it = S.it, s_sweep = S.sweep, s_maxit = S.maxit;
+  const double s_bb = S.bb, s_b2 = S.b2, s_tol2 = S.tol2, s_abs2 = S.abs2;
+  const Cx s_rho = {S.rho[K.parity][0], S.rho[K.parity][1]};
   double red[4];
   device_sums<4>(K.part, K.nblk, 0, I, sh, red);
-  const bool first_it = S.it == 0;
+  const bool first_it = s_it == 0;
   const double rr = red[0];
-  const double bb = first_it ? rr : S.bb;
-  const double b2 = (first_it && S.sweep == 0) ? red[1] : S.b2;
+  const double bb = first_it ? rr : s_bb;
+  const double b2 = (first_it && s_sweep == 0) ? red[1] : s_b2;
   const Cx rv = {red[2], red[3]};
-  const Cx rho = first_it ? Cx{rr, 0.0} : Cx{S.rho[K.parity][0], S.rho[K.parity][1]};
-  const bool conv = rr <= S.tol2 * bb || rr <= S.abs2 * b2;
-  if (conv || S.it >= S.maxit || !isfinite(rr) || (rho.re == 0.0 && rho.im == 0.0)) {
+  const Cx rho = first_it ? Cx{rr, 0.0} : s_rho;
+  const bool conv = rr <= s_tol2 * bb || rr <= s_abs2 * b2;
+  if (conv || s_it >= s_maxit || !isfinite(rr) || (rho.re == 0.0 && rho.im == 0.0)) {
     if (lead) {
       S.rr = rr;
       if (first_it) {
@@ -774,6 +779,10 @@ __global__ void __launch_bounds__(BS) k_cb_u(CArgs K) {
     cload(K.v[CP], N, H.g, p);
     cload(K.v[CV], N, H.g, v);
   }
+  // written by earlier launches only: read with the vectors
+  const Cx alpha = {S.alpha[0], S.alpha[1]};
+  const Cx rho_old = {S.rho[K.parity][0], S.rho[K.parity][1]};
+  const int s_it = S.it;
   double red[7];
   device_sums<7>(K.part, K.nblk, 1, I, sh, red);
   const Cx ts = {red[0], red[1]};
@@ -784,9 +793,7 @@ __global__ void __launch_bounds__(BS) k_cb_u(CArgs K) {
   // stop test (or the next sweep) decides
   const bool zero = omega.re == 0.0 && omega.im == 0.0;
   const Cx om_b = zero ? Cx{1e-300, 0.0} : omega;
-  const Cx alpha = {S.alpha[0], S.alpha[1]};
   const Cx rho_new = csub(rs, cmul(omega, rt));
-  const Cx rho_old = {S.rho[K.parity][0], S.rho[K.parity][1]};
   const Cx beta = cmul(cdiv(rho_new, rho_old), cdiv(alpha, om_b));
   double rr = 0.0;
   double pn[6] = {0, 0, 0, 0, 0, 0}, z[6];
@@ -821,7 +828,7 @@ __global__ void __launch_bounds__(BS) k_cb_u(CArgs K) {
     S.rho[K.parity ^ 1][1] = rho_new.im;
     S.omega[0] = om_b.re;
     S.omega[1] = om_b.im;
-    S.it = S.it + 1;
+    S.it = s_it + 1;
   }
 }
 
