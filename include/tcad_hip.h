@@ -155,6 +155,26 @@ int tcad_set_bias(tcad_batch *b, const double *h_contact_V,
 int tcad_assemble(tcad_batch *b, const double *d_psi, const double *d_n,
                   const double *d_p, double *d_F, double *d_Fnp);
 
+/* The reference's raw residual/Jacobian at (psi, n, p) and the voltages
+ * set by tcad_set_bias, without Dirichlet elimination or scaling:
+ *   d_F     3 planes of N_total: F after the contact and gate rows;
+ *   d_Fn,
+ *   d_Fp    N_total each: raw F_n, F_p per node before the contact rows;
+ *   d_Jraw  ne planes of N_total in the stencil-block layout of
+ *           tcad_export_jacobian (ne = 9 + 10*dim_max): every node's
+ *           Jacobian row entries BEFORE contact rows become Dirichlet
+ *           rows.  At a contact node these are the raw rows the
+ *           reference's current_rows_for captures; the reference's J
+ *           holds e_k there (the caller stamps them).  Entries coupling
+ *           to a contact neighbour are kept (not eliminated).
+ * The handle's Newton matrix is overwritten, as by tcad_assemble.
+ * Replaces: Device2D._residual_jacobian (device2d.py:1402-1920, its
+ * F, J, F_n, F_p outputs and the current_rows_for capture :1751-1758),
+ * Device3D._residual_jacobian (device3d.py:1485-1978). */
+int tcad_residual_jacobian(tcad_batch *b, const double *d_psi, const double *d_n,
+                           const double *d_p, double *d_F, double *d_Fn, double *d_Fp,
+                           double *d_Jraw);
+
 /* Copy the last assembled Jacobian to the host in stencil-block form:
  * h_blocks[e * N_total + k] for e < 9 + 5*2*dim_max (entries 0..8: the
  * 3x3 node block, then per neighbour direction x-,x+,y-,y+,z-,z+ the

@@ -27,13 +27,19 @@ Unknowns are interleaved (psi, n, p) per node, node k = j*Nx + i in 2D
 and k = kk*Nx*Ny + j*Nx + i in 3D (mesh2d.py:1-8, mesh3d.py:1-9).  All
 quantities are in the reference's scaled units (psi/VT, n/Ns, x/LD).
 
-Pinning: 2D by golden fixtures generated from the reference itself
-(tests/golden/make_golden.py -> tests/golden/*.npz): residual F and
-Jacobian J at a perturbed state, equilibrium psi, bias-point psi/n/p,
-Newton iteration counts and terminal currents.  3D by invariants
-(tests/test_oracle_dd.py): a z-invariant 3D device equals the pinned 2D
-oracle, axis permutation symmetry (the same device built along x and
-along z), finite-difference Jacobian, and terminal-current conservation.
+Pinning: golden fixtures generated from the reference itself, matched
+bit for bit (tests/test_oracle_golden.py).  2D: tests/golden/make_golden.py
+(reference Device2D).  3D: tests/golden/make_golden_3d.py (reference
+Device3D, its five compiled assembly blocks replaced by the reference's
+own pure-NumPy bodies, device3d.py:190-258, 320-408).  Each fixture holds
+the residual F, the raw Jacobian J, the edge currents and F_n/F_p at a
+perturbed state (3D also the equilibrium Poisson F/J), and the
+equilibrium psi, bias-point psi/n/p, Newton iteration counts and terminal
+currents of a sweep: 3D covers a tri-gate (y- and z-normal gates, body
+contact overlapping source and drain) at Vd = 0.05 and 0.7 V and slabs
+with x- and z-normal gates.  tests/test_oracle_dd3d.py adds invariants
+(z-invariant 3D equals 2D, axis permutation, finite-difference Jacobian,
+current conservation).
 """
 import warnings
 
@@ -399,6 +405,11 @@ class _DeviceND:
                 strip += [3 * kk, 3 * kk + 1, 3 * kk + 2]
         if strip:
             strip = np.unique(np.concatenate(strip))
+            if nd == 3:
+                # device3d.py:1910-1922: the eliminated rows are listed by
+                # component block (every contact node's psi row, then n, then p)
+                ck = strip[0::3] // 3
+                strip = np.concatenate([3 * ck, 3 * ck + 1, 3 * ck + 2])
             keep = ~np.isin(rows, strip)
             rows = np.concatenate([rows[keep], strip])
             cols = np.concatenate([cols[keep], strip])

@@ -55,6 +55,7 @@ struct AsmArgs {
   double *ac_raw;                   // 21 planes: raw (n + p)-row sums, node then directions
   double *F, *Fnp, *A64, *Dinv64, *bs, *rowscale;
   float *A32, *Dinv32;
+  double *Jpre;                     // optional: raw rows before the contact rows (ne planes)
   int *flag;                        // [0] any, [1 + d] per device
   const Scal *scal;                 // non-NULL: skip devices not active
 };
@@ -237,6 +238,18 @@ __device__ void assemble_node(const AsmArgs &A, const DevInfo &I, long long k, i
     const double Vg_s = A.gate_Vg[gate] / I.VT, Vfb_s = A.gate_Vfb[gate] / I.VT;
     Fpsi = Fpsi + kw * (Vg_s - Vfb_s - (psi - A.gn_psib[gt]));
     D[0] = D[0] + -kw;
+  }
+
+  if (A.Jpre) {
+    // the node's raw Jacobian row entries before contact rows become
+    // Dirichlet rows (device2d.py:1742-1760, device3d.py:1875-1912):
+    // node block, then per direction the five coupling entries
+#pragma unroll
+    for (int e = 0; e < 9; ++e) A.Jpre[e * N + g] = D[e];
+    const int ndir = (A.ne - 9) / 5;
+    for (int dir = 0; dir < ndir; ++dir)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) A.Jpre[(9 + 5 * dir + m) * N + g] = off[dir][m];
   }
 
   const int ct = A.cidx[g];
@@ -494,8 +507,9 @@ static AsmArgs make_args(Batch &b, const double *psi, const double *n, const dou
 
 int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
                     double *F, double *Fnp, bool want_fp32, bool only_active, int res_only,
-                    const TransientArgs *tr, const AcAssembly *ac) {
+                    const TransientArgs *tr, const AcAssembly *ac, double *Jpre) {
   AsmArgs A = make_args(b, psi, n, p);
+  A.Jpre = Jpre;
   if (ac) {
     A.ac = ac->mode;
     A.acg = ac->acg;

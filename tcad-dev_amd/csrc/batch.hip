@@ -1281,6 +1281,23 @@ int tcad_continuity_residuals(tcad_batch *bp, const double *psi, const double *n
   return TCAD_OK;
 }
 
+int tcad_residual_jacobian(tcad_batch *bp, const double *psi, const double *n, const double *p,
+                           double *F, double *Fn, double *Fp, double *Jraw) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi || !n || !p || !F || !Fn || !Fp || !Jraw) {
+    tcad::set_error("tcad_residual_jacobian: null argument");
+    return TCAD_E_INVALID;
+  }
+  tcad::TransientArgs tr{};
+  tr.Fn_raw = Fn;
+  tr.Fp_raw = Fp;
+  int rc = tcad::launch_assemble(b, psi, n, p, F, nullptr, false, false, 0, &tr, nullptr, Jraw);
+  if (rc) return rc;
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  return tcad::check_flag(b);
+}
+
 int tcad_terminal_currents(tcad_batch *bp, const double *psi, const double *n, const double *p,
                            double *h_I) {
   GUARD(bp);

@@ -256,7 +256,7 @@ struct AcAssembly {
 int launch_assemble(Batch &b, const double *psi, const double *n, const double *p,
                     double *F, double *Fnp, bool want_fp32, bool only_active = false,
                     int res_only = 0, const TransientArgs *tr = nullptr,
-                    const AcAssembly *ac = nullptr);
+                    const AcAssembly *ac = nullptr, double *Jpre = nullptr);
 int launch_surface_mobility(Batch &b, const double *psi, bool only_active = false);
 int launch_assemble_poisson(Batch &b, const double *psi, double *F);
 int launch_edge_currents(Batch &b, const double *psi, const double *n, const double *p,

@@ -95,6 +95,7 @@ def lib():
         "tcad_assemble": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "tcad_export_jacobian": ([vp, c_double_p, c_double_p, c_double_p, c_int32_p],
                                  ctypes.c_int),
+        "tcad_residual_jacobian": ([vp, vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "tcad_solve_linear": ([vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, vp,
                                c_int32_p, c_double_p], ctypes.c_int),
         "tcad_solve_equilibrium": ([vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
@@ -160,7 +161,8 @@ def iptr(a):
 def exported_symbols():
     """Names declared in include/tcad_hip.h that the library must export."""
     return ["tcad_batch_create", "tcad_batch_destroy", "tcad_batch_nodes", "tcad_set_bias",
-            "tcad_assemble", "tcad_export_jacobian", "tcad_solve_linear",
+            "tcad_assemble", "tcad_export_jacobian", "tcad_residual_jacobian",
+            "tcad_solve_linear",
             "tcad_solve_equilibrium", "tcad_solve_bias", "tcad_terminal_currents",
             "tcad_edge_currents", "tcad_surface_mobility", "tcad_time_kernel",
             "tcad_transient_step", "tcad_continuity_residuals", "tcad_y_parameters",

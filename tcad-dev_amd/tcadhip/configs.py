@@ -194,6 +194,57 @@ def trigate_3d(nx=64, ny=48, nz=48, Lg=20 * NM, Lsd=15 * NM, H=30 * NM, W=10 * N
                       {}, contacts, gates, sweep, name)
 
 
+def gated_slab_3d(axis, nx, ny, nz, L=60 * NM, depth=30 * NM, width=20 * NM, tox=2 * NM,
+                  Na=1e17, Nsd=1e19, Vd=0.1, vg_list=(0.0, 0.5, 1.0), name=None):
+    """3D MOS slab whose single gate face is normal to `axis` ('x' or
+    'z'): the face at index 0 of that axis, over the middle half of the
+    channel.  The channel runs along y for an x-normal gate and along x
+    for a z-normal gate; abrupt n+ source/drain over the first and last
+    quarter of the channel length on a p body.  Contacts: source and
+    drain over the two end faces of the channel axis, body over the
+    face opposite the gate.  The mesh is geometric away from the gate
+    face (first cell depth/40) and uniform along the other two axes."""
+    ax = {"x": 0, "y": 1, "z": 2}
+    g = ax[axis]
+    c = 1 if axis == "x" else 0                     # channel axis
+    w = 3 - g - c                                   # the remaining axis
+    n = (nx, ny, nz)
+    lengths = [0.0, 0.0, 0.0]
+    lengths[g], lengths[c], lengths[w] = depth, L, width
+    coords = [None, None, None]
+    coords[g] = geometric_mesh(depth, n[g], depth / 40)
+    coords[c] = uniform_mesh(L, n[c])
+    coords[w] = uniform_mesh(width, n[w])
+    shape = (nz, ny, nx)
+    cc = coords[c]
+    sd = (cc < 0.25 * L) | (cc > 0.75 * L)
+    prof = np.where(sd, Nsd, -Na)
+    bshape = [1, 1, 1]
+    bshape[2 - c] = n[c]
+    doping = np.broadcast_to(prof.reshape(bshape), shape).copy()
+
+    def face(axis_idx, value, sel_axis=None, sel=None):
+        ranges = [np.arange(nx), np.arange(ny), np.arange(nz)]
+        ranges[axis_idx] = np.array([value])
+        if sel_axis is not None:
+            ranges[sel_axis] = sel
+        I, J, K = np.meshgrid(*ranges, indexing="ij")
+        return (K.ravel(), J.ravel(), I.ravel())
+
+    chan = np.where((cc > 0.25 * L) & (cc < 0.75 * L))[0]
+    contacts = [ContactSpec("source", face(c, 0)),
+                ContactSpec("drain", face(c, n[c] - 1)),
+                ContactSpec("body", face(g, n[g] - 1))]
+    Vfb = flatband_voltage(-Na, tox, "n+poly", 300.0, SILICON)
+    gates = [GateSpec("gate", face(g, 0, c, chan), tox, Vfb, axis)]
+    sweep = [{"drain": float(Vd), "gate": float(v)} for v in vg_list]
+    return DeviceSpec(coords, doping, None,
+                      dict(doping_mobility=True, srh=True, auger=True, bgn=True,
+                           surface_mobility=False),
+                      {}, contacts, gates, sweep,
+                      name or f"slab_{axis}gate_{nx}x{ny}x{nz}")
+
+
 def config3_trigate(Vd=0.05):
     """64x48x48 tri-gate, Vg 0 -> 0.8 V in 17 points at the given Vd."""
     return trigate_3d(Vd=Vd, name=f"trigate_64x48x48_vd{Vd}")

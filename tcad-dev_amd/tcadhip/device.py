@@ -578,12 +578,84 @@ class _GPUDevice:
         _lib.check(_lib.lib().tcad_surface_mobility(B.handle, t.data_ptr()),
                    "tcad_surface_mobility")
 
-    def _residual_jacobian(self, psi, n, p, voltages):
-        """GPU assembly at (psi, n, p).  Returns (F, Jd, Jn, Jp, Fnp):
-        F interleaved (3N) exactly as the reference's first output; Jd the
-        Dirichlet-eliminated Jacobian (what eliminate_csr returns,
-        dirichlet.py:161-191) as scipy CSR; per-axis scaled SG edge
-        currents (node-shaped, last entry along each axis 0); and the raw
+    def _edge_shaped(self, Jh, carrier):
+        """Node-shaped edge planes -> the reference's per-axis edge arrays,
+        e.g. (Ny, Nx-1) along x in 2D."""
+        out = []
+        for a in range(self.dim):
+            ax = self.dim - 1 - a
+            sl = [slice(None)] * self.dim
+            sl[ax] = slice(None, -1)
+            out.append(Jh[a, carrier].reshape(self.shape)[tuple(sl)])
+        return out
+
+    def _residual_jacobian(self, psi, n, p, voltages, theta=None, current_rows_for=None):
+        """The reference's operator: device2d.py:1402-1920 returns
+        (F, J, Jn_x, Jn_y, Jp_x, Jp_y, F_n, F_p), device3d.py:1485-1978
+        (F, J, Jn_x, Jn_y, Jn_z, Jp_x, Jp_y, Jp_z, F_n, F_p).  F is the
+        interleaved residual after the contact and gate rows, J the raw
+        (not eliminated, not scaled) Jacobian as scipy CSR with unit rows
+        at contact unknowns, the edge currents scaled SG fluxes in edge
+        shapes, F_n and F_p the raw continuity residuals.  Sets
+        `_dirichlet_rows` (the eliminated rows, in the reference's order)
+        and, for `current_rows_for` node indices, `_raw_current_rows` =
+        (rows, cols, vals) of their raw continuity rows 3k+1, 3k+2 before
+        the contact rows replace them (device2d.py:1751-1758).  Assembled
+        on the GPU by tcad_residual_jacobian; theta (the energy-balance
+        temperature) is not part of this path."""
+        if theta is not None:
+            raise NotImplementedError("energy balance (theta) is not part of this path")
+        torch = _torch()
+        self._set_voltages(voltages)
+        B = self._gpu()
+        B.set_bias()
+        N = self.N
+        dev = [torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64).ravel()).to("cuda")
+               for v in (psi, n, p)]
+        ne = 9 + 10 * max(s.dim for s in B.setups)
+        F = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+        Fn = torch.zeros(N, dtype=torch.float64, device="cuda")
+        Fp = torch.zeros(N, dtype=torch.float64, device="cuda")
+        Jr = torch.zeros(ne * N, dtype=torch.float64, device="cuda")
+        L = _lib.lib()
+        _lib.check(L.tcad_residual_jacobian(B.handle, dev[0].data_ptr(), dev[1].data_ptr(),
+                                            dev[2].data_ptr(), F.data_ptr(), Fn.data_ptr(),
+                                            Fp.data_ptr(), Jr.data_ptr()),
+                   "tcad_residual_jacobian")
+        Jq = torch.zeros(2 * self.dim * N, dtype=torch.float64, device="cuda")
+        _lib.check(L.tcad_edge_currents(B.handle, dev[0].data_ptr(), dev[1].data_ptr(),
+                                        dev[2].data_ptr(), Jq.data_ptr()), "tcad_edge_currents")
+        Jh = Jq.cpu().numpy().reshape(self.dim, 2, N)
+        Fh = F.cpu().numpy().reshape(3, N).T.ravel()
+        rows, cols, vals = raw_stencil_coo(Jr.cpu().numpy().reshape(ne, N), self._s)
+        contact = np.unique(np.concatenate(
+            [np.ravel_multi_index(bc.idx, self.shape) for bc in self.bcs.values()
+             if isinstance(bc, DirichletBC)] or [np.zeros(0, dtype=np.int64)]))
+        if current_rows_for is not None:
+            k_req = np.asarray(current_rows_for, dtype=int)
+            sel = np.isin(rows, np.concatenate([3 * k_req + 1, 3 * k_req + 2]))
+            self._raw_current_rows = (rows[sel].copy(), cols[sel].copy(), vals[sel].copy())
+        if self.dim == 3:
+            # device3d.py:1910-1922: component blocks, psi rows then n then p
+            strip = np.concatenate([3 * contact, 3 * contact + 1, 3 * contact + 2])
+        else:
+            strip = np.unique(np.concatenate([3 * contact, 3 * contact + 1, 3 * contact + 2]))
+        keep = ~np.isin(rows, strip)
+        import scipy.sparse as sp
+        J = sp.csr_matrix((np.concatenate([vals[keep], np.ones(strip.size)]),
+                           (np.concatenate([rows[keep], strip]),
+                            np.concatenate([cols[keep], strip]))), shape=(3 * N, 3 * N))
+        self._dirichlet_rows = strip
+        Fn_h = Fn.cpu().numpy().reshape(self.shape)
+        Fp_h = Fp.cpu().numpy().reshape(self.shape)
+        return (Fh, J, *self._edge_shaped(Jh, 0), *self._edge_shaped(Jh, 1), Fn_h, Fp_h)
+
+    def eliminated_system(self, psi, n, p, voltages):
+        """The Newton system the GPU solves, at (psi, n, p): (F, Jd, Jn,
+        Jp, Fnp) with F as in _residual_jacobian, Jd the Dirichlet-
+        eliminated Jacobian (what eliminate_csr returns, dirichlet.py:
+        161-191) as scipy CSR, per-axis scaled SG edge currents
+        (node-shaped, last entry along each axis 0) and the raw
         pre-Dirichlet F_n + F_p per node."""
         torch = _torch()
         self._set_voltages(voltages)
@@ -616,6 +688,39 @@ class _GPUDevice:
         return (Fh, Jd, [Jh[a, 0].reshape(self.shape) for a in range(self.dim)],
                 [Jh[a, 1].reshape(self.shape) for a in range(self.dim)],
                 Fnp.cpu().numpy().reshape(self.shape))
+
+
+_PATTERN = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 2)]
+
+
+def raw_stencil_coo(blocks, setup):
+    """COO (rows, cols, vals) of the raw Jacobian rows exported in
+    stencil-block form (tcad_residual_jacobian): every node's 3x3 block
+    and, per existing neighbour, the five coupling entries; interleaved
+    unknown numbering 3k + c."""
+    N = setup.N
+    idx = np.arange(N).reshape(setup.shape)
+    k = np.arange(N)
+    rows, cols, vals = [], [], []
+    for r in range(3):
+        for c in range(3):
+            rows.append(3 * k + r); cols.append(3 * k + c); vals.append(blocks[3 * r + c])
+    for a in range(setup.dim):
+        ax = setup.dim - 1 - a
+        for side in (0, 1):
+            dirn = 2 * a + side
+            sl_self = [slice(None)] * setup.dim
+            sl_nb = [slice(None)] * setup.dim
+            if side == 0:
+                sl_self[ax] = slice(1, None); sl_nb[ax] = slice(None, -1)
+            else:
+                sl_self[ax] = slice(None, -1); sl_nb[ax] = slice(1, None)
+            ks = idx[tuple(sl_self)].ravel()
+            kn = idx[tuple(sl_nb)].ravel()
+            for m, (r, c) in enumerate(_PATTERN):
+                rows.append(3 * ks + r); cols.append(3 * kn + c)
+                vals.append(blocks[9 + 5 * dirn + m][ks])
+    return np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
 
 
 def stencil_to_csr(blocks, rowscale, n, p, setup):

@@ -1,5 +1,5 @@
-"""3D oracle pinned by invariants (the reference's Device3D needs its
-compiled _core, which cannot be built here):
+"""3D oracle invariants, beside the bit-exact reference fixtures of
+tests/test_oracle_golden.py (made by tests/golden/make_golden_3d.py):
 
 * a z-invariant 3D device equals the golden-pinned 2D oracle
   (the reference's own gate, tests/test_validation_3d.py:68-109, 156-187):
