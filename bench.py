@@ -1,17 +1,26 @@
-"""Benchmark: converged bias-point solves/s of the 2D n-MOSFET Id-Vg sweep.
+"""Benchmark: converged bias-point solves/s of the 3D tri-gate Id-Vg sweeps.
 
-Workload: 200x120 MOSFET (0.2 nm first cell at the Si/oxide interface,
-L = 100 nm, t_ox = 2 nm, N_A = 1e17, 1e20 Gaussian S/D, Lombardi surface
-mobility), Vd = 0.05 V, Vg 0 -> 1.0 V in 21 warm-started points, as
-mosfet.id_vg_sweep runs it (equilibrium, then 21 solve_bias, drain
-current after each).  One step = one full sweep (equilibrium + 21
-converged bias points + terminal currents).  Inputs are built once and
-resident on the GPU before the timed region.
+Headline workload (the largest single-GPU configuration): the 64x48x48
+tri-gate (147,456 nodes, 442,368 unknowns; fin 10 nm wide and 30 nm tall,
+channel 1e16, source/drain 1e20 with 5 nm Gaussian lateral roll-off,
+1 nm oxide, top and side gates), Vd in {0.05, 0.7} V x Vg 0 -> 0.8 V in 17
+warm-started points, as finfet3d.id_vg_sweep_3d runs it (equilibrium, then
+every Vg point, terminal currents after each).  One step = both sweeps
+(2 equilibria + 34 converged bias points + their currents).  Inputs are
+built once and resident on the GPU before the timed region.
 
-Prints ONE JSON line.  `value` is the faster of the two GPU paths
-(FP64 BiCGStab, or FP32-stored matrix BiCGStab inside FP64 refinement)
-whose drain currents are within 1e-6 relative of the FP64 CPU oracle on
-the oracle's sample points and of the FP64 GPU path on every point.
+Prints ONE JSON line.  `value` is the FP64 path's bias points/s over the
+timed steps.  Parity is checked where the reference's own numbers exist:
+the reduced 24x18x14 tri-gate at both drain biases against the reference
+fixtures (tests/golden/make_golden_3d.py), strict 1e-6 with no floor.
+`roofline` is the dominant kernel's in-solver average (HIP events around
+every launch of it during one extra, untimed step).  `cpu_baseline` is
+the FP64 CPU restatement at full size with the reference's iterative
+method for large 3D systems (GMRES(100) + node block-Jacobi, rtol 1e-10),
+on the first bias point from the GPU's equilibrium state, under a stated
+cap.  Extras (--extras): the 60x40 diode sweep, the 200x120 MOSFET sweep
+(FP64 and FP32-matrix paths), small-signal AC, 256 batched diodes and the
+serving loop at sustainable and saturating arrival rates.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -30,312 +39,305 @@ import numpy as np  # noqa: E402
 
 METRIC = ("converged bias-point solves/sec at <=1e-6 current err vs FP64; "
           "p50/p99 latency")
+HBM_PEAK_GBPS = 8000.0
+# algorithmic bytes per node of the fused BiCGStab update kernel k_bicg_u:
+# reads x, phat, shat, s, t, p, v (7 x 3 FP64), writes x, r, p, phat
+# (4 x 3 FP64), reads the node's 3 rows of its tile inverse (36 planes of
+# 4 FP16 + 3 FP32 scales)
+K_BICG_U_BYTES_PER_NODE = 7 * 24 + 4 * 24 + 36 * 8 + 3 * 4
+PROBE_KERNELS = {0: "k_bicg_spmv<double>", 1: "k_bicg_s_tmem<double>",
+                 2: "k_bicg_spmv_t<double>", 3: "k_bicg_u_tmem<double>"}
 
 
-def run_sweep(dev, spec, opts, lat=None):
-    """One step: equilibrium + every bias point; returns drain currents."""
+def _say(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _sync():
     import torch
-    dev.solve_equilibrium(opts)
-    out = []
-    for bias in spec.sweep:
-        t0 = time.perf_counter()
-        dev.solve_bias(bias, opts)
-        I = dev.terminal_currents()
-        torch.cuda.synchronize()
-        if lat is not None:
-            lat.append(time.perf_counter() - t0)
-        out.append(I)
-    return out
-
-
-def measure(spec, linsolve, steps, warmup, rtol=1e-10, forcing=False):
-    import torch
-    from tcadhip.device import NewtonOptions, device_from_spec
-    opts = NewtonOptions(linsolve=linsolve, linsolve_rtol=rtol, forcing=forcing)
-    dev = device_from_spec(spec)
-    for _ in range(warmup):
-        run_sweep(dev, spec, opts)
     torch.cuda.synchronize()
-    lat = []
-    its, kry = [], []
+
+
+def _lib():
+    from tcadhip import _lib as L
+    return L
+
+
+# ---------------------------------------------------------------------------
+# headline: the tri-gate sweeps
+# ---------------------------------------------------------------------------
+def trigate_step(devs, specs, opts, eq_opts, lat=None, record=None):
+    """One step: for each drain bias, equilibrium + every Vg point."""
+    for dev, spec in zip(devs, specs):
+        dev.solve_equilibrium(eq_opts)
+        for bias in spec.sweep:
+            t0 = time.perf_counter()
+            dev.solve_bias(bias, opts)
+            I = dev.terminal_currents()
+            if lat is not None:
+                lat.append(time.perf_counter() - t0)
+            if record is not None:
+                record.append((spec.name, dev.last_iterations, dev.last_krylov_iterations,
+                               I["drain"]))
+
+
+def bench_trigate(steps, warmup, mode, points=None, probe=3):
+    from tcadhip import configs
+    from tcadhip.device import NewtonOptions, device_from_spec
+    specs = [configs.config3_trigate(0.05), configs.config3_trigate(0.7)]
+    if points:
+        for s in specs:
+            s.sweep = s.sweep[:points]
+    devs = [device_from_spec(s) for s in specs]
+    opts = NewtonOptions(linsolve=mode)
+    eq_opts = NewtonOptions(max_iter=400)
+    for _ in range(warmup):
+        trigate_step(devs, specs, opts, eq_opts)
+    _sync()
+    lat, rec = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
-        cur = run_sweep(dev, spec, opts, lat)
-    torch.cuda.synchronize()
+        rec = []
+        trigate_step(devs, specs, opts, eq_opts, lat, rec)
+    _sync()
     dt = time.perf_counter() - t0
-    # per-point Newton / Krylov counts of the last sweep (re-run, untimed)
-    dev.solve_equilibrium(opts)
-    for bias in spec.sweep:
-        dev.solve_bias(bias, opts)
-        its.append(dev.last_iterations)
-        kry.append(dev.last_krylov_iterations)
-    drain = np.array([c["drain"] for c in cur])
-    return dict(dev=dev, seconds=dt, latencies=np.array(lat), drain=drain,
-                newton=its, krylov=kry)
+    npts = sum(len(s.sweep) for s in specs)
+    out = dict(seconds=dt, points_per_step=npts, latencies=np.array(lat), record=rec,
+               devs=devs, specs=specs, eq_iterations=[d.last_eq_iterations for d in devs])
+    if probe is not None and probe >= 0:
+        # in-solver duration of one kernel: one more (untimed) step with HIP
+        # events around every launch of it
+        import ctypes
+        L = _lib()
+        for d in devs:
+            L.check(L.lib().tcad_probe_kernel(d._gpu().handle, probe), "tcad_probe_kernel")
+        trigate_step(devs, specs, opts, eq_opts)
+        tot_n, tot_us = 0, 0.0
+        for d in devs:
+            n = ctypes.c_int64()
+            us = ctypes.c_double()
+            L.check(L.lib().tcad_probe_read(d._gpu().handle, ctypes.byref(n), ctypes.byref(us)),
+                    "tcad_probe_read")
+            L.check(L.lib().tcad_probe_kernel(d._gpu().handle, -1), "tcad_probe_kernel")
+            tot_n += n.value
+            tot_us += n.value * us.value
+        out["probe"] = dict(kernel=probe, launches=tot_n,
+                            avg_us=tot_us / tot_n if tot_n else None)
+    return out
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on this workload from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
-    path = os.path.join(REPO, "profiles", "mosfet_200x120_spmv_hbm_traffic.md")
-    if not os.path.exists(path):
-        return None
-    for ln in open(path):
-        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
-        if len(cells) == 7 and cells[0] == f"`{kernel}`":
-            return float(cells[6]) * 1e6
-    return None
-
-
-def kernel_timing(handle, which, reps=1000):
-    """Average launch time [us] of one library kernel on a batch handle
-    (tcad_time_kernel: an untimed burst, then `reps` timed launches)."""
-    import ctypes
-    from tcadhip import _lib
-    out = ctypes.c_double()
-    _lib.check(_lib.lib().tcad_time_kernel(handle, which, reps, ctypes.byref(out)),
-               "tcad_time_kernel")
-    return out.value
-
-
-def spmv_bytes(spec, w=8):
-    """Algorithmic bytes of one stencil SpMV launch over a 2D device: the
-    matrix (9 node-block entries per node + 5 per directed neighbour link,
-    width w) + the source, product and dotted vectors (3 FP64 planes
-    each)."""
-    nx, ny = spec.coords[0].size, spec.coords[1].size
-    N = nx * ny
-    edges = 2 * ((nx - 1) * ny + nx * (ny - 1))
-    return (9 * N + 5 * edges) * w + 3 * N * 8 * 3
-
-
-def _cpu_diode(idx):
-    """One batch diode's continuation sweep on the FP64 CPU oracle (NumPy +
-    SciPy SuperLU), in a worker process of bench_batch's CPU baseline;
-    returns (final currents in contact order, seconds)."""
-    import warnings
-    warnings.simplefilter("ignore")
-    from tcadhip import configs
-    from oracle_build import oracle_sweep
-    spec = configs.batch_diodes()[idx]
-    t0 = time.perf_counter()
-    _, _, cur, _ = oracle_sweep(spec)
-    return np.asarray(cur[-1]).tolist(), time.perf_counter() - t0
-
-
-def bench_trigate(vds=(0.05, 0.7)):
-    """3D tri-gate 64x48x48 (147,456 nodes, 442,368 unknowns): equilibrium
-    + 17 Vg points at each Vd, FP64 path; Newton counts per point."""
-    import torch
-    from tcadhip import configs
+def reduced_trigate_parity(mode):
+    """The reduced 24x18x14 tri-gate at both drain biases against the
+    reference's own sweeps (committed fixtures): Newton counts and every
+    contact current per point, strict 1e-6 of the largest current."""
+    from golden.fixtures import load
     from tcadhip.device import NewtonOptions, device_from_spec
     out = {}
-    total_pts, total_s = 0, 0.0
-    for Vd in vds:
-        spec = configs.config3_trigate(Vd)
+    worst, pts, strict, newton_ok = 0.0, 0, 0, True
+    for name in ("trigate_24x18x14_vd005_sweep.npz", "trigate_24x18x14_vd07_sweep.npz"):
+        spec, d = load(name)
         dev = device_from_spec(spec)
-        opts = NewtonOptions()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
         dev.solve_equilibrium(NewtonOptions(max_iter=400))
-        its, ids = [], []
+        its, cur = [], []
         for bias in spec.sweep:
-            dev.solve_bias(bias, opts)
+            dev.solve_bias(bias, NewtonOptions(linsolve=mode))
             its.append(dev.last_iterations)
-            ids.append(dev.terminal_current("drain"))
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        total_pts += len(spec.sweep)
-        total_s += dt
-        out[f"vd{Vd}"] = {"points": len(spec.sweep), "seconds": round(dt, 3),
-                          "newton_per_point": its,
-                          "id_first_last_A": [ids[0], ids[-1]]}
-    out["points_per_s"] = round(total_pts / total_s, 4)
-    out["nodes"] = 147456
-    # the CPU oracle's direct LU is out of reach at 442,368 unknowns: the
-    # GPU/CPU comparison runs on a reduced tri-gate (same builder, 24x18x14,
-    # Vd = 0.05 V, Vg 0 -> 0.8 V in 5 points), equilibrium included on both
-    from oracle_build import oracle_sweep
-    rs = configs.trigate_3d(nx=24, ny=18, nz=14, Vd=0.05, vg_list=[0.0, 0.2, 0.4, 0.6, 0.8])
-    gcur = None
-    for rep in range(2):                       # the first run warms up
-        dev = device_from_spec(rs)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        dev.solve_equilibrium(NewtonOptions(max_iter=400))
-        gcur = []
-        for bias in rs.sweep:
-            dev.solve_bias(bias, opts)
-            gcur.append([dev.terminal_current(c.name) for c in rs.contacts])
-        torch.cuda.synchronize()
-        dt_gpu = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    _, _, ocur, _ = oracle_sweep(rs)
-    dt_cpu = time.perf_counter() - t0
-    gcur = np.array(gcur)
-    err = float(max(np.abs(gcur[p] - ocur[p]).max() / np.abs(ocur[p]).max()
-                    for p in range(len(rs.sweep))))
-    out["reduced_24x18x14_vs_cpu_oracle"] = {
-        "points": len(rs.sweep), "gpu_points_per_s": round(len(rs.sweep) / dt_gpu, 3),
-        "cpu_oracle_points_per_s": round(len(rs.sweep) / dt_cpu, 4), "cpu_cores": 1,
-        "cpu": "FP64 oracle (NumPy + SciPy SuperLU), equilibrium + 5 points",
-        "max_current_err_vs_oracle_rel_to_max": err}
+            I = dev.terminal_currents()
+            cur.append([I[c.name] for c in spec.contacts])
+        cur = np.array(cur)
+        ref = d["currents"]
+        for p in range(len(cur)):
+            e = float(np.abs(cur[p] - ref[p]).max() / np.abs(ref[p]).max())
+            worst = max(worst, e)
+            strict += int(e <= 1e-6)
+            pts += 1
+        newton_ok = newton_ok and bool(np.all(np.abs(np.array(its) - d["newton_iterations"]) <= 1))
+        out[name.split("_sweep")[0]] = {"newton_gpu": its,
+                                        "newton_reference": d["newton_iterations"].tolist()}
+    out.update({"points": pts, "points_within_1e-6_no_floor": strict,
+                "max_rel_current_err_vs_reference": worst, "newton_within_1": newton_ok})
     return out
 
 
-def bench_batch(lanes=1, sample=8):
-    """256 seeded diodes (40x30..160x100 nodes, -2.0..0.8 V, 0.1 V
-    continuation steps), batched in `lanes` streams, both precisions;
-    the plain FP64 path (one device at a time) on an evenly spaced
-    sample.  Unit of work: one device converged at its target bias."""
-    import torch
-    from tcadhip import configs, sweep
-    from tcadhip.device import NewtonOptions, setup_from_spec
-    specs = configs.batch_diodes()
-    res = {}
-    for mode in ("bicgstab_mixed", "bicgstab"):
-        setups = [setup_from_spec(s) for s in specs]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r, bss = sweep.run_lanes(setups, [s.sweep for s in specs], NewtonOptions(linsolve=mode),
-                                 lanes=lanes)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        res[mode] = dict(seconds=dt, results=r, rounds=sum(b.rounds for b in bss),
-                         rebuilds=sum(b.rebuilds for b in bss))
-        print(f"[bench] batch {mode}: {len(specs) / dt:.3f} devices/s", file=sys.stderr,
-              flush=True)
-    idx = list(range(0, len(specs), len(specs) // sample))[:sample]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    alone = {}
-    for d in idx:
-        alone[d] = sweep.BatchedSweep([setup_from_spec(specs[d])], [specs[d].sweep],
-                                      NewtonOptions()).run()[0]
-    torch.cuda.synchronize()
-    dt_alone = time.perf_counter() - t0
-    plain = len(idx) / dt_alone
-
-    def final(r):
-        return np.array([list(c.values()) for c in r.currents[-1:]]).ravel()
-
-    # the headline's roofline kernel over the whole batch (1.7 M nodes),
-    # where bandwidth rather than launch latency bounds it
-    from tcadhip.device import GPUBatch
-    B = GPUBatch([setup_from_spec(s) for s in specs])
-    B.solve_equilibrium(NewtonOptions())
-    B.solve_bias(NewtonOptions(max_iter=1))
-    us = kernel_timing(B.handle, 0)
-    nbytes = sum(spmv_bytes(s) for s in specs)
-    roof = {"kernel": "k_bicg_spmv<double>", "bytes_per_launch": int(nbytes),
-            "avg_launch_us": round(us, 3),
-            "achieved_GBps": round(nbytes / (us * 1e-6) / 1e9, 1),
-            "frac_of_8000": round(nbytes / (us * 1e-6) / 1e9 / 8000.0, 4)}
-    del B
-    out = {"devices": len(specs), "nodes": int(sum(s.N for s in specs)),
-           "continuation_solves": int(sum(len(s.sweep) for s in specs)), "lanes": lanes,
-           "plain_fp64_devices_per_s": round(plain, 4),
-           "plain_sample": f"{len(idx)} evenly spaced devices solved one at a time",
-           "spmv_roofline_whole_batch": roof}
-    # CPU baseline: the FP64 oracle, one device per worker process (one
-    # worker per host core, at most 16: one GPU's share of the host), on an
-    # evenly spaced sample; the rate is bounded by the sample's slowest
-    # device, so the mean per-device time is reported too
-    import multiprocessing as mp
-    workers = max(1, min(16, os.cpu_count() or 1))
-    cidx = list(range(0, len(specs), max(1, len(specs) // workers)))[:workers]
-    t0 = time.perf_counter()
-    with mp.get_context("spawn").Pool(workers) as pool:
-        cres = pool.map(_cpu_diode, cidx, chunksize=1)
-    dt_cpu = time.perf_counter() - t0
-    cok = True
-    for d, (ccur, _) in zip(cidx, cres):
-        names = [c.name for c in specs[d].contacts]
-        got = np.array([res["bicgstab"]["results"][d].currents[-1][nm] for nm in names])
-        ref = np.array(ccur)
-        cok = cok and bool(np.all(np.abs(got - ref) <= 1e-6 * np.abs(ref).max()
-                                  + 10.0 * abs(ref.sum())))
-    out["cpu_oracle"] = {"devices_per_s": round(len(cidx) / dt_cpu, 4), "workers": workers,
-                         "mean_seconds_per_device": round(float(np.mean([t for _, t in cres])), 2),
-                         "sample": f"{len(cidx)} evenly spaced devices, FP64 oracle (NumPy + "
-                                   f"SciPy SuperLU), one per worker process",
-                         "gpu_fp64_batch_meets_1e-6_vs_oracle": cok}
-    for mode, r in res.items():
-        fails = sum(1 for x in r["results"] if x.failed)
-        err, ok = 0.0, True
-        for d in idx:
-            got, ref = final(r["results"][d]), final(alone[d])
-            err = max(err, float(np.max(np.abs(got - ref) / np.abs(ref))))
-            # reverse-bias currents are resolvable only to the FP64
-            # solve's own conservation error |sum_c I| (the CPU oracle's
-            # direct solve shows the same imbalance): 1e-6 of the largest
-            # current plus 10x that floor, as the parity tests use
-            ok = ok and bool(np.all(np.abs(got - ref) <= 1e-6 * np.abs(ref).max()
-                                    + 10.0 * abs(ref.sum())))
-        out[mode] = {"devices_per_s": round(len(specs) / r["seconds"], 4),
-                     "seconds": round(r["seconds"], 3), "failed": fails,
-                     "rounds": r["rounds"], "compactions": r["rebuilds"],
-                     "vs_plain_fp64": round(len(specs) / r["seconds"] / plain, 2),
-                     "max_rel_current_err_vs_fp64_on_sample": err,
-                     "meets_1e-6_with_conservation_floor": ok}
-    return out
-
-
-def bench_serve(count=30, rate=2.0, check=3, lanes=2):
-    """Serving: seeded Poisson arrivals of mixed requests (2D diode and
-    MOSFET sweeps, 3D tri-gate sweeps, 1k-200k nodes, 5-40 points each),
-    continuous batching in a 32 GiB pool split into `lanes` independent
-    lanes (own HIP stream each); `check` of them re-solved alone must give
-    the same bits."""
-    import torch
-    from tcadhip import serve
+def cpu_trigate_baseline(gpu_dev, spec, cap_s, gpu_newton):
+    """The FP64 CPU restatement at full size (NumPy assembly, SciPy
+    GMRES(100) + node block-Jacobi at rtol 1e-10, one core), first Vg point
+    of the Vd = 0.05 V sweep, warm-started from the GPU's equilibrium state;
+    stops after the first Newton iteration past `cap_s`."""
+    import warnings
+    warnings.simplefilter("ignore")
+    from oracle import physics as oph
+    from oracle_build import oracle_device
     from tcadhip.device import NewtonOptions
-    reqs = serve.request_stream(count, rate=rate, seed=20261203)
-    S = serve.Server(serve.nodes_for_budget(32 << 30), opts=NewtonOptions(), lanes=lanes)
-    torch.cuda.synchronize()
+    gpu_dev.solve_equilibrium(NewtonOptions(max_iter=400))
+    odev = oracle_device(spec)
+    odev.psi, odev.n, odev.p = gpu_dev.psi, gpu_dev.n, gpu_dev.p
+    opts = oph.NewtonOptions(linsolve="gmres", linsolve_rtol=1e-10, block_size=3,
+                             direct_fallback=False)
     t0 = time.perf_counter()
-    served = S.run(reqs, realtime=True)
-    torch.cuda.synchronize()
+    err = None
+    try:
+        odev.solve_bias(spec.sweep[0], opts, deadline=t0 + cap_s)
+    except Exception as e:                    # a GMRES that does not converge
+        err = f"{type(e).__name__}: {e}"
     dt = time.perf_counter() - t0
-    lat = np.array([o.latency for o in served])
-    pts = sum(len(o.result.currents) for o in served)
-    same = True
+    done = bool(odev.last_converged) if err is None and not odev.last_timed_out else False
+    its = odev.last_iterations or 0
+    if done:
+        value = 1.0 / dt
+        how = f"converged in {its} Newton iterations, {dt:.1f} s"
+    else:
+        # not finished under the cap: the rate the iterations done imply,
+        # with the GPU's Newton count for the same point
+        value = (its / max(gpu_newton, 1)) / dt if its else 0.0
+        how = (f"NOT finished within the {cap_s:.0f} s cap: {its} of the {gpu_newton} Newton "
+               f"iterations the GPU needs in {dt:.1f} s" + (f" ({err})" if err else "")
+               + "; value = the fraction done per second")
+    return {"value": round(value, 6), "unit": "bias points/s", "cores": 1, "kind": "port",
+            "completed": done, "krylov_iterations": odev.last_krylov_iterations,
+            "sample": "FP64 oracle restatement (NumPy assembly + SciPy GMRES(100) with node "
+                      "block-Jacobi, rtol 1e-10, the reference's method for large 3D systems) "
+                      "on the full 64x48x48 tri-gate, first point (Vd 0.05 V, Vg 0 V) from the "
+                      f"GPU's equilibrium state: {how}"}
+
+
+# ---------------------------------------------------------------------------
+# extras
+# ---------------------------------------------------------------------------
+def bench_diode(steps=3, warmup=1):
+    """The 60x40 diode, 0 -> 0.6 V in 13 points, FP64, against the
+    reference's own sweep (fixture) and the CPU oracle's time for it."""
+    from golden.fixtures import load
+    from oracle_build import oracle_sweep
+    from tcadhip.device import NewtonOptions, device_from_spec
+    spec, d = load("diode_60x40_sweep.npz")
+    dev = device_from_spec(spec)
+    opts = NewtonOptions()
+    names = [c.name for c in spec.contacts]
+
+    def sweep(lat=None):
+        dev.solve_equilibrium(opts)
+        cur, its = [], []
+        for bias in spec.sweep:
+            t0 = time.perf_counter()
+            dev.solve_bias(bias, opts)
+            I = dev.terminal_currents()
+            if lat is not None:
+                lat.append(time.perf_counter() - t0)
+            cur.append([I[n] for n in names])
+            its.append(dev.last_iterations)
+        return np.array(cur), its
+    for _ in range(warmup):
+        sweep()
+    _sync()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cur, its = sweep(lat)
+    _sync()
+    dt = time.perf_counter() - t0
+    ref = d["currents"]
+    errs = [float(np.abs(cur[p] - ref[p]).max() / np.abs(ref[p]).max()) for p in range(len(ref))]
+    cons = [float(abs(ref[p].sum()) / np.abs(ref[p]).max()) for p in range(len(ref))]
     t1 = time.perf_counter()
-    alone_pts = 0
-    for o, r in list(zip(served, reqs))[:check]:
-        a = serve.serve_alone(r.spec, NewtonOptions())
-        alone_pts += len(a.currents)
-        same = same and a.newton == o.result.newton and all(
-            x[k] == y[k] for x, y in zip(a.currents, o.result.currents) for k in x)
-    torch.cuda.synchronize()
-    dt_alone = time.perf_counter() - t1
-    return {"requests": count, "arrival_rate_per_s": rate, "lanes": lanes,
-            "kinds": {k: sum(r.kind == k for r in reqs) for k in ("diode", "mosfet", "trigate")},
-            "bias_points": pts, "wall_s": round(dt, 3),
-            "points_per_s": round(pts / dt, 4),
-            "request_latency_p50_s": round(float(np.percentile(lat, 50)), 3),
-            "request_latency_p99_s": round(float(np.percentile(lat, 99)), 3),
-            "failed": sum(1 for o in served if o.result.failed),
-            "rounds": S.rounds, "batch_rebuilds": S.rebuilds, "max_resident": S.max_resident,
-            "alone_equals_mix_on_first": check, "alone_equals_mix": bool(same),
-            "plain_fp64_points_per_s_on_those": round(alone_pts / dt_alone, 4)}
+    oracle_sweep(spec)
+    dt_cpu = time.perf_counter() - t1
+    lat_ms = np.array(lat) * 1e3
+    return {"points": len(spec.sweep), "points_per_s": round(len(spec.sweep) * steps / dt, 3),
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+            "newton_gpu": its, "newton_reference": d["newton_iterations"].tolist(),
+            "points_within_1e-6_no_floor": int(sum(e <= 1e-6 for e in errs)),
+            "max_rel_current_err_vs_reference": max(errs),
+            "points_under_floor": int(sum(e > 1e-6 and e <= c + 1e-6 for e, c in zip(errs, cons))),
+            "reference_conservation_rel_max": max(cons),
+            "cpu_oracle_points_per_s": round(len(spec.sweep) / dt_cpu, 3),
+            "cpu_sample": f"FP64 oracle (NumPy + SciPy SuperLU, 1 core), equilibrium + 13 points, "
+                          f"{dt_cpu:.2f} s"}
 
 
-def bench_ac(dev, spec, freqs=tuple(np.logspace(6, 12, 7)), cpu_freqs=2):
-    """Small-signal Y(f) of the MOSFET at the last point of its sweep
-    (Vg = 1.0 V, Vd = 0.05 V for the full 21-point sweep): 4 ports x
-    len(freqs) complex solves on the GPU, f_T;
-    the FP64 oracle (SciPy SuperLU per frequency) on the first `cpu_freqs`
-    frequencies from the same operating point, for time and agreement."""
-    import torch
+def bench_mosfet(steps=2, warmup=1, modes=("bicgstab", "bicgstab_mixed"), cpu_points=3):
+    """The 200x120 MOSFET Id-Vg sweep (21 points), FP64 and FP32-matrix
+    paths; the CPU oracle on the first points for time and agreement."""
+    from oracle_build import oracle_device
+    from tcadhip import configs
+    from tcadhip.device import NewtonOptions, device_from_spec
+    spec = configs.config2_mosfet()
+    out = {}
+    ref64 = None
+    for mode in modes:
+        dev = device_from_spec(spec)
+        opts = NewtonOptions(linsolve=mode)
+
+        def sweep(lat=None):
+            dev.solve_equilibrium(opts)
+            cur, its = [], []
+            for bias in spec.sweep:
+                t0 = time.perf_counter()
+                dev.solve_bias(bias, opts)
+                cur.append(dev.terminal_currents()["drain"])
+                if lat is not None:
+                    lat.append(time.perf_counter() - t0)
+                its.append(dev.last_iterations)
+            return np.array(cur), its
+        for _ in range(warmup):
+            sweep()
+        _sync()
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            cur, its = sweep(lat)
+        _sync()
+        dt = time.perf_counter() - t0
+        if ref64 is None:
+            ref64 = cur
+        lat_ms = np.array(lat) * 1e3
+        out[mode] = {"points_per_s": round(len(spec.sweep) * steps / dt, 3),
+                     "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+                     "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                     "newton_per_point": its,
+                     "max_rel_drain_err_vs_fp64_gpu": float(np.max(np.abs(cur - ref64)
+                                                                   / np.abs(ref64)))}
+        out[mode]["_drain"] = cur
+    odev = oracle_device(spec)
+    t0 = time.perf_counter()
+    odev.solve_equilibrium()
+    ocur = []
+    for bias in spec.sweep[:cpu_points]:
+        odev.solve_bias(bias)
+        ocur.append(odev.terminal_current("drain"))
+    dt_cpu = time.perf_counter() - t0
+    ocur = np.array(ocur)
+    for mode in modes:
+        e = float(np.max(np.abs(out[mode]["_drain"][:cpu_points] - ocur) / np.abs(ocur)))
+        out[mode]["max_rel_drain_err_vs_oracle"] = e
+        out[mode]["meets_1e-6_no_floor"] = bool(
+            e <= 1e-6 and out[mode]["max_rel_drain_err_vs_fp64_gpu"] <= 1e-6)
+        del out[mode]["_drain"]
+    out["cpu_oracle_points_per_s"] = round(cpu_points / dt_cpu, 4)
+    out["cpu_sample"] = (f"FP64 oracle (NumPy + SciPy SuperLU, 1 core), equilibrium + first "
+                         f"{cpu_points} of 21 points, {dt_cpu:.1f} s")
+    return out
+
+
+def bench_ac(freqs=tuple(np.logspace(6, 12, 7)), cpu_freqs=2):
+    """Small-signal Y(f) of the 200x120 MOSFET at Vg = 1.0 V, Vd = 0.05 V:
+    4 ports x 7 frequencies on the GPU, f_T; the FP64 oracle (SciPy SuperLU
+    per frequency) on the first frequencies from the same operating point."""
     from oracle import ac as oac
     from oracle_build import oracle_device
-    from tcadhip import ac
+    from tcadhip import ac, configs
+    from tcadhip.device import NewtonOptions, device_from_spec
+    spec = configs.config2_mosfet()
+    dev = device_from_spec(spec)
+    dev.solve_equilibrium(NewtonOptions())
+    for bias in spec.sweep:
+        dev.solve_bias(bias, NewtonOptions())
     ac.y_parameters(dev, [freqs[0]])                   # warm-up
-    torch.cuda.synchronize()
+    _sync()
     t0 = time.perf_counter()
     res = ac.y_parameters(dev, list(freqs))
-    torch.cuda.synchronize()
+    _sync()
     dt = time.perf_counter() - t0
     odev = oracle_device(spec)
     odev.psi, odev.n, odev.p = dev.psi, dev.n, dev.p
@@ -362,6 +364,161 @@ def bench_ac(dev, spec, freqs=tuple(np.logspace(6, 12, 7)), cpu_freqs=2):
             "max_err_vs_oracle_rel_to_max_Y": err}
 
 
+def _cpu_diode(idx):
+    """One batch diode's continuation sweep on the FP64 CPU oracle (NumPy +
+    SciPy SuperLU), in a worker process of bench_batch's CPU baseline;
+    returns (final currents in contact order, seconds)."""
+    import warnings
+    warnings.simplefilter("ignore")
+    from current_bar import oracle_sweep
+    from tcadhip import configs
+    spec = configs.batch_diodes()[idx]
+    t0 = time.perf_counter()
+    _, cur = oracle_sweep(spec)
+    return np.asarray(cur[-1]).tolist(), time.perf_counter() - t0
+
+
+def bench_batch(lanes=1, sample=8):
+    """256 seeded diodes (40x30..160x100 nodes, -2.0..0.8 V, 0.1 V
+    continuation steps), batched, both precisions; the plain FP64 path (one
+    device at a time) on an evenly spaced sample.  Unit of work: one device
+    converged at its target bias."""
+    from tcadhip import configs, sweep
+    from tcadhip.device import NewtonOptions, setup_from_spec
+    specs = configs.batch_diodes()
+    res = {}
+    for mode in ("bicgstab_mixed", "bicgstab"):
+        setups = [setup_from_spec(s) for s in specs]
+        _sync()
+        t0 = time.perf_counter()
+        r, bss = sweep.run_lanes(setups, [s.sweep for s in specs], NewtonOptions(linsolve=mode),
+                                 lanes=lanes)
+        _sync()
+        dt = time.perf_counter() - t0
+        res[mode] = dict(seconds=dt, results=r, rounds=sum(b.rounds for b in bss),
+                         rebuilds=sum(b.rebuilds for b in bss))
+        _say(f"batch {mode}: {len(specs) / dt:.3f} devices/s")
+    idx = list(range(0, len(specs), len(specs) // sample))[:sample]
+    _sync()
+    t0 = time.perf_counter()
+    alone = {}
+    for d in idx:
+        alone[d] = sweep.BatchedSweep([setup_from_spec(specs[d])], [specs[d].sweep],
+                                      NewtonOptions()).run()[0]
+    _sync()
+    plain = len(idx) / (time.perf_counter() - t0)
+
+    def final(r, names):
+        return np.array([r.currents[-1][nm] for nm in names])
+
+    import multiprocessing as mp
+    workers = max(1, min(16, os.cpu_count() or 1))
+    cidx = list(range(0, len(specs), max(1, len(specs) // workers)))[:workers]
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(workers) as pool:
+        cres = pool.map(_cpu_diode, cidx, chunksize=1)
+    dt_cpu = time.perf_counter() - t0
+    out = {"devices": len(specs), "nodes": int(sum(s.N for s in specs)),
+           "continuation_solves": int(sum(len(s.sweep) for s in specs)), "lanes": lanes,
+           "plain_fp64_devices_per_s": round(plain, 4),
+           "plain_sample": f"{len(idx)} evenly spaced devices solved one at a time",
+           "cpu_oracle": {"devices_per_s": round(len(cidx) / dt_cpu, 4), "workers": workers,
+                          "mean_seconds_per_device": round(float(np.mean([t for _, t in cres])),
+                                                           2),
+                          "sample": f"{len(cidx)} evenly spaced devices, FP64 oracle (NumPy + "
+                                    f"SciPy SuperLU), one per worker process"}}
+    for mode, r in res.items():
+        strict = floor_pts = 0
+        worst = 0.0
+        for d, (ccur, _) in zip(cidx, cres):
+            names = [c.name for c in specs[d].contacts]
+            got = final(r["results"][d], names)
+            ref = np.array(ccur)
+            e = float(np.abs(got - ref).max())
+            big = float(np.abs(ref).max())
+            worst = max(worst, e / big)
+            if e <= 1e-6 * big:
+                strict += 1
+            elif e <= 1e-6 * big + abs(ref.sum()) + abs(got.sum()):
+                floor_pts += 1
+        e64 = 0.0
+        for d in idx:
+            names = [c.name for c in specs[d].contacts]
+            got, ref = final(r["results"][d], names), final(alone[d], names)
+            e64 = max(e64, float(np.abs(got - ref).max() / np.abs(ref).max()))
+        out[mode] = {"devices_per_s": round(len(specs) / r["seconds"], 4),
+                     "seconds": round(r["seconds"], 3),
+                     "failed": sum(1 for x in r["results"] if x.failed),
+                     "rounds": r["rounds"], "compactions": r["rebuilds"],
+                     "vs_plain_fp64": round(len(specs) / r["seconds"] / plain, 2),
+                     "max_rel_current_err_vs_fp64_alone": e64,
+                     "vs_cpu_oracle_devices": len(cidx),
+                     "meets_1e-6_no_floor": strict,
+                     "under_conservation_floor_only": floor_pts,
+                     "max_rel_current_err_vs_oracle": worst}
+    return out
+
+
+def bench_serve(count=16, lanes=2, check=3, budget_gib=32):
+    """Serving: seeded Poisson arrivals of mixed requests (2D diode and
+    MOSFET sweeps, 3D tri-gate sweeps, 2k-200k nodes, 5-40 points each),
+    continuous batching in an HBM-budgeted pool with `lanes` lanes.
+    First the saturation throughput (every request present at t = 0), then
+    the same stream at 30% and 60% of that rate, where the server keeps up;
+    `check` requests re-solved alone must give the same bits."""
+    from tcadhip import serve
+    from tcadhip.device import NewtonOptions
+    cap = serve.nodes_for_budget(budget_gib << 30)
+    base = serve.request_stream(count, rate=1.0, seed=20261203)
+    pts_total = sum(len(r.spec.sweep) for r in base)
+    out = {"requests": count, "lanes": lanes, "pool_nodes": cap,
+           "kinds": {k: sum(r.kind == k for r in base) for k in ("diode", "mosfet", "trigate")},
+           "nodes_range": [min(r.spec.N for r in base), max(r.spec.N for r in base)],
+           "bias_points": pts_total}
+
+    def run(rate):
+        reqs = serve.request_stream(count, rate=rate, seed=20261203) if rate else \
+            [serve.Request(r.rid, r.spec, 0.0, r.kind) for r in base]
+        S = serve.Server(cap, opts=NewtonOptions(), lanes=lanes)
+        _sync()
+        t0 = time.perf_counter()
+        served = S.run(reqs, realtime=True)
+        _sync()
+        dt = time.perf_counter() - t0
+        lat = np.array([o.latency for o in served])
+        per_pt = np.array([o.latency / max(1, len(o.result.currents)) for o in served])
+        return served, reqs, {
+            "arrival_rate_per_s": rate, "wall_s": round(dt, 3),
+            "points_per_s": round(pts_total / dt, 3),
+            "request_latency_p50_s": round(float(np.percentile(lat, 50)), 3),
+            "request_latency_p99_s": round(float(np.percentile(lat, 99)), 3),
+            "latency_per_point_p50_ms": round(1e3 * float(np.percentile(per_pt, 50)), 2),
+            "latency_per_point_p99_ms": round(1e3 * float(np.percentile(per_pt, 99)), 2),
+            "failed": sum(1 for o in served if o.result.failed),
+            "rounds": S.rounds, "batch_rebuilds": S.rebuilds, "max_resident": S.max_resident}
+    served, reqs, sat = run(None)
+    out["saturation"] = sat
+    req_rate = count / sat["wall_s"]
+    out["saturation_requests_per_s"] = round(req_rate, 4)
+    for frac in (0.3, 0.6):
+        _, _, r = run(round(frac * req_rate, 4))
+        r["fraction_of_saturation"] = frac
+        out[f"load_{int(frac * 100)}pct"] = r
+    same = True
+    t1 = time.perf_counter()
+    alone_pts = 0
+    for o, r in list(zip(served, reqs))[:check]:
+        a = serve.serve_alone(r.spec, NewtonOptions())
+        alone_pts += len(a.currents)
+        same = same and a.newton == o.result.newton and all(
+            x[k] == y[k] for x, y in zip(a.currents, o.result.currents) for k in x)
+    _sync()
+    out["alone_equals_mix_on_first"] = check
+    out["alone_equals_mix"] = bool(same)
+    out["plain_fp64_points_per_s_on_those"] = round(alone_pts / (time.perf_counter() - t1), 4)
+    return out
+
+
 def copy_ceiling(mib=1024, reps=10):
     """On-box bandwidth ceiling: a plain device-to-device copy of `mib` MiB,
     bytes read + written per second (GB/s), timed with HIP events."""
@@ -383,51 +540,46 @@ def copy_ceiling(mib=1024, reps=10):
     return round(gbps, 1)
 
 
-def cpu_baseline(spec, npoints, budget_s=30.0):
-    """The FP64 oracle (NumPy + SciPy SuperLU, single thread) on the
-    first `npoints` bias points of the same sweep, equilibrium included."""
-    from oracle_build import oracle_device
-    dev = oracle_device(spec)
-    t0 = time.perf_counter()
-    dev.solve_equilibrium()
-    cur = []
-    done = 0
-    for bias in spec.sweep[:npoints]:
-        dev.solve_bias(bias)
-        cur.append(dev.terminal_current("drain"))
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return done, dt, np.array(cur)
+def measured_traffic(profile, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(REPO, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) >= 7 and cells[0] == f"`{kernel}`":
+            try:
+                return float(cells[6]) * 1e6
+            except ValueError:
+                return None
+    return None
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-points", type=int, default=5)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", default="bicgstab", choices=("bicgstab", "bicgstab_mixed"),
+                    help="linear-solve path of the headline")
     ap.add_argument("--points", type=int, default=None,
-                    help="first K bias points only (short profiling runs)")
-    ap.add_argument("--modes", default="bicgstab,bicgstab_mixed")
-    ap.add_argument("--rtol", type=float, default=1e-10,
-                    help="relative residual of each Newton linear solve")
-    ap.add_argument("--forcing", action="store_true",
-                    help="Newton forcing terms (inexact early solves)")
-    ap.add_argument("--no-extras", action="store_true",
-                    help="skip the AC, 3D tri-gate, batched-diode and serving workloads")
-    ap.add_argument("--extras", default="ac,trigate,batch,serve",
-                    help="which of the extra workloads to run (comma-separated)")
+                    help="first K Vg points of each sweep only (short profiling runs)")
+    ap.add_argument("--probe", type=int, default=3,
+                    help="BiCGStab kernel timed in-solver for the roofline (-1: none)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cap", type=float, default=75.0,
+                    help="wall-clock cap [s] of the full-size CPU baseline")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--extras", default="diode,mosfet,ac,batch,serve",
+                    help="extra workloads (comma-separated; '' for none)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (the reference "
                          "kernel for scaling HBM traffic counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every MI355X box here has one, and this "
-              f"workload (one device sweep) is not sharded; --gpus {args.gpus} is not "
-              f"supported.", file=sys.stderr)
+              f"workload is not sharded; --gpus {args.gpus} is not supported.", file=sys.stderr)
         sys.exit(2)
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     import torch
@@ -444,63 +596,17 @@ def main():
         return
     import warnings
     warnings.simplefilter("ignore")
-    from tcadhip import configs
 
-    spec = configs.config2_mosfet()
-    if args.points:
-        spec.sweep = spec.sweep[:args.points]
-    npts = len(spec.sweep)
-    res = {}
-    modes = args.modes.split(",")
-    for mode in modes:
-        print(f"[bench] {mode}: warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
-              flush=True)
-        res[mode] = measure(spec, mode, args.steps, args.warmup, args.rtol,
-                            forcing=args.forcing)
-        r = res[mode]
-        print(f"[bench] {mode}: {npts * args.steps / r['seconds']:.3f} points/s "
-              f"newton={r['newton']} krylov={r['krylov']}", file=sys.stderr, flush=True)
-
-    ref64 = res["bicgstab"]["drain"] if "bicgstab" in res else res[modes[0]]["drain"]
-    cpu = None
-    if not args.no_cpu:
-        print(f"[bench] cpu oracle on {args.cpu_points} points", file=sys.stderr, flush=True)
-        done, dt, ccur = cpu_baseline(spec, args.cpu_points)
-        cpu = dict(points=done, seconds=dt, drain=ccur)
-
-    def err_vs(a, b):
-        return float(np.max(np.abs(a - b) / np.abs(b)))
-
-    chosen, best = None, -1.0
-    for mode, r in res.items():
-        ok = err_vs(r["drain"], ref64) <= 1e-6
-        if cpu is not None:
-            r["err_vs_oracle"] = err_vs(r["drain"][:cpu["points"]], cpu["drain"])
-            ok = ok and r["err_vs_oracle"] <= 1e-6
-        r["ok"] = ok
-        rate = npts * args.steps / r["seconds"]
-        if ok and rate > best:
-            chosen, best = mode, rate
-    if chosen is None:
-        chosen = "bicgstab"
-        best = npts * args.steps / res[chosen]["seconds"]
-    r = res[chosen]
-
-    # roofline: the stencil SpMV inside BiCGStab dominates the kernel time;
-    # algorithmic bytes per launch = N * (9 + 5 * n_neighbour_blocks) * w
-    # matrix entries + 3 reads of the source vector + 3 writes of the
-    # product + 3 reads of the dotted vector (FP64 vectors), per node
-    dev = r["dev"]
-    w = 4 if chosen == "bicgstab_mixed" else 8
-    nbytes = spmv_bytes(spec, w)
-    N = spec.N
-    which = 1 if chosen == "bicgstab_mixed" else 0
-    us = kernel_timing(dev._gpu().handle, which)
-    achieved = nbytes / (us * 1e-6) / 1e9
+    _say(f"tri-gate 64x48x48, {args.mode}: warmup {args.warmup}, steps {args.steps}")
+    r = bench_trigate(args.steps, args.warmup, args.mode, args.points, args.probe)
+    npts = r["points_per_step"]
+    value = npts * args.steps / r["seconds"]
+    _say(f"tri-gate: {value:.3f} points/s")
+    N = r["specs"][0].N
     lat_ms = r["latencies"] * 1e3
     line = {
         "metric": METRIC,
-        "value": round(best, 4),
+        "value": round(value, 4),
         "unit": "bias points/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -509,43 +615,58 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 matrix / f64 vectors + f64 residual" if chosen == "bicgstab_mixed" else "f64",
+        "dtype": "f64" if args.mode == "bicgstab" else "f32 matrix / f64 vectors + f64 residual",
         "data": "synthetic",
-        "config": {"workload": "mosfet_200x120_idvg_vd0.05_vg0-1_21pts", "nodes": N,
-                   "unknowns": 3 * N, "bias_points_per_step": npts, "path": chosen},
+        "config": {"workload": "trigate_64x48x48_idvg_vd0.05+0.7_vg0-0.8_34pts", "nodes": N,
+                   "unknowns": 3 * N, "bias_points_per_step": npts, "path": args.mode,
+                   "precond": "2x1x8-node tile block-Jacobi (FP16 rows, FP32 scales)"},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-        "paths": {m: {"points_per_s": round(npts * args.steps / v["seconds"], 4),
-                      "newton_per_point": v["newton"], "krylov_per_point": v["krylov"],
-                      "max_rel_err_vs_fp64_gpu": err_vs(v["drain"], ref64),
-                      "max_rel_err_vs_oracle": v.get("err_vs_oracle"),
-                      "meets_1e-6": v["ok"]} for m, v in res.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_bicg_spmv<%s>" % ("float" if w == 4 else "double"),
-                     "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / 8000.0, 5),
-                     "traffic": measured_traffic("k_bicg_spmv<double>") if w == 8 else None,
-                     "bytes_per_launch": nbytes, "avg_launch_us": round(us, 3),
-                     "copy_ceiling_GBps": copy_ceiling()},
-        "cpu_baseline": None if cpu is None else {
-            "value": round(cpu["points"] / cpu["seconds"], 5), "unit": "bias points/s",
-            "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle (NumPy + SciPy SuperLU) equilibrium + first "
-                      f"{cpu['points']} of {npts} Vg points of the same 200x120 sweep, "
-                      f"{cpu['seconds']:.1f} s"},
+        "newton_per_point": [x[1] for x in r["record"]],
+        "krylov_per_point": [x[2] for x in r["record"]],
+        "equilibrium_newton": r["eq_iterations"],
+        "drain_current_A": {"vd0.05": [x[3] for x in r["record"] if "vd0.05" in x[0]],
+                            "vd0.7": [x[3] for x in r["record"] if "vd0.7" in x[0]]},
     }
-    extras = [] if args.no_extras else args.extras.split(",")
-    if "ac" in extras:
-        print("[bench] small-signal AC", file=sys.stderr, flush=True)
-        line["ac_mosfet_200x120"] = bench_ac(dev, spec)
-    if "trigate" in extras:
-        print("[bench] 3D tri-gate 64x48x48", file=sys.stderr, flush=True)
-        line["trigate_3d_64x48x48"] = bench_trigate()
-    if "batch" in extras:
-        print("[bench] 256 batched diodes", file=sys.stderr, flush=True)
-        line["batched_diodes_256"] = bench_batch()
-    if "serve" in extras:
-        print("[bench] serving", file=sys.stderr, flush=True)
-        line["serving"] = bench_serve()
+    if "probe" in r and r["probe"]["avg_us"]:
+        pr = r["probe"]
+        kname = PROBE_KERNELS[pr["kernel"]]
+        if pr["kernel"] == 3:
+            nbytes = K_BICG_U_BYTES_PER_NODE * N
+            achieved = nbytes / (pr["avg_us"] * 1e-6) / 1e9
+            line["roofline"] = {
+                "bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": measured_traffic("trigate_64x48x48_hbm_traffic.md", kname),
+                "bytes_per_launch": nbytes, "bytes_per_node": K_BICG_U_BYTES_PER_NODE,
+                "avg_launch_us_in_solver": round(pr["avg_us"], 3),
+                "launches_timed": pr["launches"],
+                "timing": "HIP events around every launch of the kernel in one untimed step",
+                "copy_ceiling_GBps": copy_ceiling()}
+        else:
+            line["probe"] = {"kernel": kname, "avg_us": round(pr["avg_us"], 3),
+                             "launches": pr["launches"]}
+    if not args.no_parity:
+        _say("reduced tri-gate vs the reference fixtures")
+        line["parity_reduced_24x18x14_vs_reference"] = reduced_trigate_parity(args.mode)
+    line["cpu_baseline"] = None
+    if not args.no_cpu:
+        gpu_newton = r["record"][0][1] if r["record"] else 5
+        _say(f"CPU baseline at full size (cap {args.cpu_cap:.0f} s)")
+        line["cpu_baseline"] = cpu_trigate_baseline(r["devs"][0], r["specs"][0], args.cpu_cap,
+                                                    gpu_newton)
+    del r
+    torch.cuda.empty_cache()
+    extras = [e for e in args.extras.split(",") if e]
+    for name, fn in (("diode", bench_diode), ("mosfet", bench_mosfet), ("ac", bench_ac),
+                     ("batch", bench_batch), ("serve", bench_serve)):
+        if name in extras:
+            _say(f"extra: {name}")
+            key = {"diode": "diode_60x40", "mosfet": "mosfet_200x120",
+                   "ac": "ac_mosfet_200x120", "batch": "batched_diodes_256",
+                   "serve": "serving"}[name]
+            line[key] = fn()
+            torch.cuda.empty_cache()
     print(json.dumps(line), flush=True)
 
 

@@ -310,6 +310,47 @@ int tcad_surface_mobility(tcad_batch *b, const double *d_psi);
  * reset, included in the time).  Has no reference counterpart. */
 int tcad_time_kernel(tcad_batch *b, int32_t which, int32_t reps, double *h_avg_us);
 
+/* Iterative solve of a caller-supplied sparse system A x = b in CSR form
+ * (n rows, device arrays d_indptr[n+1], d_indices[nnz], d_data[nnz] with
+ * int32 indices; d_b[n]; d_x[n] holds the initial guess on entry -- zero
+ * it for none -- and the solution on return).
+ *   method     TCAD_CSR_BICGSTAB or TCAD_CSR_GMRES (restart length
+ *              `restart`, <= 0 for min(100, n); at most 100);
+ *   precond    TCAD_CSR_PREC_BLOCK_JACOBI with block_size 1 or 3 (the exact
+ *              inverse of each node's diagonal block), or TCAD_CSR_PREC_ILU0
+ *              (level-scheduled ILU(0); needs sorted indices and a stored
+ *              diagonal in every row);
+ *   maxiter    BiCGStab iterations, or GMRES restart cycles;
+ *   rtol       relative to ||b||; the result is accepted when the true
+ *              residual ||b - A x|| / ||b|| <= max(rtol, 1e-6).
+ * Returns TCAD_OK, or TCAD_E_NOT_CONVERGED (x holds the last iterate),
+ * TCAD_E_NONFINITE (singular block / pivot), TCAD_E_INVALID.  h_iters and
+ * h_resid (may be NULL) receive the Krylov iterations and the final true
+ * relative residual.  `stream` is a hipStream_t (NULL = default).  The
+ * workspace is allocated and freed inside the call.  Replaces:
+ * linsolve.solve_linear for methods "gmres" / "bicgstab" with
+ * block_size (node block-Jacobi, linsolve.py:363-427) or without it (an
+ * incomplete factorisation, linsolve.py:429-585), linsolve.py:725-985. */
+#define TCAD_CSR_BICGSTAB 0
+#define TCAD_CSR_GMRES 1
+#define TCAD_CSR_PREC_BLOCK_JACOBI 0
+#define TCAD_CSR_PREC_ILU0 1
+int tcad_csr_solve(int32_t n, const int32_t *d_indptr, const int32_t *d_indices,
+                   const double *d_data, const double *d_b, double *d_x, int32_t method,
+                   int32_t precond, int32_t block_size, double rtol, int32_t maxiter,
+                   int32_t restart, void *stream, int32_t *h_iters, double *h_resid);
+
+/* Bench instrumentation, in-solver kernel timing: from this call on,
+ * every launch of one BiCGStab kernel inside the solves of this batch is
+ * bracketed by a pair of HIP events on the batch stream (which: 0 the
+ * stencil SpMV, 1 the alpha / s kernel, 2 the second SpMV, 3 the fused
+ * update kernel, whichever preconditioner variant runs; -1 stops).
+ * tcad_probe_read waits for the stream and returns the launches bracketed
+ * since the last read and their average duration [microseconds].  Has no
+ * reference counterpart. */
+int tcad_probe_kernel(tcad_batch *b, int32_t which);
+int tcad_probe_read(tcad_batch *b, int64_t *h_launches, double *h_avg_us);
+
 /* Wall-clock-free counters of the last call: kernel launches issued. */
 int64_t tcad_launch_count(const tcad_batch *b);
 

@@ -41,6 +41,7 @@ with x- and z-normal gates.  tests/test_oracle_dd3d.py adds invariants
 (z-invariant 3D equals 2D, axis permutation, finite-difference Jacobian,
 current conservation).
 """
+import time
 import warnings
 
 import numpy as np
@@ -429,10 +430,14 @@ class _DeviceND:
             else:
                 bc.Vg = V
 
-    def solve_bias(self, voltages=None, opts=None):
+    def solve_bias(self, voltages=None, opts=None, deadline=None):
         """device2d.py:1923-2191 / device3d.py:1981-2271 (plain path: one
-        stage, no line search, density floor 1e-10)."""
+        stage, no line search, density floor 1e-10).  `deadline` (a
+        time.perf_counter() value) stops the Newton loop after the first
+        iteration that ends past it, unconverged, with last_timed_out set
+        (a cap for timing runs; the reference has none)."""
         opts = opts or ph.NewtonOptions()
+        self.last_timed_out = False
         if self.psi is None:
             self.solve_equilibrium(opts)
         self._set_voltages(voltages)
@@ -466,6 +471,9 @@ class _DeviceND:
             n, p = n_new, p_new
             if err < opts.tol_update:
                 converged = True
+                break
+            if deadline is not None and time.perf_counter() > deadline:
+                self.last_timed_out = True
                 break
         else:
             warnings.warn(f"Newton did not converge; last update {err:.2e}")
@@ -581,4 +589,6 @@ class Device3D(_DeviceND):
             self.last_krylov_iterations += info["iterations"]
             return x
         except LinearSolveError:
+            if not opts.direct_fallback:
+                raise
             return solve_linear(Jd, rhs, method="direct")[0]

@@ -102,6 +102,9 @@ class NewtonOptions:
     linsolve: str = "direct"
     linsolve_rtol: float = 1e-10
     block_size: int = 3
+    # 3D iterative solves fall back to direct on LinearSolveError
+    # (device3d.py:2154-2168); off where a direct solve is out of reach
+    direct_fallback: bool = True
 
 
 def mobility_caughey_thomas(N, mat, T, carrier):

@@ -24,7 +24,7 @@ PY = os.environ.get("TCADHIP_PY")          # patch height (default 4)
 T3 = os.environ.get("TCADHIP_T3")          # 3D tile x and y extent (default "2x1")
 OBJ = os.path.join(HERE, "_build" + (f"_bs{BS}" if BS else "") + (f"_py{PY}" if PY else "")
                    + (f"_t3{T3}" if T3 else ""))
-SOURCES = ["assemble.hip", "krylov.hip", "batch.hip", "ac.hip"]
+SOURCES = ["assemble.hip", "krylov.hip", "batch.hip", "ac.hip", "csr.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -769,6 +769,7 @@ static void free_batch(Batch *b) {
                   b->d_flag, b->d_mask, b->cand, b->d_Icand, b->T4, b->T4s, b->P16, b->d_dts};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
+  for (hipEvent_t e : b->probe_ev) (void)hipEventDestroy(e);
   for (int i = 0; i < 9; ++i)
     if (b->v64[i]) (void)hipFree(b->v64[i]);
   if (b->ir_r) (void)hipFree(b->ir_r);
@@ -1353,6 +1354,40 @@ int tcad_time_kernel(tcad_batch *bp, int32_t which, int32_t reps, double *h_avg_
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *h_avg_us = 1000.0 * ms / reps;
+  return TCAD_OK;
+}
+
+int tcad_probe_kernel(tcad_batch *bp, int32_t which) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (which < -1 || which > 3) {
+    tcad::set_error("tcad_probe_kernel: which must be -1..3");
+    return TCAD_E_INVALID;
+  }
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  b.probe = which;
+  b.probe_used = 0;
+  return TCAD_OK;
+}
+
+int tcad_probe_read(tcad_batch *bp, int64_t *h_launches, double *h_avg_us) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!h_launches || !h_avg_us) {
+    tcad::set_error("tcad_probe_read: null output");
+    return TCAD_E_INVALID;
+  }
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  double total = 0.0;
+  const size_t pairs = b.probe_used / 2;
+  for (size_t i = 0; i < pairs; ++i) {
+    float ms = 0.f;
+    TCAD_HIP_CHECK(hipEventElapsedTime(&ms, b.probe_ev[2 * i], b.probe_ev[2 * i + 1]));
+    total += ms;
+  }
+  *h_launches = (int64_t)pairs;
+  *h_avg_us = pairs ? 1000.0 * total / (double)pairs : 0.0;
+  b.probe_used = 0;
   return TCAD_OK;
 }
 

@@ -50,6 +50,8 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "node_helpers.h"
 
 namespace tcad {
@@ -768,21 +770,32 @@ static int precond_variant(const Batch &b) {
   return b.nblk <= kTileRegBlocks ? PV_TILE_REG : PV_TILE_MEM;
 }
 
-#define TCAD_PV_LAUNCH(name)                                                               \
+#define TCAD_PV_LAUNCH(name, probe_id)                                                     \
   template <typename T>                                                                    \
   static void launch_##name(Batch &b, const KArgs<T> &K) {                                 \
     const dim3 g(b.nblk), t(BS);                                                           \
+    hipEvent_t e0 = nullptr, e1 = nullptr;                                                 \
+    const bool pr = probe_id >= 0 && probe_events(b, probe_id, e0, e1);                    \
     switch (precond_variant(b)) {                                                          \
-      case PV_NODE: hipLaunchKernelGGL(name##_node<T>, g, t, 0, b.stream, K); break;       \
-      case PV_TILE_REG: hipLaunchKernelGGL(name##_treg<T>, g, t, 0, b.stream, K); break;   \
-      default: hipLaunchKernelGGL(name##_tmem<T>, g, t, 0, b.stream, K); break;            \
+      case PV_NODE:                                                                        \
+        if (pr) hipExtLaunchKernelGGL(name##_node<T>, g, t, 0, b.stream, e0, e1, 0, K);   \
+        else hipLaunchKernelGGL(name##_node<T>, g, t, 0, b.stream, K);                     \
+        break;                                                                             \
+      case PV_TILE_REG:                                                                    \
+        if (pr) hipExtLaunchKernelGGL(name##_treg<T>, g, t, 0, b.stream, e0, e1, 0, K);   \
+        else hipLaunchKernelGGL(name##_treg<T>, g, t, 0, b.stream, K);                     \
+        break;                                                                             \
+      default:                                                                             \
+        if (pr) hipExtLaunchKernelGGL(name##_tmem<T>, g, t, 0, b.stream, e0, e1, 0, K);   \
+        else hipLaunchKernelGGL(name##_tmem<T>, g, t, 0, b.stream, K);                     \
+        break;                                                                             \
     }                                                                                      \
     b.launches++;                                                                          \
   }
 
-TCAD_PV_LAUNCH(k_bicg_init)
-TCAD_PV_LAUNCH(k_bicg_s)
-TCAD_PV_LAUNCH(k_bicg_u)
+TCAD_PV_LAUNCH(k_bicg_init, -1)
+TCAD_PV_LAUNCH(k_bicg_s, 1)
+TCAD_PV_LAUNCH(k_bicg_u, 3)
 
 // Reset the per-device Krylov state for a new solve; devices not active
 // are marked done.  per_dev: each device's own target S.eta2 (not below
@@ -839,13 +852,36 @@ int launch_tile_setup(Batch &b, bool only_active) {
   return TCAD_OK;
 }
 
+// Events of the next probed launch: true (and e0, e1 set) when `which` is
+// the probed kernel; the launch then goes through hipExtLaunchKernelGGL,
+// whose events take the kernel's own start and end times.
+bool probe_events(Batch &b, int which, hipEvent_t &e0, hipEvent_t &e1) {
+  if (b.probe != which) return false;
+  while (b.probe_ev.size() < b.probe_used + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) { b.probe = -1; return false; }
+    b.probe_ev.push_back(e);
+  }
+  e0 = b.probe_ev[b.probe_used];
+  e1 = b.probe_ev[b.probe_used + 1];
+  b.probe_used += 2;
+  return true;
+}
+
 template <typename T>
 static void launch_iteration(Batch &b, KArgs<T> K, int parity) {
   const dim3 g(b.nblk), t(BS);
   K.parity = parity;
-  hipLaunchKernelGGL(k_bicg_spmv<T>, g, t, 0, b.stream, K);
+  hipEvent_t e0, e1;
+  if (probe_events(b, 0, e0, e1))
+    hipExtLaunchKernelGGL(k_bicg_spmv<T>, g, t, 0, b.stream, e0, e1, 0, K);
+  else
+    hipLaunchKernelGGL(k_bicg_spmv<T>, g, t, 0, b.stream, K);
   launch_k_bicg_s<T>(b, K);
-  hipLaunchKernelGGL(k_bicg_spmv_t<T>, g, t, 0, b.stream, K);
+  if (probe_events(b, 2, e0, e1))
+    hipExtLaunchKernelGGL(k_bicg_spmv_t<T>, g, t, 0, b.stream, e0, e1, 0, K);
+  else
+    hipLaunchKernelGGL(k_bicg_spmv_t<T>, g, t, 0, b.stream, K);
   launch_k_bicg_u<T>(b, K);
   b.launches += 2;
 }

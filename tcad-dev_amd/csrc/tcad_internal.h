@@ -211,7 +211,16 @@ struct Batch {
   const double *last_n = nullptr, *last_p = nullptr;   // state of the last assembly
   const double *last_psi = nullptr;
   std::vector<int> dev_ncontact, dev_ngate;
+  // in-solver timing of one BiCGStab kernel (tcad_probe_kernel): every
+  // launch of kernel `probe` (0 SpMV, 1 alpha/s, 2 second SpMV, 3 update;
+  // -1 off) is bracketed by a pair of events on the batch stream
+  int probe = -1;
+  std::vector<hipEvent_t> probe_ev;
+  size_t probe_used = 0;
 };
+
+// start/stop events of a probed launch (krylov.hip); false unless b.probe == which
+bool probe_events(Batch &b, int which, hipEvent_t &e0, hipEvent_t &e1);
 
 // error handling (tcad_capi.cpp)
 void set_error(const std::string &msg);

@@ -27,6 +27,11 @@ LIN_BICGSTAB_MIXED = 1
 LIN_FORCING = 0x100
 PRECOND_NODE = 0x200
 
+CSR_BICGSTAB = 0
+CSR_GMRES = 1
+CSR_PREC_BLOCK_JACOBI = 0
+CSR_PREC_ILU0 = 1
+
 NEWTON_OK = 0
 NEWTON_MAX_ITER = 1
 NEWTON_LINEAR_FAIL = 2
@@ -117,6 +122,11 @@ def lib():
         "tcad_edge_currents": ([vp, vp, vp, vp, vp], ctypes.c_int),
         "tcad_surface_mobility": ([vp, vp], ctypes.c_int),
         "tcad_time_kernel": ([vp, ctypes.c_int32, ctypes.c_int32, c_double_p], ctypes.c_int),
+        "tcad_probe_kernel": ([vp, ctypes.c_int32], ctypes.c_int),
+        "tcad_csr_solve": ([ctypes.c_int32, vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_int32,
+                            ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, vp,
+                            c_int32_p, c_double_p], ctypes.c_int),
+        "tcad_probe_read": ([vp, ctypes.POINTER(ctypes.c_int64), c_double_p], ctypes.c_int),
         "tcad_last_error": ([], ctypes.c_char_p),
         "tcad_version": ([], ctypes.c_char_p),
     }
@@ -166,4 +176,5 @@ def exported_symbols():
             "tcad_solve_equilibrium", "tcad_solve_bias", "tcad_terminal_currents",
             "tcad_edge_currents", "tcad_surface_mobility", "tcad_time_kernel",
             "tcad_transient_step", "tcad_continuity_residuals", "tcad_y_parameters",
+            "tcad_probe_kernel", "tcad_probe_read", "tcad_csr_solve",
             "tcad_launch_count", "tcad_last_error", "tcad_version"]

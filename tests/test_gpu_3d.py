@@ -1,6 +1,7 @@
-"""3D tri-gate on the GPU: a mid-size mesh against the live CPU oracle, and
-the full 64x48x48 mesh (442,368 unknowns, where the oracle's direct solve
-is out of reach) through invariants.
+"""3D tri-gate on the GPU beyond the reference fixtures of
+tests/test_gpu_parity.py: ragged patch edges against the live CPU oracle,
+and the full 64x48x48 mesh (442,368 unknowns, where the oracle's direct
+solve is out of reach) through invariants.
 
 Bars: Newton counts within +-1 of the oracle's; terminal currents within
 1e-6 * max_c |I_ref| per point (the tri-gate's currents are not
@@ -43,21 +44,6 @@ def _sweep(spec, opts):
         its.append(dev.last_iterations)
         cur.append([dev.terminal_current(c.name) for c in spec.contacts])
     return dev, np.array(its), np.array(cur)
-
-
-@pytest.mark.parametrize("mode", ["bicgstab", "bicgstab_mixed"])
-def test_trigate_mid_vs_oracle(mode):
-    configs, NewtonOptions, _ = _mods()
-    _say(f"[3d] trigate 24x18x14 Vd=0.05 Vg 0,0.8 {mode} vs oracle")
-    spec = configs.trigate_3d(nx=24, ny=18, nz=14, Vd=0.05, vg_list=[0.0, 0.8])
-    odev, oits, ocur, ost = oracle_sweep(spec, keep=(1,))
-    dev, its, cur = _sweep(spec, NewtonOptions(linsolve=mode))
-    assert np.all(np.abs(its - oits) <= 1), (its, oits)
-    for p in range(len(spec.sweep)):
-        err = np.abs(cur[p] - ocur[p]).max() / np.abs(ocur[p]).max()
-        _say(f"  point {p}: max rel current error {err:.2e}")
-        assert err <= 1e-6, (p, cur[p], ocur[p])
-    assert np.abs(dev.psi - ost[1][0]).max() <= 1e-9
 
 
 def test_trigate_full_size_invariants():

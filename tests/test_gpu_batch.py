@@ -74,14 +74,13 @@ def test_batched_sweep_vs_oracle():
     opts = NewtonOptions()
     res = sweep.BatchedSweep([setup_from_spec(s) for s in specs],
                              [s.sweep for s in specs], opts).run()
+    from current_bar import check, oracle_sweep
     for d, s in enumerate(specs):
         with warnings.catch_warnings():
             warnings.simplefilter("ignore")
-            _, rits, rcur, _ = oracle_sweep(s)
+            rits, rcur = oracle_sweep(s)
         got = np.array([[c[k.name] for k in s.contacts] for c in res[d].currents])
-        for row, ref in zip(got, rcur):
-            tol = 1e-6 * np.abs(ref).max() + 10.0 * abs(ref.sum())
-            assert np.abs(row - ref).max() <= tol, (d, row, ref)
+        check(got, rcur, s.name, say=_say)
         assert np.all(np.abs(np.array(res[d].newton) - rits) <= 1)
 
 

@@ -1,28 +1,30 @@
 """GPU parity: the HIP path (through the C ABI) against the reference's
-golden fixtures and the CPU oracle.
+golden fixtures (tests/golden/make_golden.py for Device2D,
+make_golden_3d.py for Device3D) and the CPU oracle, which reproduces every
+fixture bit for bit.
 
 Tolerances (stated per quantity):
   * residual F at a given state: max |F_gpu - F_ref| <= 1e-12 * max |F_ref|
     (same arithmetic; only expm1/log/exp differ by a few ulp from glibc);
-  * eliminated Jacobian: max |Jd_gpu - Jd_ref| <= 1e-12 * max |Jd_ref| per row;
+    F_n, F_p and the edge currents likewise;
+  * raw Jacobian J: per row max |J_gpu - J_ref| <= 1e-12 * max |J_ref row|;
   * converged states: psi atol 1e-9 (scaled, i.e. 2.6e-11 V), n and p
     relative 1e-7 (the reference's own bars, tests/test_native_linsolve.py:118-137);
-  * terminal currents: |I_gpu - I_ref| <= 1e-6 * max_c |I_ref| + floor, per
-    bias point, with floor = 10 x the reference's own current-conservation
-    error |sum_c I_ref| at that point (currents below the reference's
-    rounding noise are not resolvable by either side);
+  * terminal currents: 1e-6 of the largest current per bias point plus an
+    absolute floor, the two states' own current-conservation errors
+    (tests/current_bar.py states it); every point prints which bar held;
   * Newton iteration counts within +-1 of the reference's.
 """
-import os
-import warnings
+import functools
 
 import numpy as np
 import pytest
 import scipy.sparse as sp
 
+from current_bar import check, contacts_overlap, oracle_sweep
 from golden.fixtures import load
-from oracle.linsolve import eliminate_csr
-from oracle_build import oracle_device, oracle_sweep
+from oracle import physics as oph
+from oracle_build import oracle_device
 
 pytestmark = pytest.mark.gpu
 
@@ -39,13 +41,20 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def current_tol(ref_row):
-    ref_row = np.asarray(ref_row)
-    return 1e-6 * np.abs(ref_row).max() + 10.0 * abs(ref_row.sum())
+SNAPSHOTS = ["diode_14x9_fj.npz", "mosfet_16x12_fj.npz", "trigate_12x10x8_fj.npz",
+             "slab_xgate_10x12x9_fj.npz", "slab_zgate_12x9x10_fj.npz"]
 
 
-@pytest.mark.parametrize("name", ["diode_14x9_fj.npz", "mosfet_16x12_fj.npz"])
+def _rows_close(A, B, rel=1e-12):
+    diff = abs(A - B).max(axis=1).toarray().ravel()
+    scale = abs(B).max(axis=1).toarray().ravel()
+    return np.all(diff <= rel * scale), float(np.max(diff / np.maximum(scale, 1e-300)))
+
+
+@pytest.mark.parametrize("name", SNAPSHOTS)
 def test_residual_jacobian_vs_reference(name):
+    """The reference-shaped operator: (F, raw J, edge currents, F_n, F_p)
+    of device2d.py:1920 / device3d.py:1978, and current_rows_for."""
     NewtonOptions, device_from_spec = _gpu()
     _say(f"[F/J] {name}")
     spec, d = load(name)
@@ -54,26 +63,52 @@ def test_residual_jacobian_vs_reference(name):
     dev._set_voltages(volts)
     if spec.models["surface_mobility"]:
         dev._update_surface_mobility(d["in_psi"])
-    F, Jd, Jn, Jp, Fnp = dev._residual_jacobian(d["in_psi"], d["in_n"], d["in_p"], volts)
-    Fref = d["F"]
-    assert np.abs(F - Fref).max() <= 1e-12 * np.abs(Fref).max()
-    Jr = sp.csr_matrix((d["J_data"], d["J_indices"], d["J_indptr"]), shape=(F.size, F.size))
-    Jdr, rhs = eliminate_csr(Jr, -Fref, d["dirichlet_rows"])
-    diff = abs(Jd - Jdr).max(axis=1).toarray().ravel()
-    scale = abs(Jdr).max(axis=1).toarray().ravel()
-    assert np.all(diff <= 1e-12 * scale), (diff / scale).max()
-    for a, key in enumerate(("Jnx", "Jny")):
-        ref = d[key]
-        got = Jn[a][tuple(slice(0, s) for s in ref.shape)]
-        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max() + 1e-300
-    Fn_p_ref = d["Fn"] + d["Fp"]
-    assert np.abs(Fnp - Fn_p_ref).max() <= 1e-12 * (np.abs(d["Fn"]).max() + np.abs(d["Fp"]).max())
+    contacts = [c.name for c in spec.contacts]
+    first = spec.contacts[0]
+    k_req = np.ravel_multi_index(first.idx, spec.shape)
+    out = dev._residual_jacobian(d["in_psi"], d["in_n"], d["in_p"],
+                                 {k: v for k, v in volts.items() if k in contacts},
+                                 current_rows_for=k_req)
+    nd = spec.dim
+    assert len(out) == 4 + 2 * nd
+    F, J = out[0], out[1]
+    Jn, Jp = out[2:2 + nd], out[2 + nd:2 + 2 * nd]
+    Fn, Fp = out[-2], out[-1]
+    assert np.abs(F - d["F"]).max() <= 1e-12 * np.abs(d["F"]).max()
+    Jr = sp.csr_matrix((d["J_data"], d["J_indices"], d["J_indptr"]), shape=J.shape)
+    ok, worst = _rows_close(J, Jr)
+    _say(f"  raw J worst row error {worst:.2e}")
+    assert ok
+    for a, ax in enumerate("xyz"[:nd]):
+        for got, key in ((Jn[a], f"Jn{ax}"), (Jp[a], f"Jp{ax}")):
+            ref = d[key]
+            assert got.shape == ref.shape
+            assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max() + 1e-300
+    scale = np.abs(d["Fn"]).max() + np.abs(d["Fp"]).max()
+    assert np.abs(Fn - d["Fn"]).max() <= 1e-12 * scale
+    assert np.abs(Fp - d["Fp"]).max() <= 1e-12 * scale
+    assert np.array_equal(dev._dirichlet_rows, d["dirichlet_rows"])
+    # the raw continuity rows of the first contact, against the oracle's
+    # capture (the oracle's F/J equal the fixture's bit for bit)
+    odev = oracle_device(spec)
+    odev._set_voltages(volts)
+    if spec.models["surface_mobility"]:
+        odev._update_surface_mobility(d["in_psi"])
+    odev._residual_jacobian(d["in_psi"], d["in_n"], d["in_p"],
+                            {k: v for k, v in volts.items() if k in contacts},
+                            current_rows_for=k_req)
+    shape = (F.size, F.size)
+    G = sp.csr_matrix((dev._raw_current_rows[2], dev._raw_current_rows[:2]), shape=shape)
+    O = sp.csr_matrix((odev._raw_current_rows[2], odev._raw_current_rows[:2]), shape=shape)
+    ok, worst = _rows_close(G, O)
+    _say(f"  current rows of {first.name}: worst row error {worst:.2e}")
+    assert ok
 
 
 def _run_sweep(spec, opts):
     NewtonOptions, device_from_spec = _gpu()
     dev = device_from_spec(spec)
-    dev.solve_equilibrium(opts)
+    dev.solve_equilibrium(NewtonOptions(max_iter=400) if spec.dim == 3 else opts)
     its, cur, states = [], [], []
     for bias in spec.sweep:
         dev.solve_bias(bias, opts)
@@ -83,13 +118,17 @@ def _run_sweep(spec, opts):
     return dev, np.array(its), np.array(cur), states
 
 
-SWEEPS = ["diode_60x40_sweep.npz", "mosfet_48x32_sweep.npz", "diode_24x16_defaults_sweep.npz"]
+SWEEPS = ["diode_60x40_sweep.npz", "mosfet_48x32_sweep.npz", "diode_24x16_defaults_sweep.npz",
+          "trigate_24x18x14_vd005_sweep.npz", "trigate_24x18x14_vd07_sweep.npz",
+          "slab_xgate_10x12x9_sweep.npz", "slab_zgate_12x9x10_sweep.npz"]
 
 
-def _equilibrated_counts(spec):
-    from oracle import physics as oph
-    _, its, _, _ = oracle_sweep(spec, opts=oph.NewtonOptions(linsolve="direct_equilibrated"))
-    return its
+@functools.lru_cache(maxsize=None)
+def _equilibrated_counts(name):
+    """The oracle's Newton counts with the equilibrated direct solve."""
+    spec, d = load(name)
+    eq_its, _ = oracle_sweep(spec, oph.NewtonOptions(linsolve="direct_equilibrated"))
+    return eq_its
 
 
 @pytest.mark.parametrize("mode", ["bicgstab", "bicgstab_mixed"])
@@ -98,69 +137,73 @@ def test_sweep_vs_reference(name, mode):
     NewtonOptions, _ = _gpu()
     _say(f"[sweep] {name} {mode}")
     spec, d = load(name)
+    eq_its = _equilibrated_counts(name)
     opts = NewtonOptions(linsolve=mode)
     dev, its, cur, states = _run_sweep(spec, opts)
     assert abs(dev.last_eq_iterations - int(d["eq_iterations"])) <= 1
     ref_its = d["newton_iterations"]
+    _say(f"  Newton gpu {its.tolist()} reference {ref_its.tolist()}")
     # Where the reference's SuperLU on the unscaled Jacobian loses digits
     # (deep reverse bias), its Newton count drifts from the exact-step
     # count; there the GPU is held to the equilibrated direct oracle
     # (same algorithm, well-conditioned solve) and to never needing more
     # iterations than the reference.
-    eq_its = _equilibrated_counts(spec)
     for p in range(len(its)):
         if abs(eq_its[p] - ref_its[p]) <= 1:
             assert abs(its[p] - ref_its[p]) <= 1, (p, its, ref_its)
         else:
             assert abs(its[p] - eq_its[p]) <= 1 and its[p] <= ref_its[p], (p, its, eq_its)
-    ref = d["currents"]
-    for p in range(len(spec.sweep)):
-        tol = current_tol(ref[p])
-        assert np.abs(cur[p] - ref[p]).max() <= tol, (p, cur[p], ref[p], tol)
+    n, strict, fl = check(cur, d["currents"], name, contacts_overlap(spec), say=_say)
+    _say(f"  {strict}/{n} points within 1e-6 with no floor, {fl} needed the floor")
     for k in d.files:
         if k.startswith("psi_") and k != "psi_eq":
             p = int(k.split("_")[1])
-            psi, n, pp = states[p]
+            psi, nn, pp = states[p]
             assert np.abs(psi - d[f"psi_{p}"]).max() <= 1e-9
-            assert (np.abs(n - d[f"n_{p}"]) / d[f"n_{p}"]).max() <= 1e-7
-            assert (np.abs(pp - d[f"p_{p}"]) / d[f"p_{p}"]).max() <= 1e-7
+            # densities above 1e-10 (scaled), the reference's own mask
+            # (tests/test_native_linsolve.py:131-133)
+            for got, key in ((nn, f"n_{p}"), (pp, f"p_{p}")):
+                ref = d[key]
+                m = ref > 1e-10
+                if m.any():
+                    assert (np.abs(got[m] - ref[m]) / ref[m]).max() <= 1e-7
 
 
-def test_equilibrium_vs_reference():
+@pytest.mark.parametrize("name", ["diode_60x40_sweep.npz", "mosfet_48x32_sweep.npz",
+                                  "trigate_24x18x14_vd005_sweep.npz",
+                                  "slab_xgate_10x12x9_sweep.npz"])
+def test_equilibrium_vs_reference(name):
     NewtonOptions, device_from_spec = _gpu()
-    _say("[eq] diode_60x40 and mosfet_48x32")
-    for name in ("diode_60x40_sweep.npz", "mosfet_48x32_sweep.npz"):
-        spec, d = load(name)
-        dev = device_from_spec(spec)
-        dev.solve_equilibrium(NewtonOptions())
-        assert abs(dev.last_eq_iterations - int(d["eq_iterations"])) <= 1
-        assert np.abs(dev.psi - d["psi_eq"]).max() <= 1e-9
+    _say(f"[eq] {name}")
+    spec, d = load(name)
+    dev = device_from_spec(spec)
+    dev.solve_equilibrium(NewtonOptions(max_iter=400))
+    assert abs(dev.last_eq_iterations - int(d["eq_iterations"])) <= 1
+    assert np.abs(dev.psi - d["psi_eq"]).max() <= 1e-9
 
 
-def test_trigate_small_vs_oracle():
-    """3D tri-gate (reduced mesh) against the live CPU oracle."""
+def test_mosfet_200x120_vs_oracle():
+    """The MOSFET workload at its full 200 x 120 size (24,000 nodes, 0.2 nm
+    first cell), Vd = 0.05 V, Vg 0, 0.5 and 1.0 V warm-started on both
+    sides, against the live oracle (its SuperLU solve is the reference's)."""
     NewtonOptions, _ = _gpu()
     from tcadhip import configs
-    _say("[3d] trigate 14x10x8 Vd=0.05, Vg 0,0.4,0.8")
-    spec = configs.trigate_3d(nx=14, ny=10, nz=8, vg_list=[0.0, 0.4, 0.8])
-    odev, oits, ocur, ost = oracle_sweep(spec, keep=(2,))
-    dev, its, cur, states = _run_sweep(spec, NewtonOptions())
-    assert np.all(np.abs(its - oits) <= 1), (its, oits)
-    for p in range(len(spec.sweep)):
-        # overlapping contact node lists: no conservation floor here
-        tol = 1e-6 * np.abs(ocur[p]).max()
-        assert np.abs(cur[p] - ocur[p]).max() <= tol, (p, cur[p], ocur[p])
-    psi, n, pp = states[2]
-    assert np.abs(psi - ost[2][0]).max() <= 1e-9
-    assert (np.abs(n - ost[2][1]) / ost[2][1]).max() <= 1e-7
+    spec = configs.mosfet_2d(vg_list=[0.0, 0.5, 1.0], name="mosfet_200x120")
+    _say("[C2] mosfet 200x120, Vg 0 / 0.5 / 1.0: oracle")
+    oits, ocur = oracle_sweep(spec)
+    _say("  gpu")
+    dev, its, cur, _ = _run_sweep(spec, NewtonOptions())
+    _say(f"  Newton gpu {its.tolist()} oracle {oits.tolist()}")
+    assert np.all(np.abs(its - oits) <= 1)
+    n, strict, fl = check(cur, ocur, "mosfet_200x120", say=_say)
+    assert strict == n
 
 
 def test_batch_equals_alone():
     """Devices solved together in one batch give the same results as alone."""
     NewtonOptions, device_from_spec = _gpu()
     from tcadhip import configs
-    from tcadhip.device import DeviceSetup, GPUBatch
-    from tcadhip import physics as ph
+    from tcadhip.device import GPUBatch
     _say("[batch] 3 diodes together vs alone")
     specs = configs.batch_diodes(count=3, nx_range=(20, 40), ny_range=(12, 20))
     opts = NewtonOptions()

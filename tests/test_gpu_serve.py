@@ -59,3 +59,58 @@ def test_serving_lanes_alone_equals_mix():
         assert sv.result.newton == alone.newton
         for a, b in zip(sv.result.currents, alone.currents):
             assert all(a[k] == b[k] for k in a), (r.spec.name, a, b)
+
+
+def test_serving_at_workload_sizes():
+    """Serving at its stated sizes: a seeded stream with a 2D diode
+    (50 x 40 = 2,000 nodes), a 2D MOSFET (100 x 60 = 6,000 nodes, 0.2 nm
+    first cell) and a 3D tri-gate (32 x 24 x 24 = 18,432 nodes), plus a
+    second diode, 5 points each, in a pool that cannot hold all four at
+    once.  Every request gives the same bits alone and in the mix; the 2D
+    requests match the oracle (Newton +-1, currents to the bar of
+    tests/current_bar.py)."""
+    serve, NewtonOptions = _mods()
+    from tcadhip import configs
+    from current_bar import check, oracle_sweep
+    rng = np.random.default_rng(20261204)
+    arrivals = np.cumsum(rng.exponential(0.05, size=4))
+    specs = [
+        ("diode", configs.diode_2d(50, 40, Na=3e17, Nd=2e16, xj_frac=0.45,
+                                   biases=[-0.4 * (s + 1) for s in range(5)],
+                                   models=dict(doping_mobility=True, srh=True, auger=True,
+                                               bgn=True, surface_mobility=False),
+                                   material={}, name="req0_diode_50x40")),
+        ("mosfet", configs.mosfet_2d(100, 60, Vd=0.5, vg_list=np.linspace(0.0, 1.0, 5),
+                                     name="req1_mosfet_100x60")),
+        ("trigate", configs.trigate_3d(32, 24, 24, Vd=0.7, vg_list=np.linspace(0.0, 0.8, 5),
+                                       name="req2_trigate_32x24x24")),
+        ("diode", configs.diode_2d(60, 40, Na=1e16, Nd=1e18, xj_frac=0.6,
+                                   biases=[0.12 * (s + 1) for s in range(5)],
+                                   models=dict(doping_mobility=True, srh=True, auger=True,
+                                               bgn=True, surface_mobility=False),
+                                   material={}, name="req3_diode_60x40")),
+    ]
+    reqs = [serve.Request(i, sp, float(t), kind) for i, ((kind, sp), t)
+            in enumerate(zip(specs, arrivals))]
+    cap = reqs[2].spec.N + reqs[1].spec.N + 100
+    print(f"[serve-sizes] {[r.spec.N for r in reqs]} nodes, pool {cap}", flush=True)
+    opts = NewtonOptions()
+    S = serve.Server(capacity_nodes=cap, opts=opts)
+    served = S.run(reqs, realtime=False)
+    print(f"  rounds {S.rounds} rebuilds {S.rebuilds} max resident {S.max_resident}",
+          flush=True)
+    assert S.max_resident >= 2 and S.rebuilds >= 2
+    for r, sv in zip(reqs, served):
+        mix = sv.result
+        assert not mix.failed, r.spec.name
+        alone = serve.serve_alone(r.spec, opts)
+        assert mix.newton == alone.newton, r.spec.name
+        for a, b in zip(mix.currents, alone.currents):
+            assert all(a[k] == b[k] for k in a), (r.spec.name, a, b)
+        if r.spec.dim == 2:
+            oits, ocur = oracle_sweep(r.spec)
+            names = [c.name for c in r.spec.contacts]
+            got = np.array([[c[nm] for nm in names] for c in mix.currents])
+            print(f"  {r.spec.name}: Newton {mix.newton} oracle {oits.tolist()}", flush=True)
+            assert np.all(np.abs(np.array(mix.newton) - oits) <= 1)
+            check(got, ocur, r.spec.name, say=lambda m: print(m, flush=True))
