@@ -340,6 +340,22 @@ int tcad_csr_solve(int32_t n, const int32_t *d_indptr, const int32_t *d_indices,
                    int32_t precond, int32_t block_size, double rtol, int32_t maxiter,
                    int32_t restart, void *stream, int32_t *h_iters, double *h_resid);
 
+/* tcad_csr_solve without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and kernels as tcad_csr_solve, with the workspace in
+ * a fixed 64 MiB device array of the library (one per GPU, held until the
+ * process ends).  Block-Jacobi only (TCAD_CSR_PREC_ILU0 is refused);
+ * systems whose workspace exceeds the array are refused with
+ * TCAD_E_INVALID (BiCGStab with 3-blocks: about 11 n words, so n up to
+ * about 760,000; GMRES(m): about (m + 7) n words).  Every call on one GPU
+ * uses the same array: calls that overlap on one GPU race. */
+int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_indices,
+                           const double *d_data, const double *d_b, double *d_x,
+                           int32_t method, int32_t precond, int32_t block_size, double rtol,
+                           int32_t maxiter, int32_t restart, void *stream, int32_t *h_iters,
+                           double *h_resid);
+
 /* Bench instrumentation, in-solver kernel timing: from this call on,
  * every launch of one BiCGStab kernel inside the solves of this batch is
  * bracketed by a pair of HIP events on the batch stream (which: 0 the

@@ -23,13 +23,17 @@
 // only every `chunk` BiCGStab iterations (and once per GMRES step for
 // the small Hessenberg problem).
 //
-// tcad_csr_solve allocates its workspace per call.  Roofline: HBM bound; per BiCGStab iteration two SpMVs
+// Two entry points share every kernel: tcad_csr_solve allocates its
+// workspace per call; tcad_csr_solve_noalloc runs in `scratch`, a
+// file-scope device array of fixed size, for callers that cannot
+// allocate.  Roofline: HBM bound; per BiCGStab iteration two SpMVs
 // (12 B per stored entry + 16 B per row) and about 20 vector passes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,7 +44,7 @@ namespace csr {
 
 constexpr int TB = 256;           // threads per block of the vector kernels
 constexpr int NPART = 512;        // blocks (partial sums) of every reduction
-constexpr int MAXM = 100;         // GMRES restart length cap
+constexpr int MAXM = 100;         // GMRES restart length cap of the scratch variant
 constexpr int SUB = 8;            // lanes per row in the SpMV
 
 struct Mat {
@@ -759,6 +763,39 @@ static int run(Solve &s, const double *b, double *x, double rtol, int maxiter, i
   return TCAD_OK;
 }
 
+// ---- the no-allocation variant's device scratch ----------------------------
+// 64 MiB of FP64 words on every GPU the library runs on, held until the
+// process ends.  Calls on one GPU share it: overlapping calls race.
+constexpr size_t SCRATCH_WORDS = size_t(8) << 20;
+static __device__ __align__(256) double scratch[SCRATCH_WORDS];
+constexpr int MAX_DEVICES = 64;
+static double *g_scratch[MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device, looked up once per
+// device (the host-side symbol is no device address)
+static int scratch_ptr(double **out) {
+  int dev = 0;
+  TCAD_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) {
+    set_error("tcad_csr_solve_noalloc: device index out of range");
+    return TCAD_E_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("tcad_csr_solve_noalloc: hipGetSymbolAddress failed: ") +
+                hipGetErrorString(e));
+      return TCAD_E_HIP;
+    }
+    g_scratch[dev] = (double *)p;
+  }
+  *out = g_scratch[dev];
+  return TCAD_OK;
+}
+
 }  // namespace csr
 }  // namespace tcad
 
@@ -829,6 +866,45 @@ int tcad_csr_solve(int32_t n, const int32_t *d_indptr, const int32_t *d_indices,
   (void)hipStreamSynchronize(s.st);
   (void)hipFree(base);
   return rc;
+}
+
+int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_indices,
+                           const double *d_data, const double *d_b, double *d_x, int32_t method,
+                           int32_t precond, int32_t block_size, double rtol, int32_t maxiter,
+                           int32_t restart, void *stream, int32_t *h_iters, double *h_resid) {
+  using namespace tcad;
+  using namespace tcad::csr;
+  int rc = check_args(n, d_indptr, d_indices, d_data, d_b, d_x, method, precond, block_size,
+                      rtol, maxiter, "tcad_csr_solve_noalloc");
+  if (rc) return rc;
+  if (precond != TCAD_CSR_PREC_BLOCK_JACOBI) {
+    set_error("tcad_csr_solve_noalloc: only block-Jacobi runs without allocation (ILU(0) "
+              "needs tcad_csr_solve)");
+    return TCAD_E_INVALID;
+  }
+  const int m = method == TCAD_CSR_GMRES ? std::min(restart > 0 ? restart : 100, n) : 0;
+  if (method == TCAD_CSR_GMRES && m > MAXM) {
+    set_error("tcad_csr_solve_noalloc: GMRES restart length is limited to 100");
+    return TCAD_E_INVALID;
+  }
+  const size_t words = words_needed(n, block_size, method, m) + 32 * 16;
+  if (words > SCRATCH_WORDS) {
+    set_error("tcad_csr_solve_noalloc: the system needs " + std::to_string(words * 8) +
+              " bytes of workspace; the fixed scratch holds " +
+              std::to_string(SCRATCH_WORDS * 8) + " (use tcad_csr_solve)");
+    return TCAD_E_INVALID;
+  }
+  double *base = nullptr;
+  if ((rc = scratch_ptr(&base))) return rc;
+  Solve s{};
+  s.A = Mat{n, d_indptr, d_indices, d_data};
+  s.n = n;
+  s.bs = block_size;
+  s.method = method;
+  s.prec = 0;
+  s.st = (hipStream_t)stream;
+  carve(base, n, s.bs, method, m, s.W);
+  return run(s, d_b, d_x, rtol, maxiter, m, h_iters, h_resid);
 }
 
 }  // extern "C"

@@ -126,6 +126,10 @@ def lib():
         "tcad_csr_solve": ([ctypes.c_int32, vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_int32,
                             ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, vp,
                             c_int32_p, c_double_p], ctypes.c_int),
+        "tcad_csr_solve_noalloc": ([ctypes.c_int32, vp, vp, vp, vp, vp, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                    ctypes.c_int32, ctypes.c_int32, vp, c_int32_p, c_double_p],
+                                   ctypes.c_int),
         "tcad_probe_read": ([vp, ctypes.POINTER(ctypes.c_int64), c_double_p], ctypes.c_int),
         "tcad_last_error": ([], ctypes.c_char_p),
         "tcad_version": ([], ctypes.c_char_p),
@@ -176,5 +180,5 @@ def exported_symbols():
             "tcad_solve_equilibrium", "tcad_solve_bias", "tcad_terminal_currents",
             "tcad_edge_currents", "tcad_surface_mobility", "tcad_time_kernel",
             "tcad_transient_step", "tcad_continuity_residuals", "tcad_y_parameters",
-            "tcad_probe_kernel", "tcad_probe_read", "tcad_csr_solve",
+            "tcad_probe_kernel", "tcad_probe_read", "tcad_csr_solve", "tcad_csr_solve_noalloc",
             "tcad_launch_count", "tcad_last_error", "tcad_version"]

@@ -52,8 +52,12 @@ def _prepare(A, b):
 
 
 def solve_linear(A, b, *, method="bicgstab", rtol=1e-10, atol=0.0, maxiter=500, x0=None,
-                 restart=None, block_size=None, precond="auto", stream=None):
-    """Solve A x = b on the GPU; returns (x, info) with x a NumPy array."""
+                 restart=None, block_size=None, precond="auto", stream=None, noalloc=False):
+    """Solve A x = b on the GPU; returns (x, info) with x a NumPy array.
+
+    noalloc=True runs the allocation-free entry point
+    (tcad_csr_solve_noalloc: block-Jacobi only, a fixed 64 MiB device
+    workspace per GPU) instead of tcad_csr_solve."""
     if method == "direct":
         raise ValueError("method='direct' is a host factorisation (SciPy SuperLU); the GPU "
                          "solve_linear runs 'gmres' and 'bicgstab'")
@@ -82,7 +86,7 @@ def solve_linear(A, b, *, method="bicgstab", rtol=1e-10, atol=0.0, maxiter=500, 
     torch.cuda.current_stream().synchronize()
     its = ctypes.c_int32()
     res = ctypes.c_double()
-    fn = _lib.lib().tcad_csr_solve
+    fn = _lib.lib().tcad_csr_solve_noalloc if noalloc else _lib.lib().tcad_csr_solve
     meth = _lib.CSR_GMRES if method == "gmres" else _lib.CSR_BICGSTAB
     rc = fn(n, ptr.data_ptr(), idx.data_ptr(), val.data_ptr(), bd.data_ptr(), x.data_ptr(),
             meth, prec, bs, float(rt), int(maxiter), int(restart or 0),
@@ -92,6 +96,6 @@ def solve_linear(A, b, *, method="bicgstab", rtol=1e-10, atol=0.0, maxiter=500, 
             f"{method} did not converge within {maxiter} iterations (relative residual="
             f"{res.value:.3e}, target rtol={rtol:.3e}) -- refusing to return the unconverged "
             f"iterate")
-    _lib.check(rc, "tcad_csr_solve")
+    _lib.check(rc, "tcad_csr_solve_noalloc" if noalloc else "tcad_csr_solve")
     return x.cpu().numpy(), {"method": method, "iterations": int(its.value), "converged": True,
                              "residual": float(res.value), "backend": "hip"}
