@@ -81,7 +81,8 @@ def trigate_step(devs, specs, opts, eq_opts, lat=None, record=None):
                                I["drain"]))
 
 
-def bench_trigate(steps, warmup, mode, points=None, probe=3):
+def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, precond="tile",
+                  tune=True):
     from tcadhip import configs
     from tcadhip.device import NewtonOptions, device_from_spec
     specs = [configs.config3_trigate(0.05), configs.config3_trigate(0.7)]
@@ -89,8 +90,17 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3):
         for s in specs:
             s.sweep = s.sweep[:points]
     devs = [device_from_spec(s) for s in specs]
-    opts = NewtonOptions(linsolve=mode)
+    opts = NewtonOptions(linsolve=mode, forcing=forcing, precond=precond)
     eq_opts = NewtonOptions(max_iter=400)
+    tuning = None
+    if tune:
+        # the autotuner times its candidates on the second Vg point of the
+        # Vd = 0.05 V sweep, warm-started from the first (untimed, before the
+        # warm-up steps)
+        from tcadhip import tune as tuner
+        devs[0].solve_equilibrium(eq_opts)
+        devs[0].solve_bias(specs[0].sweep[0], opts)
+        opts, tuning = tuner.tune(devs[0], specs[0].sweep[1], opts, say=_say)
     for _ in range(warmup):
         trigate_step(devs, specs, opts, eq_opts)
     _sync()
@@ -103,7 +113,8 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3):
     dt = time.perf_counter() - t0
     npts = sum(len(s.sweep) for s in specs)
     out = dict(seconds=dt, points_per_step=npts, latencies=np.array(lat), record=rec,
-               devs=devs, specs=specs, eq_iterations=[d.last_eq_iterations for d in devs])
+               devs=devs, specs=specs, eq_iterations=[d.last_eq_iterations for d in devs],
+               opts=opts, tuning=tuning)
     if probe is not None and probe >= 0:
         # in-solver duration of one kernel: one more (untimed) step with HIP
         # events around every launch of it
@@ -126,10 +137,11 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3):
     return out
 
 
-def reduced_trigate_parity(mode):
+def reduced_trigate_parity(opts):
     """The reduced 24x18x14 tri-gate at both drain biases against the
-    reference's own sweeps (committed fixtures): Newton counts and every
-    contact current per point, strict 1e-6 of the largest current."""
+    reference's own sweeps (committed fixtures), with the headline's solver
+    options: Newton counts and every contact current per point, strict
+    1e-6 of the largest current."""
     from golden.fixtures import load
     from tcadhip.device import NewtonOptions, device_from_spec
     out = {}
@@ -140,7 +152,7 @@ def reduced_trigate_parity(mode):
         dev.solve_equilibrium(NewtonOptions(max_iter=400))
         its, cur = [], []
         for bias in spec.sweep:
-            dev.solve_bias(bias, NewtonOptions(linsolve=mode))
+            dev.solve_bias(bias, opts)
             its.append(dev.last_iterations)
             I = dev.terminal_currents()
             cur.append([I[c.name] for c in spec.contacts])
@@ -567,6 +579,11 @@ def main():
                     help="first K Vg points of each sweep only (short profiling runs)")
     ap.add_argument("--probe", type=int, default=3,
                     help="BiCGStab kernel timed in-solver for the roofline (-1: none)")
+    ap.add_argument("--forcing", action="store_true",
+                    help="Newton forcing terms (inexact early linear solves)")
+    ap.add_argument("--precond", default="tile", choices=("tile", "node"))
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (fixed default solver options)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cap", type=float, default=75.0,
                     help="wall-clock cap [s] of the full-size CPU baseline")
@@ -598,7 +615,8 @@ def main():
     warnings.simplefilter("ignore")
 
     _say(f"tri-gate 64x48x48, {args.mode}: warmup {args.warmup}, steps {args.steps}")
-    r = bench_trigate(args.steps, args.warmup, args.mode, args.points, args.probe)
+    r = bench_trigate(args.steps, args.warmup, args.mode, args.points, args.probe,
+                      args.forcing, args.precond, tune=not args.no_tune)
     npts = r["points_per_step"]
     value = npts * args.steps / r["seconds"]
     _say(f"tri-gate: {value:.3f} points/s")
@@ -622,6 +640,10 @@ def main():
                    "precond": "2x1x8-node tile block-Jacobi (FP16 rows, FP32 scales)"},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "solver_options": {k: getattr(r["opts"], k) for k in
+                           ("linsolve", "linsolve_rtol", "precond", "forcing", "forcing_slope",
+                            "forcing_max", "tile_rows")},
+        "autotuner": r["tuning"],
         "newton_per_point": [x[1] for x in r["record"]],
         "krylov_per_point": [x[2] for x in r["record"]],
         "equilibrium_newton": r["eq_iterations"],
@@ -631,6 +653,8 @@ def main():
     if "probe" in r and r["probe"]["avg_us"]:
         pr = r["probe"]
         kname = PROBE_KERNELS[pr["kernel"]]
+        if r["opts"].tile_rows == "reg":
+            kname = kname.replace("_tmem", "_treg")
         if pr["kernel"] == 3:
             nbytes = K_BICG_U_BYTES_PER_NODE * N
             achieved = nbytes / (pr["avg_us"] * 1e-6) / 1e9
@@ -648,7 +672,7 @@ def main():
                              "launches": pr["launches"]}
     if not args.no_parity:
         _say("reduced tri-gate vs the reference fixtures")
-        line["parity_reduced_24x18x14_vs_reference"] = reduced_trigate_parity(args.mode)
+        line["parity_reduced_24x18x14_vs_reference"] = reduced_trigate_parity(r["opts"])
     line["cpu_baseline"] = None
     if not args.no_cpu:
         gpu_newton = r["record"][0][1] if r["record"] else 5

@@ -356,6 +356,21 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
                            int32_t maxiter, int32_t restart, void *stream, int32_t *h_iters,
                            double *h_resid);
 
+/* Run-time tuning of one batch (the autotuner's knobs; defaults in
+ * brackets):
+ *   TCAD_TUNE_TILE_ROWS      0 by grid size [registers up to 512 workgroups,
+ *                            memory above], 1 tile-inverse rows in registers,
+ *                            2 read at the point of use;
+ *   TCAD_TUNE_FORCING_SLOPE  c of the forcing term eta = clamp(c * err, rtol,
+ *                            eta_max) of TCAD_LIN_FORCING solves [0.01];
+ *   TCAD_TUNE_FORCING_MAX    eta_max [1e-4].
+ * Has no reference counterpart (the reference's Krylov tolerance is fixed,
+ * linsolve.py:725-727). */
+#define TCAD_TUNE_TILE_ROWS 1
+#define TCAD_TUNE_FORCING_SLOPE 2
+#define TCAD_TUNE_FORCING_MAX 3
+int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
+
 /* Bench instrumentation, in-solver kernel timing: from this call on,
  * every launch of one BiCGStab kernel inside the solves of this batch is
  * bracketed by a pair of HIP events on the batch stream (which: 0 the

@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(BS) k_newton_update(UArgs U, const double *y, 
 // whose linear solve ended above the acceptance residual stops here, before
 // its step is applied (linsolve.py:973-982 refuses such an iterate).
 __global__ void k_newton_check(UArgs U, double tol, double accept2, double rtol, double eta_max,
-                               int forcing) {
+                               int forcing, double eta_c) {
   __shared__ double sh[BS / 64];
   const int d = blockIdx.x;
   const DevInfo &I = U.info[d];
@@ -117,7 +117,7 @@ __global__ void k_newton_check(UArgs U, double tol, double accept2, double rtol,
     // (the converging step is refined to the rounding floor before it is
     // applied)
     if (forcing) {
-      const double eta = fmin(fmax(0.01 * m, rtol), eta_max);
+      const double eta = fmin(fmax(eta_c * m, rtol), eta_max);
       S.eta2 = eta * eta;
     }
   }
@@ -584,7 +584,7 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
   const bool forcing = (mode & TCAD_LIN_FORCING) != 0;
   b.tile = (mode & TCAD_PRECOND_NODE) ? 0 : 1;
   mode &= ~(TCAD_LIN_FORCING | TCAD_PRECOND_NODE);
-  const double eta_max = rtol > 1e-4 ? rtol : 1e-4;
+  const double eta_max = rtol > b.forcing_max ? rtol : b.forcing_max;
   if ((rc = launch_set_active(b, mask, forcing ? eta_max : rtol))) return rc;
   hipLaunchKernelGGL(k_stamp, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev, b.d_cidx,
                      b.d_cn_lnr, b.d_cn_n0, b.d_cn_p0, b.d_cn_contact, b.d_contact_V, b.d_scal,
@@ -612,7 +612,7 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
     hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
                        max_dpsi, 0, psi, n, p);
     hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol,
-                       accept * accept, rtol, eta_max, forcing ? 1 : 0);
+                       accept * accept, rtol, eta_max, forcing ? 1 : 0, b.forcing_c);
     b.launches += 2;
     TCAD_HIP_CHECK(hipGetLastError());
     if ((rc = sync_scalars(b))) return rc;
@@ -662,7 +662,7 @@ static int transient_impl(Batch &b, int max_iter, double tol, double max_dpsi, i
     hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
                        max_dpsi, 0, psi, n, p);
     hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol,
-                       accept * accept, rtol, 1.0, 0);
+                       accept * accept, rtol, 1.0, 0, b.forcing_c);
     b.launches += 2;
     TCAD_HIP_CHECK(hipGetLastError());
     if ((rc = sync_scalars(b))) return rc;
@@ -1355,6 +1355,29 @@ int tcad_time_kernel(tcad_batch *bp, int32_t which, int32_t reps, double *h_avg_
   (void)hipEventDestroy(e1);
   *h_avg_us = 1000.0 * ms / reps;
   return TCAD_OK;
+}
+
+int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  switch (key) {
+    case TCAD_TUNE_TILE_ROWS:
+      if (value != 0.0 && value != 1.0 && value != 2.0) break;
+      b.tile_rows = (int)value;
+      return TCAD_OK;
+    case TCAD_TUNE_FORCING_SLOPE:
+      if (!(value > 0.0 && value <= 1.0)) break;
+      b.forcing_c = value;
+      return TCAD_OK;
+    case TCAD_TUNE_FORCING_MAX:
+      if (!(value > 0.0 && value < 1.0)) break;
+      b.forcing_max = value;
+      return TCAD_OK;
+    default:
+      break;
+  }
+  tcad::set_error("tcad_set_tuning: unknown key or value out of range");
+  return TCAD_E_INVALID;
 }
 
 int tcad_probe_kernel(tcad_batch *bp, int32_t which) {

@@ -767,6 +767,8 @@ static int precond_variant(const Batch &b) {
     return -1;
   }();
   if (forced >= 0) return forced;
+  if (b.tile_rows == 1) return PV_TILE_REG;
+  if (b.tile_rows == 2) return PV_TILE_MEM;
   return b.nblk <= kTileRegBlocks ? PV_TILE_REG : PV_TILE_MEM;
 }
 

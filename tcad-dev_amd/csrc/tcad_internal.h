@@ -211,6 +211,10 @@ struct Batch {
   const double *last_n = nullptr, *last_p = nullptr;   // state of the last assembly
   const double *last_psi = nullptr;
   std::vector<int> dev_ncontact, dev_ngate;
+  // run-time tuning (tcad_set_tuning): tile-row variant (0 by grid size,
+  // 1 registers, 2 memory), forcing-term slope and cap
+  int tile_rows = 0;
+  double forcing_c = 0.01, forcing_max = 1e-4;
   // in-solver timing of one BiCGStab kernel (tcad_probe_kernel): every
   // launch of kernel `probe` (0 SpMV, 1 alpha/s, 2 second SpMV, 3 update;
   // -1 off) is bracketed by a pair of events on the batch stream

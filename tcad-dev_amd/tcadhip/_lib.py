@@ -27,6 +27,10 @@ LIN_BICGSTAB_MIXED = 1
 LIN_FORCING = 0x100
 PRECOND_NODE = 0x200
 
+TUNE_TILE_ROWS = 1
+TUNE_FORCING_SLOPE = 2
+TUNE_FORCING_MAX = 3
+
 CSR_BICGSTAB = 0
 CSR_GMRES = 1
 CSR_PREC_BLOCK_JACOBI = 0
@@ -123,6 +127,7 @@ def lib():
         "tcad_surface_mobility": ([vp, vp], ctypes.c_int),
         "tcad_time_kernel": ([vp, ctypes.c_int32, ctypes.c_int32, c_double_p], ctypes.c_int),
         "tcad_probe_kernel": ([vp, ctypes.c_int32], ctypes.c_int),
+        "tcad_set_tuning": ([vp, ctypes.c_int32, ctypes.c_double], ctypes.c_int),
         "tcad_csr_solve": ([ctypes.c_int32, vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_int32,
                             ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, vp,
                             c_int32_p, c_double_p], ctypes.c_int),
@@ -180,5 +185,5 @@ def exported_symbols():
             "tcad_solve_equilibrium", "tcad_solve_bias", "tcad_terminal_currents",
             "tcad_edge_currents", "tcad_surface_mobility", "tcad_time_kernel",
             "tcad_transient_step", "tcad_continuity_residuals", "tcad_y_parameters",
-            "tcad_probe_kernel", "tcad_probe_read", "tcad_csr_solve", "tcad_csr_solve_noalloc",
+            "tcad_probe_kernel", "tcad_probe_read", "tcad_set_tuning", "tcad_csr_solve", "tcad_csr_solve_noalloc",
             "tcad_launch_count", "tcad_last_error", "tcad_version"]

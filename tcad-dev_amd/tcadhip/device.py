@@ -54,6 +54,23 @@ class NewtonOptions:
     forcing: bool = False
     precond: str = "tile"
     verbose: bool = False
+    # run-time tuning (tcad_set_tuning): forcing eta = clamp(forcing_slope *
+    # err, linsolve_rtol, forcing_max); tile_rows "auto" | "reg" | "mem"
+    forcing_slope: float = 0.01
+    forcing_max: float = 1e-4
+    tile_rows: str = "auto"
+
+    def apply_tuning(self, handle):
+        L = _lib.lib()
+        rows = {"auto": 0, "reg": 1, "mem": 2}
+        if self.tile_rows not in rows:
+            raise ValueError(f"unknown tile_rows {self.tile_rows!r}")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_TILE_ROWS, float(rows[self.tile_rows])),
+                   "tcad_set_tuning")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FORCING_SLOPE, float(self.forcing_slope)),
+                   "tcad_set_tuning")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FORCING_MAX, float(self.forcing_max)),
+                   "tcad_set_tuning")
 
     def lin_mode(self):
         m = self.linsolve
@@ -382,6 +399,7 @@ class GPUBatch:
         status = np.zeros(self.ndev, dtype=np.int32)
         m = self._mask(active)
         self.set_bias()
+        opts.apply_tuning(self._handle)
         _lib.check(_lib.lib().tcad_solve_bias(
             self._handle, opts.max_iter, opts.tol_update, opts.max_dpsi, opts.lin_mode(),
             opts.linsolve_rtol, opts.lin_maxiter, _lib.iptr(m), self.psi.data_ptr(),

@@ -182,6 +182,11 @@ def solve_transient(device, waveforms, t_end, dt0, theta=1.0, opts=None, dt_min=
             "tcad_transient_step")
         nit, ok, status = int(cnt[0][0]), bool(cnt[1][0]), int(cnt[2][0])
         if status == _lib.NEWTON_LINEAR_FAIL:
+            # the reference's _newton_step works on copies (transient2d.py:139):
+            # a failed step leaves the device at the start of the step
+            for dst, src in zip((psi, n, p), save):
+                dst.copy_(src)
+            set_voltages(t)
             raise _lib.LinearSolveError("solve_transient: the Newton linear solve did not "
                                         "reach a relative residual of 1e-6")
         if ok:

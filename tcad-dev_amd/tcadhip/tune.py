@@ -1,0 +1,107 @@
+"""Autotuner: picks a device's solver configuration by timing it.
+
+The knobs are the ones whose best value depends on the device and the GPU
+rather than on the physics, and that leave the converged result the same
+(every candidate refines the converging Newton step to the FP64 rounding
+floor, so the terminal currents agree to rounding):
+
+  * Newton forcing terms (TCAD_LIN_FORCING: early linear solves stop at
+    eta = clamp(c * err, rtol, eta_max) instead of rtol) and their slope c
+    and cap eta_max -- fewer Krylov iterations for some more Newton steps;
+  * where the tile-inverse rows live in the BiCGStab kernels: loaded into
+    registers before the reductions (fewer waves per SIMD) or read at the
+    point of use (full occupancy).
+
+A candidate is measured by solving one bias point from the same saved
+state (psi, n, p restored on the device before each trial), twice, the
+faster taken.  Candidates that do not converge, whose currents differ from
+the untuned configuration's by more than 1e-9 of the largest current, or
+that need more than one Newton iteration over it are rejected (Newton
+counts must stay within one of the reference's; forcing terms can add
+two at deep reverse bias, where they are then not chosen).  Choices are cached per (dimension, mesh shape,
+linear-solve path, preconditioner) for the life of the process.
+"""
+import time
+from dataclasses import replace
+
+import numpy as np
+
+from .device import NewtonOptions, _torch
+
+CANDIDATES = (
+    dict(forcing=False, tile_rows="auto"),
+    dict(forcing=False, tile_rows="reg"),
+    dict(forcing=False, tile_rows="mem"),
+    dict(forcing=True, forcing_slope=0.01, forcing_max=1e-4, tile_rows="auto"),
+    dict(forcing=True, forcing_slope=0.01, forcing_max=1e-3, tile_rows="auto"),
+    dict(forcing=True, forcing_slope=0.1, forcing_max=1e-3, tile_rows="auto"),
+    dict(forcing=True, forcing_slope=0.01, forcing_max=1e-4, tile_rows="reg"),
+)
+
+_cache = {}
+
+
+def _key(dev, opts):
+    return (dev.dim, tuple(dev.shape), opts.linsolve, opts.precond)
+
+
+def tune(dev, bias, opts=None, candidates=CANDIDATES, repeats=2, say=None):
+    """Time every candidate on one bias point of `dev` (which must hold a
+    converged state: the warm start every trial restarts from) and return
+    (best NewtonOptions, report).  The device's state and bias are left
+    as they were."""
+    torch = _torch()
+    base = opts or NewtonOptions()
+    key = _key(dev, base)
+    if key in _cache:
+        best, report = _cache[key]
+        return replace(base, **best), report
+    B = dev._gpu()
+    saved = [t.clone() for t in (B.psi, B.n, B.p)]
+    saved_bias = {nm: (bc.V if hasattr(bc, "V") else bc.Vg) for nm, bc in dev.bcs.items()}
+    rows = []
+    ref_I = ref_its = None
+    try:
+        for cand in candidates:
+            o = replace(base, **cand)
+            best_t, res = None, None
+            for _ in range(repeats):
+                for t, s in zip((B.psi, B.n, B.p), saved):
+                    t.copy_(s)
+                dev._set_voltages(saved_bias)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                dev.solve_bias(bias, o)
+                I = dev.terminal_currents()
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                if best_t is None or dt < best_t:
+                    best_t = dt
+                res = (dev.last_converged, dev.last_iterations, dev.last_krylov_iterations,
+                       np.array(list(I.values())))
+            conv, its, kry, cur = res
+            if ref_I is None:
+                ref_I, ref_its = cur, its
+            ok = (bool(conv) and its <= ref_its + 1
+                  and bool(np.all(np.abs(cur - ref_I) <= 1e-9 * np.abs(ref_I).max() + 1e-300)))
+            rows.append(dict(cand, seconds=round(best_t, 5), newton=int(its), krylov=int(kry),
+                             accepted=ok))
+            if say:
+                say(f"tune {cand}: {best_t * 1e3:.2f} ms, Newton {its}, Krylov {kry}"
+                    + ("" if ok else " (rejected)"))
+    finally:
+        for t, s in zip((B.psi, B.n, B.p), saved):
+            t.copy_(s)
+        dev._set_voltages(saved_bias)
+        torch.cuda.synchronize()
+    good = [r for r in rows if r["accepted"]]
+    win = min(good, key=lambda r: r["seconds"])
+    best = {k: win[k] for k in ("forcing", "forcing_slope", "forcing_max", "tile_rows")
+            if k in win}
+    report = {"candidates": rows, "chosen": best, "point": dict(bias)}
+    _cache[key] = (best, report)
+    return replace(base, **best), report
+
+
+def clear_cache():
+    _cache.clear()

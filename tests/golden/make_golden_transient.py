@@ -6,7 +6,9 @@ travels anywhere.
 
 Only the reference pins the time-step sequence (adaptive dt), the line-
 search Newton iterates and iteration counts, and the per-step terminal
-currents.  The reference keeps no iteration counter, so its
+currents.  Each fixture also holds the state the transient starts from
+(psi_init, n_init, p_init: equilibrium, then the pre-bias if any), so a
+test can start from exactly the reference's initial state.  The reference keeps no iteration counter, so its
 transient2d._newton_step is wrapped to record the count it returns.
 
 Usage:  python tests/golden/make_golden_transient.py
@@ -46,6 +48,7 @@ def run(spec, prebias, waveforms, wf_desc, t_end, dt0, theta, output_times=None)
     dev.solve_equilibrium(opts)
     if prebias:
         dev.solve_bias(prebias, opts)
+    init = {"psi_init": dev.psi.copy(), "n_init": dev.n.copy(), "p_init": dev.p.copy()}
     ITS.clear()
     res = rt2.solve_transient(dev, waveforms, t_end, dt0, theta=theta, opts=opts,
                               output_times=output_times)
@@ -57,7 +60,7 @@ def run(spec, prebias, waveforms, wf_desc, t_end, dt0, theta, output_times=None)
            "output_times": np.array(output_times or [], dtype=float),
            "prebias_keys": np.array(sorted(prebias or {})),
            "prebias_vals": np.array([prebias[k] for k in sorted(prebias or {})], dtype=float),
-           "wf_desc": np.array(wf_desc, dtype=str)}
+           "wf_desc": np.array(wf_desc, dtype=str), **init}
     return out
 
 

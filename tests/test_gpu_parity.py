@@ -131,14 +131,21 @@ def _equilibrated_counts(name):
     return eq_its
 
 
-@pytest.mark.parametrize("mode", ["bicgstab", "bicgstab_mixed"])
-@pytest.mark.parametrize("name", SWEEPS)
+SWEEP_MODES = ([(n, m) for n in SWEEPS for m in ("bicgstab", "bicgstab_mixed")]
+               + [(n, "bicgstab_forcing") for n in SWEEPS if n.startswith(("trigate", "slab"))])
+
+
+@pytest.mark.parametrize("name,mode", SWEEP_MODES)
 def test_sweep_vs_reference(name, mode):
+    """The solver paths: FP64, FP32-stored matrix in FP64 refinement, and
+    (3D, where the autotuner picks it for the tri-gate) Newton forcing terms
+    with the tuned slope 0.1 and cap 1e-3."""
     NewtonOptions, _ = _gpu()
     _say(f"[sweep] {name} {mode}")
     spec, d = load(name)
     eq_its = _equilibrated_counts(name)
-    opts = NewtonOptions(linsolve=mode)
+    opts = (NewtonOptions(forcing=True, forcing_slope=0.1, forcing_max=1e-3)
+            if mode == "bicgstab_forcing" else NewtonOptions(linsolve=mode))
     dev, its, cur, states = _run_sweep(spec, opts)
     assert abs(dev.last_eq_iterations - int(d["eq_iterations"])) <= 1
     ref_its = d["newton_iterations"]
