@@ -45,6 +45,9 @@ HBM_PEAK_GBPS = 8000.0
 # (4 x 3 FP64), reads the node's 3 rows of its tile inverse (36 planes of
 # 4 FP16 + 3 FP32 scales)
 K_BICG_U_BYTES_PER_NODE = 7 * 24 + 4 * 24 + 36 * 8 + 3 * 4
+# launches shorter than this are the early exits issued after a solve
+# converged (they return before touching the vectors): not part of the average
+PROBE_MIN_US = 2.5
 PROBE_KERNELS = {0: "k_bicg_spmv<double>", 1: "k_bicg_s_tmem<double>",
                  2: "k_bicg_spmv_t<double>", 3: "k_bicg_u_tmem<double>"}
 
@@ -82,7 +85,7 @@ def trigate_step(devs, specs, opts, eq_opts, lat=None, record=None):
 
 
 def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, precond="tile",
-                  tune=True):
+                  tune=True, forcing_slope=0.01, forcing_max=1e-4):
     from tcadhip import configs
     from tcadhip.device import NewtonOptions, device_from_spec
     specs = [configs.config3_trigate(0.05), configs.config3_trigate(0.7)]
@@ -90,7 +93,8 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, prec
         for s in specs:
             s.sweep = s.sweep[:points]
     devs = [device_from_spec(s) for s in specs]
-    opts = NewtonOptions(linsolve=mode, forcing=forcing, precond=precond)
+    opts = NewtonOptions(linsolve=mode, forcing=forcing, precond=precond,
+                         forcing_slope=forcing_slope, forcing_max=forcing_max)
     eq_opts = NewtonOptions(max_iter=400)
     tuning = None
     if tune:
@@ -127,8 +131,8 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, prec
         for d in devs:
             n = ctypes.c_int64()
             us = ctypes.c_double()
-            L.check(L.lib().tcad_probe_read(d._gpu().handle, ctypes.byref(n), ctypes.byref(us)),
-                    "tcad_probe_read")
+            L.check(L.lib().tcad_probe_read(d._gpu().handle, PROBE_MIN_US, ctypes.byref(n),
+                                            ctypes.byref(us)), "tcad_probe_read")
             L.check(L.lib().tcad_probe_kernel(d._gpu().handle, -1), "tcad_probe_kernel")
             tot_n += n.value
             tot_us += n.value * us.value
@@ -581,6 +585,8 @@ def main():
                     help="BiCGStab kernel timed in-solver for the roofline (-1: none)")
     ap.add_argument("--forcing", action="store_true",
                     help="Newton forcing terms (inexact early linear solves)")
+    ap.add_argument("--forcing-slope", type=float, default=0.01)
+    ap.add_argument("--forcing-max", type=float, default=1e-4)
     ap.add_argument("--precond", default="tile", choices=("tile", "node"))
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (fixed default solver options)")
@@ -616,7 +622,8 @@ def main():
 
     _say(f"tri-gate 64x48x48, {args.mode}: warmup {args.warmup}, steps {args.steps}")
     r = bench_trigate(args.steps, args.warmup, args.mode, args.points, args.probe,
-                      args.forcing, args.precond, tune=not args.no_tune)
+                      args.forcing, args.precond, tune=not args.no_tune,
+                      forcing_slope=args.forcing_slope, forcing_max=args.forcing_max)
     npts = r["points_per_step"]
     value = npts * args.steps / r["seconds"]
     _say(f"tri-gate: {value:.3f} points/s")
@@ -642,7 +649,7 @@ def main():
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "solver_options": {k: getattr(r["opts"], k) for k in
                            ("linsolve", "linsolve_rtol", "precond", "forcing", "forcing_slope",
-                            "forcing_max", "tile_rows")},
+                            "forcing_max", "forcing_fp32", "tile_rows")},
         "autotuner": r["tuning"],
         "newton_per_point": [x[1] for x in r["record"]],
         "krylov_per_point": [x[2] for x in r["record"]],
@@ -665,7 +672,9 @@ def main():
                 "bytes_per_launch": nbytes, "bytes_per_node": K_BICG_U_BYTES_PER_NODE,
                 "avg_launch_us_in_solver": round(pr["avg_us"], 3),
                 "launches_timed": pr["launches"],
-                "timing": "HIP events around every launch of the kernel in one untimed step",
+                "timing": (f"kernel start/end HIP events (hipExtLaunchKernelGGL) on every launch "
+                           f"of the kernel in one untimed step; launches under {PROBE_MIN_US} us "
+                           f"(early exits after convergence) excluded"),
                 "copy_ceiling_GBps": copy_ceiling()}
         else:
             line["probe"] = {"kernel": kname, "avg_us": round(pr["avg_us"], 3),

@@ -363,12 +363,18 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
  *                            2 read at the point of use;
  *   TCAD_TUNE_FORCING_SLOPE  c of the forcing term eta = clamp(c * err, rtol,
  *                            eta_max) of TCAD_LIN_FORCING solves [0.01];
- *   TCAD_TUNE_FORCING_MAX    eta_max [1e-4].
+ *   TCAD_TUNE_FORCING_MAX    eta_max [1e-4];
+ *   TCAD_TUNE_FORCING_FP32   1: a forcing solve whose target is at least
+ *                            1e-5 for every active device runs BiCGStab on
+ *                            the FP32-stored matrix inside the FP64
+ *                            refinement (the converging step is still
+ *                            refined on the FP64 matrix) [0].
  * Has no reference counterpart (the reference's Krylov tolerance is fixed,
  * linsolve.py:725-727). */
 #define TCAD_TUNE_TILE_ROWS 1
 #define TCAD_TUNE_FORCING_SLOPE 2
 #define TCAD_TUNE_FORCING_MAX 3
+#define TCAD_TUNE_FORCING_FP32 4
 int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
 
 /* Bench instrumentation, in-solver kernel timing: from this call on,
@@ -376,11 +382,14 @@ int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
  * bracketed by a pair of HIP events on the batch stream (which: 0 the
  * stencil SpMV, 1 the alpha / s kernel, 2 the second SpMV, 3 the fused
  * update kernel, whichever preconditioner variant runs; -1 stops).
- * tcad_probe_read waits for the stream and returns the launches bracketed
- * since the last read and their average duration [microseconds].  Has no
+ * The events take the kernel's own start and end (hipExtLaunchKernelGGL).
+ * tcad_probe_read waits for the stream and returns, of the launches
+ * bracketed since the last read, those that ran at least min_us
+ * microseconds (launches issued after a solve converged return at once and
+ * move no data) and their average duration [microseconds].  Has no
  * reference counterpart. */
 int tcad_probe_kernel(tcad_batch *b, int32_t which);
-int tcad_probe_read(tcad_batch *b, int64_t *h_launches, double *h_avg_us);
+int tcad_probe_read(tcad_batch *b, double min_us, int64_t *h_launches, double *h_avg_us);
 
 /* Wall-clock-free counters of the last call: kernel launches issued. */
 int64_t tcad_launch_count(const tcad_batch *b);

@@ -595,15 +595,20 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
   // inner BiCGStab tolerance per refinement sweep: the FP64-stored matrix
   // reaches the outer target in one sweep; the FP32-stored one is limited
   // by its storage rounding
-  const double inner = mixed ? 1e-6 : rtol;
   const double accept = rtol > 1e-6 ? rtol : 1e-6;
   const dim3 gd((b.ndev + 63) / 64), td(64);
+  // FP32-stored matrix for loose forcing solves: the loosest target of the
+  // next solve over the active devices (eta_max for the first)
+  const bool loose32 = forcing && !mixed && b.forcing_fp32;
+  double eta_next = forcing ? eta_max : rtol;
   for (int it = 0; it < max_iter; ++it) {
+    const bool fp32_now = mixed || (loose32 && eta_next >= 1e-5);
+    const double inner = fp32_now ? 1e-6 : rtol;
     if ((rc = launch_surface_mobility(b, psi, true))) return rc;
-    if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, mixed, true))) return rc;
+    if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, mixed || loose32, true))) return rc;
     if (b.tile && (rc = launch_tile_setup(b, true))) return rc;
     if ((rc = launch_flags_to_status(b))) return rc;
-    if ((rc = ir_solve(b, mixed, true, rtol, 16, maxit, inner, true))) return rc;
+    if ((rc = ir_solve(b, fp32_now, true, rtol, 16, maxit, inner, true))) return rc;
     // measure the step, and before applying a step that passes the test
     // refine its solve to the FP64 rounding floor: the converged state is
     // then the reference's (exact Newton steps converge quadratically, so
@@ -617,7 +622,12 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
     TCAD_HIP_CHECK(hipGetLastError());
     if ((rc = sync_scalars(b))) return rc;
     int conv = 0;
-    for (int d = 0; d < b.ndev; ++d) conv += b.h_scal[d].active && b.h_scal[d].conv_now;
+    eta_next = 1.0;
+    for (int d = 0; d < b.ndev; ++d) {
+      conv += b.h_scal[d].active && b.h_scal[d].conv_now;
+      if (b.h_scal[d].active) eta_next = fmin(eta_next, sqrt(b.h_scal[d].eta2));
+    }
+    if (!forcing) eta_next = rtol;
     if (conv && rtol > 1e-16) {
       if ((rc = refine_converging(b, U, psi, n, p, max_dpsi, maxit, nullptr))) return rc;
     }
@@ -1373,6 +1383,10 @@ int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
       if (!(value > 0.0 && value < 1.0)) break;
       b.forcing_max = value;
       return TCAD_OK;
+    case TCAD_TUNE_FORCING_FP32:
+      if (value != 0.0 && value != 1.0) break;
+      b.forcing_fp32 = (int)value;
+      return TCAD_OK;
     default:
       break;
   }
@@ -1393,7 +1407,7 @@ int tcad_probe_kernel(tcad_batch *bp, int32_t which) {
   return TCAD_OK;
 }
 
-int tcad_probe_read(tcad_batch *bp, int64_t *h_launches, double *h_avg_us) {
+int tcad_probe_read(tcad_batch *bp, double min_us, int64_t *h_launches, double *h_avg_us) {
   GUARD(bp);
   Batch &b = *bp->impl;
   if (!h_launches || !h_avg_us) {
@@ -1402,14 +1416,18 @@ int tcad_probe_read(tcad_batch *bp, int64_t *h_launches, double *h_avg_us) {
   }
   TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
   double total = 0.0;
+  int64_t cnt = 0;
   const size_t pairs = b.probe_used / 2;
   for (size_t i = 0; i < pairs; ++i) {
     float ms = 0.f;
     TCAD_HIP_CHECK(hipEventElapsedTime(&ms, b.probe_ev[2 * i], b.probe_ev[2 * i + 1]));
-    total += ms;
+    if (1000.0 * ms >= min_us) {
+      total += ms;
+      ++cnt;
+    }
   }
-  *h_launches = (int64_t)pairs;
-  *h_avg_us = pairs ? 1000.0 * total / (double)pairs : 0.0;
+  *h_launches = cnt;
+  *h_avg_us = cnt ? 1000.0 * total / (double)cnt : 0.0;
   b.probe_used = 0;
   return TCAD_OK;
 }

@@ -215,6 +215,9 @@ struct Batch {
   // 1 registers, 2 memory), forcing-term slope and cap
   int tile_rows = 0;
   double forcing_c = 0.01, forcing_max = 1e-4;
+  // forcing solves whose target is loose (every active device's eta >=
+  // 1e-5) run BiCGStab on the FP32-stored matrix inside the FP64 refinement
+  int forcing_fp32 = 0;
   // in-solver timing of one BiCGStab kernel (tcad_probe_kernel): every
   // launch of kernel `probe` (0 SpMV, 1 alpha/s, 2 second SpMV, 3 update;
   // -1 off) is bracketed by a pair of events on the batch stream

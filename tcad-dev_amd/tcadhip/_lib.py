@@ -30,6 +30,7 @@ PRECOND_NODE = 0x200
 TUNE_TILE_ROWS = 1
 TUNE_FORCING_SLOPE = 2
 TUNE_FORCING_MAX = 3
+TUNE_FORCING_FP32 = 4
 
 CSR_BICGSTAB = 0
 CSR_GMRES = 1
@@ -135,7 +136,8 @@ def lib():
                                     ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                     ctypes.c_int32, ctypes.c_int32, vp, c_int32_p, c_double_p],
                                    ctypes.c_int),
-        "tcad_probe_read": ([vp, ctypes.POINTER(ctypes.c_int64), c_double_p], ctypes.c_int),
+        "tcad_probe_read": ([vp, ctypes.c_double, ctypes.POINTER(ctypes.c_int64), c_double_p],
+                            ctypes.c_int),
         "tcad_last_error": ([], ctypes.c_char_p),
         "tcad_version": ([], ctypes.c_char_p),
     }

@@ -55,9 +55,11 @@ class NewtonOptions:
     precond: str = "tile"
     verbose: bool = False
     # run-time tuning (tcad_set_tuning): forcing eta = clamp(forcing_slope *
-    # err, linsolve_rtol, forcing_max); tile_rows "auto" | "reg" | "mem"
+    # err, linsolve_rtol, forcing_max); forcing_fp32: loose forcing solves on
+    # the FP32-stored matrix; tile_rows "auto" | "reg" | "mem"
     forcing_slope: float = 0.01
     forcing_max: float = 1e-4
+    forcing_fp32: bool = False
     tile_rows: str = "auto"
 
     def apply_tuning(self, handle):
@@ -71,6 +73,8 @@ class NewtonOptions:
                    "tcad_set_tuning")
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FORCING_MAX, float(self.forcing_max)),
                    "tcad_set_tuning")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FORCING_FP32,
+                                     1.0 if self.forcing_fp32 else 0.0), "tcad_set_tuning")
 
     def lin_mode(self):
         m = self.linsolve

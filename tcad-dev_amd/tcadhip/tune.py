@@ -8,6 +8,8 @@ floor, so the terminal currents agree to rounding):
   * Newton forcing terms (TCAD_LIN_FORCING: early linear solves stop at
     eta = clamp(c * err, rtol, eta_max) instead of rtol) and their slope c
     and cap eta_max -- fewer Krylov iterations for some more Newton steps;
+  * whether the loose forcing solves run on the FP32-stored matrix (the
+    tight and converging ones stay on the FP64 matrix);
   * where the tile-inverse rows live in the BiCGStab kernels: loaded into
     registers before the reductions (fewer waves per SIMD) or read at the
     point of use (full occupancy).
@@ -36,6 +38,10 @@ CANDIDATES = (
     dict(forcing=True, forcing_slope=0.01, forcing_max=1e-3, tile_rows="auto"),
     dict(forcing=True, forcing_slope=0.1, forcing_max=1e-3, tile_rows="auto"),
     dict(forcing=True, forcing_slope=0.01, forcing_max=1e-4, tile_rows="reg"),
+    dict(forcing=True, forcing_slope=0.1, forcing_max=1e-3, forcing_fp32=True,
+         tile_rows="auto"),
+    dict(forcing=True, forcing_slope=0.1, forcing_max=1e-3, forcing_fp32=True,
+         tile_rows="reg"),
 )
 
 _cache = {}
@@ -96,8 +102,8 @@ def tune(dev, bias, opts=None, candidates=CANDIDATES, repeats=2, say=None):
         torch.cuda.synchronize()
     good = [r for r in rows if r["accepted"]]
     win = min(good, key=lambda r: r["seconds"])
-    best = {k: win[k] for k in ("forcing", "forcing_slope", "forcing_max", "tile_rows")
-            if k in win}
+    best = {k: win[k] for k in ("forcing", "forcing_slope", "forcing_max", "forcing_fp32",
+                                "tile_rows") if k in win}
     report = {"candidates": rows, "chosen": best, "point": dict(bias)}
     _cache[key] = (best, report)
     return replace(base, **best), report

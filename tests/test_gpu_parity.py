@@ -132,20 +132,27 @@ def _equilibrated_counts(name):
 
 
 SWEEP_MODES = ([(n, m) for n in SWEEPS for m in ("bicgstab", "bicgstab_mixed")]
-               + [(n, "bicgstab_forcing") for n in SWEEPS if n.startswith(("trigate", "slab"))])
+               + [(n, m) for n in SWEEPS if n.startswith("trigate")
+                  for m in ("bicgstab_forcing", "bicgstab_forcing32")])
 
 
 @pytest.mark.parametrize("name,mode", SWEEP_MODES)
 def test_sweep_vs_reference(name, mode):
-    """The solver paths: FP64, FP32-stored matrix in FP64 refinement, and
-    (3D, where the autotuner picks it for the tri-gate) Newton forcing terms
-    with the tuned slope 0.1 and cap 1e-3."""
+    """The solver paths: FP64, FP32-stored matrix in FP64 refinement, and,
+    on the tri-gate where the autotuner picks them, Newton forcing terms
+    with the tuned slope 0.1 and cap 1e-3, the loose solves on the FP64 or
+    the FP32-stored matrix.  (Forcing is not a universal default: on the
+    gated slabs it needs up to two Newton iterations more than the exact
+    steps, which the autotuner's one-iteration guard then rejects.)"""
     NewtonOptions, _ = _gpu()
     _say(f"[sweep] {name} {mode}")
     spec, d = load(name)
     eq_its = _equilibrated_counts(name)
-    opts = (NewtonOptions(forcing=True, forcing_slope=0.1, forcing_max=1e-3)
-            if mode == "bicgstab_forcing" else NewtonOptions(linsolve=mode))
+    if mode.startswith("bicgstab_forcing"):
+        opts = NewtonOptions(forcing=True, forcing_slope=0.1, forcing_max=1e-3,
+                             forcing_fp32=mode.endswith("32"))
+    else:
+        opts = NewtonOptions(linsolve=mode)
     dev, its, cur, states = _run_sweep(spec, opts)
     assert abs(dev.last_eq_iterations - int(d["eq_iterations"])) <= 1
     ref_its = d["newton_iterations"]
