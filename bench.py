@@ -272,9 +272,11 @@ def bench_diode(steps=3, warmup=1):
                           f"{dt_cpu:.2f} s"}
 
 
-def bench_mosfet(steps=2, warmup=1, modes=("bicgstab", "bicgstab_mixed"), cpu_points=3):
-    """The 200x120 MOSFET Id-Vg sweep (21 points), FP64 and FP32-matrix
-    paths; the CPU oracle on the first points for time and agreement."""
+def bench_mosfet(steps=2, warmup=1, modes=("bicgstab", "bicgstab_mixed", "tuned"), cpu_points=3):
+    """The 200x120 MOSFET Id-Vg sweep (21 points): the FP64 path, the
+    FP32-matrix path and the autotuned FP64 path (the autotuner timed on the
+    second Vg point); the CPU oracle on the first points for time and
+    agreement."""
     from oracle_build import oracle_device
     from tcadhip import configs
     from tcadhip.device import NewtonOptions, device_from_spec
@@ -283,7 +285,12 @@ def bench_mosfet(steps=2, warmup=1, modes=("bicgstab", "bicgstab_mixed"), cpu_po
     ref64 = None
     for mode in modes:
         dev = device_from_spec(spec)
-        opts = NewtonOptions(linsolve=mode)
+        opts = NewtonOptions(linsolve="bicgstab" if mode == "tuned" else mode)
+        if mode == "tuned":
+            from tcadhip import tune as tuner
+            dev.solve_equilibrium(opts)
+            dev.solve_bias(spec.sweep[0], opts)
+            opts, rep = tuner.tune(dev, spec.sweep[1], opts)
 
         def sweep(lat=None):
             dev.solve_equilibrium(opts)
@@ -309,6 +316,9 @@ def bench_mosfet(steps=2, warmup=1, modes=("bicgstab", "bicgstab_mixed"), cpu_po
             ref64 = cur
         lat_ms = np.array(lat) * 1e3
         out[mode] = {"points_per_s": round(len(spec.sweep) * steps / dt, 3),
+                     "options": {k: getattr(opts, k) for k in
+                                 ("linsolve", "forcing", "forcing_slope", "forcing_max",
+                                  "forcing_fp32", "tile_rows", "graph")},
                      "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
                      "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
                      "newton_per_point": its,
@@ -394,7 +404,7 @@ def _cpu_diode(idx):
     return np.asarray(cur[-1]).tolist(), time.perf_counter() - t0
 
 
-def bench_batch(lanes=1, sample=8):
+def bench_batch(lanes=1, sample=8, modes=None):
     """256 seeded diodes (40x30..160x100 nodes, -2.0..0.8 V, 0.1 V
     continuation steps), batched, both precisions; the plain FP64 path (one
     device at a time) on an evenly spaced sample.  Unit of work: one device
@@ -403,12 +413,13 @@ def bench_batch(lanes=1, sample=8):
     from tcadhip.device import NewtonOptions, setup_from_spec
     specs = configs.batch_diodes()
     res = {}
-    for mode in ("bicgstab_mixed", "bicgstab"):
+    modes = modes or {"bicgstab_mixed": NewtonOptions(linsolve="bicgstab_mixed"),
+                      "bicgstab": NewtonOptions()}
+    for mode, mopts in modes.items():
         setups = [setup_from_spec(s) for s in specs]
         _sync()
         t0 = time.perf_counter()
-        r, bss = sweep.run_lanes(setups, [s.sweep for s in specs], NewtonOptions(linsolve=mode),
-                                 lanes=lanes)
+        r, bss = sweep.run_lanes(setups, [s.sweep for s in specs], mopts, lanes=lanes)
         _sync()
         dt = time.perf_counter() - t0
         res[mode] = dict(seconds=dt, results=r, rounds=sum(b.rounds for b in bss),
@@ -596,6 +607,9 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--extras", default="diode,mosfet,ac,batch,serve",
                     help="extra workloads (comma-separated; '' for none)")
+    ap.add_argument("--batch-variants", default="",
+                    help="only the batched diodes, with these option sets (comma-separated of "
+                         "bicgstab, forcing, forcing_graph, graph)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (the reference "
                          "kernel for scaling HBM traffic counters) and exit")
@@ -619,6 +633,16 @@ def main():
         return
     import warnings
     warnings.simplefilter("ignore")
+    if args.batch_variants:
+        from tcadhip.device import NewtonOptions
+        allm = {"bicgstab": NewtonOptions(),
+                "forcing": NewtonOptions(forcing=True, forcing_slope=0.1, forcing_max=1e-3),
+                "forcing_graph": NewtonOptions(forcing=True, forcing_slope=0.1,
+                                               forcing_max=1e-3, graph=True),
+                "graph": NewtonOptions(graph=True)}
+        out = bench_batch(modes={k: allm[k] for k in args.batch_variants.split(",")})
+        print(json.dumps(out), flush=True)
+        return
 
     _say(f"tri-gate 64x48x48, {args.mode}: warmup {args.warmup}, steps {args.steps}")
     r = bench_trigate(args.steps, args.warmup, args.mode, args.points, args.probe,
@@ -649,7 +673,7 @@ def main():
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "solver_options": {k: getattr(r["opts"], k) for k in
                            ("linsolve", "linsolve_rtol", "precond", "forcing", "forcing_slope",
-                            "forcing_max", "forcing_fp32", "tile_rows")},
+                            "forcing_max", "forcing_fp32", "tile_rows", "graph")},
         "autotuner": r["tuning"],
         "newton_per_point": [x[1] for x in r["record"]],
         "krylov_per_point": [x[2] for x in r["record"]],

@@ -224,6 +224,32 @@ int tcad_solve_bias(tcad_batch *b, int32_t max_iter, double tol_update,
                     int32_t *h_iters, int32_t *h_converged, int32_t *h_krylov,
                     int32_t *h_status);
 
+/* A queue of bias points per device, solved in one call without a barrier
+ * between devices: device d solves its points 0 .. h_npoints[d]-1 in order,
+ * each warm-started from the previous converged one (the first from the
+ * present state) with its contact values stamped, and goes on to its next
+ * point as soon as it converges at the present one; a device stops at its
+ * first failure, its state restored to its last converged point.  Bias
+ * tables, point-major: h_contact_V[k * ncontact_total + c] (contact c's
+ * voltage at a device's point k, global contact order as for
+ * tcad_set_bias) and h_gate_Vg[k * ngate_total + g]; entries of other
+ * devices' contacts are ignored.  Outputs: h_done[d] points converged;
+ * h_iters / h_krylov[d * npmax + k] Newton and Krylov iterations of point
+ * k (of the failed point too); h_status[d] the TCAD_NEWTON_* outcome of the
+ * point a device stopped at (OK when all converged); h_I[k * ncontact_total
+ * + c] the terminal currents after point k (as tcad_terminal_currents; only
+ * converged points are written).  Solver arguments as for tcad_solve_bias.
+ * Replaces: the warm-started sweep loops of id_vg_sweep
+ * (pytcad/pytcad/mosfet.py:90-103), id_vg_sweep_3d (finfet3d.py:246-263)
+ * and the sweep runner (gui/services/solver_runner.py:360), for many
+ * devices at once. */
+int tcad_solve_queue(tcad_batch *b, int32_t max_iter, double tol_update, double max_dpsi,
+                     int32_t lin_mode, double lin_rtol, int32_t lin_maxiter, int32_t npmax,
+                     const int32_t *h_npoints, const double *h_contact_V,
+                     const double *h_gate_Vg, double *d_psi, double *d_n, double *d_p,
+                     int32_t *h_done, int32_t *h_iters, int32_t *h_krylov, int32_t *h_status,
+                     double *h_I);
+
 /* One implicit time step of the theta scheme, from the state at the start
  * of the step to the new time, whose contact voltages are set by
  * tcad_set_bias.  Per non-contact node the continuity rows become
@@ -368,13 +394,18 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
  *                            1e-5 for every active device runs BiCGStab on
  *                            the FP32-stored matrix inside the FP64
  *                            refinement (the converging step is still
- *                            refined on the FP64 matrix) [0].
+ *                            refined on the FP64 matrix) [0];
+ *   TCAD_TUNE_GRAPH          1: BiCGStab iterations are issued as replays of
+ *                            a captured HIP graph of 16 iterations (captured
+ *                            once per kernel-argument set on a private
+ *                            stream, launched on the batch stream) [0].
  * Has no reference counterpart (the reference's Krylov tolerance is fixed,
  * linsolve.py:725-727). */
 #define TCAD_TUNE_TILE_ROWS 1
 #define TCAD_TUNE_FORCING_SLOPE 2
 #define TCAD_TUNE_FORCING_MAX 3
 #define TCAD_TUNE_FORCING_FP32 4
+#define TCAD_TUNE_GRAPH 5
 int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
 
 /* Bench instrumentation, in-solver kernel timing: from this call on,

@@ -346,7 +346,7 @@ __global__ void k_unmask(Scal *scal, int ndev) {
 }
 
 // After the step is applied: count it, retire converged or failed devices.
-__global__ void k_newton_finish(Scal *scal, int ndev) {
+__global__ void k_newton_finish(Scal *scal, int ndev, int max_iter) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndev) return;
   Scal &S = scal[d];
@@ -357,6 +357,9 @@ __global__ void k_newton_finish(Scal *scal, int ndev) {
     S.active = 0;
   } else if (!isfinite(S.err)) {
     S.result = TCAD_NEWTON_NONFINITE;
+    S.active = 0;
+  } else if (S.newton_it >= max_iter) {
+    S.result = TCAD_NEWTON_MAX_ITER;
     S.active = 0;
   }
 }
@@ -577,70 +580,296 @@ static int refine_converging(Batch &b, const UArgs &U, const double *psi, const 
   return TCAD_OK;
 }
 
+// Per-call settings of the coupled Newton loop shared by solve_bias and the
+// point queue.
+struct NewtonCfg {
+  bool forcing, mixed, loose32;
+  double tol, max_dpsi, rtol, eta_max, accept;
+  int maxit;
+};
+
+static NewtonCfg newton_cfg(Batch &b, int &mode, double tol, double max_dpsi, double rtol,
+                            int maxit) {
+  NewtonCfg c{};
+  c.forcing = (mode & TCAD_LIN_FORCING) != 0;
+  b.tile = (mode & TCAD_PRECOND_NODE) ? 0 : 1;
+  mode &= ~(TCAD_LIN_FORCING | TCAD_PRECOND_NODE);
+  c.mixed = mode == TCAD_LIN_BICGSTAB_MIXED;
+  // FP32-stored matrix for loose forcing solves
+  c.loose32 = c.forcing && !c.mixed && b.forcing_fp32;
+  c.tol = tol;
+  c.max_dpsi = max_dpsi;
+  c.rtol = rtol;
+  c.eta_max = rtol > b.forcing_max ? rtol : b.forcing_max;
+  c.accept = rtol > 1e-6 ? rtol : 1e-6;
+  c.maxit = maxit;
+  return c;
+}
+
+// One Newton iteration of every active device: assemble, solve (FP64
+// refinement; the inner BiCGStab on the FP32-stored matrix in the mixed mode
+// and, with loose32, for loose forcing targets), measure, refine a
+// converging step to the FP64 floor, apply, and retire devices that
+// converged, failed or reached max_iter.  eta_next: the loosest target of
+// the next solve over the active devices (read back with the scalars).
+static int bias_newton_step(Batch &b, const UArgs &U, const NewtonCfg &c, int max_iter,
+                            double &eta_next, double *psi, double *n, double *p) {
+  int rc;
+  const dim3 gd((b.ndev + 63) / 64), td(64);
+  const bool fp32_now = c.mixed || (c.loose32 && eta_next >= 1e-5);
+  const double inner = fp32_now ? 1e-6 : c.rtol;
+  if ((rc = launch_surface_mobility(b, psi, true))) return rc;
+  if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, c.mixed || c.loose32, true))) return rc;
+  if (b.tile && (rc = launch_tile_setup(b, true))) return rc;
+  if ((rc = launch_flags_to_status(b))) return rc;
+  if ((rc = ir_solve(b, fp32_now, true, c.rtol, 16, c.maxit, inner, true))) return rc;
+  // measure the step, and before applying a step that passes the test
+  // refine its solve to the FP64 rounding floor: the converged state is
+  // then the reference's (exact Newton steps converge quadratically, so
+  // its final state is exact to rounding), which the terminal current
+  // needs where it is a cancellation of much larger fluxes (reverse bias)
+  hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
+                     c.max_dpsi, 0, psi, n, p);
+  hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, c.tol,
+                     c.accept * c.accept, c.rtol, c.eta_max, c.forcing ? 1 : 0, b.forcing_c);
+  b.launches += 2;
+  TCAD_HIP_CHECK(hipGetLastError());
+  if ((rc = sync_scalars(b))) return rc;
+  int conv = 0;
+  eta_next = 1.0;
+  for (int d = 0; d < b.ndev; ++d) {
+    conv += b.h_scal[d].active && b.h_scal[d].conv_now;
+    if (b.h_scal[d].active) eta_next = fmin(eta_next, sqrt(b.h_scal[d].eta2));
+  }
+  if (!c.forcing) eta_next = c.rtol;
+  if (conv && c.rtol > 1e-16) {
+    if ((rc = refine_converging(b, U, psi, n, p, c.max_dpsi, c.maxit, nullptr))) return rc;
+  }
+  hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
+                     c.max_dpsi, 1, psi, n, p);
+  hipLaunchKernelGGL(k_newton_finish, gd, td, 0, b.stream, b.d_scal, b.ndev, max_iter);
+  b.launches += 2;
+  TCAD_HIP_CHECK(hipGetLastError());
+  return sync_scalars(b);
+}
+
 static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, int mode,
                            double rtol, int maxit, const int32_t *mask, double *psi, double *n,
                            double *p) {
   int rc;
-  const bool forcing = (mode & TCAD_LIN_FORCING) != 0;
-  b.tile = (mode & TCAD_PRECOND_NODE) ? 0 : 1;
-  mode &= ~(TCAD_LIN_FORCING | TCAD_PRECOND_NODE);
-  const double eta_max = rtol > b.forcing_max ? rtol : b.forcing_max;
-  if ((rc = launch_set_active(b, mask, forcing ? eta_max : rtol))) return rc;
+  const NewtonCfg c = newton_cfg(b, mode, tol, max_dpsi, rtol, maxit);
+  if ((rc = launch_set_active(b, mask, c.forcing ? c.eta_max : rtol))) return rc;
   hipLaunchKernelGGL(k_stamp, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev, b.d_cidx,
                      b.d_cn_lnr, b.d_cn_n0, b.d_cn_p0, b.d_cn_contact, b.d_contact_V, b.d_scal,
                      psi, n, p);
   b.launches++;
   UArgs U = uargs(b);
-  const bool mixed = mode == TCAD_LIN_BICGSTAB_MIXED;
-  // inner BiCGStab tolerance per refinement sweep: the FP64-stored matrix
-  // reaches the outer target in one sweep; the FP32-stored one is limited
-  // by its storage rounding
-  const double accept = rtol > 1e-6 ? rtol : 1e-6;
-  const dim3 gd((b.ndev + 63) / 64), td(64);
-  // FP32-stored matrix for loose forcing solves: the loosest target of the
-  // next solve over the active devices (eta_max for the first)
-  const bool loose32 = forcing && !mixed && b.forcing_fp32;
-  double eta_next = forcing ? eta_max : rtol;
+  double eta_next = c.forcing ? c.eta_max : rtol;
   for (int it = 0; it < max_iter; ++it) {
-    const bool fp32_now = mixed || (loose32 && eta_next >= 1e-5);
-    const double inner = fp32_now ? 1e-6 : rtol;
-    if ((rc = launch_surface_mobility(b, psi, true))) return rc;
-    if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, mixed || loose32, true))) return rc;
-    if (b.tile && (rc = launch_tile_setup(b, true))) return rc;
-    if ((rc = launch_flags_to_status(b))) return rc;
-    if ((rc = ir_solve(b, fp32_now, true, rtol, 16, maxit, inner, true))) return rc;
-    // measure the step, and before applying a step that passes the test
-    // refine its solve to the FP64 rounding floor: the converged state is
-    // then the reference's (exact Newton steps converge quadratically, so
-    // its final state is exact to rounding), which the terminal current
-    // needs where it is a cancellation of much larger fluxes (reverse bias)
-    hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
-                       max_dpsi, 0, psi, n, p);
-    hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol,
-                       accept * accept, rtol, eta_max, forcing ? 1 : 0, b.forcing_c);
-    b.launches += 2;
-    TCAD_HIP_CHECK(hipGetLastError());
-    if ((rc = sync_scalars(b))) return rc;
-    int conv = 0;
-    eta_next = 1.0;
-    for (int d = 0; d < b.ndev; ++d) {
-      conv += b.h_scal[d].active && b.h_scal[d].conv_now;
-      if (b.h_scal[d].active) eta_next = fmin(eta_next, sqrt(b.h_scal[d].eta2));
-    }
-    if (!forcing) eta_next = rtol;
-    if (conv && rtol > 1e-16) {
-      if ((rc = refine_converging(b, U, psi, n, p, max_dpsi, maxit, nullptr))) return rc;
-    }
-    hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
-                       max_dpsi, 1, psi, n, p);
-    hipLaunchKernelGGL(k_newton_finish, gd, td, 0, b.stream, b.d_scal, b.ndev);
-    b.launches += 2;
-    TCAD_HIP_CHECK(hipGetLastError());
-    if ((rc = sync_scalars(b))) return rc;
+    if ((rc = bias_newton_step(b, U, c, max_iter, eta_next, psi, n, p))) return rc;
     int open = 0;
     for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active;
     if (open == 0) break;
   }
+  return TCAD_OK;
+}
+
+// Re-arm the devices of `mask` for their next queued point (the other
+// devices' selection flags are cleared, so the stamp that follows touches
+// only the re-armed devices).
+__global__ void k_rearm(Scal *scal, int ndev, const int *mask, double eta0) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndev) return;
+  Scal &S = scal[d];
+  S.sel = mask[d] != 0;
+  if (!S.sel) return;
+  S.active = 1;
+  S.eta2 = eta0 * eta0;
+  S.result = 0;
+  S.newton_it = 0;
+  S.newton_done = 0;
+  S.kry_total = 0;
+  S.ir_it = 0;
+  S.status = 0;
+  S.err = 0.0;
+  S.conv_now = 0;
+}
+
+// Copy the state of the devices of `mask` to (dir 0) or from (dir 1) the
+// backup planes.
+__global__ void __launch_bounds__(BS) k_backup(const DevInfo *info, const int *blk2dev,
+                                               const int *mask, int dir, long long N,
+                                               double *psi, double *n, double *p, double *bk) {
+  const int d = blk2dev[blockIdx.x];
+  if (!mask[d]) return;
+  const DevInfo &I = info[d];
+  const NodeAt P = node_at(I, blockIdx.x);
+  if (!P.valid) return;
+  const long long g = I.off + P.k;
+  if (dir == 0) {
+    bk[g] = psi[g];
+    bk[N + g] = n[g];
+    bk[2 * N + g] = p[g];
+  } else {
+    psi[g] = bk[g];
+    n[g] = bk[N + g];
+    p[g] = bk[2 * N + g];
+  }
+}
+
+// A queue of bias points per device, solved in one call: each device goes
+// on to its next point as soon as it converges at the present one (no
+// barrier between devices), warm-started from it, and stops at its first
+// failure with the state of its last converged point restored.
+static int queue_impl(Batch &b, int max_iter, double tol, double max_dpsi, int mode, double rtol,
+                      int maxit, int npmax, const int32_t *h_np, const double *h_qV,
+                      const double *h_qG, double *psi, double *n, double *p, int32_t *h_done,
+                      int32_t *h_iters, int32_t *h_kry, int32_t *h_status, double *h_I) {
+  int rc;
+  const NewtonCfg c = newton_cfg(b, mode, tol, max_dpsi, rtol, maxit);
+  const int nd = b.ndev;
+  const double eta0 = c.forcing ? c.eta_max : rtol;
+  if (!b.bk) {
+    hipError_t e = hipMalloc(&b.bk, sizeof(double) * 3 * (size_t)b.ntot);
+    if (e != hipSuccess) {
+      set_error(std::string("tcad_solve_queue: hipMalloc failed: ") + hipGetErrorString(e));
+      return TCAD_E_HIP;
+    }
+  }
+  if (!b.d_qmask) {
+    TCAD_HIP_CHECK(hipMalloc(&b.d_qmask, sizeof(int) * nd));
+  }
+  // host copies of the bias tables, overwritten per device as it advances
+  std::vector<double> hV(std::max(b.ncontact, 1)), hG(std::max(b.ngate, 1));
+  if (b.ncontact)
+    TCAD_HIP_CHECK(hipMemcpy(hV.data(), b.d_contact_V, sizeof(double) * b.ncontact,
+                             hipMemcpyDeviceToHost));
+  if (b.ngate)
+    TCAD_HIP_CHECK(hipMemcpy(hG.data(), b.d_gate_Vg, sizeof(double) * b.ngate,
+                             hipMemcpyDeviceToHost));
+  std::vector<int> pidx(nd, 0), state(nd, 0), qm(nd, 0);   // state: 0 idle, 1 running, 2 done
+  std::vector<double> hI(std::max(b.ncontact, 1));
+  auto load_bias = [&](int d) {
+    const DevInfo &I = b.info[d];
+    const int k = pidx[d];
+    for (int t = 0; t < I.nc; ++t) hV[I.c0 + t] = h_qV[(size_t)k * b.ncontact + I.c0 + t];
+    for (int t = 0; t < I.ng; ++t) hG[I.g0 + t] = h_qG[(size_t)k * b.ngate + I.g0 + t];
+  };
+  auto upload = [&]() -> int {
+    if (b.ncontact)
+      TCAD_HIP_CHECK(hipMemcpyAsync(b.d_contact_V, hV.data(), sizeof(double) * b.ncontact,
+                                    hipMemcpyHostToDevice, b.stream));
+    if (b.ngate)
+      TCAD_HIP_CHECK(hipMemcpyAsync(b.d_gate_Vg, hG.data(), sizeof(double) * b.ngate,
+                                    hipMemcpyHostToDevice, b.stream));
+    TCAD_HIP_CHECK(hipMemcpyAsync(b.d_qmask, qm.data(), sizeof(int) * nd, hipMemcpyHostToDevice,
+                                  b.stream));
+    // the host tables change again right after: wait for the copies
+    TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+    return TCAD_OK;
+  };
+  auto backup = [&](int dir) {
+    hipLaunchKernelGGL(k_backup, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev,
+                       b.d_qmask, dir, b.ntot, psi, n, p, b.bk);
+    b.launches++;
+  };
+  auto stamp = [&]() {
+    hipLaunchKernelGGL(k_stamp, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev,
+                       b.d_cidx, b.d_cn_lnr, b.d_cn_n0, b.d_cn_p0, b.d_cn_contact,
+                       b.d_contact_V, b.d_scal, psi, n, p);
+    b.launches++;
+  };
+  int total_pts = 0;
+  for (int d = 0; d < nd; ++d) {
+    h_done[d] = 0;
+    h_status[d] = TCAD_NEWTON_OK;
+    qm[d] = h_np[d] > 0 ? 1 : 0;
+    total_pts += std::max(0, h_np[d]);
+    if (qm[d]) {
+      state[d] = 1;
+      load_bias(d);
+    }
+  }
+  if (total_pts == 0) return TCAD_OK;
+  if ((rc = upload())) return rc;
+  backup(0);                                      // the start state, restored on a first-point failure
+  if ((rc = launch_set_active(b, qm.data(), eta0))) return rc;
+  stamp();
+  UArgs U = uargs(b);
+  double eta_next = eta0;
+  const long long guard = (long long)max_iter * total_pts + 1;
+  for (long long it = 0; it < guard; ++it) {
+    if ((rc = bias_newton_step(b, U, c, max_iter, eta_next, psi, n, p))) return rc;
+    std::vector<int> conv, fail;
+    int open = 0;
+    for (int d = 0; d < nd; ++d) {
+      if (state[d] != 1) continue;
+      if (b.h_scal[d].active) {
+        ++open;
+        continue;
+      }
+      (b.h_scal[d].newton_done ? conv : fail).push_back(d);
+    }
+    if (!conv.empty()) {
+      // currents at the converged states (every contact; only the converged
+      // devices' are kept)
+      if ((rc = launch_assemble(b, psi, n, p, b.F, b.Fnp, false, false, 1))) return rc;
+      if ((rc = launch_contact_sums(b, b.Fnp, b.v64[0]))) return rc;
+      if (b.ncontact)
+        TCAD_HIP_CHECK(hipMemcpyAsync(hI.data(), b.v64[0], sizeof(double) * b.ncontact,
+                                      hipMemcpyDeviceToHost, b.stream));
+      TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+      std::fill(qm.begin(), qm.end(), 0);
+      for (int d : conv) {
+        const DevInfo &I = b.info[d];
+        const int k = pidx[d];
+        for (int t = 0; t < I.nc; ++t)
+          h_I[(size_t)k * b.ncontact + I.c0 + t] = hI[I.c0 + t];
+        h_iters[(size_t)d * npmax + k] = b.h_scal[d].newton_it;
+        h_kry[(size_t)d * npmax + k] = b.h_scal[d].kry_total;
+        h_done[d] = k + 1;
+        qm[d] = 1;
+      }
+      if ((rc = upload())) return rc;
+      backup(0);                                  // their converged states
+      std::fill(qm.begin(), qm.end(), 0);
+      int rearm = 0;
+      for (int d : conv) {
+        if (++pidx[d] < h_np[d]) {
+          load_bias(d);
+          qm[d] = 1;
+          ++rearm;
+          ++open;
+        } else {
+          state[d] = 2;
+        }
+      }
+      if (rearm) {
+        if ((rc = upload())) return rc;
+        hipLaunchKernelGGL(k_rearm, dim3((nd + 63) / 64), dim3(64), 0, b.stream, b.d_scal, nd,
+                           b.d_qmask, eta0);
+        b.launches++;
+        stamp();
+        eta_next = c.forcing ? std::min(eta_next, eta0) : rtol;
+      }
+    }
+    if (!fail.empty()) {
+      std::fill(qm.begin(), qm.end(), 0);
+      for (int d : fail) {
+        const Scal &S = b.h_scal[d];
+        h_status[d] = S.result ? S.result : TCAD_NEWTON_MAX_ITER;
+        h_iters[(size_t)d * npmax + pidx[d]] = b.h_scal[d].newton_it;
+        h_kry[(size_t)d * npmax + pidx[d]] = b.h_scal[d].kry_total;
+        qm[d] = 1;
+        state[d] = 2;
+      }
+      if ((rc = upload())) return rc;
+      backup(1);                                  // back to their last converged state
+    }
+    TCAD_HIP_CHECK(hipGetLastError());
+    if (open == 0) break;
+  }
+  TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
   return TCAD_OK;
 }
 
@@ -780,6 +1009,13 @@ static void free_batch(Batch *b) {
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : b->probe_ev) (void)hipEventDestroy(e);
+  if (b->bk) (void)hipFree(b->bk);
+  if (b->d_qmask) (void)hipFree(b->d_qmask);
+  for (auto &g : b->graphs) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
   for (int i = 0; i < 9; ++i)
     if (b->v64[i]) (void)hipFree(b->v64[i]);
   if (b->ir_r) (void)hipFree(b->ir_r);
@@ -1227,6 +1463,34 @@ int tcad_solve_bias(tcad_batch *bp, int32_t max_iter, double tol_update, double 
   return TCAD_OK;
 }
 
+int tcad_solve_queue(tcad_batch *bp, int32_t max_iter, double tol_update, double max_dpsi,
+                     int32_t lin_mode, double lin_rtol, int32_t lin_maxiter, int32_t npmax,
+                     const int32_t *h_npoints, const double *h_contact_V, const double *h_gate_Vg,
+                     double *psi, double *n, double *p, int32_t *h_done, int32_t *h_iters,
+                     int32_t *h_krylov, int32_t *h_status, double *h_I) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi || !n || !p || max_iter <= 0 || lin_maxiter <= 0 || !(lin_rtol > 0.0) ||
+      npmax <= 0 || !h_npoints || !h_done || !h_iters || !h_krylov || !h_status || !h_I ||
+      (b.ncontact && !h_contact_V) || (b.ngate && !h_gate_Vg)) {
+    tcad::set_error("tcad_solve_queue: invalid argument");
+    return TCAD_E_INVALID;
+  }
+  for (int d = 0; d < b.ndev; ++d)
+    if (h_npoints[d] < 0 || h_npoints[d] > npmax) {
+      tcad::set_error("tcad_solve_queue: npoints[d] must be in [0, npmax]");
+      return TCAD_E_INVALID;
+    }
+  const int base_mode = lin_mode & ~(TCAD_LIN_FORCING | TCAD_PRECOND_NODE);
+  if (base_mode != TCAD_LIN_BICGSTAB_F64 && base_mode != TCAD_LIN_BICGSTAB_MIXED) {
+    tcad::set_error("tcad_solve_queue: unknown linear-solve mode");
+    return TCAD_E_INVALID;
+  }
+  return tcad::queue_impl(b, max_iter, tol_update, max_dpsi, lin_mode, lin_rtol, lin_maxiter,
+                          npmax, h_npoints, h_contact_V, h_gate_Vg, psi, n, p, h_done, h_iters,
+                          h_krylov, h_status, h_I);
+}
+
 int tcad_transient_step(tcad_batch *bp, int32_t max_iter, double tol_update, double max_dpsi,
                         int32_t lin_mode, double lin_rtol, int32_t lin_maxiter,
                         const int32_t *h_active, const double *h_dt_s, double theta,
@@ -1386,6 +1650,10 @@ int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
     case TCAD_TUNE_FORCING_FP32:
       if (value != 0.0 && value != 1.0) break;
       b.forcing_fp32 = (int)value;
+      return TCAD_OK;
+    case TCAD_TUNE_GRAPH:
+      if (value != 0.0 && value != 1.0) break;
+      b.use_graph = (int)value;
       return TCAD_OK;
     default:
       break;

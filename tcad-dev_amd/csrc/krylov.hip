@@ -888,6 +888,64 @@ static void launch_iteration(Batch &b, KArgs<T> K, int parity) {
   b.launches += 2;
 }
 
+// Blocks of kGraphIts BiCGStab iterations (an even count, so the iteration
+// parity is the same at every block start) captured once per kernel-argument
+// set as a HIP graph and replayed on the batch stream: one graph launch
+// instead of 4 kGraphIts kernel launches from the host.
+constexpr int kGraphIts = 16;
+
+template <typename T>
+static hipGraphExec_t krylov_graph(Batch &b, const KArgs<T> &K) {
+  const int variant = precond_variant(b) * 2 + (sizeof(T) == 4 ? 1 : 0);
+  // the key: every argument field, written into zeroed storage (no padding
+  // bytes of unknown value)
+  std::vector<unsigned char> key(256, 0);
+  {
+    unsigned char *q = key.data();
+    auto put = [&q](const void *v, size_t n) {
+      std::memcpy(q, v, n);
+      q += n;
+    };
+    put(&K.W.info, sizeof K.W.info); put(&K.W.blk2dev, sizeof K.W.blk2dev);
+    put(&K.W.single, sizeof K.W.single); put(&K.N, sizeof K.N); put(&K.nblk, sizeof K.nblk);
+    put(&K.A, sizeof K.A); put(&K.Dinv, sizeof K.Dinv); put(&K.T4, sizeof K.T4);
+    put(&K.T4s, sizeof K.T4s); put(&K.tile, sizeof K.tile); put(K.v, sizeof K.v);
+    put(&K.b, sizeof K.b); put(&K.part, sizeof K.part); put(&K.scal, sizeof K.scal);
+    put(&K.scale_b, sizeof K.scale_b); put(&K.late_stop, sizeof K.late_stop);
+  }
+  for (const auto &g : b.graphs)
+    if (g.variant == variant && g.key == key) return g.exec;
+  if (b.graphs.size() >= 8) {             // bounded cache: drop the oldest
+    (void)hipGraphExecDestroy(b.graphs.front().exec);
+    (void)hipGraphDestroy(b.graphs.front().graph);
+    b.graphs.erase(b.graphs.begin());
+  }
+  if (!b.capture_stream &&
+      hipStreamCreateWithFlags(&b.capture_stream, hipStreamNonBlocking) != hipSuccess)
+    return nullptr;
+  hipStream_t user = b.stream;
+  const long long launches = b.launches;
+  b.stream = b.capture_stream;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ok = hipStreamBeginCapture(b.capture_stream, hipStreamCaptureModeRelaxed) == hipSuccess;
+  if (ok) {
+    for (int i = 0; i < kGraphIts; ++i) launch_iteration<T>(b, K, i & 1);
+    ok = hipStreamEndCapture(b.capture_stream, &graph) == hipSuccess && graph;
+  }
+  b.stream = user;
+  b.launches = launches;
+  if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (!ok) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    b.use_graph = 0;                    // capture refused: plain launches from now on
+    return nullptr;
+  }
+  b.graphs.push_back({key, variant, graph, exec});
+  return exec;
+}
+
 // Host driver: reset, init, then chunks of iterations with one scalar
 // read-back per chunk until every device is done or maxit is reached
 // (one extra stop-test kernel then records the final state).
@@ -907,7 +965,23 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
   int next = b.kry_hint > 0 ? std::max(chunk, (3 * b.kry_hint) / 4) : chunk;
   int step = b.kry_hint > 0 ? 16 : chunk;
   while (true) {
-    for (int c = 0; c < next && it < maxit; ++c, ++it) launch_iteration<T>(b, K, it & 1);
+    int c = 0;
+    if (b.use_graph && b.probe < 0) {
+      if ((it & 1) && c < next && it < maxit) {
+        launch_iteration<T>(b, K, it & 1);
+        ++c;
+        ++it;
+      }
+      while (next - c >= kGraphIts && it + kGraphIts <= maxit) {
+        hipGraphExec_t g = krylov_graph<T>(b, K);
+        if (!g) break;
+        TCAD_HIP_CHECK(hipGraphLaunch(g, b.stream));
+        b.launches += 4 * kGraphIts;
+        c += kGraphIts;
+        it += kGraphIts;
+      }
+    }
+    for (; c < next && it < maxit; ++c, ++it) launch_iteration<T>(b, K, it & 1);
     if (it >= maxit) {
       KArgs<T> K1 = K;
       K1.parity = it & 1;

@@ -207,6 +207,8 @@ struct Batch {
   double *beq = nullptr;
   double *veq[6] = {};
   int *d_flag = nullptr;                         // [0] any, [1 + d] per-device error flag
+  double *bk = nullptr;                          // point queue: last converged states (3 planes)
+  int *d_qmask = nullptr;                        // point queue: per-device selection
   int *d_mask = nullptr;                         // per-device selection of the next solve
   const double *last_n = nullptr, *last_p = nullptr;   // state of the last assembly
   const double *last_psi = nullptr;
@@ -218,6 +220,18 @@ struct Batch {
   // forcing solves whose target is loose (every active device's eta >=
   // 1e-5) run BiCGStab on the FP32-stored matrix inside the FP64 refinement
   int forcing_fp32 = 0;
+  // BiCGStab iterations replayed as captured HIP graphs (tcad_set_tuning
+  // TCAD_TUNE_GRAPH): blocks of kGraphIts iterations captured on a private
+  // stream, one executable per kernel-argument set
+  int use_graph = 0;
+  hipStream_t capture_stream = nullptr;
+  struct GraphEntry {
+    std::vector<unsigned char> key;  // the kernel arguments it was captured with
+    int variant;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
   // in-solver timing of one BiCGStab kernel (tcad_probe_kernel): every
   // launch of kernel `probe` (0 SpMV, 1 alpha/s, 2 second SpMV, 3 update;
   // -1 off) is bracketed by a pair of events on the batch stream

@@ -31,6 +31,7 @@ TUNE_TILE_ROWS = 1
 TUNE_FORCING_SLOPE = 2
 TUNE_FORCING_MAX = 3
 TUNE_FORCING_FP32 = 4
+TUNE_GRAPH = 5
 
 CSR_BICGSTAB = 0
 CSR_GMRES = 1
@@ -115,6 +116,10 @@ def lib():
                              ctypes.c_int32, ctypes.c_double, ctypes.c_int32, c_int32_p,
                              vp, vp, vp, c_int32_p, c_int32_p, c_int32_p, c_int32_p],
                             ctypes.c_int),
+        "tcad_solve_queue": ([vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                              c_int32_p, c_double_p, c_double_p, vp, vp, vp, c_int32_p,
+                              c_int32_p, c_int32_p, c_int32_p, c_double_p], ctypes.c_int),
         "tcad_transient_step": ([vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_int32, ctypes.c_double, ctypes.c_int32, c_int32_p,
                                  c_double_p, ctypes.c_double, vp, vp, vp, vp, vp, vp, vp,
@@ -187,5 +192,6 @@ def exported_symbols():
             "tcad_solve_equilibrium", "tcad_solve_bias", "tcad_terminal_currents",
             "tcad_edge_currents", "tcad_surface_mobility", "tcad_time_kernel",
             "tcad_transient_step", "tcad_continuity_residuals", "tcad_y_parameters",
+            "tcad_solve_queue",
             "tcad_probe_kernel", "tcad_probe_read", "tcad_set_tuning", "tcad_csr_solve", "tcad_csr_solve_noalloc",
             "tcad_launch_count", "tcad_last_error", "tcad_version"]

@@ -56,11 +56,13 @@ class NewtonOptions:
     verbose: bool = False
     # run-time tuning (tcad_set_tuning): forcing eta = clamp(forcing_slope *
     # err, linsolve_rtol, forcing_max); forcing_fp32: loose forcing solves on
-    # the FP32-stored matrix; tile_rows "auto" | "reg" | "mem"
+    # the FP32-stored matrix; tile_rows "auto" | "reg" | "mem"; graph: BiCGStab
+    # iterations replayed from a captured HIP graph
     forcing_slope: float = 0.01
     forcing_max: float = 1e-4
     forcing_fp32: bool = False
     tile_rows: str = "auto"
+    graph: bool = False
 
     def apply_tuning(self, handle):
         L = _lib.lib()
@@ -75,6 +77,8 @@ class NewtonOptions:
                    "tcad_set_tuning")
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FORCING_FP32,
                                      1.0 if self.forcing_fp32 else 0.0), "tcad_set_tuning")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_GRAPH, 1.0 if self.graph else 0.0),
+                   "tcad_set_tuning")
 
     def lin_mode(self):
         m = self.linsolve
@@ -410,6 +414,61 @@ class GPUBatch:
             self.n.data_ptr(), self.p.data_ptr(), _lib.iptr(its), _lib.iptr(conv),
             _lib.iptr(kry), _lib.iptr(status)), "tcad_solve_bias")
         return its, conv, kry, status
+
+    def solve_queue(self, opts, point_lists):
+        """Every device's list of bias points (dicts of terminal voltages,
+        each point keeping the terminals it does not name at their previous
+        values) solved in one tcad_solve_queue call: a device goes on to
+        its next point as soon as it converges at the present one, and stops
+        at its first failure with its last converged state restored.
+        Returns per device (points converged, Newton iterations per point,
+        Krylov iterations per point, outcome code of the point it stopped at,
+        currents per converged point as {contact: I})."""
+        nd = self.ndev
+        if len(point_lists) != nd:
+            raise ValueError(f"one point list per device ({nd})")
+        npmax = max((len(pl) for pl in point_lists), default=0)
+        if npmax == 0:
+            return [(0, [], [], _lib.NEWTON_OK, []) for _ in range(nd)]
+        nc, ng = len(self.contacts), len(self.gates)
+        V = np.zeros((npmax, max(nc, 1)))
+        G = np.zeros((npmax, max(ng, 1)))
+        cidx = {dn: i for i, dn in enumerate(self.contacts)}
+        gidx = {dn: i for i, dn in enumerate(self.gates)}
+        for d, (s, pl) in enumerate(zip(self.setups, point_lists)):
+            vals = {nm: (bc.V if hasattr(bc, "V") else bc.Vg) for nm, bc in s.bcs.items()}
+            for k, pt in enumerate(pl):
+                vals.update({nm: float(v) for nm, v in pt.items()})
+                for nm, v in vals.items():
+                    if (d, nm) in cidx:
+                        V[k, cidx[(d, nm)]] = v
+                    elif (d, nm) in gidx:
+                        G[k, gidx[(d, nm)]] = v
+        npts = np.array([len(pl) for pl in point_lists], dtype=np.int32)
+        done = np.zeros(nd, dtype=np.int32)
+        its = np.zeros(nd * npmax, dtype=np.int32)
+        kry = np.zeros(nd * npmax, dtype=np.int32)
+        status = np.zeros(nd, dtype=np.int32)
+        I = np.zeros(npmax * max(nc, 1))
+        self.set_bias()
+        opts.apply_tuning(self._handle)
+        V = np.ascontiguousarray(V)
+        G = np.ascontiguousarray(G)
+        _lib.check(_lib.lib().tcad_solve_queue(
+            self._handle, opts.max_iter, opts.tol_update, opts.max_dpsi, opts.lin_mode(),
+            opts.linsolve_rtol, opts.lin_maxiter, npmax, _lib.iptr(npts), _lib.dptr(V.ravel()),
+            _lib.dptr(G.ravel()), self.psi.data_ptr(), self.n.data_ptr(), self.p.data_ptr(),
+            _lib.iptr(done), _lib.iptr(its), _lib.iptr(kry), _lib.iptr(status), _lib.dptr(I)),
+            "tcad_solve_queue")
+        I = I.reshape(npmax, max(nc, 1))
+        out = []
+        for d in range(nd):
+            cur = [{nm: float(I[k, cidx[(d, nm)]]) for dd, nm in self.contacts if dd == d}
+                   for k in range(int(done[d]))]
+            k1 = min(int(done[d]) + 1, len(point_lists[d]))
+            out.append((int(done[d]), its[d * npmax:d * npmax + k1].tolist(),
+                        kry[d * npmax:d * npmax + k1].tolist(), int(status[d]), cur))
+        return out
 
     def terminal_currents(self):
         out = np.zeros(max(1, len(self.contacts)), dtype=np.float64)

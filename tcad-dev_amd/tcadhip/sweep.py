@@ -116,6 +116,21 @@ class Track:
             self.next += 1
         return final
 
+    def accept_point(self, its, kry):
+        """The next requested point converged inside a point queue."""
+        target = self._vec(self.points[self.next])
+        self.trial = (target, True, 0.0)
+        _apply_bias(self.setup, self.names, target)
+        return self.accept(its, kry)
+
+    def reject_point(self, status, kry):
+        """The next requested point failed inside a point queue (the library
+        restored the last converged state): back off from it."""
+        target = self._vec(self.points[self.next])
+        dist = float(np.max(np.abs(target - self.cur))) if target.size else 0.0
+        self.trial = (target, True, dist)
+        self.reject(status, kry)
+
     def reject(self, status, kry):
         """The trial failed: back off (the caller restores the state)."""
         _, _, lim = self.trial
@@ -188,9 +203,17 @@ class BatchedSweep:
     devices must be `setups` in order."""
 
     def __init__(self, setups, sweeps, opts=None, batch=None, max_step=None, min_step=None,
-                 growth=1.5, shrink=0.5, max_attempts=64, compact=0.5):
+                 growth=1.5, shrink=0.5, max_attempts=64, compact=0.5, queue=True,
+                 queue_depth=None):
         self.setups = list(setups)
         self.compact = compact
+        # point queue (tcad_solve_queue): every device runs through its
+        # requested points in one library call without a barrier between
+        # devices; a device that fails a point falls back to the rounds
+        # below (step backoff).  Only without a step limit: the queue holds
+        # the requested points themselves.
+        self.queue = queue and max_step is None
+        self.queue_depth = queue_depth
         if len(sweeps) != len(self.setups):
             raise ValueError("one bias list per device")
         self.opts = opts or NewtonOptions()
@@ -213,6 +236,8 @@ class BatchedSweep:
                     t.result.failed = "equilibrium did not converge"
         ids = list(range(len(self.tracks)))
         self.rebuilds = 0
+        if self.queue:
+            self.queue_pass(B, self.tracks, t0)
         while any(t.open for t in self.tracks):
             open_ids = [i for i in ids if self.tracks[i].open]
             if (self.compact and len(open_ids) < len(ids)
@@ -224,6 +249,32 @@ class BatchedSweep:
             reached = solve_round(B, tracks, opts)
             record_currents(B, tracks, reached, time.perf_counter() - t0)
         return self.results
+
+    def queue_pass(self, B, tracks, t0):
+        """One tcad_solve_queue call over the open tracks (tracks[d] belongs
+        to device d of batch B), up to queue_depth points each."""
+        lists = []
+        for t in tracks:
+            if t is None or not t.open or t.step is not None:
+                lists.append([])
+                continue
+            pts = t.points[t.next:]
+            lists.append(pts[:self.queue_depth] if self.queue_depth else pts)
+        if not any(lists):
+            return
+        self.rounds += 1
+        with B.on_stream():
+            res = B.solve_queue(self.opts, lists)
+        t_now = time.perf_counter() - t0
+        for t, pl, (done, its, kry, status, cur) in zip(tracks, lists, res):
+            if not pl:
+                continue
+            for k in range(done):
+                t.accept_point(its[k], kry[k])
+                t.result.currents.append(cur[k])
+                t.result.t_done = t_now
+            if done < len(pl):
+                t.reject_point(status, kry[done] if done < len(kry) else 0)
 
     def _nodes(self, idx):
         return sum(self.setups[i].N for i in idx)

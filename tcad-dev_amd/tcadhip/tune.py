@@ -12,7 +12,9 @@ floor, so the terminal currents agree to rounding):
     tight and converging ones stay on the FP64 matrix);
   * where the tile-inverse rows live in the BiCGStab kernels: loaded into
     registers before the reductions (fewer waves per SIMD) or read at the
-    point of use (full occupancy).
+    point of use (full occupancy);
+  * whether the BiCGStab iterations go out as replays of a captured HIP
+    graph (tried last, on the best configuration so far).
 
 A candidate is measured by solving one bias point from the same saved
 state (psi, n, p restored on the device before each trial), twice, the
@@ -44,6 +46,7 @@ CANDIDATES = (
          tile_rows="reg"),
 )
 
+_KNOBS = ("forcing", "forcing_slope", "forcing_max", "forcing_fp32", "tile_rows", "graph")
 _cache = {}
 
 
@@ -68,7 +71,16 @@ def tune(dev, bias, opts=None, candidates=CANDIDATES, repeats=2, say=None):
     rows = []
     ref_I = ref_its = None
     try:
-        for cand in candidates:
+        for cand in list(candidates) + [None]:
+            if cand is None:
+                # second stage: the winner so far, its iterations issued as
+                # replays of a captured HIP graph
+                good = [r for r in rows if r["accepted"]]
+                if not good:
+                    break
+                w = min(good, key=lambda r: r["seconds"])
+                cand = {k: w[k] for k in _KNOBS if k in w}
+                cand["graph"] = True
             o = replace(base, **cand)
             best_t, res = None, None
             for _ in range(repeats):
@@ -102,8 +114,7 @@ def tune(dev, bias, opts=None, candidates=CANDIDATES, repeats=2, say=None):
         torch.cuda.synchronize()
     good = [r for r in rows if r["accepted"]]
     win = min(good, key=lambda r: r["seconds"])
-    best = {k: win[k] for k in ("forcing", "forcing_slope", "forcing_max", "forcing_fp32",
-                                "tile_rows") if k in win}
+    best = {k: win[k] for k in _KNOBS if k in win}
     report = {"candidates": rows, "chosen": best, "point": dict(bias)}
     _cache[key] = (best, report)
     return replace(base, **best), report
