@@ -45,9 +45,20 @@ HBM_PEAK_GBPS = 8000.0
 # (4 x 3 FP64), reads the node's 3 rows of its tile inverse (36 planes of
 # 4 FP16 + 3 FP32 scales)
 K_BICG_U_BYTES_PER_NODE = 7 * 24 + 4 * 24 + 36 * 8 + 3 * 4
-# launches shorter than this are the early exits issued after a solve
-# converged (they return before touching the vectors): not part of the average
-PROBE_MIN_US = 2.5
+# algorithmic bytes per node of the four 3D BiCGStab kernels (39 FP64 stencil
+# entries per node; tile rows 300 B): k_bicg_spmv, k_bicg_s, k_bicg_spmv_t, k_bicg_u
+PROBE_BYTES_PER_NODE = {0: 39 * 8 + 3 * 24, 1: 4 * 24 + 300, 2: 39 * 8 + 4 * 24,
+                        3: K_BICG_U_BYTES_PER_NODE}
+
+
+def probe_min_us(kernel, nodes):
+    """Launches of the probed kernel shorter than this are the early exits
+    issued after a solve converged (they return before reading the vectors):
+    a launch that moved its algorithmic bytes in less time would have run
+    at twice the HBM peak.  Measured with HIP events on the 64x48x48
+    tri-gate such launches take 2.5-5 us, working k_bicg_u launches 10 us
+    and more; the cut is 5.2 us there."""
+    return PROBE_BYTES_PER_NODE[kernel] * nodes / (2 * HBM_PEAK_GBPS * 1e9) * 1e6
 PROBE_KERNELS = {0: "k_bicg_spmv<double>", 1: "k_bicg_s_tmem<double>",
                  2: "k_bicg_spmv_t<double>", 3: "k_bicg_u_tmem<double>"}
 
@@ -85,7 +96,8 @@ def trigate_step(devs, specs, opts, eq_opts, lat=None, record=None):
 
 
 def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, precond="tile",
-                  tune=True, forcing_slope=0.01, forcing_max=1e-4):
+                  tune=True, forcing_slope=0.01, forcing_max=1e-4, tile_rows="auto",
+                  probe_cuts=()):
     from tcadhip import configs
     from tcadhip.device import NewtonOptions, device_from_spec
     specs = [configs.config3_trigate(0.05), configs.config3_trigate(0.7)]
@@ -94,7 +106,8 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, prec
             s.sweep = s.sweep[:points]
     devs = [device_from_spec(s) for s in specs]
     opts = NewtonOptions(linsolve=mode, forcing=forcing, precond=precond,
-                         forcing_slope=forcing_slope, forcing_max=forcing_max)
+                         forcing_slope=forcing_slope, forcing_max=forcing_max,
+                         tile_rows=tile_rows)
     eq_opts = NewtonOptions(max_iter=400)
     tuning = None
     if tune:
@@ -122,23 +135,36 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, prec
     if probe is not None and probe >= 0:
         # in-solver duration of one kernel: one more (untimed) step with HIP
         # events around every launch of it
-        import ctypes
-        L = _lib()
-        for d in devs:
-            L.check(L.lib().tcad_probe_kernel(d._gpu().handle, probe), "tcad_probe_kernel")
-        trigate_step(devs, specs, opts, eq_opts)
-        tot_n, tot_us = 0, 0.0
-        for d in devs:
-            n = ctypes.c_int64()
-            us = ctypes.c_double()
-            L.check(L.lib().tcad_probe_read(d._gpu().handle, PROBE_MIN_US, ctypes.byref(n),
-                                            ctypes.byref(us)), "tcad_probe_read")
-            L.check(L.lib().tcad_probe_kernel(d._gpu().handle, -1), "tcad_probe_kernel")
-            tot_n += n.value
-            tot_us += n.value * us.value
-        out["probe"] = dict(kernel=probe, launches=tot_n,
-                            avg_us=tot_us / tot_n if tot_n else None)
+        n, us = probe_step(devs, specs, opts, eq_opts, probe, None)
+        out["probe"] = dict(kernel=probe, launches=n, avg_us=us)
+        # the same, one step per cut: how the average depends on where the
+        # early exits are cut off
+        out["probe_cuts"] = [dict(cut_us=c, launches=m, avg_us=round(a, 3) if a else None)
+                             for c in probe_cuts
+                             for m, a in [probe_step(devs, specs, opts, eq_opts, probe, c)]]
     return out
+
+
+def probe_step(devs, specs, opts, eq_opts, probe, cut_us):
+    """One untimed step with HIP start/stop events on every launch of
+    BiCGStab kernel `probe`; returns (launches, average us) over the
+    launches longer than cut_us (default: probe_min_us)."""
+    import ctypes
+    L = _lib()
+    for d in devs:
+        L.check(L.lib().tcad_probe_kernel(d._gpu().handle, probe), "tcad_probe_kernel")
+    trigate_step(devs, specs, opts, eq_opts)
+    tot_n, tot_us = 0, 0.0
+    for d in devs:
+        n = ctypes.c_int64()
+        us = ctypes.c_double()
+        cut = probe_min_us(probe, d.N) if cut_us is None else cut_us
+        L.check(L.lib().tcad_probe_read(d._gpu().handle, cut, ctypes.byref(n), ctypes.byref(us)),
+                "tcad_probe_read")
+        L.check(L.lib().tcad_probe_kernel(d._gpu().handle, -1), "tcad_probe_kernel")
+        tot_n += n.value
+        tot_us += n.value * us.value
+    return tot_n, (tot_us / tot_n if tot_n else None)
 
 
 def reduced_trigate_parity(opts):
@@ -416,10 +442,11 @@ def bench_batch(lanes=1, sample=8, modes=None):
     modes = modes or {"bicgstab_mixed": NewtonOptions(linsolve="bicgstab_mixed"),
                       "bicgstab": NewtonOptions()}
     for mode, mopts in modes.items():
+        mopts, kw = mopts if isinstance(mopts, tuple) else (mopts, {})
         setups = [setup_from_spec(s) for s in specs]
         _sync()
         t0 = time.perf_counter()
-        r, bss = sweep.run_lanes(setups, [s.sweep for s in specs], mopts, lanes=lanes)
+        r, bss = sweep.run_lanes(setups, [s.sweep for s in specs], mopts, lanes=lanes, **kw)
         _sync()
         dt = time.perf_counter() - t0
         res[mode] = dict(seconds=dt, results=r, rounds=sum(b.rounds for b in bss),
@@ -599,6 +626,12 @@ def main():
     ap.add_argument("--forcing-slope", type=float, default=0.01)
     ap.add_argument("--forcing-max", type=float, default=1e-4)
     ap.add_argument("--precond", default="tile", choices=("tile", "node"))
+    ap.add_argument("--probe-cuts", default="",
+                    help="comma-separated early-exit cuts (us): one more probe step per cut, "
+                         "reported as probe_cuts")
+    ap.add_argument("--tile-rows", default="auto", choices=("auto", "reg", "mem"),
+                    help="tile-inverse rows in registers or read at the point of use "
+                         "(untuned runs)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (fixed default solver options)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -609,7 +642,7 @@ def main():
                     help="extra workloads (comma-separated; '' for none)")
     ap.add_argument("--batch-variants", default="",
                     help="only the batched diodes, with these option sets (comma-separated of "
-                         "bicgstab, forcing, forcing_graph, graph)")
+                         "bicgstab, forcing, forcing_graph, graph, queue, queue2, queue4)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (the reference "
                          "kernel for scaling HBM traffic counters) and exit")
@@ -639,7 +672,10 @@ def main():
                 "forcing": NewtonOptions(forcing=True, forcing_slope=0.1, forcing_max=1e-3),
                 "forcing_graph": NewtonOptions(forcing=True, forcing_slope=0.1,
                                                forcing_max=1e-3, graph=True),
-                "graph": NewtonOptions(graph=True)}
+                "graph": NewtonOptions(graph=True),
+                "queue": (NewtonOptions(), dict(queue=True)),
+                "queue2": (NewtonOptions(), dict(queue=True, queue_depth=2)),
+                "queue4": (NewtonOptions(), dict(queue=True, queue_depth=4))}
         out = bench_batch(modes={k: allm[k] for k in args.batch_variants.split(",")})
         print(json.dumps(out), flush=True)
         return
@@ -647,7 +683,9 @@ def main():
     _say(f"tri-gate 64x48x48, {args.mode}: warmup {args.warmup}, steps {args.steps}")
     r = bench_trigate(args.steps, args.warmup, args.mode, args.points, args.probe,
                       args.forcing, args.precond, tune=not args.no_tune,
-                      forcing_slope=args.forcing_slope, forcing_max=args.forcing_max)
+                      forcing_slope=args.forcing_slope, forcing_max=args.forcing_max,
+                      tile_rows=args.tile_rows,
+                      probe_cuts=[float(c) for c in args.probe_cuts.split(",") if c])
     npts = r["points_per_step"]
     value = npts * args.steps / r["seconds"]
     _say(f"tri-gate: {value:.3f} points/s")
@@ -697,12 +735,15 @@ def main():
                 "avg_launch_us_in_solver": round(pr["avg_us"], 3),
                 "launches_timed": pr["launches"],
                 "timing": (f"kernel start/end HIP events (hipExtLaunchKernelGGL) on every launch "
-                           f"of the kernel in one untimed step; launches under {PROBE_MIN_US} us "
-                           f"(early exits after convergence) excluded"),
+                           f"of the kernel in one untimed step; launches under "
+                           f"{probe_min_us(3, N):.2f} us (early exits after convergence: "
+                           f"bytes / (2 x peak)) excluded"),
                 "copy_ceiling_GBps": copy_ceiling()}
         else:
             line["probe"] = {"kernel": kname, "avg_us": round(pr["avg_us"], 3),
                              "launches": pr["launches"]}
+        if r.get("probe_cuts"):
+            line["probe_cuts"] = r["probe_cuts"]
     if not args.no_parity:
         _say("reduced tri-gate vs the reference fixtures")
         line["parity_reduced_24x18x14_vs_reference"] = reduced_trigate_parity(r["opts"])

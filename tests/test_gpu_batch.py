@@ -48,14 +48,24 @@ def _alone(spec, opts):
     return np.array(its), np.array(cur), dev
 
 
-@pytest.mark.parametrize("mode", ["bicgstab", "bicgstab_mixed"])
-def test_batched_sweep_equals_alone(mode):
+DRIVERS = {"rounds": dict(queue=False), "queue": dict(queue=True),
+           "queue2": dict(queue=True, queue_depth=2)}
+
+
+@pytest.mark.parametrize("mode,driver", [("bicgstab", "rounds"), ("bicgstab", "queue"),
+                                         ("bicgstab", "queue2"), ("bicgstab_mixed", "rounds"),
+                                         ("bicgstab_mixed", "queue")])
+def test_batched_sweep_equals_alone(mode, driver):
+    """Rounds (one masked solve per point over every device) and the point
+    queue (each device through its points without a barrier, whole list or
+    two points per pass with compaction between) give every device the bits
+    it gets alone."""
     configs, sweep, _lib, GPUBatch, NewtonOptions, _, setup_from_spec = _mods()
     specs = _small_diodes(6)
-    _say(f"[batch-sweep] 6 diodes, {mode}: batched continuation vs alone")
+    _say(f"[batch-sweep] 6 diodes, {mode}, {driver}: batched continuation vs alone")
     opts = NewtonOptions(linsolve=mode)
     setups = [setup_from_spec(s) for s in specs]
-    res = sweep.BatchedSweep(setups, [s.sweep for s in specs], opts).run()
+    res = sweep.BatchedSweep(setups, [s.sweep for s in specs], opts, **DRIVERS[driver]).run()
     for d, s in enumerate(specs):
         R = res[d]
         assert not R.failed, R.failed
