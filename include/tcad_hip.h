@@ -398,7 +398,17 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
  *   TCAD_TUNE_GRAPH          1: BiCGStab iterations are issued as replays of
  *                            a captured HIP graph of 16 iterations (captured
  *                            once per kernel-argument set on a private
- *                            stream, launched on the batch stream) [0].
+ *                            stream, launched on the batch stream) [0];
+ *   TCAD_TUNE_IR_CONTINUE    refinement sweeps after the first continue the
+ *                            previous sweep's BiCGStab (search direction and
+ *                            shadow residual kept, the iterated residual
+ *                            replaced by the FP64 true residual) instead of
+ *                            restarting it: 0 never, 1 for inner solves on
+ *                            the FP32-stored matrix, 2 always [1];
+ *   TCAD_TUNE_FP32_INNER     the residual reduction (1e-7..1e-2) an inner
+ *                            solve on the FP32-stored matrix stops at, at
+ *                            least, in each refinement sweep of a Newton
+ *                            solve [1e-6].
  * Has no reference counterpart (the reference's Krylov tolerance is fixed,
  * linsolve.py:725-727). */
 #define TCAD_TUNE_TILE_ROWS 1
@@ -406,6 +416,8 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
 #define TCAD_TUNE_FORCING_MAX 3
 #define TCAD_TUNE_FORCING_FP32 4
 #define TCAD_TUNE_GRAPH 5
+#define TCAD_TUNE_IR_CONTINUE 6
+#define TCAD_TUNE_FP32_INNER 7
 int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
 
 /* Bench instrumentation, in-solver kernel timing: from this call on,

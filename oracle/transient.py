@@ -141,7 +141,7 @@ def step_residual_jacobian(device, psi, n, p, voltages_new, n_old, p_old, F_old_
 
 
 def newton_step(device, psi0, n0, p0, voltages_new, F_old_n, F_old_p, dV, dt_s, theta, k_free,
-                opts, equilibrated=False):
+                opts, equilibrated=False, stats=None):
     """transient2d.py:138-183 / transient3d.py:152-219: Newton on the step
     residual with a direct solve, psi clip, Armijo backtracking on
     0.5 |F|^2 (up to 40 halvings), density clip; err from the clipped
@@ -150,7 +150,10 @@ def newton_step(device, psi0, n0, p0, voltages_new, F_old_n, F_old_p, dV, dt_s, 
     equilibrated=True (not a reference method) solves the same system
     after column scaling (1, n, p) and row equilibration: the exact step
     the GPU's equilibrated Krylov solve converges to, where the
-    reference's SuperLU on the unscaled Jacobian loses digits."""
+    reference's SuperLU on the unscaled Jacobian loses digits.
+    stats (test instrumentation, not in the reference): a dict that
+    receives the merit 0.5 |F|^2 at the step's start ("start") and at the
+    state the loop ends on ("end")."""
     psi, n, p = psi0.copy(), n0.copy(), p0.copy()
     shape = psi.shape
     for it in range(opts.max_iter):
@@ -161,6 +164,9 @@ def newton_step(device, psi0, n0, p0, voltages_new, F_old_n, F_old_p, dV, dt_s, 
         dn = du[1::3].reshape(shape)
         dp = du[2::3].reshape(shape)
         base = 0.5 * float(np.dot(F, F))
+        if stats is not None:
+            stats.setdefault("start", base)
+            stats["end"] = base
         lam = 1.0
         for _ in range(40):
             psi_t = psi + lam * dpsi
@@ -174,6 +180,8 @@ def newton_step(device, psi0, n0, p0, voltages_new, F_old_n, F_old_p, dV, dt_s, 
             lam *= 0.5
         else:
             lam = 0.0
+        if stats is not None and lam > 0.0:
+            stats["end"] = merit
         n_prev, p_prev = n, p
         psi = psi + lam * dpsi
         n = np.clip(n + lam * dn, 0.1 * n, 10.0 * n)
@@ -216,6 +224,7 @@ def solve_transient(device, waveforms, t_end, dt0, theta=1.0, opts=None, dt_min=
     psi, n, p = device.psi.copy(), device.n.copy(), device.p.copy()
     times, psi_h, n_h, p_h, dt_h, its = [t], [psi.copy()], [n.copy()], [p.copy()], [], []
     currents = {nm: [] for nm in contact_names}
+    merits = []
 
     def record(psi_, n_, p_, volts):
         *_, F_n, F_p = device._residual_jacobian(psi_, n_, p_, volts)
@@ -240,9 +249,11 @@ def solve_transient(device, waveforms, t_end, dt0, theta=1.0, opts=None, dt_min=
         F_old_n = F_old_p = None
         if theta != 1.0:
             *_, F_old_n, F_old_p = device._residual_jacobian(psi, n, p, voltages_at(t))
+        st = {}
         psi1, n1, p1, ok, nit = newton_step(device, psi, n, p, v_new, F_old_n, F_old_p, dV, step,
-                                            theta, k_free, opts, equilibrated)
+                                            theta, k_free, opts, equilibrated, st)
         if ok:
+            merits.append((st["start"], st["end"]))
             t = t_new
             psi, n, p = psi1, n1, p1
             times.append(t)
@@ -264,4 +275,9 @@ def solve_transient(device, waveforms, t_end, dt0, theta=1.0, opts=None, dt_min=
     final = voltages_at(t)
     for nm in contact_names:
         device.bcs[nm].V = final[nm]
-    return TransientResult(times, psi_h, n_h, p_h, currents, dt_h, its)
+    res = TransientResult(times, psi_h, n_h, p_h, currents, dt_h, its)
+    # test instrumentation: merit 0.5 |F|^2 at the start and the end of every
+    # accepted step
+    res.start_merit = np.array([m[0] for m in merits])
+    res.end_merit = np.array([m[1] for m in merits])
+    return res

@@ -617,7 +617,7 @@ static int bias_newton_step(Batch &b, const UArgs &U, const NewtonCfg &c, int ma
   int rc;
   const dim3 gd((b.ndev + 63) / 64), td(64);
   const bool fp32_now = c.mixed || (c.loose32 && eta_next >= 1e-5);
-  const double inner = fp32_now ? 1e-6 : c.rtol;
+  const double inner = fp32_now ? b.fp32_inner : c.rtol;
   if ((rc = launch_surface_mobility(b, psi, true))) return rc;
   if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, c.mixed || c.loose32, true))) return rc;
   if (b.tile && (rc = launch_tile_setup(b, true))) return rc;
@@ -1654,6 +1654,14 @@ int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
     case TCAD_TUNE_GRAPH:
       if (value != 0.0 && value != 1.0) break;
       b.use_graph = (int)value;
+      return TCAD_OK;
+    case TCAD_TUNE_IR_CONTINUE:
+      if (value != 0.0 && value != 1.0 && value != 2.0) break;
+      b.ir_cont = (int)value;
+      return TCAD_OK;
+    case TCAD_TUNE_FP32_INNER:
+      if (!(value >= 1e-7 && value <= 1e-2)) break;
+      b.fp32_inner = value;
       return TCAD_OK;
     default:
       break;

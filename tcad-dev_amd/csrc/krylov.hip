@@ -503,6 +503,35 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
   (void)lead;
   const NodeHood H = hood(I, blockIdx.x);
   const long long N = K.N;
+  if (S.cont) {
+    // continue the previous sweep's process on this sweep's rhs: x = 0,
+    // r = b / nrm (the FP64 true residual replaces the iterated one), p and
+    // phat = M^-1 p rescaled from the previous rhs scale to this one
+    // (f = nrm_prev / nrm), rhat kept; partials (r, r) at quantity 1 and
+    // (rhat, r), the first rho, at quantity 3.  One device per workgroup:
+    // the branch is uniform.
+    const double sc = 1.0 / S.nrm, f = S.nrm_prev * sc;
+    double q[2] = {0.0, 0.0}, o[2];
+    if (H.valid) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const long long ix = c * N + H.g;
+        const double bv = K.b[ix] * sc, rh = K.v[VRH][ix];
+        K.v[VX][ix] = 0.0;
+        K.v[VR][ix] = bv;
+        K.v[VP][ix] = f * K.v[VP][ix];
+        K.v[VPH][ix] = f * K.v[VPH][ix];
+        q[0] += bv * bv;
+        q[1] += rh * bv;
+      }
+    }
+    block_sums<2>(q, sh, o);
+    if (threadIdx.x == 0) {
+      K.part[(long long)K.nblk + blockIdx.x] = o[0];
+      K.part[3 * (long long)K.nblk + blockIdx.x] = o[1];
+    }
+    return;
+  }
   const bool use_tile = PV != PV_NODE && K.tile;
   TileRows TR;
   if (PV == PV_TILE_REG) tile_load(K.T4, K.T4s, N, H.g, use_tile && H.valid, TR);
@@ -619,15 +648,18 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
   }
   // the device scalars this kernel reads are written by earlier launches
   // only: read with the vectors, before the reduction
-  const int s_it = S.it, s_maxit = S.maxit;
+  const int s_it = S.it, s_maxit = S.maxit, s_cont = S.cont;
   const double s_bb = S.bb, s_rho = S.rho2[K.parity], s_tol2 = S.tol2;
   KSTOP
-  double red[2];
-  device_sums<2>(K.part, K.nblk, 1, I, sh, red);
-  const double rr = red[0], rv = red[1];
   const bool first = s_it == 0;
+  // the first iteration of a continued solve takes rho = (rhat, r) from
+  // quantity 3 (after a fresh start rhat = r and rho = (r, r))
+  double red[3];
+  if (first && s_cont) device_sums<3>(K.part, K.nblk, 1, I, sh, red);
+  else device_sums<2>(K.part, K.nblk, 1, I, sh, red);
+  const double rr = red[0], rv = red[1];
   const double bb = first ? rr : s_bb;
-  const double rho = first ? rr : s_rho;
+  const double rho = first ? (s_cont ? red[2] : rr) : s_rho;
   if (rr <= s_tol2 * bb || s_it >= s_maxit || !isfinite(rr) || rho == 0.0) {
     if (lead) {
       S.rr = rr;
@@ -803,15 +835,24 @@ TCAD_PV_LAUNCH(k_bicg_u, 3)
 // are marked done.  per_dev: each device's own target S.eta2 (not below
 // floor2) instead of tol2.
 __global__ void k_krylov_reset(Scal *scal, int ndev, double tol2, int maxit, int per_dev,
-                               double floor2) {
+                               double floor2, int cont) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndev) return;
   Scal &S = scal[d];
+  S.cont = (cont && S.active) ? S.cont_ok : 0;
   S.rho2[0] = S.rho2[1] = 1.0;
   S.alpha = 1.0;
   S.omega = 1.0;
   S.it = 0;
-  S.tol2 = per_dev ? fmax(S.eta2, floor2) : tol2;
+  // per_dev (a refinement sweep of a Newton solve): the device's target
+  // eta; with a floor (inner solves on the FP32-stored matrix, several
+  // sweeps per solve) the reduction that brings this sweep's residual
+  // (S.rr, from the refinement check) to eta relative to the Newton rhs
+  // (S.bb), not below the floor.  FP64 solves keep eta per sweep: a second
+  // sweep there follows a breakdown, and its Newton path is the one the
+  // fixtures pin.
+  const double t2 = (floor2 > 0.0 && S.rr > 0.0) ? S.eta2 * (S.bb / S.rr) : S.eta2;
+  S.tol2 = per_dev ? fmax(t2, floor2) : tol2;
   S.maxit = maxit;
   if (S.active) {
     S.status = 0;
@@ -951,9 +992,9 @@ static hipGraphExec_t krylov_graph(Batch &b, const KArgs<T> &K) {
 // (one extra stop-test kernel then records the final state).
 template <typename T>
 static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int chunk,
-                        bool per_dev = false, double floor = 0.0) {
+                        bool per_dev = false, double floor = 0.0, bool cont = false) {
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
-                     b.ndev, rtol * rtol, maxit, per_dev ? 1 : 0, floor * floor);
+                     b.ndev, rtol * rtol, maxit, per_dev ? 1 : 0, floor * floor, cont ? 1 : 0);
   b.launches++;
   launch_k_bicg_init<T>(b, K);
   // Iterations are issued in chunks between host checks of the stop flags;
@@ -1123,6 +1164,7 @@ __global__ void k_ir_check(IRArgs R, int ndev, double tol2, int per_dev) {
   device_sums<2>(R.part, R.nblk, 0, I, sh, red);
   const double rr = red[0], bb = red[1];
   if (threadIdx.x == 0) {
+    S.nrm_prev = S.nrm;
     S.nrm = sqrt(rr);
     S.rr = rr;
     S.bb = bb;
@@ -1154,15 +1196,19 @@ __global__ void __launch_bounds__(BS) k_ir_update(IRArgs R) {
   if (blockIdx.x == I.blk0 && threadIdx.x == 0) {
     S.kry_total += S.it;
     S.ir_it += 1;
+    // only a solve that converged without breakdown is continued
+    S.cont_ok = (!diverged && S.status == 0) ? 1 : 0;
   }
 }
 
-__global__ void k_ir_setup(Scal *scal, int ndev, int maxsweep) {
+// restart: a new system (y = 0), whose first sweep starts a fresh process
+__global__ void k_ir_setup(Scal *scal, int ndev, int maxsweep, int restart) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndev) return;
   scal[d].saved_active = scal[d].active;
   scal[d].ir_it = 0;
   scal[d].ir_maxsweep = maxsweep;
+  if (restart) scal[d].cont_ok = 0;
 }
 
 __global__ void k_ir_restore(Scal *scal, int ndev) {
@@ -1179,9 +1225,14 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
   R.xin = b.v64[VX];
   R.part = b.part; R.scal = b.d_scal;
   const dim3 gd((b.ndev + 63) / 64), td(64);
-  hipLaunchKernelGGL(k_ir_setup, gd, td, 0, b.stream, b.d_scal, b.ndev, maxsweep);
+  hipLaunchKernelGGL(k_ir_setup, gd, td, 0, b.stream, b.d_scal, b.ndev, maxsweep,
+                     restart ? 1 : 0);
   b.launches++;
   const int tile_mode = b.tile;
+  // continued sweeps keep phat = M^-1 p: only while the preconditioner is
+  // the one it was formed with
+  const bool cont_mode = b.ir_cont == 2 || (b.ir_cont == 1 && fp32);
+  bool cont = cont_mode;
   struct Restore {
     Batch &b;
     int t;
@@ -1202,19 +1253,25 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
     // per-device targets: the inner solve aims at the device's own target,
     // not below inner_rtol (the FP32-stored matrix's limit)
     rc = fp32 ? run_bicgstab<float>(b, make_kargs<float>(b, true), inner_rtol, inner_maxit, 8,
-                                    per_dev, inner_rtol)
+                                    per_dev, inner_rtol, cont)
               : run_bicgstab<double>(b, make_kargs<double>(b, false), inner_rtol, inner_maxit, 8,
-                                     per_dev, 0.0);
+                                     per_dev, 0.0, cont);
     if (rc) return rc;
     // a tile-preconditioned sweep that broke down or diverged (FP16 tile
     // inverses of a nearly singular tile, as some Crank-Nicolson steps at
     // reverse bias assemble) is followed by sweeps with the FP64 node
     // block-Jacobi, the reference's own preconditioner, for the rest of
     // this solve
+    // (a continued solve that breaks down is followed by a fresh one first:
+    // k_ir_update clears its cont_ok)
     if (b.tile) {
       bool broke = false;
-      for (int d = 0; d < b.ndev; ++d) broke |= b.h_scal[d].active && b.h_scal[d].status != 0;
-      if (broke) b.tile = 0;
+      for (int d = 0; d < b.ndev; ++d)
+        broke |= b.h_scal[d].active && b.h_scal[d].status != 0 && !b.h_scal[d].cont;
+      if (broke) {
+        b.tile = 0;
+        cont = false;
+      }
     }
     hipLaunchKernelGGL(k_ir_update, dim3(b.nblk), dim3(BS), 0, b.stream, R);
     b.launches++;
@@ -1521,7 +1578,7 @@ int pcg_poisson(Batch &b, double rtol, int maxit) {
   P.P16 = b.P16; P.tile = b.pcg_tile;
   P.late_stop = late_stop_ok(b);
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
-                     b.ndev, rtol * rtol, maxit, 0, 0.0);
+                     b.ndev, rtol * rtol, maxit, 0, 0.0, 0);
   hipLaunchKernelGGL(k_pcg_init, dim3(b.nblk), dim3(BS), 0, b.stream, P);
   b.launches += 2;
   // chunks between host checks as in BiCGStab: 3/4 of the last solve's

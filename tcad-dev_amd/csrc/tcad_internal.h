@@ -97,6 +97,7 @@ struct Scal {
   double rr;                        // last ||r||^2
   double rr_ref;                    // final refinement: ||r||^2 before the last sweep
   double nrm;                       // IR residual norm (rhs normalisation)
+  double nrm_prev;                  // the previous sweep's nrm (continued inner solves)
   double err;                       // Newton update measure
   double tol2;                      // stop test rr <= tol2 * bb of the current solve
   double eta2;                      // squared forcing term of this device's next Newton solve
@@ -112,6 +113,8 @@ struct Scal {
   int newton_it, newton_done, kry_total, ir_it, ir_maxsweep;
   int conv_now;                     // this Newton step's update passed the test
   int saved_active;                 // active flag kept across one refinement
+  int cont;                         // this inner solve continues the last one's Krylov process
+  int cont_ok;                      // the last inner solve may be continued
   int newton_active;                // Newton-level flag kept across the final-step refinement
   int sel;                          // device selected for the current Newton solve
   int result;                       // TCAD_NEWTON_* outcome of the current Newton solve
@@ -220,6 +223,14 @@ struct Batch {
   // forcing solves whose target is loose (every active device's eta >=
   // 1e-5) run BiCGStab on the FP32-stored matrix inside the FP64 refinement
   int forcing_fp32 = 0;
+  // refinement sweeps after the first continue the previous sweep's
+  // BiCGStab (search direction and shadow residual kept, the residual
+  // replaced by the FP64 true residual) instead of restarting it:
+  // 0 never, 1 for inner solves on the FP32-stored matrix, 2 always
+  int ir_cont = 1;
+  // relative residual reduction an inner solve on the FP32-stored matrix
+  // aims at, at least, per sweep of the Newton solves
+  double fp32_inner = 1e-6;
   // BiCGStab iterations replayed as captured HIP graphs (tcad_set_tuning
   // TCAD_TUNE_GRAPH): blocks of kGraphIts iterations captured on a private
   // stream, one executable per kernel-argument set

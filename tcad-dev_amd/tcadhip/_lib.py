@@ -32,6 +32,8 @@ TUNE_FORCING_SLOPE = 2
 TUNE_FORCING_MAX = 3
 TUNE_FORCING_FP32 = 4
 TUNE_GRAPH = 5
+TUNE_IR_CONTINUE = 6
+TUNE_FP32_INNER = 7
 
 CSR_BICGSTAB = 0
 CSR_GMRES = 1

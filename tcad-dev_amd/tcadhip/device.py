@@ -57,12 +57,17 @@ class NewtonOptions:
     # run-time tuning (tcad_set_tuning): forcing eta = clamp(forcing_slope *
     # err, linsolve_rtol, forcing_max); forcing_fp32: loose forcing solves on
     # the FP32-stored matrix; tile_rows "auto" | "reg" | "mem"; graph: BiCGStab
-    # iterations replayed from a captured HIP graph
+    # iterations replayed from a captured HIP graph; ir_continue: refinement
+    # sweeps continue the previous sweep's BiCGStab ("fp32": only inner
+    # solves on the FP32-stored matrix, "all", "never"); fp32_inner: the
+    # per-sweep residual reduction of inner solves on the FP32 matrix
     forcing_slope: float = 0.01
     forcing_max: float = 1e-4
     forcing_fp32: bool = False
     tile_rows: str = "auto"
     graph: bool = False
+    ir_continue: str = "fp32"
+    fp32_inner: float = 1e-6
 
     def apply_tuning(self, handle):
         L = _lib.lib()
@@ -78,6 +83,13 @@ class NewtonOptions:
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FORCING_FP32,
                                      1.0 if self.forcing_fp32 else 0.0), "tcad_set_tuning")
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_GRAPH, 1.0 if self.graph else 0.0),
+                   "tcad_set_tuning")
+        cont = {"never": 0, "fp32": 1, "all": 2}
+        if self.ir_continue not in cont:
+            raise ValueError(f"unknown ir_continue {self.ir_continue!r}")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_IR_CONTINUE,
+                                     float(cont[self.ir_continue])), "tcad_set_tuning")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FP32_INNER, float(self.fp32_inner)),
                    "tcad_set_tuning")
 
     def lin_mode(self):
