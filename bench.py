@@ -132,6 +132,12 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, prec
     out = dict(seconds=dt, points_per_step=npts, latencies=np.array(lat), record=rec,
                devs=devs, specs=specs, eq_iterations=[d.last_eq_iterations for d in devs],
                opts=opts, tuning=tuning)
+    if opts.forcing or opts.linsolve != "bicgstab":
+        # the exact-Newton counts of the same sweep (FP64 solves to rtol
+        # every step): the tuned path's counts are held to them +-1 per point
+        exact = []
+        trigate_step(devs, specs, NewtonOptions(precond=precond), eq_opts, None, exact)
+        out["newton_exact"] = [x[1] for x in exact]
     if probe is not None and probe >= 0:
         # in-solver duration of one kernel: one more (untimed) step with HIP
         # events around every launch of it
@@ -343,8 +349,8 @@ def bench_mosfet(steps=2, warmup=1, modes=("bicgstab", "bicgstab_mixed", "tuned"
         lat_ms = np.array(lat) * 1e3
         out[mode] = {"points_per_s": round(len(spec.sweep) * steps / dt, 3),
                      "options": {k: getattr(opts, k) for k in
-                                 ("linsolve", "forcing", "forcing_slope", "forcing_max",
-                                  "forcing_fp32", "tile_rows", "graph")},
+                                 ("linsolve", "forcing", "forcing_pow", "forcing_slope",
+                                  "forcing_max", "forcing_fp32", "tile_rows", "graph")},
                      "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
                      "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
                      "newton_per_point": its,
@@ -710,10 +716,14 @@ def main():
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "solver_options": {k: getattr(r["opts"], k) for k in
-                           ("linsolve", "linsolve_rtol", "precond", "forcing", "forcing_slope",
-                            "forcing_max", "forcing_fp32", "tile_rows", "graph")},
+                           ("linsolve", "linsolve_rtol", "precond", "forcing", "forcing_pow",
+                            "forcing_slope", "forcing_max", "forcing_fp32", "tile_rows", "graph")},
         "autotuner": r["tuning"],
         "newton_per_point": [x[1] for x in r["record"]],
+        "newton_exact_per_point": r.get("newton_exact"),
+        "newton_within_1_of_exact": (None if r.get("newton_exact") is None else
+                                     all(abs(a - b) <= 1 for a, b in
+                                         zip([x[1] for x in r["record"]], r["newton_exact"]))),
         "krylov_per_point": [x[2] for x in r["record"]],
         "equilibrium_newton": r["eq_iterations"],
         "drain_current_A": {"vd0.05": [x[3] for x in r["record"] if "vd0.05" in x[0]],

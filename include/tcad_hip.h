@@ -387,8 +387,9 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
  *   TCAD_TUNE_TILE_ROWS      0 by grid size [registers up to 512 workgroups,
  *                            memory above], 1 tile-inverse rows in registers,
  *                            2 read at the point of use;
- *   TCAD_TUNE_FORCING_SLOPE  c of the forcing term eta = clamp(c * err, rtol,
- *                            eta_max) of TCAD_LIN_FORCING solves [0.01];
+ *   TCAD_TUNE_FORCING_SLOPE  c (0 < c <= 1000) of the forcing term
+ *                            eta = clamp(c * err^q, rtol, eta_max) of
+ *                            TCAD_LIN_FORCING solves [0.01];
  *   TCAD_TUNE_FORCING_MAX    eta_max [1e-4];
  *   TCAD_TUNE_FORCING_FP32   1: a forcing solve whose target is at least
  *                            1e-5 for every active device runs BiCGStab on
@@ -408,7 +409,9 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
  *   TCAD_TUNE_FP32_INNER     the residual reduction (1e-7..1e-2) an inner
  *                            solve on the FP32-stored matrix stops at, at
  *                            least, in each refinement sweep of a Newton
- *                            solve [1e-6].
+ *                            solve [1e-6];
+ *   TCAD_TUNE_FORCING_POW    q: 1 (eta linear in the last Newton update
+ *                            measure err) or 2 (quadratic) [1].
  * Has no reference counterpart (the reference's Krylov tolerance is fixed,
  * linsolve.py:725-727). */
 #define TCAD_TUNE_TILE_ROWS 1
@@ -418,6 +421,7 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
 #define TCAD_TUNE_GRAPH 5
 #define TCAD_TUNE_IR_CONTINUE 6
 #define TCAD_TUNE_FP32_INNER 7
+#define TCAD_TUNE_FORCING_POW 8
 int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
 
 /* Bench instrumentation, in-solver kernel timing: from this call on,

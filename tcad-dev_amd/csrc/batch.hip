@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(BS) k_newton_update(UArgs U, const double *y, 
 // whose linear solve ended above the acceptance residual stops here, before
 // its step is applied (linsolve.py:973-982 refuses such an iterate).
 __global__ void k_newton_check(UArgs U, double tol, double accept2, double rtol, double eta_max,
-                               int forcing, double eta_c) {
+                               int forcing, double eta_c, int eta_pow) {
   __shared__ double sh[BS / 64];
   const int d = blockIdx.x;
   const DevInfo &I = U.info[d];
@@ -113,11 +113,12 @@ __global__ void k_newton_check(UArgs U, double tol, double accept2, double rtol,
       S.result = TCAD_NEWTON_LINEAR_FAIL;
     }
     // forcing term of the next solve: far from the solution the step only
-    // needs a few digits; near it the target tightens with the update
-    // (the converging step is refined to the rounding floor before it is
-    // applied)
+    // needs a few digits; near it the target tightens with the update,
+    // linearly (eta_pow 1) or with its square (eta_pow 2, which keeps
+    // Newton's quadratic convergence) (the converging step is refined to
+    // the rounding floor before it is applied)
     if (forcing) {
-      const double eta = fmin(fmax(eta_c * m, rtol), eta_max);
+      const double eta = fmin(fmax(eta_c * (eta_pow == 2 ? m * m : m), rtol), eta_max);
       S.eta2 = eta * eta;
     }
   }
@@ -631,7 +632,8 @@ static int bias_newton_step(Batch &b, const UArgs &U, const NewtonCfg &c, int ma
   hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
                      c.max_dpsi, 0, psi, n, p);
   hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, c.tol,
-                     c.accept * c.accept, c.rtol, c.eta_max, c.forcing ? 1 : 0, b.forcing_c);
+                     c.accept * c.accept, c.rtol, c.eta_max, c.forcing ? 1 : 0, b.forcing_c,
+                     b.forcing_pow);
   b.launches += 2;
   TCAD_HIP_CHECK(hipGetLastError());
   if ((rc = sync_scalars(b))) return rc;
@@ -901,7 +903,7 @@ static int transient_impl(Batch &b, int max_iter, double tol, double max_dpsi, i
     hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
                        max_dpsi, 0, psi, n, p);
     hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol,
-                       accept * accept, rtol, 1.0, 0, b.forcing_c);
+                       accept * accept, rtol, 1.0, 0, b.forcing_c, b.forcing_pow);
     b.launches += 2;
     TCAD_HIP_CHECK(hipGetLastError());
     if ((rc = sync_scalars(b))) return rc;
@@ -1640,7 +1642,7 @@ int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
       b.tile_rows = (int)value;
       return TCAD_OK;
     case TCAD_TUNE_FORCING_SLOPE:
-      if (!(value > 0.0 && value <= 1.0)) break;
+      if (!(value > 0.0 && value <= 1e3)) break;
       b.forcing_c = value;
       return TCAD_OK;
     case TCAD_TUNE_FORCING_MAX:
@@ -1658,6 +1660,10 @@ int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
     case TCAD_TUNE_IR_CONTINUE:
       if (value != 0.0 && value != 1.0 && value != 2.0) break;
       b.ir_cont = (int)value;
+      return TCAD_OK;
+    case TCAD_TUNE_FORCING_POW:
+      if (value != 1.0 && value != 2.0) break;
+      b.forcing_pow = (int)value;
       return TCAD_OK;
     case TCAD_TUNE_FP32_INNER:
       if (!(value >= 1e-7 && value <= 1e-2)) break;

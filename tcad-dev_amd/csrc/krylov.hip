@@ -992,7 +992,8 @@ static hipGraphExec_t krylov_graph(Batch &b, const KArgs<T> &K) {
 // (one extra stop-test kernel then records the final state).
 template <typename T>
 static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int chunk,
-                        bool per_dev = false, double floor = 0.0, bool cont = false) {
+                        bool per_dev = false, double floor = 0.0, bool cont = false,
+                        double target = 0.0) {
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
                      b.ndev, rtol * rtol, maxit, per_dev ? 1 : 0, floor * floor, cont ? 1 : 0);
   b.launches++;
@@ -1000,10 +1001,20 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
   // Iterations are issued in chunks between host checks of the stop flags;
   // a check costs a host round trip with the stream drained, an iteration
   // issued after convergence four early-exit launches.  Consecutive Newton
-  // solves need similar counts, so the first chunk is 3/4 of the last
-  // solve's, then 16, 32, 64 (without a history: chunk, doubling to 64).
+  // solves need similar counts per digit of residual reduction, so the
+  // first chunk is 3/4 of the last solve's count, scaled down by the ratio
+  // of the two targets' logarithms when this one's is looser (a loose
+  // forcing solve after a tight one needs a fraction of its iterations),
+  // then 16, 32, 64 (without a history: chunk, doubling to 64).
+  if (!(target > 0.0)) target = rtol;
+  const double lt = std::log(std::min(std::max(target, 1e-300), 0.5));
+  double predicted = (double)b.kry_hint;
+  // only ever scaled down: a continued refinement sweep needs far fewer
+  // iterations than a fresh solve to the same target, and a chunk issued
+  // past convergence costs four launches per iteration
+  if (b.kry_hint > 0 && b.kry_hint_lt < 0.0) predicted *= std::min(1.0, lt / b.kry_hint_lt);
   int it = 0;
-  int next = b.kry_hint > 0 ? std::max(chunk, (3 * b.kry_hint) / 4) : chunk;
+  int next = b.kry_hint > 0 ? std::max(chunk, (int)(0.75 * predicted)) : chunk;
   int step = b.kry_hint > 0 ? 16 : chunk;
   while (true) {
     int c = 0;
@@ -1045,6 +1056,7 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
     if (b.h_scal[d].active) failed |= b.h_scal[d].status;
   }
   b.kry_hint = failed ? 0 : most;
+  b.kry_hint_lt = lt;
   return TCAD_OK;
 }
 
@@ -1251,11 +1263,23 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
     for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active ? 1 : 0;
     if (open == 0) break;
     // per-device targets: the inner solve aims at the device's own target,
-    // not below inner_rtol (the FP32-stored matrix's limit)
+    // not below inner_rtol (the FP32-stored matrix's limit); the tightest
+    // of them (as k_krylov_reset forms them) sizes the first chunk
+    const double floor = fp32 ? inner_rtol : 0.0;
+    double target = inner_rtol;
+    if (per_dev) {
+      target = 1.0;
+      for (int d = 0; d < b.ndev; ++d) {
+        const Scal &S = b.h_scal[d];
+        if (!S.active) continue;
+        const double t2 = (floor > 0.0 && S.rr > 0.0) ? S.eta2 * (S.bb / S.rr) : S.eta2;
+        target = std::min(target, std::sqrt(std::max(t2, floor * floor)));
+      }
+    }
     rc = fp32 ? run_bicgstab<float>(b, make_kargs<float>(b, true), inner_rtol, inner_maxit, 8,
-                                    per_dev, inner_rtol, cont)
+                                    per_dev, inner_rtol, cont, target)
               : run_bicgstab<double>(b, make_kargs<double>(b, false), inner_rtol, inner_maxit, 8,
-                                     per_dev, 0.0, cont);
+                                     per_dev, 0.0, cont, target);
     if (rc) return rc;
     // a tile-preconditioned sweep that broke down or diverged (FP16 tile
     // inverses of a nearly singular tile, as some Crank-Nicolson steps at

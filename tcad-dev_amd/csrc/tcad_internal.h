@@ -158,6 +158,7 @@ struct Batch {
   // iterations of the last BiCGStab / PCG solve (slowest device): the next
   // solve's first chunk is sized from it
   int kry_hint = 0, pcg_hint = 0;
+  double kry_hint_lt = 0.0;         // ln of the relative target the kry_hint solve aimed at
   std::vector<DevInfo> info;        // host copy
   DevInfo *d_info = nullptr;
   int *d_blk2dev = nullptr;
@@ -220,6 +221,7 @@ struct Batch {
   // 1 registers, 2 memory), forcing-term slope and cap
   int tile_rows = 0;
   double forcing_c = 0.01, forcing_max = 1e-4;
+  int forcing_pow = 1;              // eta = c * err^forcing_pow
   // forcing solves whose target is loose (every active device's eta >=
   // 1e-5) run BiCGStab on the FP32-stored matrix inside the FP64 refinement
   int forcing_fp32 = 0;

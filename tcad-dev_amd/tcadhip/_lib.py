@@ -34,6 +34,7 @@ TUNE_FORCING_FP32 = 4
 TUNE_GRAPH = 5
 TUNE_IR_CONTINUE = 6
 TUNE_FP32_INNER = 7
+TUNE_FORCING_POW = 8
 
 CSR_BICGSTAB = 0
 CSR_GMRES = 1

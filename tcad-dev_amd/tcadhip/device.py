@@ -60,7 +60,8 @@ class NewtonOptions:
     # iterations replayed from a captured HIP graph; ir_continue: refinement
     # sweeps continue the previous sweep's BiCGStab ("fp32": only inner
     # solves on the FP32-stored matrix, "all", "never"); fp32_inner: the
-    # per-sweep residual reduction of inner solves on the FP32 matrix
+    # per-sweep residual reduction of inner solves on the FP32 matrix;
+    # forcing_pow: eta = forcing_slope * err^forcing_pow
     forcing_slope: float = 0.01
     forcing_max: float = 1e-4
     forcing_fp32: bool = False
@@ -68,6 +69,7 @@ class NewtonOptions:
     graph: bool = False
     ir_continue: str = "fp32"
     fp32_inner: float = 1e-6
+    forcing_pow: int = 1
 
     def apply_tuning(self, handle):
         L = _lib.lib()
@@ -90,6 +92,8 @@ class NewtonOptions:
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_IR_CONTINUE,
                                      float(cont[self.ir_continue])), "tcad_set_tuning")
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FP32_INNER, float(self.fp32_inner)),
+                   "tcad_set_tuning")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FORCING_POW, float(self.forcing_pow)),
                    "tcad_set_tuning")
 
     def lin_mode(self):

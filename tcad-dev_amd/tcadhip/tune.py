@@ -5,9 +5,11 @@ rather than on the physics, and that leave the converged result the same
 (every candidate refines the converging Newton step to the FP64 rounding
 floor, so the terminal currents agree to rounding):
 
-  * Newton forcing terms (TCAD_LIN_FORCING: early linear solves stop at
-    eta = clamp(c * err, rtol, eta_max) instead of rtol) and their slope c
-    and cap eta_max -- fewer Krylov iterations for some more Newton steps;
+  * Newton forcing terms (TCAD_LIN_FORCING: the linear solves before the
+    converging one stop at eta = clamp(c * err^q, rtol, eta_max) instead
+    of rtol, err the last Newton update measure), their slope c, power q
+    and cap eta_max -- fewer Krylov iterations for, at most, some more
+    Newton steps;
   * whether the loose forcing solves run on the FP32-stored matrix (the
     tight and converging ones stay on the FP64 matrix);
   * where the tile-inverse rows live in the BiCGStab kernels: loaded into
@@ -20,10 +22,16 @@ A candidate is measured by solving one bias point from the same saved
 state (psi, n, p restored on the device before each trial), twice, the
 faster taken.  Candidates that do not converge, whose currents differ from
 the untuned configuration's by more than 1e-9 of the largest current, or
-that need more than one Newton iteration over it are rejected (Newton
-counts must stay within one of the reference's; forcing terms can add
-two at deep reverse bias, where they are then not chosen).  Choices are cached per (dimension, mesh shape,
-linear-solve path, preconditioner) for the life of the process.
+that need more Newton iterations than it (`newton_slack`, 0 by default)
+are rejected.  The Newton counts must stay within one of the reference's
+at every point of a sweep, and the tuning point is one point: no slack
+there keeps a margin elsewhere.  Measured on the 64x48x48 tri-gate sweep
+(34 points; exact Newton 143 iterations, 23,922 Krylov, 23.3 points/s):
+linear forcing c = 0.1, eta_max = 1e-3 ran 35.0 points/s with 179 Newton
+iterations, two points at exact + 2; quadratic forcing c = 0.1,
+eta_max = 1e-3 34.3 points/s with 159, every point within exact + 1.
+Choices are cached per (dimension, mesh shape, linear-solve path,
+preconditioner) for the life of the process.
 """
 import time
 from dataclasses import replace
@@ -36,17 +44,19 @@ CANDIDATES = (
     dict(forcing=False, tile_rows="auto"),
     dict(forcing=False, tile_rows="reg"),
     dict(forcing=False, tile_rows="mem"),
-    dict(forcing=True, forcing_slope=0.01, forcing_max=1e-4, tile_rows="auto"),
-    dict(forcing=True, forcing_slope=0.01, forcing_max=1e-3, tile_rows="auto"),
-    dict(forcing=True, forcing_slope=0.1, forcing_max=1e-3, tile_rows="auto"),
-    dict(forcing=True, forcing_slope=0.01, forcing_max=1e-4, tile_rows="reg"),
-    dict(forcing=True, forcing_slope=0.1, forcing_max=1e-3, forcing_fp32=True,
+    dict(forcing=True, forcing_pow=2, forcing_slope=0.1, forcing_max=1e-3, tile_rows="auto"),
+    dict(forcing=True, forcing_pow=2, forcing_slope=0.03, forcing_max=1e-3, tile_rows="auto"),
+    dict(forcing=True, forcing_pow=2, forcing_slope=1.0, forcing_max=1e-3, tile_rows="auto"),
+    dict(forcing=True, forcing_pow=2, forcing_slope=0.1, forcing_max=1e-3, tile_rows="reg"),
+    dict(forcing=True, forcing_pow=1, forcing_slope=0.01, forcing_max=1e-4, tile_rows="auto"),
+    dict(forcing=True, forcing_pow=1, forcing_slope=0.01, forcing_max=1e-4, tile_rows="reg"),
+    dict(forcing=True, forcing_pow=1, forcing_slope=0.1, forcing_max=1e-3, tile_rows="auto"),
+    dict(forcing=True, forcing_pow=2, forcing_slope=0.1, forcing_max=1e-3, forcing_fp32=True,
          tile_rows="auto"),
-    dict(forcing=True, forcing_slope=0.1, forcing_max=1e-3, forcing_fp32=True,
-         tile_rows="reg"),
 )
 
-_KNOBS = ("forcing", "forcing_slope", "forcing_max", "forcing_fp32", "tile_rows", "graph")
+_KNOBS = ("forcing", "forcing_pow", "forcing_slope", "forcing_max", "forcing_fp32", "tile_rows",
+          "graph")
 _cache = {}
 
 
@@ -54,7 +64,7 @@ def _key(dev, opts):
     return (dev.dim, tuple(dev.shape), opts.linsolve, opts.precond)
 
 
-def tune(dev, bias, opts=None, candidates=CANDIDATES, repeats=2, say=None):
+def tune(dev, bias, opts=None, candidates=CANDIDATES, repeats=2, say=None, newton_slack=0):
     """Time every candidate on one bias point of `dev` (which must hold a
     converged state: the warm start every trial restarts from) and return
     (best NewtonOptions, report).  The device's state and bias are left
@@ -100,7 +110,7 @@ def tune(dev, bias, opts=None, candidates=CANDIDATES, repeats=2, say=None):
             conv, its, kry, cur = res
             if ref_I is None:
                 ref_I, ref_its = cur, its
-            ok = (bool(conv) and its <= ref_its + 1
+            ok = (bool(conv) and its <= ref_its + newton_slack
                   and bool(np.all(np.abs(cur - ref_I) <= 1e-9 * np.abs(ref_I).max() + 1e-300)))
             rows.append(dict(cand, seconds=round(best_t, 5), newton=int(its), krylov=int(kry),
                              accepted=ok))

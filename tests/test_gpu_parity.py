@@ -133,24 +133,31 @@ def _equilibrated_counts(name):
 
 SWEEP_MODES = ([(n, m) for n in SWEEPS for m in ("bicgstab", "bicgstab_mixed")]
                + [(n, m) for n in SWEEPS if n.startswith("trigate")
-                  for m in ("bicgstab_forcing", "bicgstab_forcing32")])
+                  for m in ("forcing_q1", "forcing_q2", "forcing_q2_fp32")])
+
+# Newton forcing terms eta = clamp(c * err^q, rtol, eta_max) with c = 0.1,
+# eta_max = 1e-3 (the autotuner's candidates on the tri-gate)
+FORCING = {"forcing_q2": dict(forcing_pow=2), "forcing_q1": dict(forcing_pow=1),
+           "forcing_q2_fp32": dict(forcing_pow=2, forcing_fp32=True)}
 
 
 @pytest.mark.parametrize("name,mode", SWEEP_MODES)
 def test_sweep_vs_reference(name, mode):
     """The solver paths: FP64, FP32-stored matrix in FP64 refinement, and,
     on the tri-gate where the autotuner picks them, Newton forcing terms
-    with the tuned slope 0.1 and cap 1e-3, the loose solves on the FP64 or
-    the FP32-stored matrix.  (Forcing is not a universal default: on the
-    gated slabs it needs up to two Newton iterations more than the exact
-    steps, which the autotuner's one-iteration guard then rejects.)"""
+    (slope 0.1, cap 1e-3) linear or quadratic in the last update, the loose
+    solves on the FP64 or the FP32-stored matrix.  (Forcing is not held
+    everywhere: on the 48x32 MOSFET, the 24x16 diode and the z-gated slab
+    the loose first solves from equilibrium cost two Newton iterations more
+    than the exact steps at the first point, linear or quadratic; the
+    autotuner decides per device, and bench.py reports the tuned counts
+    beside the exact ones.)"""
     NewtonOptions, _ = _gpu()
     _say(f"[sweep] {name} {mode}")
     spec, d = load(name)
     eq_its = _equilibrated_counts(name)
-    if mode.startswith("bicgstab_forcing"):
-        opts = NewtonOptions(forcing=True, forcing_slope=0.1, forcing_max=1e-3,
-                             forcing_fp32=mode.endswith("32"))
+    if mode in FORCING:
+        opts = NewtonOptions(forcing=True, forcing_slope=0.1, forcing_max=1e-3, **FORCING[mode])
     else:
         opts = NewtonOptions(linsolve=mode)
     dev, its, cur, states = _run_sweep(spec, opts)
