@@ -668,7 +668,10 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
   UArgs U = uargs(b);
   double eta_next = c.forcing ? c.eta_max : rtol;
   for (int it = 0; it < max_iter; ++it) {
-    if ((rc = bias_newton_step(b, U, c, max_iter, eta_next, psi, n, p))) return rc;
+    b.kry_slot = it;
+    rc = bias_newton_step(b, U, c, max_iter, eta_next, psi, n, p);
+    b.kry_slot = -1;
+    if (rc) return rc;
     int open = 0;
     for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].active;
     if (open == 0) break;
@@ -800,8 +803,12 @@ static int queue_impl(Batch &b, int max_iter, double tol, double max_dpsi, int m
   UArgs U = uargs(b);
   double eta_next = eta0;
   const long long guard = (long long)max_iter * total_pts + 1;
+  int local_it = 0;                               // Newton index at the current point (one device)
   for (long long it = 0; it < guard; ++it) {
-    if ((rc = bias_newton_step(b, U, c, max_iter, eta_next, psi, n, p))) return rc;
+    b.kry_slot = nd == 1 ? local_it++ : -1;
+    rc = bias_newton_step(b, U, c, max_iter, eta_next, psi, n, p);
+    b.kry_slot = -1;
+    if (rc) return rc;
     std::vector<int> conv, fail;
     int open = 0;
     for (int d = 0; d < nd; ++d) {
@@ -847,6 +854,7 @@ static int queue_impl(Batch &b, int max_iter, double tol, double max_dpsi, int m
         }
       }
       if (rearm) {
+        local_it = 0;
         if ((rc = upload())) return rc;
         hipLaunchKernelGGL(k_rearm, dim3((nd + 63) / 64), dim3(64), 0, b.stream, b.d_scal, nd,
                            b.d_qmask, eta0);

@@ -47,6 +47,7 @@
 // iteration, 2D (29 entries), matrix width w (8 or 4): 2 SpMV x
 // (29 w + 24 + 24 + 24) + 2 precond x (9 w + 48) + ~12 vector passes x 24.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 
@@ -993,7 +994,7 @@ static hipGraphExec_t krylov_graph(Batch &b, const KArgs<T> &K) {
 template <typename T>
 static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int chunk,
                         bool per_dev = false, double floor = 0.0, bool cont = false,
-                        double target = 0.0) {
+                        double target = 0.0, int slot = -1) {
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
                      b.ndev, rtol * rtol, maxit, per_dev ? 1 : 0, floor * floor, cont ? 1 : 0);
   b.launches++;
@@ -1008,14 +1009,21 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
   // then 16, 32, 64 (without a history: chunk, doubling to 64).
   if (!(target > 0.0)) target = rtol;
   const double lt = std::log(std::min(std::max(target, 1e-300), 0.5));
-  double predicted = (double)b.kry_hint;
+  if (slot >= Batch::kSlots) slot = -1;
+  // the k-th Newton solve of the last bias point when there is one (9/10
+  // of its count: a short first chunk costs one more round trip, a long one
+  // early-exit launches), else the last solve's (3/4 of it)
+  const bool by_slot = slot >= 0 && b.slot_hint[slot] > 0;
+  const int hint = by_slot ? b.slot_hint[slot] : b.kry_hint;
+  const double hint_lt = by_slot ? b.slot_lt[slot] : b.kry_hint_lt;
+  double predicted = (by_slot ? 0.9 : 0.75) * (double)hint;
   // only ever scaled down: a continued refinement sweep needs far fewer
   // iterations than a fresh solve to the same target, and a chunk issued
   // past convergence costs four launches per iteration
-  if (b.kry_hint > 0 && b.kry_hint_lt < 0.0) predicted *= std::min(1.0, lt / b.kry_hint_lt);
+  if (hint > 0 && hint_lt < 0.0) predicted *= std::min(1.0, lt / hint_lt);
   int it = 0;
-  int next = b.kry_hint > 0 ? std::max(chunk, (int)(0.75 * predicted)) : chunk;
-  int step = b.kry_hint > 0 ? 16 : chunk;
+  int next = hint > 0 ? std::max(chunk, (int)predicted) : chunk;
+  int step = hint > 0 ? 16 : chunk;
   while (true) {
     int c = 0;
     if (b.use_graph && b.probe < 0) {
@@ -1043,9 +1051,28 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
     int rc = sync_scalars(b);
     if (rc) return rc;
     int open = 0;
-    for (int d = 0; d < b.ndev; ++d) open += b.h_scal[d].done ? 0 : 1;
+    // the iterations the open devices still need at their average rate of
+    // residual reduction so far: the next chunk is that plus a margin, at
+    // most the geometric step (an underestimate costs one more round trip,
+    // an overestimate four early-exit launches per iteration)
+    double need = 0.0;
+    bool rated = true;
+    for (int d = 0; d < b.ndev; ++d) {
+      const Scal &S = b.h_scal[d];
+      if (S.done) continue;
+      ++open;
+      const double q = S.rr / S.bb, goal = S.tol2;
+      if (S.it > 0 && q > 0.0 && q < 1.0 && goal > 0.0 && std::isfinite(q)) {
+        const double rate = std::log(q) / S.it;
+        need = std::max(need, std::log(goal / q) / rate);
+      } else {
+        rated = false;
+      }
+    }
     if (open == 0 || it >= maxit) break;
     next = step;
+    if (rated && need > 0.0)
+      next = std::min(step, std::max(4, (int)std::ceil(need * 1.125) + 2));
     if (step < 64) step *= 2;
   }
   // a solve stopped by its cap (or failed) says nothing about the next one's
@@ -1057,6 +1084,10 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
   }
   b.kry_hint = failed ? 0 : most;
   b.kry_hint_lt = lt;
+  if (slot >= 0) {
+    b.slot_hint[slot] = failed ? 0 : most;
+    b.slot_lt[slot] = lt;
+  }
   return TCAD_OK;
 }
 
@@ -1276,10 +1307,12 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
         target = std::min(target, std::sqrt(std::max(t2, floor * floor)));
       }
     }
+    // the first sweep of a Newton solve keeps its own hint per Newton index
+    const int slot = (restart && sweep == 0) ? b.kry_slot : -1;
     rc = fp32 ? run_bicgstab<float>(b, make_kargs<float>(b, true), inner_rtol, inner_maxit, 8,
-                                    per_dev, inner_rtol, cont, target)
+                                    per_dev, inner_rtol, cont, target, slot)
               : run_bicgstab<double>(b, make_kargs<double>(b, false), inner_rtol, inner_maxit, 8,
-                                     per_dev, 0.0, cont, target);
+                                     per_dev, 0.0, cont, target, slot);
     if (rc) return rc;
     // a tile-preconditioned sweep that broke down or diverged (FP16 tile
     // inverses of a nearly singular tile, as some Crank-Nicolson steps at

@@ -159,6 +159,14 @@ struct Batch {
   // solve's first chunk is sized from it
   int kry_hint = 0, pcg_hint = 0;
   double kry_hint_lt = 0.0;         // ln of the relative target the kry_hint solve aimed at
+  // per Newton-iteration index: the k-th solve of a bias point resembles
+  // the k-th solve of the point before far more than the solve just before
+  // it (forcing targets loosen and tighten along a point's Newton steps);
+  // kry_slot: the index of the coming Newton solve (-1: none)
+  static constexpr int kSlots = 16;
+  int kry_slot = -1;
+  int slot_hint[kSlots] = {};
+  double slot_lt[kSlots] = {};
   std::vector<DevInfo> info;        // host copy
   DevInfo *d_info = nullptr;
   int *d_blk2dev = nullptr;
