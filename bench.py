@@ -97,7 +97,7 @@ def trigate_step(devs, specs, opts, eq_opts, lat=None, record=None):
 
 def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, precond="tile",
                   tune=True, forcing_slope=0.01, forcing_max=1e-4, tile_rows="auto",
-                  probe_cuts=()):
+                  probe_cuts=(), forcing_pow=1):
     from tcadhip import configs
     from tcadhip.device import NewtonOptions, device_from_spec
     specs = [configs.config3_trigate(0.05), configs.config3_trigate(0.7)]
@@ -107,7 +107,7 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, prec
     devs = [device_from_spec(s) for s in specs]
     opts = NewtonOptions(linsolve=mode, forcing=forcing, precond=precond,
                          forcing_slope=forcing_slope, forcing_max=forcing_max,
-                         tile_rows=tile_rows)
+                         forcing_pow=forcing_pow, tile_rows=tile_rows)
     eq_opts = NewtonOptions(max_iter=400)
     tuning = None
     if tune:
@@ -446,7 +446,21 @@ def bench_batch(lanes=1, sample=8, modes=None):
     specs = configs.batch_diodes()
     res = {}
     modes = modes or {"bicgstab_mixed": NewtonOptions(linsolve="bicgstab_mixed"),
-                      "bicgstab": NewtonOptions()}
+                      "bicgstab": NewtonOptions(), "tuned": None}
+    tuning = None
+    if "tuned" in modes:
+        # the autotuner on the largest diode (the second point of its sweep,
+        # warm-started from the first), its choice applied to every device
+        from tcadhip import tune as tuner
+        from tcadhip.device import device_from_spec
+        big = max(range(len(specs)), key=lambda i: specs[i].N)
+        dev = device_from_spec(specs[big])
+        dev.solve_equilibrium(NewtonOptions())
+        dev.solve_bias(specs[big].sweep[0], NewtonOptions())
+        topts, tuning = tuner.tune(dev, specs[big].sweep[1], NewtonOptions(), say=_say)
+        tuning = dict(tuning, device=big, nodes=int(specs[big].N))
+        modes["tuned"] = topts
+        del dev
     for mode, mopts in modes.items():
         mopts, kw = mopts if isinstance(mopts, tuple) else (mopts, {})
         setups = [setup_from_spec(s) for s in specs]
@@ -506,8 +520,14 @@ def bench_batch(lanes=1, sample=8, modes=None):
             names = [c.name for c in specs[d].contacts]
             got, ref = final(r["results"][d], names), final(alone[d], names)
             e64 = max(e64, float(np.abs(got - ref).max() / np.abs(ref).max()))
+        over = None
+        if mode != "bicgstab" and "bicgstab" in res:
+            # Newton counts against the exact (untuned FP64) batch, device by device
+            over = sum(int(abs(a - b) > 1) for x, y in zip(r["results"], res["bicgstab"]["results"])
+                       for a, b in zip(x.newton, y.newton))
         out[mode] = {"devices_per_s": round(len(specs) / r["seconds"], 4),
                      "seconds": round(r["seconds"], 3),
+                     "newton_points_off_exact_by_more_than_1": over,
                      "failed": sum(1 for x in r["results"] if x.failed),
                      "rounds": r["rounds"], "compactions": r["rebuilds"],
                      "vs_plain_fp64": round(len(specs) / r["seconds"] / plain, 2),
@@ -516,6 +536,11 @@ def bench_batch(lanes=1, sample=8, modes=None):
                      "meets_1e-6_no_floor": strict,
                      "under_conservation_floor_only": floor_pts,
                      "max_rel_current_err_vs_oracle": worst}
+        if mode == "tuned":
+            out[mode]["options"] = {k: getattr(modes[mode], k) for k in
+                                    ("forcing", "forcing_pow", "forcing_slope", "forcing_max",
+                                     "forcing_fp32", "tile_rows", "graph")}
+            out[mode]["autotuner"] = tuning
     return out
 
 
@@ -631,6 +656,8 @@ def main():
                     help="Newton forcing terms (inexact early linear solves)")
     ap.add_argument("--forcing-slope", type=float, default=0.01)
     ap.add_argument("--forcing-max", type=float, default=1e-4)
+    ap.add_argument("--forcing-pow", type=int, default=1, choices=(1, 2),
+                    help="forcing term linear (1) or quadratic (2) in the last Newton update")
     ap.add_argument("--precond", default="tile", choices=("tile", "node"))
     ap.add_argument("--probe-cuts", default="",
                     help="comma-separated early-exit cuts (us): one more probe step per cut, "
@@ -648,7 +675,7 @@ def main():
                     help="extra workloads (comma-separated; '' for none)")
     ap.add_argument("--batch-variants", default="",
                     help="only the batched diodes, with these option sets (comma-separated of "
-                         "bicgstab, forcing, forcing_graph, graph, queue, queue2, queue4)")
+                         "bicgstab, forcing, quad, tuned, forcing_graph, graph, queue, queue2, queue4)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (the reference "
                          "kernel for scaling HBM traffic counters) and exit")
@@ -679,6 +706,9 @@ def main():
                 "forcing_graph": NewtonOptions(forcing=True, forcing_slope=0.1,
                                                forcing_max=1e-3, graph=True),
                 "graph": NewtonOptions(graph=True),
+                "quad": NewtonOptions(forcing=True, forcing_pow=2, forcing_slope=0.1,
+                                      forcing_max=1e-3),
+                "tuned": None,
                 "queue": (NewtonOptions(), dict(queue=True)),
                 "queue2": (NewtonOptions(), dict(queue=True, queue_depth=2)),
                 "queue4": (NewtonOptions(), dict(queue=True, queue_depth=4))}
@@ -690,7 +720,7 @@ def main():
     r = bench_trigate(args.steps, args.warmup, args.mode, args.points, args.probe,
                       args.forcing, args.precond, tune=not args.no_tune,
                       forcing_slope=args.forcing_slope, forcing_max=args.forcing_max,
-                      tile_rows=args.tile_rows,
+                      forcing_pow=args.forcing_pow, tile_rows=args.tile_rows,
                       probe_cuts=[float(c) for c in args.probe_cuts.split(",") if c])
     npts = r["points_per_step"]
     value = npts * args.steps / r["seconds"]

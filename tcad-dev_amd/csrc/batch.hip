@@ -1021,6 +1021,8 @@ static void free_batch(Batch *b) {
   for (hipEvent_t e : b->probe_ev) (void)hipEventDestroy(e);
   if (b->bk) (void)hipFree(b->bk);
   if (b->d_qmask) (void)hipFree(b->d_qmask);
+  if (b->d_bmap) (void)hipFree(b->d_bmap);
+  if (b->h_bmap) (void)hipHostFree(b->h_bmap);
   for (auto &g : b->graphs) {
     (void)hipGraphExecDestroy(g.exec);
     (void)hipGraphDestroy(g.graph);

@@ -79,6 +79,11 @@ struct KArgs {
   int scale_b;                      // init: divide b by Scal::nrm (refinement rhs)
   int parity;
   int late_stop;                    // 1: finished devices return at KSTOP (late_stop_ok)
+  // launch grid: grid workgroups; with bmap, workgroup i does the work of
+  // the batch's workgroup bmap[i] (the workgroups of the devices still
+  // iterating), else workgroup i is workgroup i (grid == nblk)
+  const int *bmap;
+  int grid;
 };
 
 // Where a finished device's workgroups return.  Early (before any load)
@@ -456,14 +461,15 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
 // the loads before it stay in bounds).
 #define KPROLOGUE                                   \
   __shared__ double sh[64];                         \
-  const int d = dev_index(K.W);                     \
+  const int blk = K.bmap ? K.bmap[blockIdx.x] : (int)blockIdx.x; \
+  const int d = K.W.single ? 0 : K.W.blk2dev[blk];  \
   Scal &S = K.scal[d];                              \
   const int done0 = S.done;                         \
   const DevInfo I = dev_info(K.W, d);               \
   KARGS_EARLY(I.off, I.nx, I.ny, I.px, I.blk0)      \
   KARGS_EARLY(done0, I.dim, I.py)                   \
   if (!K.late_stop && done0) return;                \
-  const bool lead = (blockIdx.x == I.blk0) && threadIdx.x == 0;
+  const bool lead = (blk == I.blk0) && threadIdx.x == 0;
 #define KSTOP \
   if (done0) return;
 
@@ -502,7 +508,7 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
   KPROLOGUE
   KSTOP
   (void)lead;
-  const NodeHood H = hood(I, blockIdx.x);
+  const NodeHood H = hood(I, blk);
   const long long N = K.N;
   if (S.cont) {
     // continue the previous sweep's process on this sweep's rhs: x = 0,
@@ -528,8 +534,8 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
     }
     block_sums<2>(q, sh, o);
     if (threadIdx.x == 0) {
-      K.part[(long long)K.nblk + blockIdx.x] = o[0];
-      K.part[3 * (long long)K.nblk + blockIdx.x] = o[1];
+      K.part[(long long)K.nblk + blk] = o[0];
+      K.part[3 * (long long)K.nblk + blk] = o[1];
     }
     return;
   }
@@ -558,7 +564,7 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
     for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
   }
   const double s = block_sum(rr, sh);
-  if (threadIdx.x == 0) K.part[(long long)K.nblk + blockIdx.x] = s;
+  if (threadIdx.x == 0) K.part[(long long)K.nblk + blk] = s;
 }
 
 // v = A phat; partial (rhat, v) at quantity 2 (the vectors are fixed, so
@@ -568,7 +574,7 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv(KArgs<T> K) {
   KARGS_EARLY(K.A, K.v[VPH], K.v[VRH], K.v[VV], K.N)
   KPROLOGUE
   (void)lead;
-  const NodeHood H = hood(I, blockIdx.x);
+  const NodeHood H = hood(I, blk);
   const long long N = K.N;
   double s0 = 0.0, y[3] = {0, 0, 0}, a3[3] = {0, 0, 0};
   if (H.valid) {
@@ -585,7 +591,7 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv(KArgs<T> K) {
     }
   }
   const double a = block_sum(s0, sh);
-  if (threadIdx.x == 0) K.part[2 * (long long)K.nblk + blockIdx.x] = a;
+  if (threadIdx.x == 0) K.part[2 * (long long)K.nblk + blk] = a;
 }
 
 // t = A shat; partials (t, s), (t, t), (rhat, s), (rhat, t) at quantities
@@ -596,7 +602,7 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv_t(KArgs<T> K) {
   KARGS_EARLY(K.A, K.v[VSH], K.v[VS], K.v[VRH], K.N)
   KPROLOGUE
   (void)lead;
-  const NodeHood H = hood(I, blockIdx.x);
+  const NodeHood H = hood(I, blk);
   const long long N = K.N;
   double q[4] = {0.0, 0.0, 0.0, 0.0}, y[3] = {0, 0, 0}, s3[3] = {0, 0, 0}, r3[3] = {0, 0, 0};
   if (H.valid) {
@@ -624,7 +630,7 @@ __global__ void __launch_bounds__(BS) k_bicg_spmv_t(KArgs<T> K) {
   block_sums<4>(q, sh, o);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) K.part[(long long)(3 + k) * K.nblk + blockIdx.x] = o[k];
+    for (int k = 0; k < 4; ++k) K.part[(long long)(3 + k) * K.nblk + blk] = o[k];
   }
 }
 
@@ -634,7 +640,7 @@ template <typename T, int PV>
 __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
   KARGS_EARLY(K.v[VR], K.v[VV], K.T4, K.T4s, K.N)
   KPROLOGUE
-  const NodeHood H = hood(I, blockIdx.x);
+  const NodeHood H = hood(I, blk);
   const long long N = K.N;
   const bool use_tile = PV != PV_NODE && K.tile;
   TileRows TR;
@@ -711,7 +717,7 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
   KARGS_EARLY(K.v[VX], K.v[VPH], K.v[VSH], K.v[VS], K.v[VT])
   KARGS_EARLY(K.v[VP], K.v[VV], K.T4, K.T4s, K.N)
   KPROLOGUE
-  const NodeHood H = hood(I, blockIdx.x);
+  const NodeHood H = hood(I, blk);
   const long long N = K.N;
   const bool use_tile = PV != PV_NODE && K.tile;
   TileRows TR;
@@ -764,7 +770,7 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
     for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
   }
   const double a = block_sum(rr, sh);
-  if (threadIdx.x == 0) K.part[(long long)K.nblk + blockIdx.x] = a;
+  if (threadIdx.x == 0) K.part[(long long)K.nblk + blk] = a;
   if (lead) {
     S.rho2[K.parity ^ 1] = rho_new;
     S.omega = om_b;
@@ -808,7 +814,7 @@ static int precond_variant(const Batch &b) {
 #define TCAD_PV_LAUNCH(name, probe_id)                                                     \
   template <typename T>                                                                    \
   static void launch_##name(Batch &b, const KArgs<T> &K) {                                 \
-    const dim3 g(b.nblk), t(BS);                                                           \
+    const dim3 g(K.grid), t(BS);                                                           \
     hipEvent_t e0 = nullptr, e1 = nullptr;                                                 \
     const bool pr = probe_id >= 0 && probe_events(b, probe_id, e0, e1);                    \
     switch (precond_variant(b)) {                                                          \
@@ -885,7 +891,53 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
   K.b = b.ir_r;
   K.scale_b = 1;
   K.late_stop = late_stop_ok(b);
+  K.bmap = nullptr;
+  K.grid = b.nblk;
   return K;
+}
+
+// Workgroups of a batch's devices still iterating, for the Krylov launches
+// after a check: the devices that converged no longer occupy the grid (an
+// early-exit workgroup still costs a dispatch and three dependent loads).
+// Done when the open devices hold under 3/4 of the current grid; returns
+// false (grid unchanged) otherwise or if the map cannot be placed.
+template <typename T>
+static bool compact_grid(Batch &b, KArgs<T> &K) {
+  if (b.ndev < 2 || b.use_graph) return false;
+  long long open = 0;
+  for (int d = 0; d < b.ndev; ++d)
+    if (!b.h_scal[d].done) open += b.info[d].nblk;
+  if (open == 0 || 4 * open >= 3 * (long long)K.grid) return false;
+  if (!b.d_bmap) {
+    if (hipMalloc(&b.d_bmap, sizeof(int) * 2 * (size_t)b.nblk) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    if (hipHostMalloc(&b.h_bmap, sizeof(int) * 2 * (size_t)b.nblk) != hipSuccess) {
+      (void)hipGetLastError();
+      b.h_bmap = nullptr;
+      return false;
+    }
+  }
+  // two halves used in turn: the copy into one may still be pending while
+  // the host writes the other (a check drains the stream before the host
+  // writes again, so two suffice)
+  int *h = b.h_bmap + (b.bmap_half ? b.nblk : 0);
+  int *dmap = b.d_bmap + (b.bmap_half ? b.nblk : 0);
+  b.bmap_half ^= 1;
+  int n = 0;
+  for (int d = 0; d < b.ndev; ++d) {
+    if (b.h_scal[d].done) continue;
+    const DevInfo &I = b.info[d];
+    for (int k = 0; k < I.nblk; ++k) h[n++] = I.blk0 + k;
+  }
+  if (hipMemcpyAsync(dmap, h, sizeof(int) * n, hipMemcpyHostToDevice, b.stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  K.bmap = dmap;
+  K.grid = n;
+  return true;
 }
 
 int launch_tile_setup(Batch &b, bool only_active) {
@@ -914,7 +966,7 @@ bool probe_events(Batch &b, int which, hipEvent_t &e0, hipEvent_t &e1) {
 
 template <typename T>
 static void launch_iteration(Batch &b, KArgs<T> K, int parity) {
-  const dim3 g(b.nblk), t(BS);
+  const dim3 g(K.grid), t(BS);
   K.parity = parity;
   hipEvent_t e0, e1;
   if (probe_events(b, 0, e0, e1))
@@ -998,7 +1050,8 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
                      b.ndev, rtol * rtol, maxit, per_dev ? 1 : 0, floor * floor, cont ? 1 : 0);
   b.launches++;
-  launch_k_bicg_init<T>(b, K);
+  KArgs<T> KL = K;                  // its grid shrinks to the open devices as they finish
+  launch_k_bicg_init<T>(b, KL);
   // Iterations are issued in chunks between host checks of the stop flags;
   // a check costs a host round trip with the stream drained, an iteration
   // issued after convergence four early-exit launches.  Consecutive Newton
@@ -1028,12 +1081,12 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
     int c = 0;
     if (b.use_graph && b.probe < 0) {
       if ((it & 1) && c < next && it < maxit) {
-        launch_iteration<T>(b, K, it & 1);
+        launch_iteration<T>(b, KL, it & 1);
         ++c;
         ++it;
       }
       while (next - c >= kGraphIts && it + kGraphIts <= maxit) {
-        hipGraphExec_t g = krylov_graph<T>(b, K);
+        hipGraphExec_t g = krylov_graph<T>(b, KL);
         if (!g) break;
         TCAD_HIP_CHECK(hipGraphLaunch(g, b.stream));
         b.launches += 4 * kGraphIts;
@@ -1041,9 +1094,9 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
         it += kGraphIts;
       }
     }
-    for (; c < next && it < maxit; ++c, ++it) launch_iteration<T>(b, K, it & 1);
+    for (; c < next && it < maxit; ++c, ++it) launch_iteration<T>(b, KL, it & 1);
     if (it >= maxit) {
-      KArgs<T> K1 = K;
+      KArgs<T> K1 = KL;
       K1.parity = it & 1;
       launch_k_bicg_s<T>(b, K1);
     }
@@ -1070,6 +1123,7 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
       }
     }
     if (open == 0 || it >= maxit) break;
+    compact_grid<T>(b, KL);
     next = step;
     if (rated && need > 0.0)
       next = std::min(step, std::max(4, (int)std::ceil(need * 1.125) + 2));

@@ -221,6 +221,8 @@ struct Batch {
   int *d_flag = nullptr;                         // [0] any, [1 + d] per-device error flag
   double *bk = nullptr;                          // point queue: last converged states (3 planes)
   int *d_qmask = nullptr;                        // point queue: per-device selection
+  int *d_bmap = nullptr, *h_bmap = nullptr;       // Krylov grid of the open devices (2 halves)
+  int bmap_half = 0;
   int *d_mask = nullptr;                         // per-device selection of the next solve
   const double *last_n = nullptr, *last_p = nullptr;   // state of the last assembly
   const double *last_psi = nullptr;
