@@ -1266,14 +1266,12 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   TRY(dalloc(&b->T4s, 3 * N));
   TRY(dalloc(&b->P16, (size_t)TILE_NODES * N));
   {
-    // equilibrium PCG: Jacobi (default), or tile block-Jacobi with
-    // TCADHIP_PCG_PRECOND=tile (half the PCG iterations on the MOSFET and
-    // 4% on its sweep, but its equilibrium state differs from Jacobi's in the
-    // last bits, which moves the Newton path of a reverse-biased
-    // Crank-Nicolson diode ramp off the reference's: 5 instead of 8
-    // iterations at one step)
+    // equilibrium PCG: tile block-Jacobi (default: half the PCG
+    // iterations of Jacobi on the MOSFET, its sweep 5% and the tri-gate
+    // headline 2% faster), or Jacobi with TCADHIP_PCG_PRECOND=jacobi (the
+    // two equilibrium states differ in the last bits)
     const char *e = std::getenv("TCADHIP_PCG_PRECOND");
-    b->pcg_tile = (e && std::strcmp(e, "tile") == 0) ? 1 : 0;
+    b->pcg_tile = (e && std::strcmp(e, "jacobi") == 0) ? 0 : 1;
   }
   TRY(dalloc(&b->cand, 3 * N));
   TRY(dalloc(&b->d_Icand, (size_t)std::max(1, cglobal)));
