@@ -350,7 +350,7 @@ def bench_mosfet(steps=2, warmup=1, modes=("bicgstab", "bicgstab_mixed", "tuned"
         out[mode] = {"points_per_s": round(len(spec.sweep) * steps / dt, 3),
                      "options": {k: getattr(opts, k) for k in
                                  ("linsolve", "forcing", "forcing_pow", "forcing_slope",
-                                  "forcing_max", "forcing_fp32", "tile_rows", "graph")},
+                                  "forcing_max", "forcing_fp32", "tile_rows", "graph", "tile_lag")},
                      "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
                      "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
                      "newton_per_point": its,
@@ -539,7 +539,7 @@ def bench_batch(lanes=1, sample=8, modes=None):
         if mode == "tuned":
             out[mode]["options"] = {k: getattr(modes[mode], k) for k in
                                     ("forcing", "forcing_pow", "forcing_slope", "forcing_max",
-                                     "forcing_fp32", "tile_rows", "graph")}
+                                     "forcing_fp32", "tile_rows", "graph", "tile_lag")}
             out[mode]["autotuner"] = tuning
     return out
 
@@ -747,7 +747,8 @@ def main():
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "solver_options": {k: getattr(r["opts"], k) for k in
                            ("linsolve", "linsolve_rtol", "precond", "forcing", "forcing_pow",
-                            "forcing_slope", "forcing_max", "forcing_fp32", "tile_rows", "graph")},
+                            "forcing_slope", "forcing_max", "forcing_fp32", "tile_rows", "graph",
+                            "tile_lag")},
         "autotuner": r["tuning"],
         "newton_per_point": [x[1] for x in r["record"]],
         "newton_exact_per_point": r.get("newton_exact"),

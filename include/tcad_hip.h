@@ -411,7 +411,11 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
  *                            least, in each refinement sweep of a Newton
  *                            solve [1e-6];
  *   TCAD_TUNE_FORCING_POW    q: 1 (eta linear in the last Newton update
- *                            measure err) or 2 (quadratic) [1].
+ *                            measure err) or 2 (quadratic) [1];
+ *   TCAD_TUNE_TILE_LAG       a Newton step of tcad_solve_bias keeps the
+ *                            tile inverses of the step before when that
+ *                            step's update measure was below this (0: every
+ *                            step inverts its tiles) [0].
  * Has no reference counterpart (the reference's Krylov tolerance is fixed,
  * linsolve.py:725-727). */
 #define TCAD_TUNE_TILE_ROWS 1
@@ -422,6 +426,7 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
 #define TCAD_TUNE_IR_CONTINUE 6
 #define TCAD_TUNE_FP32_INNER 7
 #define TCAD_TUNE_FORCING_POW 8
+#define TCAD_TUNE_TILE_LAG 9
 int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
 
 /* Bench instrumentation, in-solver kernel timing: from this call on,

@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(BS) k_newton_update(UArgs U, const double *y, 
 // whose linear solve ended above the acceptance residual stops here, before
 // its step is applied (linsolve.py:973-982 refuses such an iterate).
 __global__ void k_newton_check(UArgs U, double tol, double accept2, double rtol, double eta_max,
-                               int forcing, double eta_c, int eta_pow) {
+                               int forcing, double eta_c, int eta_pow, double tile_lag) {
   __shared__ double sh[BS / 64];
   const int d = blockIdx.x;
   const DevInfo &I = U.info[d];
@@ -117,6 +117,10 @@ __global__ void k_newton_check(UArgs U, double tol, double accept2, double rtol,
     // linearly (eta_pow 1) or with its square (eta_pow 2, which keeps
     // Newton's quadratic convergence) (the converging step is refined to
     // the rounding floor before it is applied)
+    // the next Newton step keeps this step's tile inverses when the state
+    // moved by less than tile_lag (a preconditioner from a nearby Jacobian;
+    // the FP64 refinement still decides the solve)
+    S.tile_keep = (tile_lag > 0.0 && m < tile_lag) ? 1 : 0;
     if (forcing) {
       const double eta = fmin(fmax(eta_c * (eta_pow == 2 ? m * m : m), rtol), eta_max);
       S.eta2 = eta * eta;
@@ -427,6 +431,7 @@ __global__ void k_set_active(Scal *scal, int ndev, const int *mask, double eta0)
   S.ir_it = 0;
   S.status = 0;
   S.err = 0.0;
+  S.tile_keep = 0;
 }
 
 __global__ void k_ir_reset(Scal *scal, int ndev) {
@@ -633,7 +638,7 @@ static int bias_newton_step(Batch &b, const UArgs &U, const NewtonCfg &c, int ma
                      c.max_dpsi, 0, psi, n, p);
   hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, c.tol,
                      c.accept * c.accept, c.rtol, c.eta_max, c.forcing ? 1 : 0, b.forcing_c,
-                     b.forcing_pow);
+                     b.forcing_pow, b.tile_lag);
   b.launches += 2;
   TCAD_HIP_CHECK(hipGetLastError());
   if ((rc = sync_scalars(b))) return rc;
@@ -689,6 +694,7 @@ __global__ void k_rearm(Scal *scal, int ndev, const int *mask, double eta0) {
   S.sel = mask[d] != 0;
   if (!S.sel) return;
   S.active = 1;
+  S.tile_keep = 0;
   S.eta2 = eta0 * eta0;
   S.result = 0;
   S.newton_it = 0;
@@ -911,7 +917,7 @@ static int transient_impl(Batch &b, int max_iter, double tol, double max_dpsi, i
     hipLaunchKernelGGL(k_newton_update, dim3(b.nblk), dim3(BS), 0, b.stream, U, b.y64, psi, n, p,
                        max_dpsi, 0, psi, n, p);
     hipLaunchKernelGGL(k_newton_check, dim3(b.ndev), dim3(BS), 0, b.stream, U, tol,
-                       accept * accept, rtol, 1.0, 0, b.forcing_c, b.forcing_pow);
+                       accept * accept, rtol, 1.0, 0, b.forcing_c, b.forcing_pow, 0.0);
     b.launches += 2;
     TCAD_HIP_CHECK(hipGetLastError());
     if ((rc = sync_scalars(b))) return rc;
@@ -1672,6 +1678,10 @@ int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
     case TCAD_TUNE_FORCING_POW:
       if (value != 1.0 && value != 2.0) break;
       b.forcing_pow = (int)value;
+      return TCAD_OK;
+    case TCAD_TUNE_TILE_LAG:
+      if (!(value >= 0.0 && value < 1.0)) break;
+      b.tile_lag = value;
       return TCAD_OK;
     case TCAD_TUNE_FP32_INNER:
       if (!(value >= 1e-7 && value <= 1e-2)) break;

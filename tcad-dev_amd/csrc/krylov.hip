@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
   const int nt = threadIdx.x % 64, half = threadIdx.x / 64, c0 = half * HCOL;
   const int patch = blockIdx.x / SPP, sub = blockIdx.x % SPP;
   const int d = W.single ? 0 : W.blk2dev[patch];
-  if (scal && !scal[d].active) return;
+  if (scal && (!scal[d].active || scal[d].tile_keep)) return;
   const DevInfo I = dev_info(W, d);
   const unsigned local = (unsigned)(patch - I.blk0);
   const unsigned bx = local % (unsigned)I.px, t2 = local / (unsigned)I.px;

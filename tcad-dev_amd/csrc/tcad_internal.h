@@ -113,6 +113,7 @@ struct Scal {
   int newton_it, newton_done, kry_total, ir_it, ir_maxsweep;
   int conv_now;                     // this Newton step's update passed the test
   int saved_active;                 // active flag kept across one refinement
+  int tile_keep;                    // this Newton step keeps the last step's tile inverses
   int cont;                         // this inner solve continues the last one's Krylov process
   int cont_ok;                      // the last inner solve may be continued
   int newton_active;                // Newton-level flag kept across the final-step refinement
@@ -232,6 +233,7 @@ struct Batch {
   int tile_rows = 0;
   double forcing_c = 0.01, forcing_max = 1e-4;
   int forcing_pow = 1;              // eta = c * err^forcing_pow
+  double tile_lag = 0.0;            // keep the tile inverses after a Newton update below this
   // forcing solves whose target is loose (every active device's eta >=
   // 1e-5) run BiCGStab on the FP32-stored matrix inside the FP64 refinement
   int forcing_fp32 = 0;

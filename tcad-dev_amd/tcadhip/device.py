@@ -61,7 +61,8 @@ class NewtonOptions:
     # sweeps continue the previous sweep's BiCGStab ("fp32": only inner
     # solves on the FP32-stored matrix, "all", "never"); fp32_inner: the
     # per-sweep residual reduction of inner solves on the FP32 matrix;
-    # forcing_pow: eta = forcing_slope * err^forcing_pow
+    # forcing_pow: eta = forcing_slope * err^forcing_pow; tile_lag: Newton
+    # steps after an update below it keep the tile inverses of the step before
     forcing_slope: float = 0.01
     forcing_max: float = 1e-4
     forcing_fp32: bool = False
@@ -70,6 +71,7 @@ class NewtonOptions:
     ir_continue: str = "fp32"
     fp32_inner: float = 1e-6
     forcing_pow: int = 1
+    tile_lag: float = 0.0
 
     def apply_tuning(self, handle):
         L = _lib.lib()
@@ -94,6 +96,8 @@ class NewtonOptions:
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FP32_INNER, float(self.fp32_inner)),
                    "tcad_set_tuning")
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FORCING_POW, float(self.forcing_pow)),
+                   "tcad_set_tuning")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_TILE_LAG, float(self.tile_lag)),
                    "tcad_set_tuning")
 
     def lin_mode(self):

@@ -16,7 +16,9 @@ floor, so the terminal currents agree to rounding):
     registers before the reductions (fewer waves per SIMD) or read at the
     point of use (full occupancy);
   * whether the BiCGStab iterations go out as replays of a captured HIP
-    graph (tried last, on the best configuration so far).
+    graph, and whether a Newton step keeps the tile inverses of the step
+    before when that step's update was small (tried last, each on the best
+    configuration so far).
 
 A candidate is measured by solving one bias point from the same saved
 state (psi, n, p restored on the device before each trial), twice, the
@@ -55,8 +57,11 @@ CANDIDATES = (
          tile_rows="auto"),
 )
 
+# tried on the best configuration of the first stage, in turn
+SECOND_STAGE = (dict(graph=True), dict(tile_lag=1e-2), dict(tile_lag=1e-1))
+
 _KNOBS = ("forcing", "forcing_pow", "forcing_slope", "forcing_max", "forcing_fp32", "tile_rows",
-          "graph")
+          "graph", "tile_lag")
 _cache = {}
 
 
@@ -81,16 +86,18 @@ def tune(dev, bias, opts=None, candidates=CANDIDATES, repeats=2, say=None, newto
     rows = []
     ref_I = ref_its = None
     try:
-        for cand in list(candidates) + [None]:
-            if cand is None:
-                # second stage: the winner so far, its iterations issued as
-                # replays of a captured HIP graph
+        for cand in list(candidates) + list(SECOND_STAGE):
+            if not isinstance(cand, dict) or "forcing" not in cand:
+                # second stage: the winner so far with one more knob set (its
+                # iterations issued as replays of a captured HIP graph; its
+                # tile inverses kept over small Newton updates)
                 good = [r for r in rows if r["accepted"]]
                 if not good:
                     break
                 w = min(good, key=lambda r: r["seconds"])
+                extra = cand
                 cand = {k: w[k] for k in _KNOBS if k in w}
-                cand["graph"] = True
+                cand.update(extra)
             o = replace(base, **cand)
             best_t, res = None, None
             for _ in range(repeats):
