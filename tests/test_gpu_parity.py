@@ -133,12 +133,14 @@ def _equilibrated_counts(name):
 
 SWEEP_MODES = ([(n, m) for n in SWEEPS for m in ("bicgstab", "bicgstab_mixed")]
                + [(n, m) for n in SWEEPS if n.startswith("trigate")
-                  for m in ("forcing_q1", "forcing_q2", "forcing_q2_fp32")])
+                  for m in ("forcing_q1", "forcing_q2", "forcing_q2_fp32", "forcing_q2_lag")])
 
 # Newton forcing terms eta = clamp(c * err^q, rtol, eta_max) with c = 0.1,
 # eta_max = 1e-3 (the autotuner's candidates on the tri-gate)
 FORCING = {"forcing_q2": dict(forcing_pow=2), "forcing_q1": dict(forcing_pow=1),
-           "forcing_q2_fp32": dict(forcing_pow=2, forcing_fp32=True)}
+           "forcing_q2_fp32": dict(forcing_pow=2, forcing_fp32=True),
+           # tile inverses kept across Newton updates below 1e-2 (the headline's choice)
+           "forcing_q2_lag": dict(forcing_pow=2, tile_lag=1e-2)}
 
 
 @pytest.mark.parametrize("name,mode", SWEEP_MODES)
