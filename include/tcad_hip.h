@@ -415,7 +415,16 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
  *   TCAD_TUNE_TILE_LAG       a Newton step of tcad_solve_bias keeps the
  *                            tile inverses of the step before when that
  *                            step's update measure was below this (0: every
- *                            step inverts its tiles) [0].
+ *                            step inverts its tiles) [0];
+ *   TCAD_TUNE_LINEAR_FALLBACK 1: a Newton solve of tcad_solve_bias /
+ *                            tcad_solve_queue whose refinement ends
+ *                            unconverged is solved again, for that iteration
+ *                            only, by restarted GMRES(50) with ILU(0) on the
+ *                            device's CSR form of the same system (the
+ *                            counterpart of the reference's per-iteration
+ *                            direct fallback, device3d.py:2154-2168); only
+ *                            if that fails too does the device stop with
+ *                            TCAD_NEWTON_LINEAR_FAIL; 0: stop at once [1].
  * Has no reference counterpart (the reference's Krylov tolerance is fixed,
  * linsolve.py:725-727). */
 #define TCAD_TUNE_TILE_ROWS 1
@@ -427,6 +436,7 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
 #define TCAD_TUNE_FP32_INNER 7
 #define TCAD_TUNE_FORCING_POW 8
 #define TCAD_TUNE_TILE_LAG 9
+#define TCAD_TUNE_LINEAR_FALLBACK 10
 int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
 
 /* Bench instrumentation, in-solver kernel timing: from this call on,

@@ -543,6 +543,120 @@ int check_flag(Batch &b) {
 // ---------------------------------------------------------------------------
 // Newton drivers
 // ---------------------------------------------------------------------------
+// ---- per-iteration linear fallback ------------------------------------------
+// The CSR form of device d's equilibrated, Dirichlet-eliminated Newton
+// system (the stencil blocks of A64), unknowns interleaved u = 3 k + c
+// (k the device-local node), columns sorted: z-, y-, x- neighbour, the
+// node, x+, y+, z+ neighbour.  Row r of node k holds the node block's
+// three entries and, per neighbour that exists, psi<-psi (r = 0),
+// n<-psi, n<-n (r = 1) or p<-psi, p<-p (r = 2).
+__global__ void k_fb_gather(long long nnz, const long long *src, const double *A, double *val) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz) val[k] = A[src[k]];
+}
+
+// interleaved <-> plane layout of one device's three unknowns per node
+__global__ void k_fb_rhs(long long nd, long long N, long long off, const double *bs,
+                         double *bv) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nd) return;
+  for (int c = 0; c < 3; ++c) bv[3 * k + c] = bs[c * N + off + k];
+}
+
+__global__ void k_fb_sol(long long nd, long long N, long long off, const double *xv, double *y) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nd) return;
+  for (int c = 0; c < 3; ++c) y[c * N + off + k] = xv[3 * k + c];
+}
+
+static int build_fallback_csr(Batch &b, int d) {
+  const DevInfo &I = b.info[d];
+  const long long N = b.ntot, nd = I.n;
+  const int nx = I.nx, ny = I.ny, nz = I.dim == 3 ? I.nz : 1;
+  if (3 * nd >= (1LL << 31) / 16) {
+    set_error("linear fallback: device too large for the int32 CSR form");
+    return TCAD_E_INVALID;
+  }
+  std::vector<int> ptr(3 * nd + 1, 0), idx;
+  std::vector<long long> src;
+  idx.reserve((size_t)nd * (I.dim == 3 ? 39 : 29));
+  src.reserve(idx.capacity());
+  const long long sxy = (long long)nx * ny;
+  for (long long k = 0; k < nd; ++k) {
+    const int i = (int)(k % nx), j = (int)((k / nx) % ny), kk = (int)(k / sxy);
+    const long long g = I.off + k;
+    for (int r = 0; r < 3; ++r) {
+      auto nbr = [&](int dir, long long kn) {
+        const int e = 9 + 5 * dir;
+        if (r == 0) {
+          idx.push_back((int)(3 * kn));
+          src.push_back((long long)e * N + g);
+        } else if (r == 1) {
+          idx.push_back((int)(3 * kn));
+          src.push_back((long long)(e + 1) * N + g);
+          idx.push_back((int)(3 * kn + 1));
+          src.push_back((long long)(e + 2) * N + g);
+        } else {
+          idx.push_back((int)(3 * kn));
+          src.push_back((long long)(e + 3) * N + g);
+          idx.push_back((int)(3 * kn + 2));
+          src.push_back((long long)(e + 4) * N + g);
+        }
+      };
+      if (I.dim == 3 && kk > 0) nbr(4, k - sxy);
+      if (j > 0) nbr(2, k - nx);
+      if (i > 0) nbr(0, k - 1);
+      for (int c = 0; c < 3; ++c) {
+        idx.push_back((int)(3 * k + c));
+        src.push_back((long long)(3 * r + c) * N + g);
+      }
+      if (i < nx - 1) nbr(1, k + 1);
+      if (j < ny - 1) nbr(3, k + nx);
+      if (I.dim == 3 && kk < nz - 1) nbr(5, k + sxy);
+      ptr[3 * k + r + 1] = (int)idx.size();
+    }
+  }
+  Batch::FallbackCsr &F = b.fb[d];
+  F.n = (int)(3 * nd);
+  F.nnz = (long long)idx.size();
+  TCAD_HIP_CHECK(hipMalloc(&F.ptr, sizeof(int) * ptr.size()));
+  TCAD_HIP_CHECK(hipMalloc(&F.idx, sizeof(int) * idx.size()));
+  TCAD_HIP_CHECK(hipMalloc(&F.src, sizeof(long long) * src.size()));
+  TCAD_HIP_CHECK(hipMalloc(&F.val, sizeof(double) * idx.size()));
+  TCAD_HIP_CHECK(hipMalloc(&F.b, sizeof(double) * F.n));
+  TCAD_HIP_CHECK(hipMalloc(&F.x, sizeof(double) * F.n));
+  TCAD_HIP_CHECK(hipMemcpy(F.ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
+  TCAD_HIP_CHECK(hipMemcpy(F.idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+  TCAD_HIP_CHECK(hipMemcpy(F.src, src.data(), sizeof(long long) * src.size(),
+                           hipMemcpyHostToDevice));
+  return TCAD_OK;
+}
+
+int linear_fallback(Batch &b, int d, double rtol, int32_t *h_iters, double *h_resid) {
+  if ((int)b.fb.size() != b.ndev) b.fb.resize(b.ndev);
+  int rc;
+  if (!b.fb[d].ptr && (rc = build_fallback_csr(b, d))) return rc;
+  const Batch::FallbackCsr &F = b.fb[d];
+  const DevInfo &I = b.info[d];
+  const long long nd = I.n;
+  hipLaunchKernelGGL(k_fb_gather, dim3((unsigned)((F.nnz + 255) / 256)), dim3(256), 0, b.stream,
+                     F.nnz, F.src, b.A64, F.val);
+  hipLaunchKernelGGL(k_fb_rhs, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, b.stream, nd,
+                     b.ntot, I.off, b.bs, F.b);
+  TCAD_HIP_CHECK(hipMemsetAsync(F.x, 0, sizeof(double) * F.n, b.stream));
+  b.launches += 2;
+  TCAD_HIP_CHECK(hipGetLastError());
+  rc = tcad_csr_solve(F.n, F.ptr, F.idx, F.val, F.b, F.x, TCAD_CSR_GMRES, TCAD_CSR_PREC_ILU0, 1,
+                      rtol, 4000, 50, (void *)b.stream, h_iters, h_resid);
+  b.fallbacks++;
+  if (rc != TCAD_OK) return rc;
+  hipLaunchKernelGGL(k_fb_sol, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, b.stream, nd,
+                     b.ntot, I.off, F.x, b.y64);
+  b.launches++;
+  TCAD_HIP_CHECK(hipGetLastError());
+  return TCAD_OK;
+}
+
 // Refine the solves of the devices whose step just passed the convergence
 // test (Scal.conv_now), one 1e-6 inner sweep at a time, until the contact
 // currents at the candidate state balance to 1e-8 of the largest (toward
@@ -628,7 +742,7 @@ static int bias_newton_step(Batch &b, const UArgs &U, const NewtonCfg &c, int ma
   if ((rc = launch_assemble(b, psi, n, p, b.F, nullptr, c.mixed || c.loose32, true))) return rc;
   if (b.tile && (rc = launch_tile_setup(b, true))) return rc;
   if ((rc = launch_flags_to_status(b))) return rc;
-  if ((rc = ir_solve(b, fp32_now, true, c.rtol, 16, c.maxit, inner, true))) return rc;
+  if ((rc = ir_solve(b, fp32_now, true, c.rtol, 16, c.maxit, inner, true, true))) return rc;
   // measure the step, and before applying a step that passes the test
   // refine its solve to the FP64 rounding floor: the converged state is
   // then the reference's (exact Newton steps converge quadratically, so
@@ -1028,6 +1142,11 @@ static void free_batch(Batch *b) {
   if (b->bk) (void)hipFree(b->bk);
   if (b->d_qmask) (void)hipFree(b->d_qmask);
   if (b->d_bmap) (void)hipFree(b->d_bmap);
+  for (auto &F : b->fb) {
+    void *fp[] = {F.ptr, F.idx, F.src, F.val, F.b, F.x};
+    for (void *q : fp)
+      if (q) (void)hipFree(q);
+  }
   if (b->h_bmap) (void)hipHostFree(b->h_bmap);
   for (auto &g : b->graphs) {
     (void)hipGraphExecDestroy(g.exec);
@@ -1678,6 +1797,10 @@ int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
     case TCAD_TUNE_FORCING_POW:
       if (value != 1.0 && value != 2.0) break;
       b.forcing_pow = (int)value;
+      return TCAD_OK;
+    case TCAD_TUNE_LINEAR_FALLBACK:
+      if (value != 0.0 && value != 1.0) break;
+      b.lin_fallback = (int)value;
       return TCAD_OK;
     case TCAD_TUNE_TILE_LAG:
       if (!(value >= 0.0 && value < 1.0)) break;

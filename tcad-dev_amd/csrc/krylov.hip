@@ -1314,7 +1314,7 @@ __global__ void k_ir_restore(Scal *scal, int ndev) {
 }
 
 int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int inner_maxit,
-             double inner_rtol, bool per_dev) {
+             double inner_rtol, bool per_dev, bool fallback) {
   IRArgs R{};
   R.W = make_where(b);
   R.N = b.ntot; R.nblk = b.nblk;
@@ -1386,6 +1386,35 @@ int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int i
     }
     hipLaunchKernelGGL(k_ir_update, dim3(b.nblk), dim3(BS), 0, b.stream, R);
     b.launches++;
+  }
+  // a device whose refinement stopped unconverged (its last check, read
+  // back above, set status 2) above the acceptance residual the Newton
+  // check applies (linsolve.py:973-982: max(rtol, 1e-6), or the device's
+  // forcing target) is solved once more by the fallback method, for this
+  // Newton iteration only (device3d.py:2154-2168 falls back to a direct
+  // solve there); below it the step is taken as the reference takes an
+  // accepted iterative solve
+  if (fallback && b.lin_fallback) {
+    const double acc = std::max(rtol, 1e-6);
+    for (int d = 0; d < b.ndev; ++d) {
+      Scal &S = b.h_scal[d];
+      if (!S.saved_active || S.status != 2) continue;
+      if (S.rr <= std::max(S.eta2, acc * acc) * S.bb) continue;
+      const double target = per_dev ? std::sqrt(std::max(S.eta2, rtol * rtol)) : rtol;
+      int32_t its = 0;
+      double res = 0.0;
+      const int rc = linear_fallback(b, d, target, &its, &res);
+      if (rc != TCAD_OK && rc != TCAD_E_NOT_CONVERGED) return rc;
+      if (rc == TCAD_OK) {
+        // the device's scalars as if its refinement had ended there
+        S.rr = res * res * S.bb;
+        S.status = 0;
+        S.kry_total += its;
+        TCAD_HIP_CHECK(hipMemcpyAsync(b.d_scal + d, &S, sizeof(Scal), hipMemcpyHostToDevice,
+                                      b.stream));
+        TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+      }
+    }
   }
   hipLaunchKernelGGL(k_ir_restore, gd, td, 0, b.stream, b.d_scal, b.ndev);
   b.launches++;

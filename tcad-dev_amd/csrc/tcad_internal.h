@@ -257,6 +257,20 @@ struct Batch {
     hipGraphExec_t exec;
   };
   std::vector<GraphEntry> graphs;
+  // per-iteration linear fallback (tcad_set_tuning TCAD_TUNE_LINEAR_FALLBACK):
+  // a Newton solve whose refinement ends unconverged is solved again, for
+  // that iteration, by restarted GMRES with ILU(0) on the device's CSR form
+  // of the same equilibrated system (the pattern built on first use)
+  int lin_fallback = 1;
+  long long fallbacks = 0;          // fallback solves run (all devices, whole life)
+  struct FallbackCsr {
+    int n = 0;                      // 3 x the device's nodes
+    long long nnz = 0;
+    int *ptr = nullptr, *idx = nullptr;
+    long long *src = nullptr;       // value k = A64[src[k]]
+    double *val = nullptr, *b = nullptr, *x = nullptr;
+  };
+  std::vector<FallbackCsr> fb;
   // in-solver timing of one BiCGStab kernel (tcad_probe_kernel): every
   // launch of kernel `probe` (0 SpMV, 1 alpha/s, 2 second SpMV, 3 update;
   // -1 off) is bracketed by a pair of events on the batch stream
@@ -325,7 +339,10 @@ int bicgstab(Batch &b, bool fp32, double rtol, int maxit, bool zero_guess, int c
 // per_dev: every device's own target S.eta2 replaces rtol (and the inner
 // BiCGStab aims at it, not below inner_rtol for fp32)
 int ir_solve(Batch &b, bool fp32, bool restart, double rtol, int maxsweep, int inner_maxit,
-             double inner_rtol, bool per_dev = false);
+             double inner_rtol, bool per_dev = false, bool fallback = false);
+// the per-iteration fallback solve of device d (batch.hip): rtol relative
+// residual target; returns TCAD_OK and the solution in y64, or an error
+int linear_fallback(Batch &b, int d, double rtol, int32_t *h_iters, double *h_resid);
 int pcg_poisson(Batch &b, double rtol, int maxit);
 // inverses of the tile diagonal blocks of the last assembled matrix
 int launch_tile_setup(Batch &b, bool only_active);

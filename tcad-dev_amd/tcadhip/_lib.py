@@ -36,6 +36,7 @@ TUNE_IR_CONTINUE = 6
 TUNE_FP32_INNER = 7
 TUNE_FORCING_POW = 8
 TUNE_TILE_LAG = 9
+TUNE_LINEAR_FALLBACK = 10
 
 CSR_BICGSTAB = 0
 CSR_GMRES = 1

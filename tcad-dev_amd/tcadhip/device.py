@@ -62,7 +62,10 @@ class NewtonOptions:
     # solves on the FP32-stored matrix, "all", "never"); fp32_inner: the
     # per-sweep residual reduction of inner solves on the FP32 matrix;
     # forcing_pow: eta = forcing_slope * err^forcing_pow; tile_lag: Newton
-    # steps after an update below it keep the tile inverses of the step before
+    # steps after an update below it keep the tile inverses of the step before;
+    # linear_fallback: an unconverged Newton solve is redone, for that
+    # iteration, by GMRES + ILU(0) on the device's CSR system
+    # (device3d.py:2154-2168's per-iteration fallback)
     forcing_slope: float = 0.01
     forcing_max: float = 1e-4
     forcing_fp32: bool = False
@@ -72,6 +75,7 @@ class NewtonOptions:
     fp32_inner: float = 1e-6
     forcing_pow: int = 1
     tile_lag: float = 0.0
+    linear_fallback: bool = True
 
     def apply_tuning(self, handle):
         L = _lib.lib()
@@ -99,6 +103,8 @@ class NewtonOptions:
                    "tcad_set_tuning")
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_TILE_LAG, float(self.tile_lag)),
                    "tcad_set_tuning")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_LINEAR_FALLBACK,
+                                     1.0 if self.linear_fallback else 0.0), "tcad_set_tuning")
 
     def lin_mode(self):
         m = self.linsolve

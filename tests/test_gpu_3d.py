@@ -95,3 +95,31 @@ def test_ragged_patches_tile_and_node_precond(shape):
         assert np.abs(cur_t[p] - ocur[p]).max() <= 1e-6 * big, (p, cur_t[p], ocur[p])
     assert np.abs(dt.psi - dn.psi).max() <= 1e-9
     assert np.abs(dt.psi - ost[1][0]).max() <= 1e-9
+
+
+def test_linear_fallback_3d_vs_reference():
+    """Every Newton solve of the reduced tri-gate through the per-iteration
+    fallback (BiCGStab cut to one iteration per refinement sweep, so each
+    solve is redone by GMRES + ILU(0) on the device's CSR system, as
+    device3d.py:2154-2168 redoes a failed iterative solve directly): the
+    first two points of the reference's Vd = 0.05 V sweep, Newton counts
+    within one, every current within 1e-6 of the largest (the contacts
+    share nodes: no floor)."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a HIP device")
+    from golden.fixtures import load
+    from tcadhip.device import NewtonOptions, device_from_spec
+    spec, d = load("trigate_24x18x14_vd005_sweep.npz")
+    print("[fallback-3d] 24x18x14 tri-gate, every solve through GMRES + ILU(0)", flush=True)
+    dev = device_from_spec(spec)
+    dev.solve_equilibrium(NewtonOptions(max_iter=400))
+    weak = NewtonOptions(lin_maxiter=1)
+    for p, bias in enumerate(spec.sweep[:2]):
+        dev.solve_bias(bias, weak)
+        cur = np.array([dev.terminal_current(c.name) for c in spec.contacts])
+        ref = d["currents"][p]
+        print(f"  point {p}: Newton {dev.last_iterations} reference "
+              f"{int(d['newton_iterations'][p])}", flush=True)
+        assert abs(dev.last_iterations - int(d["newton_iterations"][p])) <= 1
+        assert np.all(np.abs(cur - ref) <= 1e-6 * np.abs(ref).max()), (cur, ref)

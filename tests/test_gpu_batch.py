@@ -146,8 +146,9 @@ def test_explicit_offsets_match_packed():
 def test_linear_failure_is_reported_per_device():
     configs, sweep, _lib, GPUBatch, NewtonOptions, device_from_spec, setup_from_spec = _mods()
     specs = _small_diodes(2, seed=9)
-    _say("[linfail] 1 Krylov iteration per refinement sweep: the device must stop, not step")
-    opts = NewtonOptions(lin_maxiter=1)
+    _say("[linfail] 1 Krylov iteration per refinement sweep, no fallback: the device must stop, "
+         "not step")
+    opts = NewtonOptions(lin_maxiter=1, linear_fallback=False)
     B = GPUBatch([setup_from_spec(s) for s in specs])
     B.solve_equilibrium(NewtonOptions())
     psi0 = B.psi.clone()
@@ -162,3 +163,39 @@ def test_linear_failure_is_reported_per_device():
     with pytest.raises(_lib.LinearSolveError):
         dev.solve_bias(specs[0].sweep[0], opts)
     del psi0
+
+
+def test_linear_fallback_per_iteration():
+    """With BiCGStab cut to one iteration per refinement sweep every Newton
+    solve fails, and each is solved again for that iteration by the
+    fallback (GMRES + ILU(0) on the device's CSR system), as the
+    reference's 3D Newton loop falls back to a direct solve per iteration
+    (device3d.py:2154-2168): the sweep then converges with the counts and
+    currents of the plain path (the fallback solves the same system to the
+    same target), batched and alone alike."""
+    configs, sweep, _lib, GPUBatch, NewtonOptions, device_from_spec, setup_from_spec = _mods()
+    specs = _small_diodes(2, seed=9)
+    _say("[linfallback] 2 diodes, every Newton solve through the fallback")
+    weak = NewtonOptions(lin_maxiter=1)
+    for s in specs:
+        dev = device_from_spec(s)
+        dev.solve_equilibrium()
+        ref = device_from_spec(s)
+        ref.solve_equilibrium()
+        for bias in s.sweep[:3]:
+            dev.solve_bias(bias, weak)
+            ref.solve_bias(bias, NewtonOptions())
+            assert dev.last_converged
+            assert abs(dev.last_iterations - ref.last_iterations) <= 1
+            got = np.array([dev.terminal_current(c.name) for c in s.contacts])
+            want = np.array([ref.terminal_current(c.name) for c in s.contacts])
+            big = np.abs(want).max()
+            floor = abs(want.sum()) + abs(got.sum())
+            assert np.all(np.abs(got - want) <= 1e-6 * big + floor), (got, want)
+    B = GPUBatch([setup_from_spec(s) for s in specs])
+    B.solve_equilibrium(NewtonOptions())
+    for s, sp in zip(B.setups, specs):
+        for nm, v in sp.sweep[0].items():
+            s.bcs[nm].V = float(v)
+    its, conv, kry, status = B.solve_bias(weak)
+    assert list(status) == [_lib.NEWTON_OK] * 2 and conv.all()
