@@ -186,8 +186,12 @@ int tcad_export_jacobian(tcad_batch *b, double *h_blocks, double *h_rowscale,
                          double *h_rhs, int32_t *h_nentries);
 
 /* Solve the last assembled Newton system; writes the unscaled update du
- * (3 planes of N_total) to d_du.  h_iters / h_resid (length ndev)
- * receive Krylov iterations and final relative residual per device.
+ * (3 planes of N_total) to d_du, unscaled with the n and p buffers that
+ * tcad_assemble was given (they must still hold that state).  lin_mode
+ * selects the precision and the preconditioner (tcad_assemble built the
+ * tile inverses).  Without an assembled system: TCAD_E_INVALID.  h_iters /
+ * h_resid (length ndev) receive Krylov iterations and final relative
+ * residual per device.
  * Replaces: linsolve.solve_linear (pytcad/pytcad/linsolve.py:725-985)
  * for the node-block-Jacobi Krylov methods. */
 int tcad_solve_linear(tcad_batch *b, int32_t mode, double rtol, int32_t maxiter,

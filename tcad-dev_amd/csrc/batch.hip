@@ -1513,6 +1513,10 @@ int tcad_solve_linear(tcad_batch *bp, int32_t mode, double rtol, int32_t maxiter
     tcad::set_error("tcad_solve_linear: rtol must be > 0 and maxiter > 0");
     return TCAD_E_INVALID;
   }
+  if (!b.last_n || !b.last_p) {
+    tcad::set_error("tcad_solve_linear: no assembled system");
+    return TCAD_E_INVALID;
+  }
   int rc = tcad::launch_set_active(b, nullptr);
   if (rc) return rc;
   const bool mixed = mode == TCAD_LIN_BICGSTAB_MIXED;
@@ -1526,13 +1530,7 @@ int tcad_solve_linear(tcad_batch *bp, int32_t mode, double rtol, int32_t maxiter
     if (h_resid) h_resid[d] = S.bb > 0 ? std::sqrt(S.rr / S.bb) : 0.0;
     if (S.status) worst = 1;
   }
-  if (d_du) {
-    if (!b.last_n || !b.last_p) {
-      tcad::set_error("tcad_solve_linear: no assembled system");
-      return TCAD_E_INVALID;
-    }
-    if ((rc = tcad::launch_unscale(b, b.last_n, b.last_p, d_du))) return rc;
-  }
+  if (d_du && (rc = tcad::launch_unscale(b, b.last_n, b.last_p, d_du))) return rc;
   TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
   if (worst) {
     tcad::set_error("Krylov solve did not reach the requested tolerance");

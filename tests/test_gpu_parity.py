@@ -109,12 +109,14 @@ def _run_sweep(spec, opts):
     NewtonOptions, device_from_spec = _gpu()
     dev = device_from_spec(spec)
     dev.solve_equilibrium(NewtonOptions(max_iter=400) if spec.dim == 3 else opts)
-    its, cur, states = [], [], []
+    its, cur, states, kry = [], [], [], 0
     for bias in spec.sweep:
         dev.solve_bias(bias, opts)
         its.append(dev.last_iterations)
+        kry += dev.last_krylov_iterations
         cur.append([dev.terminal_current(c.name) for c in spec.contacts])
         states.append((dev.psi, dev.n, dev.p))
+    _say(f"  Krylov iterations over the sweep: {kry}")
     return dev, np.array(its), np.array(cur), states
 
 
