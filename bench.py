@@ -80,24 +80,85 @@ def _lib():
 # ---------------------------------------------------------------------------
 # headline: the tri-gate sweeps
 # ---------------------------------------------------------------------------
-def trigate_step(devs, specs, opts, eq_opts, lat=None, record=None):
-    """One step: for each drain bias, equilibrium + every Vg point."""
+def trigate_step(devs, specs, opts, eq_opts, lat=None, record=None, predictor=True,
+                 opts_start=None):
+    """One step: for each drain bias, equilibrium + every Vg point, each
+    point warm-started from the previous one (with `predictor`, from the
+    secant prediction of the last two; tcadhip.sweep.device_sweep).  A
+    point's latency runs from the end of the previous point to its drain
+    current."""
+    from tcadhip import sweep
     for dev, spec in zip(devs, specs):
         dev.solve_equilibrium(eq_opts)
-        for bias in spec.sweep:
-            t0 = time.perf_counter()
-            dev.solve_bias(bias, opts)
+        t_last = [time.perf_counter()]
+
+        def on_point(k, its, kry, dev=dev, spec=spec, t_last=t_last):
             I = dev.terminal_currents()
+            t = time.perf_counter()
             if lat is not None:
-                lat.append(time.perf_counter() - t0)
+                lat.append(t - t_last[0])
+            t_last[0] = t
             if record is not None:
-                record.append((spec.name, dev.last_iterations, dev.last_krylov_iterations,
-                               I["drain"]))
+                record.append((spec.name, its, kry, I["drain"]))
+
+        sweep.device_sweep(dev, spec.sweep, opts, predictor=predictor, on_point=on_point,
+                           opts_start=opts_start)
+
+
+# Newton iterations a tuning candidate may need above the untuned solve when
+# both start from the predicted state: that start usually saves one iteration
+# against the previous converged state (the reference's start), so such a
+# candidate still needs no more iterations than the untuned solve from
+# there, and the sweep stays within +-1 of the exact counts
+PREDICTED_SLACK = 1
+
+
+def tuning_start(dev, sw, opts, eq_opts, predictor):
+    """Leave `dev` at the start the sweep gives the point the autotuner
+    times, and return that point's index: the second point warm-started
+    from the first, or with the predictor the third, from its secant
+    prediction off the first two."""
+    from tcadhip import sweep
+    dev.solve_equilibrium(eq_opts)
+    dev.solve_bias(sw[0], opts)
+    if not predictor or len(sw) < 3:
+        return 1
+    B = dev._gpu()
+    names = list(dev._s.bcs)
+    with B.on_stream():
+        prev = (B.psi.clone(), B.n.clone(), B.p.clone())
+    v0 = sweep.bias_vector(dev._s, names)
+    dev.solve_bias(sw[1], opts)
+    v1 = sweep.bias_vector(dev._s, names)
+    dev._set_voltages(sw[2])
+    f = sweep.secant_ratio(v0, v1, sweep.bias_vector(dev._s, names))
+    with B.on_stream():
+        B.predict(prev, np.array([f]))
+    B.stream_sync()
+    return 2
+
+
+def tune_sweep(dev, sw, opts, eq_opts, predictor):
+    """The autotuner for a sweep: the options of the points that start from
+    the previous converged state, tuned on the second point warm-started
+    from the first, and with the predictor those of the predicted points,
+    tuned on the third point from its prediction (newton_slack
+    PREDICTED_SLACK).  Returns (options of predicted points, options of the
+    other points, report)."""
+    from tcadhip import tune as tuner
+    tuning_start(dev, sw, opts, eq_opts, False)
+    warm, rep_w = tuner.tune(dev, sw[1], opts, say=_say, start="warm")
+    if not predictor or len(sw) < 3:
+        return warm, warm, rep_w
+    k = tuning_start(dev, sw, opts, eq_opts, True)
+    pred, rep_p = tuner.tune(dev, sw[k], opts, say=_say, newton_slack=PREDICTED_SLACK,
+                             start="predicted")
+    return pred, warm, {"predicted_start": rep_p, "warm_start": rep_w}
 
 
 def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, precond="tile",
                   tune=True, forcing_slope=0.01, forcing_max=1e-4, tile_rows="auto",
-                  probe_cuts=(), forcing_pow=1):
+                  probe_cuts=(), forcing_pow=1, predictor=True):
     from tcadhip import configs
     from tcadhip.device import NewtonOptions, device_from_spec
     specs = [configs.config3_trigate(0.05), configs.config3_trigate(0.7)]
@@ -110,48 +171,51 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, prec
                          forcing_pow=forcing_pow, tile_rows=tile_rows)
     eq_opts = NewtonOptions(max_iter=400)
     tuning = None
+    opts_start = None
     if tune:
-        # the autotuner times its candidates on the second Vg point of the
-        # Vd = 0.05 V sweep, warm-started from the first (untimed, before the
-        # warm-up steps)
-        from tcadhip import tune as tuner
-        devs[0].solve_equilibrium(eq_opts)
-        devs[0].solve_bias(specs[0].sweep[0], opts)
-        opts, tuning = tuner.tune(devs[0], specs[0].sweep[1], opts, say=_say)
+        # the autotuner times its candidates on Vg points of the Vd = 0.05 V
+        # sweep started as the sweep starts them (untimed, before the warm-up
+        # steps; tune_sweep)
+        opts, opts_start, tuning = tune_sweep(devs[0], specs[0].sweep, opts, eq_opts,
+                                              predictor)
     for _ in range(warmup):
-        trigate_step(devs, specs, opts, eq_opts)
+        trigate_step(devs, specs, opts, eq_opts, predictor=predictor, opts_start=opts_start)
     _sync()
     lat, rec = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         rec = []
-        trigate_step(devs, specs, opts, eq_opts, lat, rec)
+        trigate_step(devs, specs, opts, eq_opts, lat, rec, predictor, opts_start)
     _sync()
     dt = time.perf_counter() - t0
     npts = sum(len(s.sweep) for s in specs)
     out = dict(seconds=dt, points_per_step=npts, latencies=np.array(lat), record=rec,
                devs=devs, specs=specs, eq_iterations=[d.last_eq_iterations for d in devs],
-               opts=opts, tuning=tuning)
-    if opts.forcing or opts.linsolve != "bicgstab":
-        # the exact-Newton counts of the same sweep (FP64 solves to rtol
-        # every step): the tuned path's counts are held to them +-1 per point
+               opts=opts, tuning=tuning, opts_start=opts_start)
+    if opts.forcing or opts.linsolve != "bicgstab" or predictor:
+        # the exact-Newton counts of the plain sweep (FP64 solves to rtol
+        # every step, each point started from the previous one as the
+        # reference starts it): the headline's counts are held to them +-1
+        # per point, and its currents to theirs
         exact = []
-        trigate_step(devs, specs, NewtonOptions(precond=precond), eq_opts, None, exact)
+        trigate_step(devs, specs, NewtonOptions(precond=precond), eq_opts, None, exact, False)
         out["newton_exact"] = [x[1] for x in exact]
+        out["drain_exact"] = [x[3] for x in exact]
     if probe is not None and probe >= 0:
         # in-solver duration of one kernel: one more (untimed) step with HIP
         # events around every launch of it
-        n, us = probe_step(devs, specs, opts, eq_opts, probe, None)
+        n, us = probe_step(devs, specs, opts, eq_opts, probe, None, predictor, opts_start)
         out["probe"] = dict(kernel=probe, launches=n, avg_us=us)
         # the same, one step per cut: how the average depends on where the
         # early exits are cut off
         out["probe_cuts"] = [dict(cut_us=c, launches=m, avg_us=round(a, 3) if a else None)
                              for c in probe_cuts
-                             for m, a in [probe_step(devs, specs, opts, eq_opts, probe, c)]]
+                             for m, a in [probe_step(devs, specs, opts, eq_opts, probe, c,
+                                                     predictor, opts_start)]]
     return out
 
 
-def probe_step(devs, specs, opts, eq_opts, probe, cut_us):
+def probe_step(devs, specs, opts, eq_opts, probe, cut_us, predictor=True, opts_start=None):
     """One untimed step with HIP start/stop events on every launch of
     BiCGStab kernel `probe`; returns (launches, average us) over the
     launches longer than cut_us (default: probe_min_us)."""
@@ -159,7 +223,7 @@ def probe_step(devs, specs, opts, eq_opts, probe, cut_us):
     L = _lib()
     for d in devs:
         L.check(L.lib().tcad_probe_kernel(d._gpu().handle, probe), "tcad_probe_kernel")
-    trigate_step(devs, specs, opts, eq_opts)
+    trigate_step(devs, specs, opts, eq_opts, predictor=predictor, opts_start=opts_start)
     tot_n, tot_us = 0, 0.0
     for d in devs:
         n = ctypes.c_int64()
@@ -173,12 +237,13 @@ def probe_step(devs, specs, opts, eq_opts, probe, cut_us):
     return tot_n, (tot_us / tot_n if tot_n else None)
 
 
-def reduced_trigate_parity(opts):
+def reduced_trigate_parity(opts, predictor=True, opts_start=None):
     """The reduced 24x18x14 tri-gate at both drain biases against the
     reference's own sweeps (committed fixtures), with the headline's solver
-    options: Newton counts and every contact current per point, strict
-    1e-6 of the largest current."""
+    options and sweep predictor: Newton counts and every contact current
+    per point, strict 1e-6 of the largest current."""
     from golden.fixtures import load
+    from tcadhip import sweep
     from tcadhip.device import NewtonOptions, device_from_spec
     out = {}
     worst, pts, strict, newton_ok = 0.0, 0, 0, True
@@ -187,11 +252,14 @@ def reduced_trigate_parity(opts):
         dev = device_from_spec(spec)
         dev.solve_equilibrium(NewtonOptions(max_iter=400))
         its, cur = [], []
-        for bias in spec.sweep:
-            dev.solve_bias(bias, opts)
-            its.append(dev.last_iterations)
+
+        def on_point(k, n_it, kry, dev=dev, spec=spec):
+            its.append(n_it)
             I = dev.terminal_currents()
             cur.append([I[c.name] for c in spec.contacts])
+
+        sweep.device_sweep(dev, spec.sweep, opts, predictor=predictor, on_point=on_point,
+                           opts_start=opts_start)
         cur = np.array(cur)
         ref = d["currents"]
         for p in range(len(cur)):
@@ -304,13 +372,18 @@ def bench_diode(steps=3, warmup=1):
                           f"{dt_cpu:.2f} s"}
 
 
+MOPTS = ("linsolve", "forcing", "forcing_pow", "forcing_slope", "forcing_max", "forcing_fp32",
+         "tile_rows", "graph", "tile_lag")
+
+
 def bench_mosfet(steps=2, warmup=1, modes=("bicgstab", "bicgstab_mixed", "tuned"), cpu_points=3):
-    """The 200x120 MOSFET Id-Vg sweep (21 points): the FP64 path, the
-    FP32-matrix path and the autotuned FP64 path (the autotuner timed on the
-    second Vg point); the CPU oracle on the first points for time and
-    agreement."""
+    """The 200x120 MOSFET Id-Vg sweep (21 points): the FP64 path and the
+    FP32-matrix path with the reference's warm start, and the autotuned
+    FP64 path with the secant predictor (the autotuner timed on the third
+    Vg point from its predicted start); the CPU oracle on the first points
+    for time and agreement."""
     from oracle_build import oracle_device
-    from tcadhip import configs
+    from tcadhip import configs, sweep as sweepmod
     from tcadhip.device import NewtonOptions, device_from_spec
     spec = configs.config2_mosfet()
     out = {}
@@ -318,22 +391,26 @@ def bench_mosfet(steps=2, warmup=1, modes=("bicgstab", "bicgstab_mixed", "tuned"
     for mode in modes:
         dev = device_from_spec(spec)
         opts = NewtonOptions(linsolve="bicgstab" if mode == "tuned" else mode)
+        predictor = mode == "tuned"
+        opts_start = None
         if mode == "tuned":
-            from tcadhip import tune as tuner
-            dev.solve_equilibrium(opts)
-            dev.solve_bias(spec.sweep[0], opts)
-            opts, rep = tuner.tune(dev, spec.sweep[1], opts)
+            opts, opts_start, rep = tune_sweep(dev, spec.sweep, opts, opts, True)
 
         def sweep(lat=None):
             dev.solve_equilibrium(opts)
             cur, its = [], []
-            for bias in spec.sweep:
-                t0 = time.perf_counter()
-                dev.solve_bias(bias, opts)
+            t_last = [time.perf_counter()]
+
+            def on_point(k, n_it, kry):
                 cur.append(dev.terminal_currents()["drain"])
+                t = time.perf_counter()
                 if lat is not None:
-                    lat.append(time.perf_counter() - t0)
-                its.append(dev.last_iterations)
+                    lat.append(t - t_last[0])
+                t_last[0] = t
+                its.append(n_it)
+
+            sweepmod.device_sweep(dev, spec.sweep, opts, predictor=predictor, on_point=on_point,
+                                  opts_start=opts_start)
             return np.array(cur), its
         for _ in range(warmup):
             sweep()
@@ -348,9 +425,10 @@ def bench_mosfet(steps=2, warmup=1, modes=("bicgstab", "bicgstab_mixed", "tuned"
             ref64 = cur
         lat_ms = np.array(lat) * 1e3
         out[mode] = {"points_per_s": round(len(spec.sweep) * steps / dt, 3),
-                     "options": {k: getattr(opts, k) for k in
-                                 ("linsolve", "forcing", "forcing_pow", "forcing_slope",
-                                  "forcing_max", "forcing_fp32", "tile_rows", "graph", "tile_lag")},
+                     "sweep_start": "secant predictor" if predictor else "plain warm start",
+                     "options": {k: getattr(opts, k) for k in MOPTS},
+                     "options_unpredicted_points": (None if opts_start is None else
+                                                    {k: getattr(opts_start, k) for k in MOPTS}),
                      "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
                      "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
                      "newton_per_point": its,
@@ -445,21 +523,26 @@ def bench_batch(lanes=1, sample=8, modes=None):
     from tcadhip.device import NewtonOptions, setup_from_spec
     specs = configs.batch_diodes()
     res = {}
+    # plain batch (FP64, FP32-stored matrix), the secant predictor between
+    # continuation steps, and the autotuner's choice with the predictor
     modes = modes or {"bicgstab_mixed": NewtonOptions(linsolve="bicgstab_mixed"),
-                      "bicgstab": NewtonOptions(), "tuned": None}
+                      "bicgstab": NewtonOptions(),
+                      "predictor": (NewtonOptions(), dict(predictor=True)), "tuned": None}
     tuning = None
     if "tuned" in modes:
-        # the autotuner on the largest diode (the second point of its sweep,
-        # warm-started from the first), its choice applied to every device
+        # the autotuner on the largest diode (a point of its sweep started as
+        # the predicted sweep starts it), its choice applied to every device
         from tcadhip import tune as tuner
         from tcadhip.device import device_from_spec
         big = max(range(len(specs)), key=lambda i: specs[i].N)
         dev = device_from_spec(specs[big])
-        dev.solve_equilibrium(NewtonOptions())
-        dev.solve_bias(specs[big].sweep[0], NewtonOptions())
-        topts, tuning = tuner.tune(dev, specs[big].sweep[1], NewtonOptions(), say=_say)
-        tuning = dict(tuning, device=big, nodes=int(specs[big].N))
-        modes["tuned"] = topts
+        k = tuning_start(dev, specs[big].sweep, NewtonOptions(), NewtonOptions(), True)
+        # no slack here: a round solves every open device with one option
+        # set, predicted or not, so a candidate accepted with the slack would
+        # also run on the devices' first, unpredicted points
+        topts, tuning = tuner.tune(dev, specs[big].sweep[k], NewtonOptions(), say=_say)
+        tuning = dict(tuning, device=big, nodes=int(specs[big].N), point=k)
+        modes["tuned"] = (topts, dict(predictor=True))
         del dev
     for mode, mopts in modes.items():
         mopts, kw = mopts if isinstance(mopts, tuple) else (mopts, {})
@@ -522,12 +605,15 @@ def bench_batch(lanes=1, sample=8, modes=None):
             e64 = max(e64, float(np.abs(got - ref).max() / np.abs(ref).max()))
         over = None
         if mode != "bicgstab" and "bicgstab" in res:
-            # Newton counts against the exact (untuned FP64) batch, device by device
-            over = sum(int(abs(a - b) > 1) for x, y in zip(r["results"], res["bicgstab"]["results"])
+            # Newton counts against the exact (untuned FP64, plain warm start)
+            # batch, device by device: points that needed more than one
+            # iteration more (the predictor's starts save iterations)
+            over = sum(int(a - b > 1) for x, y in zip(r["results"], res["bicgstab"]["results"])
                        for a, b in zip(x.newton, y.newton))
         out[mode] = {"devices_per_s": round(len(specs) / r["seconds"], 4),
                      "seconds": round(r["seconds"], 3),
-                     "newton_points_off_exact_by_more_than_1": over,
+                     "newton_total": int(sum(sum(x.newton) for x in r["results"])),
+                     "newton_points_above_exact_plus_1": over,
                      "failed": sum(1 for x in r["results"] if x.failed),
                      "rounds": r["rounds"], "compactions": r["rebuilds"],
                      "vs_plain_fp64": round(len(specs) / r["seconds"] / plain, 2),
@@ -536,8 +622,11 @@ def bench_batch(lanes=1, sample=8, modes=None):
                      "meets_1e-6_no_floor": strict,
                      "under_conservation_floor_only": floor_pts,
                      "max_rel_current_err_vs_oracle": worst}
+        mo = modes[mode][0] if isinstance(modes[mode], tuple) else modes[mode]
+        if isinstance(modes[mode], tuple):
+            out[mode]["sweep"] = modes[mode][1]
         if mode == "tuned":
-            out[mode]["options"] = {k: getattr(modes[mode], k) for k in
+            out[mode]["options"] = {k: getattr(mo, k) for k in
                                     ("forcing", "forcing_pow", "forcing_slope", "forcing_max",
                                      "forcing_fp32", "tile_rows", "graph", "tile_lag")}
             out[mode]["autotuner"] = tuning
@@ -564,7 +653,7 @@ def bench_serve(count=16, lanes=2, check=3, budget_gib=32):
     def run(rate):
         reqs = serve.request_stream(count, rate=rate, seed=20261203) if rate else \
             [serve.Request(r.rid, r.spec, 0.0, r.kind) for r in base]
-        S = serve.Server(cap, opts=NewtonOptions(), lanes=lanes)
+        S = serve.Server(cap, opts=NewtonOptions(), lanes=lanes, predictor=True)
         _sync()
         t0 = time.perf_counter()
         served = S.run(reqs, realtime=True)
@@ -590,14 +679,19 @@ def bench_serve(count=16, lanes=2, check=3, budget_gib=32):
         r["fraction_of_saturation"] = frac
         out[f"load_{int(frac * 100)}pct"] = r
     same = True
-    t1 = time.perf_counter()
-    alone_pts = 0
     for o, r in list(zip(served, reqs))[:check]:
-        a = serve.serve_alone(r.spec, NewtonOptions())
-        alone_pts += len(a.currents)
+        a = serve.serve_alone(r.spec, NewtonOptions(), predictor=True)
         same = same and a.newton == o.result.newton and all(
             x[k] == y[k] for x, y in zip(a.currents, o.result.currents) for k in x)
+    # the plain FP64 path on the same requests: one at a time, each point
+    # started from the previous converged state (no predictor)
     _sync()
+    t1 = time.perf_counter()
+    alone_pts = 0
+    for r in reqs[:check]:
+        alone_pts += len(serve.serve_alone(r.spec, NewtonOptions()).currents)
+    _sync()
+    out["sweep_start"] = "secant predictor"
     out["alone_equals_mix_on_first"] = check
     out["alone_equals_mix"] = bool(same)
     out["plain_fp64_points_per_s_on_those"] = round(alone_pts / (time.perf_counter() - t1), 4)
@@ -665,6 +759,9 @@ def main():
     ap.add_argument("--tile-rows", default="auto", choices=("auto", "reg", "mem"),
                     help="tile-inverse rows in registers or read at the point of use "
                          "(untuned runs)")
+    ap.add_argument("--no-predictor", action="store_true",
+                    help="start each Vg point from the previous converged state (the "
+                         "reference's warm start) instead of the secant prediction")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (fixed default solver options)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -709,6 +806,7 @@ def main():
                 "quad": NewtonOptions(forcing=True, forcing_pow=2, forcing_slope=0.1,
                                       forcing_max=1e-3),
                 "tuned": None,
+                "predictor": (NewtonOptions(), dict(predictor=True)),
                 "queue": (NewtonOptions(), dict(queue=True)),
                 "queue2": (NewtonOptions(), dict(queue=True, queue_depth=2)),
                 "queue4": (NewtonOptions(), dict(queue=True, queue_depth=4))}
@@ -721,7 +819,8 @@ def main():
                       args.forcing, args.precond, tune=not args.no_tune,
                       forcing_slope=args.forcing_slope, forcing_max=args.forcing_max,
                       forcing_pow=args.forcing_pow, tile_rows=args.tile_rows,
-                      probe_cuts=[float(c) for c in args.probe_cuts.split(",") if c])
+                      probe_cuts=[float(c) for c in args.probe_cuts.split(",") if c],
+                      predictor=not args.no_predictor)
     npts = r["points_per_step"]
     value = npts * args.steps / r["seconds"]
     _say(f"tri-gate: {value:.3f} points/s")
@@ -742,19 +841,30 @@ def main():
         "data": "synthetic",
         "config": {"workload": "trigate_64x48x48_idvg_vd0.05+0.7_vg0-0.8_34pts", "nodes": N,
                    "unknowns": 3 * N, "bias_points_per_step": npts, "path": args.mode,
-                   "precond": "2x1x8-node tile block-Jacobi (FP16 rows, FP32 scales)"},
+                   "precond": "2x1x8-node tile block-Jacobi (FP16 rows, FP32 scales)",
+                   "sweep_start": ("plain warm start" if args.no_predictor else
+                                   "secant predictor from the last two converged points")},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "solver_options": {k: getattr(r["opts"], k) for k in
                            ("linsolve", "linsolve_rtol", "precond", "forcing", "forcing_pow",
                             "forcing_slope", "forcing_max", "forcing_fp32", "tile_rows", "graph",
                             "tile_lag")},
+        "solver_options_unpredicted_points": (
+            None if r["opts_start"] is None else
+            {k: getattr(r["opts_start"], k) for k in ("forcing", "forcing_pow", "forcing_slope",
+                                                      "forcing_max", "forcing_fp32", "tile_rows",
+                                                      "graph", "tile_lag")}),
         "autotuner": r["tuning"],
         "newton_per_point": [x[1] for x in r["record"]],
         "newton_exact_per_point": r.get("newton_exact"),
         "newton_within_1_of_exact": (None if r.get("newton_exact") is None else
                                      all(abs(a - b) <= 1 for a, b in
                                          zip([x[1] for x in r["record"]], r["newton_exact"]))),
+        "max_rel_drain_err_vs_exact_plain": (
+            None if r.get("drain_exact") is None else
+            float(max(abs(a - b) / abs(b) for a, b in
+                      zip([x[3] for x in r["record"]], r["drain_exact"])))),
         "krylov_per_point": [x[2] for x in r["record"]],
         "equilibrium_newton": r["eq_iterations"],
         "drain_current_A": {"vd0.05": [x[3] for x in r["record"] if "vd0.05" in x[0]],
@@ -787,7 +897,8 @@ def main():
             line["probe_cuts"] = r["probe_cuts"]
     if not args.no_parity:
         _say("reduced tri-gate vs the reference fixtures")
-        line["parity_reduced_24x18x14_vs_reference"] = reduced_trigate_parity(r["opts"])
+        line["parity_reduced_24x18x14_vs_reference"] = reduced_trigate_parity(
+            r["opts"], not args.no_predictor, r["opts_start"])
     line["cpu_baseline"] = None
     if not args.no_cpu:
         gpu_newton = r["record"][0][1] if r["record"] else 5

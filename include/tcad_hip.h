@@ -155,6 +155,22 @@ int tcad_set_bias(tcad_batch *b, const double *h_contact_V,
 int tcad_assemble(tcad_batch *b, const double *d_psi, const double *d_n,
                   const double *d_p, double *d_F, double *d_Fnp);
 
+/* Secant predictor of the next bias point's Newton start, from the state at
+ * the sweep's previous converged point (d_*_prev) and the present one
+ * (d_psi, d_n, d_p, updated in place).  Per device, h_ratio[d] (host,
+ * length ndev) is the step to the next point over the step from the
+ * previous one (the bias vectors parallel); a device with h_ratio[d] <= 0
+ * is not touched, and ratios above 8 or non-finite are refused
+ * (TCAD_E_INVALID).  psi moves linearly, n and p geometrically (linearly
+ * in log n, log p), each density by at most the Newton update's clip
+ * [0.1x, 10x].  Asynchronous on the handle's stream.  Replaces: the secant
+ * (tangent) predictor of continuation.py:368-404 (arc_length_sweep, psi
+ * only there), applied to the warm start that id_vg_sweep (mosfet.py:90-103)
+ * and id_vg_sweep_3d (finfet3d.py:246-263) take from the previous point. */
+int tcad_predict_state(tcad_batch *b, const double *d_psi_prev, const double *d_n_prev,
+                       const double *d_p_prev, double *d_psi, double *d_n, double *d_p,
+                       const double *h_ratio);
+
 /* The reference's raw residual/Jacobian at (psi, n, p) and the voltages
  * set by tcad_set_bias, without Dirichlet elimination or scaling:
  *   d_F     3 planes of N_total: F after the contact and gate rows;

@@ -461,6 +461,34 @@ __global__ void __launch_bounds__(BS) k_stamp(const DevInfo *info, const int *bl
   p[g] = p0[t];
 }
 
+// Secant predictor of a sweep's next Newton start from its last two
+// converged points (the tangent step of continuation.py:368-404, there in
+// psi only; here psi linearly and the densities geometrically, i.e.
+// linearly in log n and log p, which tracks the quasi-Fermi levels).  Per
+// device ratio f = step to the next point / step from the previous one
+// (f <= 0: the device is not touched): psi += f (psi - psi_prev),
+// n *= clamp((n / n_prev)^f, 0.1, 10), p likewise -- the Newton update's own
+// density clip (device2d.py:2083-2100), so a prediction moves no density
+// by more than one Newton step could.
+__global__ void __launch_bounds__(BS) k_predict(const DevInfo *info, const int *blk2dev,
+                                                const double *ratio, const double *psi0,
+                                                const double *n0, const double *p0, double *psi,
+                                                double *n, double *p) {
+  const int d = blk2dev[blockIdx.x];
+  const double f = ratio[d];
+  if (!(f > 0.0)) return;
+  const DevInfo &I = info[d];
+  const NodeAt P = node_at(I, blockIdx.x);
+  if (!P.valid) return;
+  const long long g = I.off + P.k;
+  const double ps = psi[g], nn = n[g], pp = p[g];
+  const double rn = f == 1.0 ? nn / n0[g] : pow(nn / n0[g], f);
+  const double rp = f == 1.0 ? pp / p0[g] : pow(pp / p0[g], f);
+  psi[g] = ps + f * (ps - psi0[g]);
+  n[g] = nn * fmin(fmax(rn, 0.1), 10.0);
+  p[g] = pp * fmin(fmax(rp, 0.1), 10.0);
+}
+
 // Bulk-neutral guess arcsinh(C / (2 nie_s)) (device2d.py:744-759).
 __global__ void __launch_bounds__(BS) k_bulk_guess(const DevInfo *info, const int *blk2dev,
                                                    const double *C, const double *nie_s,
@@ -1135,7 +1163,8 @@ static void free_batch(Batch *b) {
                   b->d_cscale, b->d_c2dev, b->d_gptr, b->d_gnode, b->d_surf[0], b->d_surf[1], b->d_surf[2],
                   b->d_surf[3], b->A64, b->A32, b->Dinv64, b->Dinv32, b->bs, b->rowscale,
                   b->colscale, b->y64, b->F, b->Fnp, b->part, b->d_scal, b->Aeq, b->beq,
-                  b->d_flag, b->d_mask, b->cand, b->d_Icand, b->T4, b->T4s, b->P16, b->d_dts};
+                  b->d_flag, b->d_mask, b->d_ratio, b->cand, b->d_Icand, b->T4, b->T4s,
+                  b->P16, b->d_dts};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : b->probe_ev) (void)hipEventDestroy(e);
@@ -1386,6 +1415,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   for (int i = 0; i < 6; ++i) TRY(dalloc(&b->veq[i], N));
   TRY(dalloc(&b->d_flag, (size_t)ndev + 1));
   TRY(dalloc(&b->d_mask, (size_t)ndev));
+  TRY(dalloc(&b->d_ratio, (size_t)ndev));
   TRY(dalloc(&b->d_dts, (size_t)ndev));
   TRY(dalloc(&b->T4, (size_t)TILE_PLANES * N));
   TRY(dalloc(&b->T4s, 3 * N));
@@ -1468,6 +1498,29 @@ int tcad_set_bias(tcad_batch *bp, const double *h_contact_V, const double *h_gat
     TCAD_HIP_CHECK(hipMemcpyAsync(b.d_gate_Vfb, h_gate_Vfb, sizeof(double) * b.ngate,
                                   hipMemcpyHostToDevice, b.stream));
   TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));
+  return TCAD_OK;
+}
+
+int tcad_predict_state(tcad_batch *bp, const double *psi_prev, const double *n_prev,
+                       const double *p_prev, double *psi, double *n, double *p,
+                       const double *h_ratio) {
+  GUARD(bp);
+  Batch &b = *bp->impl;
+  if (!psi_prev || !n_prev || !p_prev || !psi || !n || !p || !h_ratio) {
+    tcad::set_error("tcad_predict_state: null pointer");
+    return TCAD_E_INVALID;
+  }
+  for (int d = 0; d < b.ndev; ++d)
+    if (!std::isfinite(h_ratio[d]) || h_ratio[d] > 8.0) {
+      tcad::set_error("tcad_predict_state: ratio must be finite and at most 8");
+      return TCAD_E_INVALID;
+    }
+  TCAD_HIP_CHECK(hipMemcpyAsync(b.d_ratio, h_ratio, sizeof(double) * b.ndev,
+                                hipMemcpyHostToDevice, b.stream));
+  hipLaunchKernelGGL(tcad::k_predict, dim3(b.nblk), dim3(tcad::BS), 0, b.stream, b.d_info,
+                     b.d_blk2dev, b.d_ratio, psi_prev, n_prev, p_prev, psi, n, p);
+  b.launches++;
+  TCAD_HIP_CHECK(hipGetLastError());
   return TCAD_OK;
 }
 

@@ -225,6 +225,7 @@ struct Batch {
   int *d_bmap = nullptr, *h_bmap = nullptr;       // Krylov grid of the open devices (2 halves)
   int bmap_half = 0;
   int *d_mask = nullptr;                         // per-device selection of the next solve
+  double *d_ratio = nullptr;                     // per-device secant ratio (tcad_predict_state)
   const double *last_n = nullptr, *last_p = nullptr;   // state of the last assembly
   const double *last_psi = nullptr;
   std::vector<int> dev_ncontact, dev_ngate;

@@ -109,6 +109,7 @@ def lib():
         "tcad_launch_count": ([vp], ctypes.c_int64),
         "tcad_set_bias": ([vp, c_double_p, c_double_p, c_double_p], ctypes.c_int),
         "tcad_assemble": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
+        "tcad_predict_state": ([vp, vp, vp, vp, vp, vp, vp, c_double_p], ctypes.c_int),
         "tcad_export_jacobian": ([vp, c_double_p, c_double_p, c_double_p, c_int32_p],
                                  ctypes.c_int),
         "tcad_residual_jacobian": ([vp, vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
@@ -192,7 +193,7 @@ def iptr(a):
 def exported_symbols():
     """Names declared in include/tcad_hip.h that the library must export."""
     return ["tcad_batch_create", "tcad_batch_destroy", "tcad_batch_nodes", "tcad_set_bias",
-            "tcad_assemble", "tcad_export_jacobian", "tcad_residual_jacobian",
+            "tcad_assemble", "tcad_predict_state", "tcad_export_jacobian", "tcad_residual_jacobian",
             "tcad_solve_linear",
             "tcad_solve_equilibrium", "tcad_solve_bias", "tcad_terminal_currents",
             "tcad_edge_currents", "tcad_surface_mobility", "tcad_time_kernel",

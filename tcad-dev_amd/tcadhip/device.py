@@ -441,6 +441,20 @@ class GPUBatch:
             _lib.iptr(kry), _lib.iptr(status)), "tcad_solve_bias")
         return its, conv, kry, status
 
+    def predict(self, prev, ratios):
+        """Secant predictor of every device's next Newton start
+        (tcad_predict_state): prev = (psi, n, p) planes holding each
+        device's state at its previous converged point, the present state in
+        self.psi/n/p; ratios[d] = step to the next point / step from the
+        previous one (<= 0: device d is not touched)."""
+        r = np.ascontiguousarray(ratios, dtype=np.float64)
+        if r.shape != (self.ndev,):
+            raise ValueError(f"one ratio per device ({self.ndev})")
+        _lib.check(_lib.lib().tcad_predict_state(
+            self._handle, prev[0].data_ptr(), prev[1].data_ptr(), prev[2].data_ptr(),
+            self.psi.data_ptr(), self.n.data_ptr(), self.p.data_ptr(), _lib.dptr(r)),
+            "tcad_predict_state")
+
     def solve_queue(self, opts, point_lists):
         """Every device's list of bias points (dicts of terminal voltages,
         each point keeping the terminals it does not name at their previous

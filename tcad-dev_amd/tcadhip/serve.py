@@ -38,7 +38,7 @@ import numpy as np
 
 from . import _lib
 from .device import GPUBatch, NewtonOptions, _torch, setup_from_spec
-from .sweep import SweepResult, Track, record_currents, solve_round
+from .sweep import Secant, SweepResult, Track, record_currents, solve_round
 
 # library workspace per node of a batch (upper bound over 2D/3D): the
 # stencil-block matrix in FP64 and FP32 (39 entries in 3D), its node block
@@ -151,6 +151,9 @@ class _Lane:
         self.B = None
         self.rounds = self.rebuilds = self.max_resident = 0
         self.load = 0                   # outstanding node-points routed here
+        # the secant predictor's previous states live in whole pool-sized
+        # planes (the round snapshots), indexed by each device's offset
+        self.secant = Secant() if server.predictor else None
 
     def _ctx(self):
         import contextlib
@@ -202,7 +205,7 @@ class _Lane:
                 if mask[d] and (status[d] != _lib.NEWTON_OK or not conv[d]):
                     m["track"].result.failed = "equilibrium did not converge"
         tracks = [m["track"] for m in self.members]
-        reached = solve_round(self.B, tracks, srv.opts)
+        reached = solve_round(self.B, tracks, srv.opts, self.secant)
         self.rounds += 1
         record_currents(self.B, tracks, reached, now())
         keep = []
@@ -232,10 +235,15 @@ class Server:
     request goes to the lane with the least outstanding work (node count
     x points) that can hold it."""
 
-    def __init__(self, capacity_nodes, opts=None, eq_opts=None, max_step=None, lanes=1):
+    def __init__(self, capacity_nodes, opts=None, eq_opts=None, max_step=None, lanes=1,
+                 predictor=False):
         self.opts = opts or NewtonOptions()
         self.eq_opts = eq_opts or NewtonOptions(max_iter=400)
         self.max_step = max_step
+        # secant predictor between a request's continuation steps
+        # (tcadhip.sweep; a request's results alone and in the mix stay
+        # the same bits: the prediction of a device reads only its own range)
+        self.predictor = predictor
         torch = _torch()
         self.lanes = [_Lane(int(capacity_nodes) // lanes, self,
                             None if lanes == 1 else torch.cuda.Stream())
@@ -402,7 +410,7 @@ def request_stream(count, rate, seed, mix=("diode", "mosfet", "trigate"), max_no
     return out
 
 
-def serve_alone(spec, opts=None, eq_opts=None):
+def serve_alone(spec, opts=None, eq_opts=None, predictor=False):
     """The same request on its own (a batch of one), for comparison."""
     from .sweep import BatchedSweep
     setup = setup_from_spec(spec)
@@ -410,5 +418,5 @@ def serve_alone(spec, opts=None, eq_opts=None):
     eq_opts = eq_opts or NewtonOptions(max_iter=400)
     B = GPUBatch([setup])
     its, conv, status = B.solve_equilibrium(eq_opts)
-    bs = BatchedSweep([setup], [spec.sweep], opts, batch=B)
+    bs = BatchedSweep([setup], [spec.sweep], opts, batch=B, predictor=predictor)
     return bs.run(equilibrium=False)[0]

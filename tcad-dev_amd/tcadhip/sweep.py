@@ -11,6 +11,18 @@ state and the step toward the next point is halved; a device whose step
 shrinks below `min_step` is reported failed, never returned as converged.
 A step that succeeds grows by `growth` up to the full distance.
 
+With `predictor=True` a device's Newton start is not its last converged
+state itself but the secant prediction from its last two converged points
+(tcad_predict_state: psi linearly, n and p geometrically, within the
+Newton update's density clip), when the next bias step is parallel to the
+last one -- the tangent predictor of continuation.py:368-404
+(arc_length_sweep), applied to the warm-started voltage sweep.  The
+converged states are the same (the Newton test and the final-step
+refinement decide them); a point typically needs one Newton iteration
+fewer.  A predicted start that fails is handled like any failed step
+(state restored, step halved) and the retry starts from the converged
+state.
+
 The devices share every kernel launch: one round issues one masked
 tcad_solve_bias over all devices that still have points to reach; the
 others are masked out and keep their state untouched.  Masked devices
@@ -27,7 +39,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .device import GPUBatch, NewtonOptions, _torch
+from .device import GPUBatch, NewtonOptions, _raise_outcome, _torch
 
 
 @dataclass
@@ -41,6 +53,27 @@ class SweepResult:
     backoffs: int = 0                               # failed attempts
     failed: str = ""                                # reason when the sweep stopped early
     t_done: float = 0.0                             # wall time when the last point finished
+
+
+def secant_ratio(prev, cur, nxt, fmax=2.0):
+    """Ratio f of the bias step cur -> nxt to the step prev -> cur (vectors
+    of terminal voltages) when the two are parallel and point the same way,
+    0 otherwise (no prediction); f is capped at fmax."""
+    if prev is None:
+        return 0.0
+    dp, dn = cur - prev, nxt - cur
+    a = float(dp @ dp)
+    if a == 0.0:
+        return 0.0
+    f = float(dn @ dp) / a
+    if not 0.0 < f <= fmax or np.abs(dn - f * dp).max() > 1e-9 * np.abs(dn).max():
+        return 0.0
+    return f
+
+
+def bias_vector(setup, names):
+    """Present terminal voltages of a device, in the order of `names`."""
+    return np.array([_bc_value(setup, nm) for nm in names])
 
 
 def _bc_value(setup, name):
@@ -69,6 +102,7 @@ class Track:
         self.growth, self.shrink = growth, shrink
         self.max_attempts = max_attempts
         self.cur = self._vec({})         # last converged bias (terminals' present values)
+        self.prev = None                 # the converged bias before it (secant predictor)
         self.next = 0                    # index of the next requested point
         self.step = None                 # current step length (max-norm, V)
         self.attempts = 0
@@ -96,11 +130,19 @@ class Track:
         self.trial = (self.cur + frac * (target - self.cur), frac >= 1.0, lim)
         _apply_bias(self.setup, self.names, self.trial[0])
 
+    def secant_ratio(self):
+        """Ratio of the proposed step to the last converged one (0: no
+        prediction)."""
+        if self.trial is None:
+            return 0.0
+        return secant_ratio(self.prev, self.cur, self.trial[0])
+
     def accept(self, its, kry):
         """The trial converged; returns True when it was a requested point."""
         vec, final, _ = self.trial
         R = self.result
         self.kry += int(kry)
+        self.prev = self.cur
         self.cur = vec
         R.steps += 1
         self.attempts = 0
@@ -136,6 +178,7 @@ class Track:
         _, _, lim = self.trial
         R = self.result
         self.kry += int(kry)
+        self.prev = None
         _apply_bias(self.setup, self.names, self.cur)
         R.backoffs += 1
         self.attempts += 1
@@ -149,10 +192,21 @@ class Track:
                         f"{self.points[self.next]}")
 
 
-def solve_round(B, tracks, opts):
+class Secant:
+    """The secant predictor's memory for the devices of a batch: whole
+    planes holding each device's state at its previous converged point
+    (the snapshot a round takes before it solves)."""
+
+    def __init__(self):
+        self.planes = None
+
+
+def solve_round(B, tracks, opts, secant=None):
     """One masked Newton solve over the open tracks (tracks[d] belongs to
     device d of batch B); returns the indices that reached a requested
-    point.  Failed devices get their state restored."""
+    point.  Failed devices get their state restored.  With `secant` (a
+    Secant), each open device whose step continues its last one starts
+    from the secant prediction."""
     todo = [d for d, t in enumerate(tracks) if t is not None and t.open]
     if not todo:
         return []
@@ -164,6 +218,12 @@ def solve_round(B, tracks, opts):
         # whole-plane snapshot: three copies, cheaper in launches than one
         # per device; only failed devices' ranges are restored from it
         snap = (B.psi.clone(), B.n.clone(), B.p.clone())
+        if secant is not None and secant.planes is not None:
+            ratios = np.zeros(B.ndev)
+            for d in todo:
+                ratios[d] = tracks[d].secant_ratio()
+            if (ratios > 0).any():
+                B.predict(secant.planes, ratios)
         its, conv, kry, status = B.solve_bias(opts, active=mask)
         reached = []
         for d in todo:
@@ -177,8 +237,67 @@ def solve_round(B, tracks, opts):
                 B.n[sl].copy_(snap[1][sl])
                 B.p[sl].copy_(snap[2][sl])
                 t.reject(status[d], kry[d])
+        if secant is not None:
+            # each converged device's previous point is the snapshot's
+            # state; a failed device's prediction is off until it converges
+            # twice more (Track.reject clears its previous bias)
+            secant.planes = snap
         B.stream_sync()
     return reached
+
+
+def device_sweep(dev, biases, opts=None, predictor=True, on_point=None, opts_start=None):
+    """The reference's sweep loop (id_vg_sweep, mosfet.py:90-103;
+    id_vg_sweep_3d, finfet3d.py:246-263) on one solved GPU device: every
+    bias point warm-started from the previous converged one, with
+    `predictor` from the secant prediction of the last two (when the bias
+    step continues the last one).  A predicted start that does not converge
+    is solved again from the previous converged state, as the reference
+    starts.  opts_start (default opts) solves the points that start from
+    the previous converged state (Newton options tuned on predicted starts,
+    such as forcing terms, may cost iterations from a farther start).
+    on_point(k, its, kry) is called after each point (the device holds its
+    converged state then).  Returns per point (Newton iterations, Krylov
+    iterations), both attempts counted."""
+    opts = opts or NewtonOptions()
+    opts_start = opts_start or opts
+    B = dev._gpu()
+    names = list(dev._s.bcs)
+
+    cur, prev, prev_state = bias_vector(dev._s, names), None, None
+    out = []
+    for k, bias in enumerate(biases):
+        dev._set_voltages(bias)
+        nxt = bias_vector(dev._s, names)
+        f = secant_ratio(prev, cur, nxt) if predictor else 0.0
+        snap = None
+        if predictor:
+            with B.on_stream():
+                snap = (B.psi.clone(), B.n.clone(), B.p.clone())
+                if f > 0.0:
+                    B.predict(prev_state, np.array([f]))
+        its, conv, kry, status = B.solve_bias(opts if f > 0.0 else opts_start)
+        its_tot, kry_tot = int(its[0]), int(kry[0])
+        if f > 0.0 and not (conv[0] and status[0] == _lib.NEWTON_OK):
+            with B.on_stream():
+                B.psi.copy_(snap[0])
+                B.n.copy_(snap[1])
+                B.p.copy_(snap[2])
+            its, conv, kry, status = B.solve_bias(opts_start)
+            its_tot += int(its[0])
+            kry_tot += int(kry[0])
+        dev.last_iterations = its_tot
+        dev.last_krylov_iterations = kry_tot
+        dev.last_converged = bool(conv[0])
+        _raise_outcome(int(status[0]), "device_sweep")
+        if not dev.last_converged:
+            raise RuntimeError(f"device_sweep: Newton did not converge at {bias}")
+        out.append((its_tot, kry_tot))
+        if on_point is not None:
+            on_point(k, its_tot, kry_tot)
+        prev, cur, prev_state = cur, nxt, snap
+    B.stream_sync()
+    return out
 
 
 def record_currents(B, tracks, reached, now):
@@ -204,9 +323,12 @@ class BatchedSweep:
 
     def __init__(self, setups, sweeps, opts=None, batch=None, max_step=None, min_step=None,
                  growth=1.5, shrink=0.5, max_attempts=64, compact=0.5, queue=False,
-                 queue_depth=None):
+                 queue_depth=None, predictor=False):
         self.setups = list(setups)
         self.compact = compact
+        # secant predictor between a device's rounds (not across queue
+        # passes, whose points the library runs itself)
+        self.secant = Secant() if predictor else None
         # queue=True: point queue (tcad_solve_queue): every device runs
         # through up to queue_depth of its requested points per library call
         # without a barrier between devices; a device that fails a point
@@ -252,7 +374,7 @@ class BatchedSweep:
                 self.queue_pass(B, tracks, t0)
                 continue
             self.rounds += 1
-            reached = solve_round(B, tracks, opts)
+            reached = solve_round(B, tracks, opts, self.secant)
             record_currents(B, tracks, reached, time.perf_counter() - t0)
         return self.results
 
@@ -279,6 +401,8 @@ class BatchedSweep:
                 t.accept_point(its[k], kry[k])
                 t.result.currents.append(cur[k])
                 t.result.t_done = t_now
+            # the predictor's planes do not follow the queue's points
+            t.prev = None
             if done < len(pl):
                 t.reject_point(status, kry[done] if done < len(kry) else 0)
 

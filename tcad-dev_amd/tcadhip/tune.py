@@ -69,14 +69,19 @@ def _key(dev, opts):
     return (dev.dim, tuple(dev.shape), opts.linsolve, opts.precond)
 
 
-def tune(dev, bias, opts=None, candidates=CANDIDATES, repeats=2, say=None, newton_slack=0):
-    """Time every candidate on one bias point of `dev` (which must hold a
-    converged state: the warm start every trial restarts from) and return
-    (best NewtonOptions, report).  The device's state and bias are left
-    as they were."""
+def tune(dev, bias, opts=None, candidates=CANDIDATES, repeats=2, say=None, newton_slack=0,
+         start="warm"):
+    """Time every candidate on one bias point of `dev` (which must hold the
+    start every trial restarts from: a converged state, or the secant
+    prediction from two, named by `start`, "warm" or "predicted") and
+    return (best NewtonOptions, report).  A candidate is accepted when its
+    currents agree with the untuned solve's and it needs at most
+    newton_slack Newton iterations more.  The device's state and bias are
+    left as they were.  Results are cached per device shape, solver and
+    start."""
     torch = _torch()
     base = opts or NewtonOptions()
-    key = _key(dev, base)
+    key = _key(dev, base) + (start,)
     if key in _cache:
         best, report = _cache[key]
         return replace(base, **best), report

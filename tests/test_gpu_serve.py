@@ -19,21 +19,25 @@ def _mods():
     return serve, NewtonOptions
 
 
-@pytest.mark.parametrize("mode", ["bicgstab", "bicgstab_mixed"])
-def test_serving_alone_equals_mix(mode):
+@pytest.mark.parametrize("mode,predictor", [("bicgstab", False), ("bicgstab_mixed", False),
+                                            ("bicgstab", True)])
+def test_serving_alone_equals_mix(mode, predictor):
+    """Each request gives the bits alone that it gives in the mix, with the
+    reference's warm start and with the secant predictor between its
+    continuation steps."""
     serve, NewtonOptions = _mods()
     reqs = serve.request_stream(7, rate=40.0, seed=23, small=True, max_points=8)
     kinds = sorted({r.kind for r in reqs})
-    print(f"[serve] {len(reqs)} requests {kinds}, {mode}", flush=True)
+    print(f"[serve] {len(reqs)} requests {kinds}, {mode}, predictor {predictor}", flush=True)
     big = sorted(r.spec.N for r in reqs)[-3:]
     opts = NewtonOptions(linsolve=mode)
-    S = serve.Server(capacity_nodes=sum(big), opts=opts)
+    S = serve.Server(capacity_nodes=sum(big), opts=opts, predictor=predictor)
     served = S.run(reqs, realtime=False)
     print(f"  rounds {S.rounds} rebuilds {S.rebuilds} max resident {S.max_resident}",
           flush=True)
     assert S.max_resident >= 2 and S.rebuilds >= 2
     for r, sv in zip(reqs, served):
-        alone = serve.serve_alone(r.spec, opts)
+        alone = serve.serve_alone(r.spec, opts, predictor=predictor)
         mix = sv.result
         assert mix.failed == alone.failed, (r.spec.name, mix.failed, alone.failed)
         assert mix.newton == alone.newton, r.spec.name
