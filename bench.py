@@ -156,9 +156,27 @@ def tune_sweep(dev, sw, opts, eq_opts, predictor):
     return pred, warm, {"predicted_start": rep_p, "warm_start": rep_w}
 
 
+def parse_options(text):
+    """NewtonOptions fields from "key=value,key=value" (bench.py --set and
+    --start-set), converted to each field's type."""
+    import dataclasses
+    from tcadhip.device import NewtonOptions
+    types = {f.name: f.type for f in dataclasses.fields(NewtonOptions)}
+    out = {}
+    for item in filter(None, (t.strip() for t in text.split(","))):
+        k, v = item.split("=", 1)
+        if k not in types:
+            raise SystemExit(f"unknown NewtonOptions field {k!r}")
+        t = types[k] if isinstance(types[k], type) else \
+            {"bool": bool, "int": int, "float": float, "str": str}[types[k]]
+        out[k] = (v.lower() in ("1", "true", "yes")) if t is bool else t(v)
+    return out
+
+
 def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, precond="tile",
                   tune=True, forcing_slope=0.01, forcing_max=1e-4, tile_rows="auto",
-                  probe_cuts=(), forcing_pow=1, predictor=True):
+                  probe_cuts=(), forcing_pow=1, predictor=True, overrides=None,
+                  start_overrides=None):
     from tcadhip import configs
     from tcadhip.device import NewtonOptions, device_from_spec
     specs = [configs.config3_trigate(0.05), configs.config3_trigate(0.7)]
@@ -172,6 +190,13 @@ def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, prec
     eq_opts = NewtonOptions(max_iter=400)
     tuning = None
     opts_start = None
+    if overrides or start_overrides:
+        # a fixed configuration (profiling runs): the options of the
+        # predicted points, and of the points without a prediction
+        from dataclasses import replace
+        base = opts
+        opts = replace(base, **(overrides or {}))
+        opts_start = replace(base, **(start_overrides or {}))
     if tune:
         # the autotuner times its candidates on Vg points of the Vd = 0.05 V
         # sweep started as the sweep starts them (untimed, before the warm-up
@@ -759,6 +784,11 @@ def main():
     ap.add_argument("--tile-rows", default="auto", choices=("auto", "reg", "mem"),
                     help="tile-inverse rows in registers or read at the point of use "
                          "(untuned runs)")
+    ap.add_argument("--set", default="",
+                    help="NewtonOptions of the headline's predicted points, key=value,... "
+                         "(with --no-tune: a fixed configuration, e.g. the autotuner's choice)")
+    ap.add_argument("--start-set", default="",
+                    help="NewtonOptions of the points without a prediction, key=value,...")
     ap.add_argument("--no-predictor", action="store_true",
                     help="start each Vg point from the previous converged state (the "
                          "reference's warm start) instead of the secant prediction")
@@ -820,7 +850,8 @@ def main():
                       forcing_slope=args.forcing_slope, forcing_max=args.forcing_max,
                       forcing_pow=args.forcing_pow, tile_rows=args.tile_rows,
                       probe_cuts=[float(c) for c in args.probe_cuts.split(",") if c],
-                      predictor=not args.no_predictor)
+                      predictor=not args.no_predictor,
+                      overrides=parse_options(args.set), start_overrides=parse_options(args.start_set))
     npts = r["points_per_step"]
     value = npts * args.steps / r["seconds"]
     _say(f"tri-gate: {value:.3f} points/s")
