@@ -837,6 +837,9 @@ def main():
                                       forcing_max=1e-3),
                 "tuned": None,
                 "predictor": (NewtonOptions(), dict(predictor=True)),
+                "node": NewtonOptions(precond="node"),
+                "node_predictor": (NewtonOptions(precond="node"), dict(predictor=True)),
+                "predictor_graph": (NewtonOptions(graph=True), dict(predictor=True)),
                 "queue": (NewtonOptions(), dict(queue=True)),
                 "queue2": (NewtonOptions(), dict(queue=True, queue_depth=2)),
                 "queue4": (NewtonOptions(), dict(queue=True, queue_depth=4))}

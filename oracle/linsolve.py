@@ -10,7 +10,7 @@ has the same solution as the row-stamped one; `solve_linear("direct")`
 is SciPy's spsolve (the reference's own call, linsolve.py:860); the
 block-Jacobi Krylov path against direct within 1e-6 and block-Jacobi
 exact on a block-diagonal matrix to 1e-10 (the reference's own bars,
-tests/test_m22_linsolve.py:58-95, 271-298).
+pytcad/tests/test_m22_linsolve.py:58-95, 271-298).
 """
 import warnings
 

@@ -3,9 +3,9 @@ codes and the batched continuation sweep.
 
 Bars: a device's results in a batch are the same bits as alone (the
 kernels never mix devices and every reduction runs in a fixed order per
-device); against the CPU oracle, terminal currents within
-1e-6 * max_c |I| + 10 * |sum_c I| per point (the oracle's own conservation
-error is the floor) and Newton counts within +-1.
+device); against the CPU oracle, terminal currents within the current bar
+of tests/current_bar.py (1e-6 of the largest current plus each side's own
+conservation error, |sum_c I|, as the floor) and Newton counts within +-1.
 """
 import warnings
 

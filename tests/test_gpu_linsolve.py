@@ -1,5 +1,5 @@
 """The GPU solve_linear (tcadhip.linsolve, tcad_csr_solve) against the
-reference's own bars for its Krylov methods (tests/test_m22_linsolve.py):
+reference's own bars for its Krylov methods (pytcad/tests/test_m22_linsolve.py):
 
 * G3: GMRES / BiCGStab agree with the direct solve (SciPy spsolve, the
   reference's "direct") within 1e-6 relative, on random diagonally

@@ -9,7 +9,7 @@ Tolerances (stated per quantity):
     F_n, F_p and the edge currents likewise;
   * raw Jacobian J: per row max |J_gpu - J_ref| <= 1e-12 * max |J_ref row|;
   * converged states: psi atol 1e-9 (scaled, i.e. 2.6e-11 V), n and p
-    relative 1e-7 (the reference's own bars, tests/test_native_linsolve.py:118-137);
+    relative 1e-7 (the reference's own bars, pytcad/tests/test_native_linsolve.py:118-137);
   * terminal currents: 1e-6 of the largest current per bias point plus an
     absolute floor, the two states' own current-conservation errors
     (tests/current_bar.py states it); every point prints which bar held;
@@ -186,7 +186,7 @@ def test_sweep_vs_reference(name, mode):
             psi, nn, pp = states[p]
             assert np.abs(psi - d[f"psi_{p}"]).max() <= 1e-9
             # densities above 1e-10 (scaled), the reference's own mask
-            # (tests/test_native_linsolve.py:131-133)
+            # (pytcad/tests/test_native_linsolve.py:131-133)
             for got, key in ((nn, f"n_{p}"), (pp, f"p_{p}")):
                 ref = d[key]
                 m = ref > 1e-10

@@ -2,12 +2,12 @@
 tests/test_oracle_golden.py (made by tests/golden/make_golden_3d.py):
 
 * a z-invariant 3D device equals the golden-pinned 2D oracle
-  (the reference's own gate, tests/test_validation_3d.py:68-109, 156-187):
+  (the reference's own gate, pytcad/tests/test_validation_3d.py:68-109, 156-187):
   psi identical in every z-plane, I_3D = I_2D * Lz, same Newton counts;
 * axis-permutation symmetry: the same device built along x and along z
   gives the permuted solution (pins the z-axis code on the x-axis code);
-* finite-difference Jacobian (tests/test_validation_3d.py:112-153);
-* terminal-current conservation (tests/test_validation_3d.py:190-217).
+* finite-difference Jacobian (pytcad/tests/test_validation_3d.py:112-153);
+* terminal-current conservation (pytcad/tests/test_validation_3d.py:190-217).
 """
 import numpy as np
 

@@ -6,15 +6,15 @@ reference's own bars and by what the definitions fix:
   identity that the eliminated system has the row-stamped system's
   solution;
 * solve_linear("direct") is SciPy's spsolve bit for bit
-  (tests/test_m22_linsolve.py:33-55);
+  (pytcad/tests/test_m22_linsolve.py:33-55);
 * the node block-Jacobi preconditioner equals each node's dense block
-  solve to 1e-10 (tests/test_m22_linsolve.py:271-298), on non-symmetric
+  solve to 1e-10 (pytcad/tests/test_m22_linsolve.py:271-298), on non-symmetric
   blocks so a transposed block fails;
 * GMRES and BiCGStab with it agree with the direct solve within 1e-6 on
   random diagonally dominant systems and on a real device Jacobian
-  (tests/test_m22_linsolve.py:58-95);
+  (pytcad/tests/test_m22_linsolve.py:58-95);
 * a Krylov solve that cannot converge raises LinearSolveError, never
-  returns a half-solved state (tests/test_m22_linsolve.py:98-121).
+  returns a half-solved state (pytcad/tests/test_m22_linsolve.py:98-121).
 """
 import warnings
 

@@ -24,7 +24,7 @@ def _dB_exact(x):
 
 @pytest.mark.parametrize("scale", [1e-6, 1e-3, 1.0, 10.0, 100.0, 600.0])
 def test_bernoulli_vs_mpmath(scale):
-    """The reference's scale sweep (tests/test_physics_kernels_accel_parity.py:24-46)."""
+    """The reference's scale sweep (pytcad/tests/test_physics_kernels_accel_parity.py:24-46)."""
     rng = np.random.default_rng(int(scale * 1000) + 1)
     x = np.concatenate([rng.uniform(-1, 1, 200) * scale, [1e-4, -1e-4, 0.0, 9.999e-5]])
     B = ph.bernoulli(x)
