@@ -795,7 +795,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (fixed default solver options)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-cap", type=float, default=75.0,
+    ap.add_argument("--cpu-cap", type=float, default=30.0,
                     help="wall-clock cap [s] of the full-size CPU baseline")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--extras", default="diode,mosfet,ac,batch,serve",
