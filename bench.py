@@ -562,10 +562,12 @@ def bench_batch(lanes=1, sample=8, modes=None):
         big = max(range(len(specs)), key=lambda i: specs[i].N)
         dev = device_from_spec(specs[big])
         k = tuning_start(dev, specs[big].sweep, NewtonOptions(), NewtonOptions(), True)
-        # no slack here: a round solves every open device with one option
-        # set, predicted or not, so a candidate accepted with the slack would
-        # also run on the devices' first, unpredicted points
-        topts, tuning = tuner.tune(dev, specs[big].sweep[k], NewtonOptions(), say=_say)
+        # a round solves every open device with one option set, predicted or
+        # not: forcing terms (accepted with the predicted start's slack) then
+        # apply only to the devices whose start was predicted
+        topts, tuning = tuner.tune(dev, specs[big].sweep[k], NewtonOptions(), say=_say,
+                                   newton_slack=PREDICTED_SLACK, start="predicted")
+        topts.forcing_predicted = topts.forcing
         tuning = dict(tuning, device=big, nodes=int(specs[big].N), point=k)
         modes["tuned"] = (topts, dict(predictor=True))
         del dev
@@ -653,7 +655,8 @@ def bench_batch(lanes=1, sample=8, modes=None):
         if mode == "tuned":
             out[mode]["options"] = {k: getattr(mo, k) for k in
                                     ("forcing", "forcing_pow", "forcing_slope", "forcing_max",
-                                     "forcing_fp32", "tile_rows", "graph", "tile_lag")}
+                                     "forcing_fp32", "tile_rows", "graph", "tile_lag",
+                                     "forcing_predicted")}
             out[mode]["autotuner"] = tuning
     return out
 

@@ -444,7 +444,13 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
  *                            counterpart of the reference's per-iteration
  *                            direct fallback, device3d.py:2154-2168); only
  *                            if that fails too does the device stop with
- *                            TCAD_NEWTON_LINEAR_FAIL; 0: stop at once [1].
+ *                            TCAD_NEWTON_LINEAR_FAIL; 0: stop at once [1];
+ *   TCAD_TUNE_FORCING_PREDICTED 1: in a TCAD_LIN_FORCING solve of
+ *                            tcad_solve_bias only the devices whose start
+ *                            the last tcad_predict_state moved (ratio > 0)
+ *                            use forcing terms; the others solve every
+ *                            Newton step to lin_rtol, as without the flag
+ *                            [0].
  * Has no reference counterpart (the reference's Krylov tolerance is fixed,
  * linsolve.py:725-727). */
 #define TCAD_TUNE_TILE_ROWS 1
@@ -457,6 +463,7 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
 #define TCAD_TUNE_FORCING_POW 8
 #define TCAD_TUNE_TILE_LAG 9
 #define TCAD_TUNE_LINEAR_FALLBACK 10
+#define TCAD_TUNE_FORCING_PREDICTED 11
 int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
 
 /* Bench instrumentation, in-solver kernel timing: from this call on,

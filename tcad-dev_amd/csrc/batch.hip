@@ -121,7 +121,7 @@ __global__ void k_newton_check(UArgs U, double tol, double accept2, double rtol,
     // moved by less than tile_lag (a preconditioner from a nearby Jacobian;
     // the FP64 refinement still decides the solve)
     S.tile_keep = (tile_lag > 0.0 && m < tile_lag) ? 1 : 0;
-    if (forcing) {
+    if (forcing && S.forced) {
       const double eta = fmin(fmax(eta_c * (eta_pow == 2 ? m * m : m), rtol), eta_max);
       S.eta2 = eta * eta;
     }
@@ -416,12 +416,18 @@ __global__ void k_newton_finalize(UArgs U, double tol, int add_krylov) {
 }
 
 // Select the devices of the next Newton solve (mask NULL = all) and reset
-// their counters.
-__global__ void k_set_active(Scal *scal, int ndev, const int *mask, double eta0) {
+// their counters.  eta_exact < 0: every device starts at the target eta0 and
+// may use forcing terms; eta_exact >= 0 (forcing terms for predicted starts
+// only): a device whose start tcad_predict_state did not move solves every
+// Newton step to eta_exact.  The predicted flag is consumed here.
+__global__ void k_set_active(Scal *scal, int ndev, const int *mask, double eta0,
+                             double eta_exact) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndev) return;
   Scal &S = scal[d];
-  S.eta2 = eta0 * eta0;
+  S.forced = eta_exact < 0.0 || S.predicted;
+  S.predicted = 0;
+  S.eta2 = S.forced ? eta0 * eta0 : eta_exact * eta_exact;
   S.sel = mask ? (mask[d] != 0) : 1;
   S.active = S.sel;
   S.result = 0;
@@ -489,6 +495,12 @@ __global__ void __launch_bounds__(BS) k_predict(const DevInfo *info, const int *
   p[g] = pp * fmin(fmax(rp, 0.1), 10.0);
 }
 
+// Flag the devices whose start k_predict moved.
+__global__ void k_mark_predicted(Scal *scal, int ndev, const double *ratio) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < ndev) scal[d].predicted = ratio[d] > 0.0;
+}
+
 // Bulk-neutral guess arcsinh(C / (2 nie_s)) (device2d.py:744-759).
 __global__ void __launch_bounds__(BS) k_bulk_guess(const DevInfo *info, const int *blk2dev,
                                                    const double *C, const double *nie_s,
@@ -537,12 +549,12 @@ int sync_scalars(Batch &b) {
   return TCAD_OK;
 }
 
-int launch_set_active(Batch &b, const int32_t *h_mask, double eta0) {
+int launch_set_active(Batch &b, const int32_t *h_mask, double eta0, double eta_exact) {
   if (h_mask)
     TCAD_HIP_CHECK(hipMemcpyAsync(b.d_mask, h_mask, sizeof(int) * b.ndev, hipMemcpyHostToDevice,
                                   b.stream));
   hipLaunchKernelGGL(k_set_active, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
-                     b.ndev, h_mask ? (const int *)b.d_mask : nullptr, eta0);
+                     b.ndev, h_mask ? (const int *)b.d_mask : nullptr, eta0, eta_exact);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
   return TCAD_OK;
@@ -807,7 +819,9 @@ static int solve_bias_impl(Batch &b, int max_iter, double tol, double max_dpsi, 
                            double *p) {
   int rc;
   const NewtonCfg c = newton_cfg(b, mode, tol, max_dpsi, rtol, maxit);
-  if ((rc = launch_set_active(b, mask, c.forcing ? c.eta_max : rtol))) return rc;
+  if ((rc = launch_set_active(b, mask, c.forcing ? c.eta_max : rtol,
+                              c.forcing && b.forcing_pred_only ? rtol : -1.0)))
+    return rc;
   hipLaunchKernelGGL(k_stamp, dim3(b.nblk), dim3(BS), 0, b.stream, b.d_info, b.d_blk2dev, b.d_cidx,
                      b.d_cn_lnr, b.d_cn_n0, b.d_cn_p0, b.d_cn_contact, b.d_contact_V, b.d_scal,
                      psi, n, p);
@@ -1519,7 +1533,9 @@ int tcad_predict_state(tcad_batch *bp, const double *psi_prev, const double *n_p
                                 hipMemcpyHostToDevice, b.stream));
   hipLaunchKernelGGL(tcad::k_predict, dim3(b.nblk), dim3(tcad::BS), 0, b.stream, b.d_info,
                      b.d_blk2dev, b.d_ratio, psi_prev, n_prev, p_prev, psi, n, p);
-  b.launches++;
+  hipLaunchKernelGGL(tcad::k_mark_predicted, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream,
+                     b.d_scal, b.ndev, b.d_ratio);
+  b.launches += 2;
   TCAD_HIP_CHECK(hipGetLastError());
   return TCAD_OK;
 }
@@ -1856,6 +1872,10 @@ int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
     case TCAD_TUNE_TILE_LAG:
       if (!(value >= 0.0 && value < 1.0)) break;
       b.tile_lag = value;
+      return TCAD_OK;
+    case TCAD_TUNE_FORCING_PREDICTED:
+      if (value != 0.0 && value != 1.0) break;
+      b.forcing_pred_only = (int)value;
       return TCAD_OK;
     case TCAD_TUNE_FP32_INNER:
       if (!(value >= 1e-7 && value <= 1e-2)) break;

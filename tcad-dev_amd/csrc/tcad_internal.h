@@ -118,6 +118,9 @@ struct Scal {
   int cont_ok;                      // the last inner solve may be continued
   int newton_active;                // Newton-level flag kept across the final-step refinement
   int sel;                          // device selected for the current Newton solve
+  int predicted;                    // tcad_predict_state moved this device's start (consumed
+                                    // by the next solve's selection)
+  int forced;                       // the current Newton solve applies forcing terms
   int result;                       // TCAD_NEWTON_* outcome of the current Newton solve
 };
 
@@ -235,6 +238,7 @@ struct Batch {
   double forcing_c = 0.01, forcing_max = 1e-4;
   int forcing_pow = 1;              // eta = c * err^forcing_pow
   double tile_lag = 0.0;            // keep the tile inverses after a Newton update below this
+  int forcing_pred_only = 0;        // forcing terms only for devices with a predicted start
   // forcing solves whose target is loose (every active device's eta >=
   // 1e-5) run BiCGStab on the FP32-stored matrix inside the FP64 refinement
   int forcing_fp32 = 0;
@@ -366,7 +370,8 @@ int launch_bulk_guess(Batch &b, double *psi);
 int launch_slaved_densities(Batch &b, const double *psi, double *n, double *p);
 // select the devices of the next Newton solve (h_mask NULL = all); eta0 is
 // the forcing term of each device's first Newton solve
-int launch_set_active(Batch &b, const int32_t *h_mask, double eta0 = 0.0);
+int launch_set_active(Batch &b, const int32_t *h_mask, double eta0 = 0.0,
+                      double eta_exact = -1.0);
 int sync_scalars(Batch &b);
 // TCAD_E_NONFINITE (and clears the flags) if an assembly flagged a
 // non-finite or singular node block

@@ -37,6 +37,7 @@ TUNE_FP32_INNER = 7
 TUNE_FORCING_POW = 8
 TUNE_TILE_LAG = 9
 TUNE_LINEAR_FALLBACK = 10
+TUNE_FORCING_PREDICTED = 11
 
 CSR_BICGSTAB = 0
 CSR_GMRES = 1

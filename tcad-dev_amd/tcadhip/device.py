@@ -65,7 +65,9 @@ class NewtonOptions:
     # steps after an update below it keep the tile inverses of the step before;
     # linear_fallback: an unconverged Newton solve is redone, for that
     # iteration, by GMRES + ILU(0) on the device's CSR system
-    # (device3d.py:2154-2168's per-iteration fallback)
+    # (device3d.py:2154-2168's per-iteration fallback); forcing_predicted:
+    # forcing terms only for devices whose start the secant predictor moved
+    # (the others solve every Newton step to linsolve_rtol)
     forcing_slope: float = 0.01
     forcing_max: float = 1e-4
     forcing_fp32: bool = False
@@ -76,6 +78,7 @@ class NewtonOptions:
     forcing_pow: int = 1
     tile_lag: float = 0.0
     linear_fallback: bool = True
+    forcing_predicted: bool = False
 
     def apply_tuning(self, handle):
         L = _lib.lib()
@@ -105,6 +108,8 @@ class NewtonOptions:
                    "tcad_set_tuning")
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_LINEAR_FALLBACK,
                                      1.0 if self.linear_fallback else 0.0), "tcad_set_tuning")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FORCING_PREDICTED,
+                                     1.0 if self.forcing_predicted else 0.0), "tcad_set_tuning")
 
     def lin_mode(self):
         m = self.linsolve

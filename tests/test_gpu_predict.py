@@ -17,7 +17,13 @@ sweep's:
     may save more than one iteration where the bias steps are coarse (the
     reduced tri-gate fixtures step Vg by 0.2 V: 5 -> 3 at the last point);
   * a batch with the predictor gives every device the bits it gets alone
-    with the predictor.
+    with the predictor;
+  * forcing terms for predicted starts only (NewtonOptions.
+    forcing_predicted, TCAD_TUNE_FORCING_PREDICTED): in one solve of a batch
+    whose starts were predicted for some devices only, the predicted
+    devices get the bits of the same solve with forcing terms for all, the
+    others the bits of the solve without forcing terms (Newton and Krylov
+    counts, psi, n, p).
 """
 import numpy as np
 import pytest
@@ -135,3 +141,55 @@ def test_predicted_batch_equals_alone():
         ref = np.array([[c[x.name] for x in s.contacts] for c in plain[k].currents])
         _say(f"  device {k}: Newton predicted {sum(R.newton)} plain {sum(plain[k].newton)}")
         check(got, ref, f"device {k} vs plain", say=_say)
+
+
+def test_forcing_for_predicted_starts_only():
+    torch, configs, sweep, GPUBatch, NewtonOptions, _, setup_from_spec = _mods()
+    specs = configs.batch_diodes(count=3, seed=21, nx_range=(20, 40), ny_range=(12, 24))
+    setups = [setup_from_spec(s) for s in specs]
+    B = GPUBatch(setups)
+    B.solve_equilibrium(NewtonOptions())
+    volts = (0.15, 0.3, 0.45)                 # anode bias of the three points
+
+    def bias(k):
+        for s in setups:
+            s.bcs["anode"].V = volts[k]
+
+    def clone():
+        return tuple(t.clone() for t in (B.psi, B.n, B.p))
+
+    bias(0)
+    its, conv, _, _ = B.solve_bias(NewtonOptions())
+    assert conv.all()
+    prev = clone()
+    bias(1)
+    its, conv, _, _ = B.solve_bias(NewtonOptions())
+    assert conv.all()
+    start = clone()
+    bias(2)
+    f = np.array([sweep.secant_ratio(np.array([volts[0]]), np.array([volts[1]]),
+                                     np.array([volts[2]]))] * 3)
+    assert np.allclose(f, 1.0), f
+    f[1] = 0.0                                # device 1 starts unpredicted
+    _say(f"[forcing predicted only] ratios {f.tolist()}")
+    runs = {}
+    for key, opts in (("pred_only", NewtonOptions(forcing=True, forcing_predicted=True)),
+                      ("forcing", NewtonOptions(forcing=True)), ("exact", NewtonOptions())):
+        for t, v in zip((B.psi, B.n, B.p), start):
+            t.copy_(v)
+        B.predict(prev, f)
+        its, conv, kry, status = B.solve_bias(opts)
+        B.stream_sync()
+        assert conv.all() and (status == 0).all(), (key, conv, status)
+        runs[key] = (its.copy(), kry.copy(), [t.cpu().numpy() for t in (B.psi, B.n, B.p)])
+        _say(f"  {key}: Newton {its.tolist()} Krylov {kry.tolist()}")
+    for d in range(3):
+        ref = runs["exact" if f[d] == 0.0 else "forcing"]
+        got = runs["pred_only"]
+        sl = B.device_slice(d)
+        assert got[0][d] == ref[0][d] and got[1][d] == ref[1][d], (d, got[:2], ref[:2])
+        for a, b in zip(got[2], ref[2]):
+            assert np.array_equal(a[sl], b[sl]), d
+    # forcing terms change the predicted devices' Krylov work, so the
+    # comparison above tells the two option sets apart
+    assert any(runs["forcing"][1][d] != runs["exact"][1][d] for d in (0, 2)), runs
