@@ -899,14 +899,20 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
 // Workgroups of a batch's devices still iterating, for the Krylov launches
 // after a check: the devices that converged no longer occupy the grid (an
 // early-exit workgroup still costs a dispatch and three dependent loads).
-// Done when the open devices hold under 3/4 of the current grid; returns
-// false (grid unchanged) otherwise or if the map cannot be placed.
+// At the start of a solve (initial) the open devices are the active ones
+// (the host copy of the scalars is current then, and the reset marks every
+// other device done); after a check, those not done.  Done when the open
+// devices hold under 3/4 of the current grid; returns false (grid
+// unchanged) otherwise or if the map cannot be placed.
 template <typename T>
-static bool compact_grid(Batch &b, KArgs<T> &K) {
-  if (b.ndev < 2 || b.use_graph) return false;
+static bool compact_grid(Batch &b, KArgs<T> &K, bool initial = false) {
+  if (b.ndev < 2) return false;
+  auto is_open = [&b, initial](int d) {
+    return initial ? b.h_scal[d].active != 0 : !b.h_scal[d].done;
+  };
   long long open = 0;
   for (int d = 0; d < b.ndev; ++d)
-    if (!b.h_scal[d].done) open += b.info[d].nblk;
+    if (is_open(d)) open += b.info[d].nblk;
   if (open == 0 || 4 * open >= 3 * (long long)K.grid) return false;
   if (!b.d_bmap) {
     if (hipMalloc(&b.d_bmap, sizeof(int) * 2 * (size_t)b.nblk) != hipSuccess) {
@@ -927,7 +933,7 @@ static bool compact_grid(Batch &b, KArgs<T> &K) {
   b.bmap_half ^= 1;
   int n = 0;
   for (int d = 0; d < b.ndev; ++d) {
-    if (b.h_scal[d].done) continue;
+    if (!is_open(d)) continue;
     const DevInfo &I = b.info[d];
     for (int k = 0; k < I.nblk; ++k) h[n++] = I.blk0 + k;
   }
@@ -1050,7 +1056,11 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
   hipLaunchKernelGGL(k_krylov_reset, dim3((b.ndev + 63) / 64), dim3(64), 0, b.stream, b.d_scal,
                      b.ndev, rtol * rtol, maxit, per_dev ? 1 : 0, floor * floor, cont ? 1 : 0);
   b.launches++;
-  KArgs<T> KL = K;                  // its grid shrinks to the open devices as they finish
+  // its grid: the workgroups of the devices this solve selects (the host
+  // copy of the scalars is current on entry; the reset marks the others
+  // done), shrinking to those still iterating as devices finish
+  KArgs<T> KL = K;
+  compact_grid<T>(b, KL, true);
   launch_k_bicg_init<T>(b, KL);
   // Iterations are issued in chunks between host checks of the stop flags;
   // a check costs a host round trip with the stream drained, an iteration
@@ -1079,7 +1089,9 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
   int step = hint > 0 ? 16 : chunk;
   while (true) {
     int c = 0;
-    if (b.use_graph && b.probe < 0) {
+    // graph replays cover the whole grid (their kernel arguments are
+    // captured): only while the grid is not compacted
+    if (b.use_graph && b.probe < 0 && !KL.bmap) {
       if ((it & 1) && c < next && it < maxit) {
         launch_iteration<T>(b, KL, it & 1);
         ++c;
