@@ -539,11 +539,13 @@ def _cpu_diode(idx):
     return np.asarray(cur[-1]).tolist(), time.perf_counter() - t0
 
 
-def bench_batch(lanes=1, sample=8, modes=None):
+def bench_batch(lanes=2, sample=8, modes=None):
     """256 seeded diodes (40x30..160x100 nodes, -2.0..0.8 V, 0.1 V
     continuation steps), batched, both precisions; the plain FP64 path (one
     device at a time) on an evenly spaced sample.  Unit of work: one device
-    converged at its target bias."""
+    converged at its target bias.  Two lanes (half batches on two streams,
+    whose launches overlap): 2-4% above one lane in every mode measured,
+    three lanes no better than two in the plain modes."""
     from tcadhip import configs, sweep
     from tcadhip.device import NewtonOptions, setup_from_spec
     specs = configs.batch_diodes()
