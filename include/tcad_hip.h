@@ -450,7 +450,15 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
  *                            the last tcad_predict_state moved (ratio > 0)
  *                            use forcing terms; the others solve every
  *                            Newton step to lin_rtol, as without the flag
- *                            [0].
+ *                            [0];
+ *   TCAD_TUNE_TILE_RB        1: the tile preconditioner orders its tiles
+ *                            red-black (a checkerboard over the tile grid)
+ *                            and applies the black tiles' inverses after the
+ *                            red ones, to the residual less the couplings to
+ *                            the red tiles' results (forward block
+ *                            Gauss-Seidel): about half the BiCGStab
+ *                            iterations of 0, plain tile block-Jacobi, for
+ *                            two more launches per iteration [1].
  * Has no reference counterpart (the reference's Krylov tolerance is fixed,
  * linsolve.py:725-727). */
 #define TCAD_TUNE_TILE_ROWS 1
@@ -464,6 +472,7 @@ int tcad_csr_solve_noalloc(int32_t n, const int32_t *d_indptr, const int32_t *d_
 #define TCAD_TUNE_TILE_LAG 9
 #define TCAD_TUNE_LINEAR_FALLBACK 10
 #define TCAD_TUNE_FORCING_PREDICTED 11
+#define TCAD_TUNE_TILE_RB 12
 int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
 
 /* Bench instrumentation, in-solver kernel timing: from this call on,

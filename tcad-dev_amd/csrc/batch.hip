@@ -1873,6 +1873,10 @@ int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
       if (!(value >= 0.0 && value < 1.0)) break;
       b.tile_lag = value;
       return TCAD_OK;
+    case TCAD_TUNE_TILE_RB:
+      if (value != 0.0 && value != 1.0) break;
+      b.tile_rb = (int)value;
+      return TCAD_OK;
     case TCAD_TUNE_FORCING_PREDICTED:
       if (value != 0.0 && value != 1.0) break;
       b.forcing_pred_only = (int)value;

@@ -79,6 +79,11 @@ struct KArgs {
   int scale_b;                      // init: divide b by Scal::nrm (refinement rhs)
   int parity;
   int late_stop;                    // 1: finished devices return at KSTOP (late_stop_ok)
+  // red-black tile ordering (tile != 0): the preconditioner kernels apply
+  // the red tiles' inverses only, k_bicg_black the black tiles' after them
+  // (v[rb_src] -> v[rb_dst]; rb_init: the solve's first application,
+  // skipped by a continued solve, which rescales the last one)
+  int rb, rb_src, rb_dst, rb_init;
   // launch grid: grid workgroups; with bmap, workgroup i does the work of
   // the batch's workgroup bmap[i] (the workgroups of the devices still
   // iterating), else workgroup i is workgroup i (grid == nblk)
@@ -482,22 +487,159 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
 // grids, where more waves in flight hide the latency instead).
 enum { PV_NODE = 0, PV_TILE_REG = 1, PV_TILE_MEM = 2 };
 
-// z = M^-1 x at this thread's node (every thread of the workgroup calls
-// it: the tile variant holds barriers)
+// z = M^-1 x at this thread's node where `on` (every thread of the
+// workgroup calls it: the tile variant holds barriers)
 template <typename T, int PV>
 __device__ __forceinline__ void apply_prec(const KArgs<T> &K, const NodeHood &H, int dim,
-                                           bool use_tile, const TileRows &TR, const double x[3],
-                                           double z[3]) {
+                                           bool use_tile, bool on, const TileRows &TR,
+                                           const double x[3], double z[3]) {
   if (!use_tile) {
-    if (H.valid) prec3<T>(K.Dinv, K.N, H.g, x, z);
+    if (on) prec3<T>(K.Dinv, K.N, H.g, x, z);
     return;
   }
   __shared__ double tv[BS * 3];
   if (PV == PV_TILE_REG) {
-    tile_apply(TR, dim, H.valid, x, z, tv);
+    tile_apply(TR, dim, on, x, z, tv);
   } else {
     const TileMem TM{K.T4, K.T4s, K.N, H.g};
-    tile_apply(TM, dim, H.valid, x, z, tv);
+    tile_apply(TM, dim, on, x, z, tv);
+  }
+}
+
+// Red-black tile ordering: a tile is black when the sum of its tile
+// coordinates is odd.  Tiles that share a face differ in colour, so every
+// coupling that leaves a tile joins a red and a black one.
+__device__ __forceinline__ bool tile_black(int dim, const NodeHood &H) {
+  if (dim == 3) return ((H.i / T3X + H.j / T3Y + H.kk / T3Z) & 1) != 0;
+  return ((H.i / TILE_X + H.j / TILE_Y) & 1) != 0;
+}
+
+// Whether the neighbour in direction dir (x-, x+, y-, y+, z-, z+) lies in
+// another tile.
+__device__ __forceinline__ bool leaves_tile(int dim, const NodeHood &H, int dir) {
+  const int tx = dim == 3 ? T3X : TILE_X, ty = dim == 3 ? T3Y : TILE_Y;
+  switch (dir) {
+    case 0: return H.i % tx == 0;
+    case 1: return H.i % tx == tx - 1;
+    case 2: return H.j % ty == 0;
+    case 3: return H.j % ty == ty - 1;
+    case 4: return H.kk % T3Z == 0;
+    default: return H.kk % T3Z == T3Z - 1;
+  }
+}
+
+// With red-black ordering one launch applies the tiles of one colour, and
+// two lanes share each of their nodes: the lane at the same place in the
+// x-adjacent tile (the other colour, lane ^ TILE_X in 2D, ^ T3X in 3D)
+// takes the second half of the node's 48 tile columns, so every lane of
+// the workgroup works.  The halves are summed with one exchange.
+constexpr int QH = TU / 8;                  // quads (4 columns) per half row
+static_assert((TILE_X & (TILE_X - 1)) == 0 && (T3X & (T3X - 1)) == 0 &&
+                  (PX / TILE_X) % 2 == 0 && (PX3 / T3X) % 2 == 0 && TU % 8 == 0,
+              "a tile's x-neighbour in the patch is lane ^ tile width");
+__device__ __forceinline__ int mate_mask(int dim) { return dim == 3 ? T3X : TILE_X; }
+
+// The node this lane works for, the columns it takes and its tile's LDS
+// slot: its own node (primary: its tile has the colour being applied) or
+// its mate's.
+struct PairSlot {
+  long long g;
+  bool act;
+  int q0, vt;
+};
+
+__device__ __forceinline__ PairSlot pair_slot(int dim, const NodeHood &H, bool primary, bool on) {
+  int tile, l;
+  tile_slot(dim, tile, l);
+  const int m = mate_mask(dim);
+  const long long gm = __shfl_xor(H.g, m);
+  const int onm = __shfl_xor(on ? 1 : 0, m);
+  PairSlot P;
+  P.g = primary ? H.g : gm;
+  P.act = primary ? on : onm != 0;
+  P.q0 = primary ? 0 : QH;
+  P.vt = primary ? tile : (tile ^ 1);
+  return P;
+}
+
+// The half rows of a lane, loaded into registers before the reductions
+// (PV_TILE_REG: the first 3 QH entries of TileRows, whose registers the
+// whole rows would take without the red-black ordering), or read at the
+// point of use.
+__device__ __forceinline__ void half_load(const tile4 *T4, const float *T4s, long long N,
+                                          const PairSlot &P, TileRows &R) {
+  if (!P.act) return;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < QH; ++k) R.t[r * QH + k] = T4[(long long)(r * (TU / 4) + P.q0 + k) * N + P.g];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) R.sc[r] = T4s[(long long)r * N + P.g];
+}
+
+struct HalfMem {
+  const tile4 *T4;
+  const float *T4s;
+  long long N, g;
+  int q0;
+};
+
+__device__ __forceinline__ tile4 half_row(const TileRows &R, int r, int k) { return R.t[r * QH + k]; }
+__device__ __forceinline__ tile4 half_row(const HalfMem &M, int r, int k) {
+  return M.T4[(long long)(r * (TU / 4) + M.q0 + k) * M.N + M.g];
+}
+__device__ __forceinline__ float half_scale(const TileRows &R, int r) { return R.sc[r]; }
+__device__ __forceinline__ float half_scale(const HalfMem &M, int r) {
+  return M.T4s[(long long)r * M.N + M.g];
+}
+
+// z = T^-1 x on the tiles of one colour, two lanes per node: lanes with
+// `on` stage their x (their tile has the colour); z is valid on those lanes.
+// Every lane calls it (barriers and the exchange).
+template <typename Rows>
+__device__ __forceinline__ void pair_apply(const Rows &R, const PairSlot &P, int dim, bool on,
+                                           const double x[3], double z[3], double *tv) {
+  int tile, l;
+  tile_slot(dim, tile, l);
+  __syncthreads();                          // tv free (a previous use is done)
+#pragma unroll
+  for (int c = 0; c < 3; ++c) tv[tile * TU + 3 * l + c] = on ? x[c] : 0.0;
+  __syncthreads();
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (P.act) {
+    const double *v = tv + P.vt * TU;
+#pragma unroll
+    for (int k = 0; k < QH; ++k) {
+      const int q = P.q0 + k;
+      const double v0 = v[4 * q], v1 = v[4 * q + 1], v2 = v[4 * q + 2], v3 = v[4 * q + 3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const tile4 t = half_row(R, r, k);
+        acc[r] += (double)t.x * v0 + (double)t.y * v1 + (double)t.z * v2 + (double)t.w * v3;
+      }
+    }
+  }
+  const int m = mate_mask(dim);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double other = __shfl_xor(acc[r], m);
+    // the node's own half first, then its mate's (the same order on
+    // either lane of the pair)
+    z[r] = (P.q0 == 0 ? acc[r] + other : other + acc[r]) * (double)half_scale(R, r);
+  }
+}
+
+// The same for one colour of the red-black ordering (pair_apply).
+template <typename T, int PV>
+__device__ __forceinline__ void apply_prec_rb(const KArgs<T> &K, int dim, const PairSlot &P,
+                                              bool on, const TileRows &HR, const double x[3],
+                                              double z[3]) {
+  __shared__ double tv[BS * 3];
+  if (PV == PV_TILE_REG) {
+    pair_apply(HR, P, dim, on, x, z, tv);
+  } else {
+    const HalfMem HM{K.T4, K.T4s, K.N, P.g, P.q0};
+    pair_apply(HM, P, dim, on, x, z, tv);
   }
 }
 
@@ -540,8 +682,19 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
     return;
   }
   const bool use_tile = PV != PV_NODE && K.tile;
+  // red-black ordering: this launch applies the red tiles only, two lanes
+  // per node
+  const bool rb = use_tile && K.rb;
+  const bool black = tile_black(I.dim, H);
+  const bool on = H.valid && !(rb && black);
   TileRows TR;
-  if (PV == PV_TILE_REG) tile_load(K.T4, K.T4s, N, H.g, use_tile && H.valid, TR);
+  PairSlot PS{};
+  if (rb) {
+    PS = pair_slot(I.dim, H, !black, on);
+    if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, N, PS, TR);
+  } else if (PV == PV_TILE_REG) {
+    tile_load(K.T4, K.T4s, N, H.g, use_tile && on, TR);
+  }
   const double sc = K.scale_b ? 1.0 / S.nrm : 1.0;
   double b3[3] = {0, 0, 0}, rr = 0.0;
   if (H.valid) {
@@ -558,8 +711,9 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
     }
   }
   double z[3];
-  apply_prec<T, PV>(K, H, I.dim, use_tile, TR, b3, z);
-  if (H.valid) {
+  if (rb) apply_prec_rb<T, PV>(K, I.dim, PS, on, TR, b3, z);
+  else apply_prec<T, PV>(K, H, I.dim, use_tile, on, TR, b3, z);
+  if (on) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
   }
@@ -643,8 +797,17 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
   const NodeHood H = hood(I, blk);
   const long long N = K.N;
   const bool use_tile = PV != PV_NODE && K.tile;
+  const bool rb = use_tile && K.rb;
+  const bool black = tile_black(I.dim, H);
+  const bool on = H.valid && !(rb && black);
   TileRows TR;
-  if (PV == PV_TILE_REG) tile_load(K.T4, K.T4s, N, H.g, use_tile && H.valid, TR);
+  PairSlot PS{};
+  if (rb) {
+    PS = pair_slot(I.dim, H, !black, on);
+    if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, N, PS, TR);
+  } else if (PV == PV_TILE_REG) {
+    tile_load(K.T4, K.T4s, N, H.g, use_tile && on, TR);
+  }
   double r[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   if (H.valid) {
 #pragma unroll
@@ -692,8 +855,9 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
       K.v[VS][c * N + H.g] = s3[c];
     }
   }
-  apply_prec<T, PV>(K, H, I.dim, use_tile, TR, s3, z);
-  if (H.valid) {
+  if (rb) apply_prec_rb<T, PV>(K, I.dim, PS, on, TR, s3, z);
+  else apply_prec<T, PV>(K, H, I.dim, use_tile, on, TR, s3, z);
+  if (on) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) K.v[VSH][c * N + H.g] = z[c];
   }
@@ -720,8 +884,17 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
   const NodeHood H = hood(I, blk);
   const long long N = K.N;
   const bool use_tile = PV != PV_NODE && K.tile;
+  const bool rb = use_tile && K.rb;
+  const bool black = tile_black(I.dim, H);
+  const bool on = H.valid && !(rb && black);
   TileRows TR;
-  if (PV == PV_TILE_REG) tile_load(K.T4, K.T4s, N, H.g, use_tile && H.valid, TR);
+  PairSlot PS{};
+  if (rb) {
+    PS = pair_slot(I.dim, H, !black, on);
+    if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, N, PS, TR);
+  } else if (PV == PV_TILE_REG) {
+    tile_load(K.T4, K.T4s, N, H.g, use_tile && on, TR);
+  }
   double x[3] = {0, 0, 0}, ph[3] = {0, 0, 0}, shh[3] = {0, 0, 0}, s[3] = {0, 0, 0},
          t[3] = {0, 0, 0}, p[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   if (H.valid) {
@@ -764,8 +937,9 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
     }
   }
   double z[3];
-  apply_prec<T, PV>(K, H, I.dim, use_tile, TR, pn, z);
-  if (H.valid) {
+  if (rb) apply_prec_rb<T, PV>(K, I.dim, PS, on, TR, pn, z);
+  else apply_prec<T, PV>(K, H, I.dim, use_tile, on, TR, pn, z);
+  if (on) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
   }
@@ -775,6 +949,56 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
     S.rho2[K.parity ^ 1] = rho_new;
     S.omega = om_b;
     S.it = s_it + 1;
+  }
+}
+
+// The black tiles of the red-black tile preconditioner (forward block
+// Gauss-Seidel over the two colours): z = T^-1 (x - C z_red) on every
+// black tile, x = v[rb_src], z = v[rb_dst] (its red tiles written by the
+// launch before), C the couplings of the tile's nodes to the neighbouring
+// tiles, all red.  Halves the BiCGStab iterations of tile block-Jacobi on
+// the equilibrated Newton systems (CPU emulation, 200x120 MOSFET: 152 ->
+// 82, 300 -> 138; 32x24x24 tri-gate: 89 -> 47, 100 -> 48) for two more
+// launches per iteration.  Each black node's product is shared with the
+// red lane beside it (pair_apply).
+template <typename T, int PV>
+__device__ __forceinline__ void bicg_black_body(const KArgs<T> &K) {
+  KPROLOGUE
+  KSTOP
+  (void)lead;
+  (void)sh;
+  // a continued solve's first direction is the last solve's, rescaled by
+  // the init kernel (uniform per device: one device per workgroup)
+  if (K.rb_init && S.cont) return;
+  const NodeHood H = hood(I, blk);
+  const long long N = K.N;
+  const bool black = tile_black(I.dim, H);
+  const bool on = H.valid && black;
+  const PairSlot PS = pair_slot(I.dim, H, black, on);
+  TileRows HR;
+  if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, N, PS, HR);
+  const double *src = K.v[K.rb_src];
+  double *dst = K.v[K.rb_dst];
+  double x[3] = {0, 0, 0};
+  if (on) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[c] = src[c * N + H.g];
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      if (!H.nb[dir] || !leaves_tile(I.dim, H, dir)) continue;
+      const long long gn = H.nbg[dir];
+      const T *o = K.A + (long long)(9 + 5 * dir) * N + H.g;
+      const double zp = dst[gn], zn = dst[N + gn], zq = dst[2 * N + gn];
+      x[0] -= (double)o[0] * zp;
+      x[1] -= (double)o[N] * zp + (double)o[2 * N] * zn;
+      x[2] -= (double)o[3 * N] * zp + (double)o[4 * N] * zq;
+    }
+  }
+  double z[3];
+  apply_prec_rb<T, PV>(K, I.dim, PS, on, HR, x, z);
+  if (on) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dst[c * N + H.g] = z[c];
   }
 }
 
@@ -790,6 +1014,7 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
 TCAD_PV_KERNELS(k_bicg_init, bicg_init_body)
 TCAD_PV_KERNELS(k_bicg_s, bicg_s_body)
 TCAD_PV_KERNELS(k_bicg_u, bicg_u_body)
+TCAD_PV_KERNELS(k_bicg_black, bicg_black_body)
 
 // Variant of the preconditioner kernels for this batch: node-only when no
 // device applies tiles; else registers up to kTileRegBlocks workgroups
@@ -837,6 +1062,17 @@ static int precond_variant(const Batch &b) {
 TCAD_PV_LAUNCH(k_bicg_init, -1)
 TCAD_PV_LAUNCH(k_bicg_s, 1)
 TCAD_PV_LAUNCH(k_bicg_u, 3)
+TCAD_PV_LAUNCH(k_bicg_black, -1)
+
+// The black tiles' application after a kernel that applied the red ones.
+template <typename T>
+static void launch_black(Batch &b, KArgs<T> K, int src, int dst, int init) {
+  if (!K.tile || !K.rb) return;
+  K.rb_src = src;
+  K.rb_dst = dst;
+  K.rb_init = init;
+  launch_k_bicg_black<T>(b, K);
+}
 
 // Reset the per-device Krylov state for a new solve; devices not active
 // are marked done.  per_dev: each device's own target S.eta2 (not below
@@ -891,6 +1127,7 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
   K.b = b.ir_r;
   K.scale_b = 1;
   K.late_stop = late_stop_ok(b);
+  K.rb = b.tile_rb;
   K.bmap = nullptr;
   K.grid = b.nblk;
   return K;
@@ -980,11 +1217,13 @@ static void launch_iteration(Batch &b, KArgs<T> K, int parity) {
   else
     hipLaunchKernelGGL(k_bicg_spmv<T>, g, t, 0, b.stream, K);
   launch_k_bicg_s<T>(b, K);
+  launch_black<T>(b, K, VS, VSH, 0);
   if (probe_events(b, 2, e0, e1))
     hipExtLaunchKernelGGL(k_bicg_spmv_t<T>, g, t, 0, b.stream, e0, e1, 0, K);
   else
     hipLaunchKernelGGL(k_bicg_spmv_t<T>, g, t, 0, b.stream, K);
   launch_k_bicg_u<T>(b, K);
+  launch_black<T>(b, K, VP, VPH, 0);
   b.launches += 2;
 }
 
@@ -1012,6 +1251,7 @@ static hipGraphExec_t krylov_graph(Batch &b, const KArgs<T> &K) {
     put(&K.T4s, sizeof K.T4s); put(&K.tile, sizeof K.tile); put(K.v, sizeof K.v);
     put(&K.b, sizeof K.b); put(&K.part, sizeof K.part); put(&K.scal, sizeof K.scal);
     put(&K.scale_b, sizeof K.scale_b); put(&K.late_stop, sizeof K.late_stop);
+    put(&K.rb, sizeof K.rb);
   }
   for (const auto &g : b.graphs)
     if (g.variant == variant && g.key == key) return g.exec;
@@ -1062,6 +1302,7 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
   KArgs<T> KL = K;
   compact_grid<T>(b, KL, true);
   launch_k_bicg_init<T>(b, KL);
+  launch_black<T>(b, KL, VP, VPH, 1);
   // Iterations are issued in chunks between host checks of the stop flags;
   // a check costs a host round trip with the stream drained, an iteration
   // issued after convergence four early-exit launches.  Consecutive Newton

@@ -157,6 +157,7 @@ __device__ __forceinline__ double device_sum(const double *part, int nblk_total,
 struct NodeHood {
   bool valid;
   long long g;
+  int i, j, kk;                     // device-local coordinates
   bool nb[6];
   long long nbg[6];
 };
@@ -167,6 +168,7 @@ __device__ __forceinline__ NodeHood hood(const DevInfo &I, int blk) {
   H.valid = P.valid;
   H.g = I.off + P.k;
   const int i = P.i, j = P.j, kk = P.kk;
+  H.i = i; H.j = j; H.kk = kk;
   const long long sy = I.nx, sz = (long long)I.nx * I.ny;
   H.nb[0] = i > 0;          H.nbg[0] = H.g - 1;
   H.nb[1] = i < I.nx - 1;   H.nbg[1] = H.g + 1;

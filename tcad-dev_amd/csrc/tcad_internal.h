@@ -238,6 +238,7 @@ struct Batch {
   double forcing_c = 0.01, forcing_max = 1e-4;
   int forcing_pow = 1;              // eta = c * err^forcing_pow
   double tile_lag = 0.0;            // keep the tile inverses after a Newton update below this
+  int tile_rb = 1;                  // red-black tile ordering of the tile preconditioner
   int forcing_pred_only = 0;        // forcing terms only for devices with a predicted start
   // forcing solves whose target is loose (every active device's eta >=
   // 1e-5) run BiCGStab on the FP32-stored matrix inside the FP64 refinement

@@ -38,6 +38,7 @@ TUNE_FORCING_POW = 8
 TUNE_TILE_LAG = 9
 TUNE_LINEAR_FALLBACK = 10
 TUNE_FORCING_PREDICTED = 11
+TUNE_TILE_RB = 12
 
 CSR_BICGSTAB = 0
 CSR_GMRES = 1

@@ -67,7 +67,11 @@ class NewtonOptions:
     # iteration, by GMRES + ILU(0) on the device's CSR system
     # (device3d.py:2154-2168's per-iteration fallback); forcing_predicted:
     # forcing terms only for devices whose start the secant predictor moved
-    # (the others solve every Newton step to linsolve_rtol)
+    # (the others solve every Newton step to linsolve_rtol); tile_rb:
+    # red-black tile ordering of the tile preconditioner (the black tiles'
+    # inverses applied after the red ones', to the residual less their
+    # couplings to them: forward block Gauss-Seidel) instead of tile
+    # block-Jacobi
     forcing_slope: float = 0.01
     forcing_max: float = 1e-4
     forcing_fp32: bool = False
@@ -79,6 +83,7 @@ class NewtonOptions:
     tile_lag: float = 0.0
     linear_fallback: bool = True
     forcing_predicted: bool = False
+    tile_rb: bool = True
 
     def apply_tuning(self, handle):
         L = _lib.lib()
@@ -110,6 +115,8 @@ class NewtonOptions:
                                      1.0 if self.linear_fallback else 0.0), "tcad_set_tuning")
         _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_FORCING_PREDICTED,
                                      1.0 if self.forcing_predicted else 0.0), "tcad_set_tuning")
+        _lib.check(L.tcad_set_tuning(handle, _lib.TUNE_TILE_RB, 1.0 if self.tile_rb else 0.0),
+                   "tcad_set_tuning")
 
     def lin_mode(self):
         m = self.linsolve
