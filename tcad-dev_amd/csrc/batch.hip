@@ -1431,8 +1431,9 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   TRY(dalloc(&b->d_mask, (size_t)ndev));
   TRY(dalloc(&b->d_ratio, (size_t)ndev));
   TRY(dalloc(&b->d_dts, (size_t)ndev));
-  TRY(dalloc(&b->T4, (size_t)TILE_PLANES * N));
-  TRY(dalloc(&b->T4s, 3 * N));
+  // tile-inverse rows by patch slot (workgroups x BS; tile_row_slot)
+  TRY(dalloc(&b->T4, (size_t)TILE_PLANES * b->nblk * BS));
+  TRY(dalloc(&b->T4s, (size_t)3 * b->nblk * BS));
   TRY(dalloc(&b->P16, (size_t)TILE_NODES * N));
   {
     // equilibrium PCG: tile block-Jacobi (default: half the PCG

@@ -71,6 +71,7 @@ struct KArgs {
   const T *Dinv;                    // preconditioner storage precision
   const tile4 *T4;                  // tile inverses (FP16 rows), tile != 0
   const float *T4s;                 // their per-row scales
+  long long NS;                     // their plane stride (workgroups x BS; tile_row_slot)
   int tile;                         // 1: tile block-Jacobi, 0: node block-Jacobi
   double *v[9];                     // Krylov vectors, always FP64
   const double *b;
@@ -134,11 +135,13 @@ __device__ __forceinline__ void prec3(const T *Di, long long N, long long g, con
 // Tile block-Jacobi.  Every 16-node tile (4 x 4 in 2D, T3X x T3Y x T3Z in
 // 3D) is one diagonal block of the preconditioner: the scaled Newton matrix
 // restricted to the tile (couplings leaving the tile dropped), inverted
-// exactly.  The inverse rows of node k are TILE_PLANES planes of four FP16
-// entries, T4[(r * TILE_U / 4 + q) * N + g] (row r of the node, columns
+// exactly.  The inverse rows of a node are TILE_PLANES planes of four FP16
+// entries, T4[(r * TILE_U / 4 + q) * NS + slot] (row r of the node, columns
 // 4q..4q+3 of the tile, tile unknown u = 3 l + c with l the node's place in
 // the tile, x fastest), each row scaled by its largest magnitude, kept in
-// T4s[r * N + g].  Measured BiCGStab iterations to 1e-10 on the
+// T4s[r * NS + slot]; NS = workgroups x BS, and the slot of a node is
+// tile_row_slot's (patch-major, each colour of the red-black ordering
+// contiguous in its patch).  Measured BiCGStab iterations to 1e-10 on the
 // equilibrated systems of the 200x120 MOSFET (Vg = 0.5 V): node
 // block-Jacobi ~630, 4 x 4 tiles ~255 (4 x 8 tiles ~185 for twice the
 // bytes: slower on the GPU).  On a 32x24x24 tri-gate: node block-Jacobi
@@ -150,6 +153,28 @@ __device__ __forceinline__ void prec3(const T *Di, long long N, long long g, con
 // tiles: 19,540 iterations, but 8-node x rows, 16.0 points/s).
 // ---------------------------------------------------------------------------
 constexpr int TU = TILE_U;
+
+// Red-black tile ordering: a tile is black when the sum of its tile
+// coordinates is odd.  Tiles that share a face differ in colour, so every
+// coupling that leaves a tile joins a red and a black one.
+__device__ __forceinline__ bool tile_black_at(int dim, int i, int j, int kk) {
+  if (dim == 3) return ((i / T3X + j / T3Y + kk / T3Z) & 1) != 0;
+  return ((i / TILE_X + j / TILE_Y) & 1) != 0;
+}
+
+// Storage slot of the tile-inverse rows of the node at lane t of workgroup
+// (patch) blk, whose tile has colour `black`: the patch's BS slots hold its
+// red tiles' nodes, then its black tiles' (each in lane order: every x row
+// of a patch alternates tile colours, so each colour fills half of every
+// row).  A launch over the tiles of one colour then reads whole cache
+// lines, where the node order interleaves the colours every tile width
+// (2 nodes, 16 B of FP16 rows, in 3D).
+__device__ __forceinline__ long long tile_row_slot(int dim, long long blk, int t, bool black) {
+  const int px = dim == 3 ? PX3 : PX, tw = dim == 3 ? T3X : TILE_X;
+  const int il = t % px, row = t / px;
+  const int rank = row * (px / 2) + ((il / tw) >> 1) * tw + il % tw;
+  return blk * BS + (black ? BS / 2 : 0) + rank;
+}
 // tile setup: a 64-thread workgroup inverts STPW tiles side by side in x
 constexpr int STPW = 64 / TILE_NODES;
 constexpr int SCOLS = STPW * TILE_X;        // node columns of one 2D setup workgroup
@@ -306,9 +331,9 @@ __device__ __forceinline__ int setup_lane(int dim, int tq, int l) {
   return (l / TILE_X) * SCOLS + tq * TILE_X + l % TILE_X;
 }
 
-__global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const double *A,
-                                                    const Scal *scal, tile4 *T4, float *T4s,
-                                                    int *flag) {
+__global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, long long NS,
+                                                    const double *A, const Scal *scal,
+                                                    tile4 *T4, float *T4s, int *flag) {
   __shared__ double M[3 * TU * 64];
   __shared__ float amx[2 * 64 * 3];           // row maxima of each half
   const int nt = threadIdx.x % 64, half = threadIdx.x / 64, c0 = half * HCOL;
@@ -325,6 +350,11 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
   const int *ln = SN.ln;
   const bool valid = i < I.nx && j < I.ny && kk < I.nz;
   const long long g = I.off + ((long long)kk * I.ny + j) * I.nx + i;
+  // the node's lane in its patch's node workgroups, and its row slot
+  const int lane = I.dim == 3
+      ? ((kk - (int)bz * PZ3) * PY3 + (j - (int)by * PY3)) * PX3 + (i - (int)bx * PX3)
+      : (j - (int)by * PY) * PX + (i - (int)bx * PX);
+  const long long ts = tile_row_slot(I.dim, patch, lane, tile_black_at(I.dim, i, j, kk));
 #pragma unroll
   for (int e = 0; e < HCOL; ++e)
 #pragma unroll
@@ -443,7 +473,7 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
       const float amax = fmaxf(amx[nt * 3 + r], amx[(64 + nt) * 3 + r]);
       const float sc = amax > 0.0f && isfinite(amax) ? amax : 1.0f;
       const double inv = 1.0 / (double)sc;
-      if (half == 0) T4s[(long long)r * N + g] = sc;
+      if (half == 0) T4s[(long long)r * NS + ts] = sc;
 #pragma unroll
       for (int q = c0 / 4; q < (c0 + HCOL) / 4; ++q) {
         tile4 h;
@@ -451,7 +481,7 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, const 
         h.y = (_Float16)(TMAT(nt, r, 4 * q + 1) * inv);
         h.z = (_Float16)(TMAT(nt, r, 4 * q + 2) * inv);
         h.w = (_Float16)(TMAT(nt, r, 4 * q + 3) * inv);
-        T4[(long long)(r * (TU / 4) + q) * N + g] = h;
+        T4[(long long)(r * (TU / 4) + q) * NS + ts] = h;
       }
     }
   }
@@ -491,8 +521,8 @@ enum { PV_NODE = 0, PV_TILE_REG = 1, PV_TILE_MEM = 2 };
 // workgroup calls it: the tile variant holds barriers)
 template <typename T, int PV>
 __device__ __forceinline__ void apply_prec(const KArgs<T> &K, const NodeHood &H, int dim,
-                                           bool use_tile, bool on, const TileRows &TR,
-                                           const double x[3], double z[3]) {
+                                           bool use_tile, bool on, long long ts,
+                                           const TileRows &TR, const double x[3], double z[3]) {
   if (!use_tile) {
     if (on) prec3<T>(K.Dinv, K.N, H.g, x, z);
     return;
@@ -501,17 +531,13 @@ __device__ __forceinline__ void apply_prec(const KArgs<T> &K, const NodeHood &H,
   if (PV == PV_TILE_REG) {
     tile_apply(TR, dim, on, x, z, tv);
   } else {
-    const TileMem TM{K.T4, K.T4s, K.N, H.g};
+    const TileMem TM{K.T4, K.T4s, K.NS, ts};
     tile_apply(TM, dim, on, x, z, tv);
   }
 }
 
-// Red-black tile ordering: a tile is black when the sum of its tile
-// coordinates is odd.  Tiles that share a face differ in colour, so every
-// coupling that leaves a tile joins a red and a black one.
 __device__ __forceinline__ bool tile_black(int dim, const NodeHood &H) {
-  if (dim == 3) return ((H.i / T3X + H.j / T3Y + H.kk / T3Z) & 1) != 0;
-  return ((H.i / TILE_X + H.j / TILE_Y) & 1) != 0;
+  return tile_black_at(dim, H.i, H.j, H.kk);
 }
 
 // Whether the neighbour in direction dir (x-, x+, y-, y+, z-, z+) lies in
@@ -543,19 +569,22 @@ __device__ __forceinline__ int mate_mask(int dim) { return dim == 3 ? T3X : TILE
 // slot: its own node (primary: its tile has the colour being applied) or
 // its mate's.
 struct PairSlot {
-  long long g;
+  long long slot;                   // the node's tile-row slot
   bool act;
   int q0, vt;
 };
 
-__device__ __forceinline__ PairSlot pair_slot(int dim, const NodeHood &H, bool primary, bool on) {
+__device__ __forceinline__ PairSlot pair_slot(int dim, long long blk, const NodeHood &H,
+                                              bool primary, bool on) {
   int tile, l;
   tile_slot(dim, tile, l);
   const int m = mate_mask(dim);
-  const long long gm = __shfl_xor(H.g, m);
   const int onm = __shfl_xor(on ? 1 : 0, m);
+  // the mate's tile has the other colour
+  const bool black = tile_black(dim, H);
   PairSlot P;
-  P.g = primary ? H.g : gm;
+  P.slot = primary ? tile_row_slot(dim, blk, threadIdx.x, black)
+                   : tile_row_slot(dim, blk, threadIdx.x ^ m, !black);
   P.act = primary ? on : onm != 0;
   P.q0 = primary ? 0 : QH;
   P.vt = primary ? tile : (tile ^ 1);
@@ -572,9 +601,10 @@ __device__ __forceinline__ void half_load(const tile4 *T4, const float *T4s, lon
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
-    for (int k = 0; k < QH; ++k) R.t[r * QH + k] = T4[(long long)(r * (TU / 4) + P.q0 + k) * N + P.g];
+    for (int k = 0; k < QH; ++k)
+      R.t[r * QH + k] = T4[(long long)(r * (TU / 4) + P.q0 + k) * N + P.slot];
 #pragma unroll
-  for (int r = 0; r < 3; ++r) R.sc[r] = T4s[(long long)r * N + P.g];
+  for (int r = 0; r < 3; ++r) R.sc[r] = T4s[(long long)r * N + P.slot];
 }
 
 struct HalfMem {
@@ -638,7 +668,7 @@ __device__ __forceinline__ void apply_prec_rb(const KArgs<T> &K, int dim, const 
   if (PV == PV_TILE_REG) {
     pair_apply(HR, P, dim, on, x, z, tv);
   } else {
-    const HalfMem HM{K.T4, K.T4s, K.N, P.g, P.q0};
+    const HalfMem HM{K.T4, K.T4s, K.NS, P.slot, P.q0};
     pair_apply(HM, P, dim, on, x, z, tv);
   }
 }
@@ -687,13 +717,14 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
   const bool rb = use_tile && K.rb;
   const bool black = tile_black(I.dim, H);
   const bool on = H.valid && !(rb && black);
+  const long long tslot = tile_row_slot(I.dim, blk, threadIdx.x, black);
   TileRows TR;
   PairSlot PS{};
   if (rb) {
-    PS = pair_slot(I.dim, H, !black, on);
-    if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, N, PS, TR);
+    PS = pair_slot(I.dim, blk, H, !black, on);
+    if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, K.NS, PS, TR);
   } else if (PV == PV_TILE_REG) {
-    tile_load(K.T4, K.T4s, N, H.g, use_tile && on, TR);
+    tile_load(K.T4, K.T4s, K.NS, tslot, use_tile && on, TR);
   }
   const double sc = K.scale_b ? 1.0 / S.nrm : 1.0;
   double b3[3] = {0, 0, 0}, rr = 0.0;
@@ -712,7 +743,7 @@ __device__ __forceinline__ void bicg_init_body(const KArgs<T> &K) {
   }
   double z[3];
   if (rb) apply_prec_rb<T, PV>(K, I.dim, PS, on, TR, b3, z);
-  else apply_prec<T, PV>(K, H, I.dim, use_tile, on, TR, b3, z);
+  else apply_prec<T, PV>(K, H, I.dim, use_tile, on, tslot, TR, b3, z);
   if (on) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
@@ -800,13 +831,14 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
   const bool rb = use_tile && K.rb;
   const bool black = tile_black(I.dim, H);
   const bool on = H.valid && !(rb && black);
+  const long long tslot = tile_row_slot(I.dim, blk, threadIdx.x, black);
   TileRows TR;
   PairSlot PS{};
   if (rb) {
-    PS = pair_slot(I.dim, H, !black, on);
-    if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, N, PS, TR);
+    PS = pair_slot(I.dim, blk, H, !black, on);
+    if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, K.NS, PS, TR);
   } else if (PV == PV_TILE_REG) {
-    tile_load(K.T4, K.T4s, N, H.g, use_tile && on, TR);
+    tile_load(K.T4, K.T4s, K.NS, tslot, use_tile && on, TR);
   }
   double r[3] = {0, 0, 0}, v[3] = {0, 0, 0};
   if (H.valid) {
@@ -856,7 +888,7 @@ __device__ __forceinline__ void bicg_s_body(const KArgs<T> &K) {
     }
   }
   if (rb) apply_prec_rb<T, PV>(K, I.dim, PS, on, TR, s3, z);
-  else apply_prec<T, PV>(K, H, I.dim, use_tile, on, TR, s3, z);
+  else apply_prec<T, PV>(K, H, I.dim, use_tile, on, tslot, TR, s3, z);
   if (on) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) K.v[VSH][c * N + H.g] = z[c];
@@ -887,13 +919,14 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
   const bool rb = use_tile && K.rb;
   const bool black = tile_black(I.dim, H);
   const bool on = H.valid && !(rb && black);
+  const long long tslot = tile_row_slot(I.dim, blk, threadIdx.x, black);
   TileRows TR;
   PairSlot PS{};
   if (rb) {
-    PS = pair_slot(I.dim, H, !black, on);
-    if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, N, PS, TR);
+    PS = pair_slot(I.dim, blk, H, !black, on);
+    if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, K.NS, PS, TR);
   } else if (PV == PV_TILE_REG) {
-    tile_load(K.T4, K.T4s, N, H.g, use_tile && on, TR);
+    tile_load(K.T4, K.T4s, K.NS, tslot, use_tile && on, TR);
   }
   double x[3] = {0, 0, 0}, ph[3] = {0, 0, 0}, shh[3] = {0, 0, 0}, s[3] = {0, 0, 0},
          t[3] = {0, 0, 0}, p[3] = {0, 0, 0}, v[3] = {0, 0, 0};
@@ -938,7 +971,7 @@ __device__ __forceinline__ void bicg_u_body(const KArgs<T> &K) {
   }
   double z[3];
   if (rb) apply_prec_rb<T, PV>(K, I.dim, PS, on, TR, pn, z);
-  else apply_prec<T, PV>(K, H, I.dim, use_tile, on, TR, pn, z);
+  else apply_prec<T, PV>(K, H, I.dim, use_tile, on, tslot, TR, pn, z);
   if (on) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) K.v[VPH][c * N + H.g] = z[c];
@@ -974,9 +1007,9 @@ __device__ __forceinline__ void bicg_black_body(const KArgs<T> &K) {
   const long long N = K.N;
   const bool black = tile_black(I.dim, H);
   const bool on = H.valid && black;
-  const PairSlot PS = pair_slot(I.dim, H, black, on);
+  const PairSlot PS = pair_slot(I.dim, blk, H, black, on);
   TileRows HR;
-  if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, N, PS, HR);
+  if (PV == PV_TILE_REG) half_load(K.T4, K.T4s, K.NS, PS, HR);
   const double *src = K.v[K.rb_src];
   double *dst = K.v[K.rb_dst];
   double x[3] = {0, 0, 0};
@@ -1123,6 +1156,7 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
   K.Dinv = fp32 ? (const T *)b.Dinv32 : (const T *)b.Dinv64;
   K.T4 = b.T4;
   K.T4s = b.T4s;
+  K.NS = (long long)b.nblk * BS;
   K.tile = b.tile;
   K.b = b.ir_r;
   K.scale_b = 1;
@@ -1185,6 +1219,7 @@ static bool compact_grid(Batch &b, KArgs<T> &K, bool initial = false) {
 
 int launch_tile_setup(Batch &b, bool only_active) {
   hipLaunchKernelGGL(k_tile_setup, dim3(SPP * b.nblk), dim3(128), 0, b.stream, make_where(b), b.ntot,
+                     (long long)b.nblk * BS,
                      b.A64, only_active ? b.d_scal : nullptr, b.T4, b.T4s, b.d_flag);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
