@@ -1177,7 +1177,7 @@ static void free_batch(Batch *b) {
                   b->d_cscale, b->d_c2dev, b->d_gptr, b->d_gnode, b->d_surf[0], b->d_surf[1], b->d_surf[2],
                   b->d_surf[3], b->A64, b->A32, b->Dinv64, b->Dinv32, b->bs, b->rowscale,
                   b->colscale, b->y64, b->F, b->Fnp, b->part, b->d_scal, b->Aeq, b->beq,
-                  b->d_flag, b->d_mask, b->d_ratio, b->cand, b->d_Icand, b->T4, b->T4s,
+                  b->d_flag, b->d_mask, b->d_ratio, b->cand, b->d_Icand, b->T4, b->T4s, b->CPL,
                   b->P16, b->d_dts};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -1434,6 +1434,7 @@ static int create_impl(const tcad_device_desc *devs, int ndev, const tcad_static
   // tile-inverse rows by patch slot (workgroups x BS; tile_row_slot)
   TRY(dalloc(&b->T4, (size_t)TILE_PLANES * b->nblk * BS));
   TRY(dalloc(&b->T4s, (size_t)3 * b->nblk * BS));
+  TRY(dalloc(&b->CPL, (size_t)CPL_GROUPS * 5 * b->nblk * (BS / 2)));
   TRY(dalloc(&b->P16, (size_t)TILE_NODES * N));
   {
     // equilibrium PCG: tile block-Jacobi (default: half the PCG

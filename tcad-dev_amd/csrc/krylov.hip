@@ -72,6 +72,7 @@ struct KArgs {
   const tile4 *T4;                  // tile inverses (FP16 rows), tile != 0
   const float *T4s;                 // their per-row scales
   long long NS;                     // their plane stride (workgroups x BS; tile_row_slot)
+  const float *CPL;                 // black tiles' couplings to the red ones (cpl_index)
   int tile;                         // 1: tile block-Jacobi, 0: node block-Jacobi
   double *v[9];                     // Krylov vectors, always FP64
   const double *b;
@@ -174,6 +175,35 @@ __device__ __forceinline__ long long tile_row_slot(int dim, long long blk, int t
   const int il = t % px, row = t / px;
   const int rank = row * (px / 2) + ((il / tw) >> 1) * tw + il % tw;
   return blk * BS + (black ? BS / 2 : 0) + rank;
+}
+
+// Whether the neighbour of node (i, j, kk) in direction dir (x-, x+, y-,
+// y+, z-, z+) lies in another tile (at the device's edge: where it would).
+__device__ __forceinline__ bool leaves_tile_at(int dim, int i, int j, int kk, int dir) {
+  const int tx = dim == 3 ? T3X : TILE_X, ty = dim == 3 ? T3Y : TILE_Y;
+  switch (dir) {
+    case 0: return i % tx == 0;
+    case 1: return i % tx == tx - 1;
+    case 2: return j % ty == 0;
+    case 3: return j % ty == ty - 1;
+    case 4: return dim == 3 && kk % T3Z == 0;
+    default: return dim == 3 && kk % T3Z == T3Z - 1;
+  }
+}
+
+// The black tiles' couplings to the red tiles, FP32, for the black-tile
+// kernel: for every black node, its out-of-tile directions in direction
+// order (at most CPL_GROUPS: one side per axis, both along an axis the tile
+// is one node wide), 5 entries each (the stencil coupling entries psi<-psi,
+// n<-psi, n<-n, p<-psi, p<-p), zero where the neighbour is outside the
+// device.  Entry (grp * 5 + e) of black slot s sits at
+// (grp * 5 + e) * NC + cpl_index(s), NC = workgroups x BS / 2: contiguous
+// per patch, like the tile rows.
+static_assert((TILE_X > 1) + (TILE_Y > 1) <= 2 &&
+                  (T3X > 1 ? 1 : 2) + (T3Y > 1 ? 1 : 2) + (T3Z > 1 ? 1 : 2) <= CPL_GROUPS,
+              "at most CPL_GROUPS out-of-tile directions per node");
+__device__ __forceinline__ long long cpl_index(long long slot) {
+  return (slot / BS) * (BS / 2) + (slot % BS) - BS / 2;
 }
 // tile setup: a 64-thread workgroup inverts STPW tiles side by side in x
 constexpr int STPW = 64 / TILE_NODES;
@@ -333,7 +363,8 @@ __device__ __forceinline__ int setup_lane(int dim, int tq, int l) {
 
 __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, long long NS,
                                                     const double *A, const Scal *scal,
-                                                    tile4 *T4, float *T4s, int *flag) {
+                                                    tile4 *T4, float *T4s, float *CPL,
+                                                    int *flag) {
   __shared__ double M[3 * TU * 64];
   __shared__ float amx[2 * 64 * 3];           // row maxima of each half
   const int nt = threadIdx.x % 64, half = threadIdx.x / 64, c0 = half * HCOL;
@@ -354,7 +385,24 @@ __global__ void __launch_bounds__(128) k_tile_setup(Where W, long long N, long l
   const int lane = I.dim == 3
       ? ((kk - (int)bz * PZ3) * PY3 + (j - (int)by * PY3)) * PX3 + (i - (int)bx * PX3)
       : (j - (int)by * PY) * PX + (i - (int)bx * PX);
-  const long long ts = tile_row_slot(I.dim, patch, lane, tile_black_at(I.dim, i, j, kk));
+  const bool blk_tile = tile_black_at(I.dim, i, j, kk);
+  const long long ts = tile_row_slot(I.dim, patch, lane, blk_tile);
+  if (valid && blk_tile && half == 1 && CPL) {
+    // the black node's couplings to the red tiles around its tile
+    const long long NC = NS / 2, ci = cpl_index(ts);
+    const bool exists[6] = {i > 0, i < I.nx - 1, j > 0, j < I.ny - 1,
+                            I.dim == 3 && kk > 0, I.dim == 3 && kk < I.nz - 1};
+    int grp = 0;
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      if (!leaves_tile_at(I.dim, i, j, kk, dir)) continue;
+#pragma unroll
+      for (int e = 0; e < 5; ++e)
+        CPL[(long long)(grp * 5 + e) * NC + ci] =
+            exists[dir] ? (float)A[(long long)(9 + 5 * dir + e) * N + g] : 0.0f;
+      ++grp;
+    }
+  }
 #pragma unroll
   for (int e = 0; e < HCOL; ++e)
 #pragma unroll
@@ -543,15 +591,7 @@ __device__ __forceinline__ bool tile_black(int dim, const NodeHood &H) {
 // Whether the neighbour in direction dir (x-, x+, y-, y+, z-, z+) lies in
 // another tile.
 __device__ __forceinline__ bool leaves_tile(int dim, const NodeHood &H, int dir) {
-  const int tx = dim == 3 ? T3X : TILE_X, ty = dim == 3 ? T3Y : TILE_Y;
-  switch (dir) {
-    case 0: return H.i % tx == 0;
-    case 1: return H.i % tx == tx - 1;
-    case 2: return H.j % ty == 0;
-    case 3: return H.j % ty == ty - 1;
-    case 4: return H.kk % T3Z == 0;
-    default: return H.kk % T3Z == T3Z - 1;
-  }
+  return leaves_tile_at(dim, H.i, H.j, H.kk, dir);
 }
 
 // With red-black ordering one launch applies the tiles of one colour, and
@@ -1016,15 +1056,22 @@ __device__ __forceinline__ void bicg_black_body(const KArgs<T> &K) {
   if (on) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) x[c] = src[c * N + H.g];
+    // the couplings from the compact FP32 store (k_tile_setup), by
+    // out-of-tile direction in direction order
+    const long long NC = K.NS / 2;
+    const float *o = K.CPL + cpl_index(PS.slot);
+    int grp = 0;
 #pragma unroll
     for (int dir = 0; dir < 6; ++dir) {
-      if (!H.nb[dir] || !leaves_tile(I.dim, H, dir)) continue;
+      if (!leaves_tile(I.dim, H, dir)) continue;
+      const int gq = grp++;
+      if (!H.nb[dir]) continue;
       const long long gn = H.nbg[dir];
-      const T *o = K.A + (long long)(9 + 5 * dir) * N + H.g;
+      const float *og = o + (long long)(gq * 5) * NC;
       const double zp = dst[gn], zn = dst[N + gn], zq = dst[2 * N + gn];
-      x[0] -= (double)o[0] * zp;
-      x[1] -= (double)o[N] * zp + (double)o[2 * N] * zn;
-      x[2] -= (double)o[3 * N] * zp + (double)o[4 * N] * zq;
+      x[0] -= (double)og[0] * zp;
+      x[1] -= (double)og[NC] * zp + (double)og[2 * NC] * zn;
+      x[2] -= (double)og[3 * NC] * zp + (double)og[4 * NC] * zq;
     }
   }
   double z[3];
@@ -1157,6 +1204,7 @@ static KArgs<T> make_kargs(Batch &b, bool fp32) {
   K.T4 = b.T4;
   K.T4s = b.T4s;
   K.NS = (long long)b.nblk * BS;
+  K.CPL = b.CPL;
   K.tile = b.tile;
   K.b = b.ir_r;
   K.scale_b = 1;
@@ -1220,7 +1268,7 @@ static bool compact_grid(Batch &b, KArgs<T> &K, bool initial = false) {
 int launch_tile_setup(Batch &b, bool only_active) {
   hipLaunchKernelGGL(k_tile_setup, dim3(SPP * b.nblk), dim3(128), 0, b.stream, make_where(b), b.ntot,
                      (long long)b.nblk * BS,
-                     b.A64, only_active ? b.d_scal : nullptr, b.T4, b.T4s, b.d_flag);
+                     b.A64, only_active ? b.d_scal : nullptr, b.T4, b.T4s, b.CPL, b.d_flag);
   b.launches++;
   TCAD_HIP_CHECK(hipGetLastError());
   return TCAD_OK;

@@ -67,6 +67,7 @@ constexpr int T3X = TCAD_T3X, T3Y = TCAD_T3Y, T3Z = TILE_NODES / (T3X * T3Y);
 constexpr int PY3 = T3Y, PZ3 = T3Z, PX3 = BS / (PY3 * PZ3);
 constexpr int TILE_U = 3 * TILE_NODES;          // unknowns per tile
 constexpr int TILE_PLANES = 3 * TILE_U / 4;     // tile4 planes per node
+constexpr int CPL_GROUPS = 4;                   // out-of-tile directions per node, at most
 static_assert(PX % TILE_X == 0 && PY % TILE_Y == 0, "2D patches must hold whole tiles");
 static_assert(T3X * T3Y * T3Z == TILE_NODES && PX3 * PY3 * PZ3 == BS && PX3 % T3X == 0,
               "3D patches must hold whole 16-node tiles");
@@ -199,6 +200,7 @@ struct Batch {
   double *Dinv64 = nullptr;
   float *Dinv32 = nullptr;
   tile4 *T4 = nullptr;                           // tile inverses, TILE_PLANES planes (FP16)
+  float *CPL = nullptr;                          // black tiles' couplings to the red ones
   float *T4s = nullptr;                          // their per-row scales, 3 planes
   double *P16 = nullptr;                         // equilibrium PCG tile-inverse rows, 16 planes
   int pcg_tile = 0;                              // 1: tile block-Jacobi PCG, 0: Jacobi
