@@ -40,27 +40,65 @@ import numpy as np  # noqa: E402
 METRIC = ("converged bias-point solves/sec at <=1e-6 current err vs FP64; "
           "p50/p99 latency")
 HBM_PEAK_GBPS = 8000.0
-# algorithmic bytes per node of the fused BiCGStab update kernel k_bicg_u:
-# reads x, phat, shat, s, t, p, v (7 x 3 FP64), writes x, r, p, phat
-# (4 x 3 FP64), reads the node's 3 rows of its tile inverse (36 planes of
-# 4 FP16 + 3 FP32 scales)
-K_BICG_U_BYTES_PER_NODE = 7 * 24 + 4 * 24 + 36 * 8 + 3 * 4
-# algorithmic bytes per node of the four 3D BiCGStab kernels (39 FP64 stencil
-# entries per node; tile rows 300 B): k_bicg_spmv, k_bicg_s, k_bicg_spmv_t, k_bicg_u
-PROBE_BYTES_PER_NODE = {0: 39 * 8 + 3 * 24, 1: 4 * 24 + 300, 2: 39 * 8 + 4 * 24,
-                        3: K_BICG_U_BYTES_PER_NODE}
-
-
-def probe_min_us(kernel, nodes):
-    """Launches of the probed kernel shorter than this are the early exits
-    issued after a solve converged (they return before reading the vectors):
-    a launch that moved its algorithmic bytes in less time would have run
-    at twice the HBM peak.  Measured with HIP events on the 64x48x48
-    tri-gate such launches take 2.5-5 us, working k_bicg_u launches 10 us
-    and more; the cut is 5.2 us there."""
-    return PROBE_BYTES_PER_NODE[kernel] * nodes / (2 * HBM_PEAK_GBPS * 1e9) * 1e6
+# Algorithmic bytes of the BiCGStab kernels on one 3D device (each distinct
+# array element once): 39 FP64 stencil entries per node, FP64 vectors of 3
+# planes (24 B per node), a node's tile-inverse rows 36 x 4 FP16 + 3 FP32
+# scales (300 B).
+TILE_ROW_BYTES = 36 * 8 + 3 * 4
 PROBE_KERNELS = {0: "k_bicg_spmv<double>", 1: "k_bicg_s_tmem<double>",
-                 2: "k_bicg_spmv_t<double>", 3: "k_bicg_u_tmem<double>"}
+                 2: "k_bicg_spmv_t<double>", 3: "k_bicg_u_tmem<double>",
+                 4: "k_bicg_black_tmem<double>"}
+
+
+def krylov_bytes(shape, rb=True):
+    """Algorithmic bytes per launch of BiCGStab kernel 0..4 on a 3D device
+    of shape (nz, ny, nx) with 2x1x8-node tiles:
+      0 k_bicg_spmv: matrix 312, p-hat 24, v 24 (written), r-hat 24;
+      1 k_bicg_s: r, v 48, s 24 (written), s-hat 24 (written) and the tile
+        rows of the red tiles' nodes with the red-black ordering (rb), of
+        every node without;
+      2 k_bicg_spmv_t: matrix 312, s-hat 24, t 24 (written), s, r-hat 48;
+      3 k_bicg_u: x, p-hat, s-hat, s, t, p, v 168, x, r, p 72 (written),
+        p-hat 24 (written) and tile rows as kernel 1;
+      4 k_bicg_black (rb only): for each black-tile node its source 24,
+        tile rows 300, result 24 (written) and 20 (5 FP32) per existing
+        neighbour in another tile, and each such red neighbour's value
+        (24) once."""
+    nz, ny, nx = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    g = (k * ny + j) * nx + i
+    N = nx * ny * nz
+    black = ((i // 2 + j + k // 8) & 1) == 1
+    nb = int(black.sum()) if rb else 0
+    nr = N - nb
+    out = {0: N * (312 + 72), 2: N * (312 + 96),
+           1: N * 72 + nr * (24 + TILE_ROW_BYTES), 3: N * 240 + nr * (24 + TILE_ROW_BYTES)}
+    if not rb:
+        return out
+    moves = [((i % 2 == 0) & (i > 0), -1), ((i % 2 == 1) & (i < nx - 1), 1),
+             (j > 0, -nx), (j < ny - 1, nx),
+             ((k % 8 == 0) & (k > 0), -nx * ny), ((k % 8 == 7) & (k < nz - 1), nx * ny)]
+    adj = np.zeros(N, dtype=bool)
+    links = 0
+    for cond, step in moves:
+        sel = black & cond
+        links += int(sel.sum())
+        adj[g[sel] + step] = True
+    out[4] = nb * (24 + TILE_ROW_BYTES + 24) + links * 20 + int(adj.sum()) * 24
+    return out
+
+
+# Launches issued after a solve converged return before reading the vectors:
+# with HIP events on the 64x48x48 tri-gate they take 2.5-5 us, the working
+# launches of every BiCGStab kernel 10 us and more.
+EARLY_EXIT_US = 5.2
+
+
+def probe_min_us(kernel, shape):
+    """Launches of the probed kernel shorter than this are left out as early
+    exits: under EARLY_EXIT_US, or so short that their algorithmic bytes
+    would have moved at twice the HBM peak."""
+    return max(EARLY_EXIT_US, krylov_bytes(shape)[kernel] / (2 * HBM_PEAK_GBPS * 1e9) * 1e6)
 
 
 def _say(msg):
@@ -173,7 +211,7 @@ def parse_options(text):
     return out
 
 
-def bench_trigate(steps, warmup, mode, points=None, probe=3, forcing=False, precond="tile",
+def bench_trigate(steps, warmup, mode, points=None, probe=4, forcing=False, precond="tile",
                   tune=True, forcing_slope=0.01, forcing_max=1e-4, tile_rows="auto",
                   probe_cuts=(), forcing_pow=1, predictor=True, overrides=None,
                   start_overrides=None):
@@ -253,7 +291,7 @@ def probe_step(devs, specs, opts, eq_opts, probe, cut_us, predictor=True, opts_s
     for d in devs:
         n = ctypes.c_int64()
         us = ctypes.c_double()
-        cut = probe_min_us(probe, d.N) if cut_us is None else cut_us
+        cut = probe_min_us(probe, tuple(specs[0].shape)) if cut_us is None else cut_us
         L.check(L.lib().tcad_probe_read(d._gpu().handle, cut, ctypes.byref(n), ctypes.byref(us)),
                 "tcad_probe_read")
         L.check(L.lib().tcad_probe_kernel(d._gpu().handle, -1), "tcad_probe_kernel")
@@ -774,8 +812,9 @@ def main():
                     help="linear-solve path of the headline")
     ap.add_argument("--points", type=int, default=None,
                     help="first K Vg points of each sweep only (short profiling runs)")
-    ap.add_argument("--probe", type=int, default=3,
-                    help="BiCGStab kernel timed in-solver for the roofline (-1: none)")
+    ap.add_argument("--probe", type=int, default=4,
+                    help="BiCGStab kernel timed in-solver for the roofline: 0 SpMV, 1 s, "
+                         "2 second SpMV, 3 update, 4 black tiles (the most GPU time; -1: none)")
     ap.add_argument("--forcing", action="store_true",
                     help="Newton forcing terms (inexact early linear solves)")
     ap.add_argument("--forcing-slope", type=float, default=0.01)
@@ -864,6 +903,7 @@ def main():
     value = npts * args.steps / r["seconds"]
     _say(f"tri-gate: {value:.3f} points/s")
     N = r["specs"][0].N
+    specs0 = r["specs"][0]
     lat_ms = r["latencies"] * 1e3
     line = {
         "metric": METRIC,
@@ -914,24 +954,21 @@ def main():
         kname = PROBE_KERNELS[pr["kernel"]]
         if r["opts"].tile_rows == "reg":
             kname = kname.replace("_tmem", "_treg")
-        if pr["kernel"] == 3:
-            nbytes = K_BICG_U_BYTES_PER_NODE * N
-            achieved = nbytes / (pr["avg_us"] * 1e-6) / 1e9
-            line["roofline"] = {
-                "bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                "traffic": measured_traffic("trigate_64x48x48_hbm_traffic.md", kname),
-                "bytes_per_launch": nbytes, "bytes_per_node": K_BICG_U_BYTES_PER_NODE,
-                "avg_launch_us_in_solver": round(pr["avg_us"], 3),
-                "launches_timed": pr["launches"],
-                "timing": (f"kernel start/end HIP events (hipExtLaunchKernelGGL) on every launch "
-                           f"of the kernel in one untimed step; launches under "
-                           f"{probe_min_us(3, N):.2f} us (early exits after convergence: "
-                           f"bytes / (2 x peak)) excluded"),
-                "copy_ceiling_GBps": copy_ceiling()}
-        else:
-            line["probe"] = {"kernel": kname, "avg_us": round(pr["avg_us"], 3),
-                             "launches": pr["launches"]}
+        shape = tuple(specs0.shape)
+        nbytes = krylov_bytes(shape, r["opts"].tile_rb)[pr["kernel"]]
+        achieved = nbytes / (pr["avg_us"] * 1e-6) / 1e9
+        line["roofline"] = {
+            "bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": measured_traffic("trigate_64x48x48_hbm_traffic.md", kname),
+            "bytes_per_launch": nbytes, "bytes_per_node": round(nbytes / N, 1),
+            "avg_launch_us_in_solver": round(pr["avg_us"], 3),
+            "launches_timed": pr["launches"],
+            "timing": (f"kernel start/end HIP events (hipExtLaunchKernelGGL) on every launch "
+                       f"of the kernel in one untimed step; launches under "
+                       f"{probe_min_us(pr['kernel'], shape):.2f} us (early exits after "
+                       f"convergence: bytes / (2 x peak)) excluded"),
+            "copy_ceiling_GBps": copy_ceiling()}
         if r.get("probe_cuts"):
             line["probe_cuts"] = r["probe_cuts"]
     if not args.no_parity:

@@ -479,7 +479,9 @@ int tcad_set_tuning(tcad_batch *b, int32_t key, double value);
  * every launch of one BiCGStab kernel inside the solves of this batch is
  * bracketed by a pair of HIP events on the batch stream (which: 0 the
  * stencil SpMV, 1 the alpha / s kernel, 2 the second SpMV, 3 the fused
- * update kernel, whichever preconditioner variant runs; -1 stops).
+ * update kernel, 4 the black-tile kernel of the red-black tile ordering
+ * (both of its launches per iteration), whichever preconditioner variant
+ * runs; -1 stops).
  * The events take the kernel's own start and end (hipExtLaunchKernelGGL).
  * tcad_probe_read waits for the stream and returns, of the launches
  * bracketed since the last read, those that ran at least min_us

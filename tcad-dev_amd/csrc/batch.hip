@@ -1897,8 +1897,8 @@ int tcad_set_tuning(tcad_batch *bp, int32_t key, double value) {
 int tcad_probe_kernel(tcad_batch *bp, int32_t which) {
   GUARD(bp);
   Batch &b = *bp->impl;
-  if (which < -1 || which > 3) {
-    tcad::set_error("tcad_probe_kernel: which must be -1..3");
+  if (which < -1 || which > 4) {
+    tcad::set_error("tcad_probe_kernel: which must be -1..4");
     return TCAD_E_INVALID;
   }
   TCAD_HIP_CHECK(hipStreamSynchronize(b.stream));

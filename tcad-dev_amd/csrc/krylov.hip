@@ -1142,7 +1142,7 @@ static int precond_variant(const Batch &b) {
 TCAD_PV_LAUNCH(k_bicg_init, -1)
 TCAD_PV_LAUNCH(k_bicg_s, 1)
 TCAD_PV_LAUNCH(k_bicg_u, 3)
-TCAD_PV_LAUNCH(k_bicg_black, -1)
+TCAD_PV_LAUNCH(k_bicg_black, 4)
 
 // The black tiles' application after a kernel that applied the red ones.
 template <typename T>
