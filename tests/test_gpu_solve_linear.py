@@ -2,7 +2,8 @@
 Newton system (linsolve.solve_linear's iterative contract,
 pytcad/pytcad/linsolve.py:725-985), with each preconditioner the library
 offers: the reference's node block-Jacobi and the 16-node tile inverses,
-with the FP64 and the FP32-stored matrix.
+applied in red-black order (the default) and as block-Jacobi
+(TCAD_TUNE_TILE_RB 0), with the FP64 and the FP32-stored matrix.
 
 The system is the reference's own: the Newton matrix at the fixtures'
 perturbed states (tests/golden/make_golden*.py), assembled on the GPU
@@ -25,7 +26,8 @@ from oracle import dd
 pytestmark = pytest.mark.gpu
 
 CASES = ["mosfet_16x12_fj.npz", "trigate_12x10x8_fj.npz", "slab_zgate_12x9x10_fj.npz"]
-MODES = [("node", False), ("tile", False), ("node", True), ("tile", True)]
+MODES = [("node", False), ("tile", False), ("tile-jacobi", False), ("node", True), ("tile", True),
+         ("tile-jacobi", True)]
 
 
 def _say(msg):
@@ -75,6 +77,8 @@ def test_solve_linear_preconditioners(name):
     for kind, mixed in MODES:
         mode = _lib.LIN_BICGSTAB_MIXED if mixed else _lib.LIN_BICGSTAB_F64
         mode |= _lib.PRECOND_NODE if kind == "node" else 0
+        _lib.check(L.tcad_set_tuning(B.handle, _lib.TUNE_TILE_RB,
+                                     0.0 if kind == "tile-jacobi" else 1.0), "tcad_set_tuning")
         du = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
         its = (ctypes.c_int32 * 1)()
         res = (ctypes.c_double * 1)()
