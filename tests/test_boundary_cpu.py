@@ -70,3 +70,17 @@ def test_struct_layout_matches_header(tmp_path):
     want += [ctypes.sizeof(_lib.StaticFields)]
     want += [getattr(_lib.StaticFields, f).offset for f in sfields]
     assert got == want
+
+
+def test_header_constants_match_binding():
+    """Every integer constant the header defines has the same value in the
+    ctypes binding (TCAD_E_* and TCAD_OK under their own names, the others
+    without the TCAD_ prefix)."""
+    src = open(HEADER).read()
+    defs = dict(re.findall(r"^#define\s+(TCAD_\w+)\s+\(?(-?(?:0x)?[0-9a-fA-F]+)\)?", src, re.M))
+    assert defs, "no constants parsed"
+    for name, text in defs.items():
+        value = int(text, 0)
+        py = name if name == "TCAD_OK" or name.startswith("TCAD_E_") else name[len("TCAD_"):]
+        assert hasattr(_lib, py), f"{name} has no binding constant {py}"
+        assert getattr(_lib, py) == value, (name, getattr(_lib, py), value)
