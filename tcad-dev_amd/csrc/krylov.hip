@@ -1313,7 +1313,8 @@ static void launch_iteration(Batch &b, KArgs<T> K, int parity) {
 // Blocks of kGraphIts BiCGStab iterations (an even count, so the iteration
 // parity is the same at every block start) captured once per kernel-argument
 // set as a HIP graph and replayed on the batch stream: one graph launch
-// instead of 4 kGraphIts kernel launches from the host.
+// instead of 4 kGraphIts (6 kGraphIts with red-black tiles) kernel launches
+// from the host.
 constexpr int kGraphIts = 16;
 
 template <typename T>
@@ -1388,7 +1389,8 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
   launch_black<T>(b, KL, VP, VPH, 1);
   // Iterations are issued in chunks between host checks of the stop flags;
   // a check costs a host round trip with the stream drained, an iteration
-  // issued after convergence four early-exit launches.  Consecutive Newton
+  // issued after convergence four (six with red-black tiles) early-exit
+  // launches.  Consecutive Newton
   // solves need similar counts per digit of residual reduction, so the
   // first chunk is 3/4 of the last solve's count, scaled down by the ratio
   // of the two targets' logarithms when this one's is looser (a loose
@@ -1406,7 +1408,7 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
   double predicted = (by_slot ? 0.9 : 0.75) * (double)hint;
   // only ever scaled down: a continued refinement sweep needs far fewer
   // iterations than a fresh solve to the same target, and a chunk issued
-  // past convergence costs four launches per iteration
+  // past convergence costs four to six launches per iteration
   if (hint > 0 && hint_lt < 0.0) predicted *= std::min(1.0, lt / hint_lt);
   int it = 0;
   int next = hint > 0 ? std::max(chunk, (int)predicted) : chunk;
@@ -1425,7 +1427,7 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
         hipGraphExec_t g = krylov_graph<T>(b, KL);
         if (!g) break;
         TCAD_HIP_CHECK(hipGraphLaunch(g, b.stream));
-        b.launches += 4 * kGraphIts;
+        b.launches += (KL.tile && KL.rb ? 6 : 4) * kGraphIts;
         c += kGraphIts;
         it += kGraphIts;
       }
@@ -1443,7 +1445,7 @@ static int run_bicgstab(Batch &b, const KArgs<T> &K, double rtol, int maxit, int
     // the iterations the open devices still need at their average rate of
     // residual reduction so far: the next chunk is that plus a margin, at
     // most the geometric step (an underestimate costs one more round trip,
-    // an overestimate four early-exit launches per iteration)
+    // an overestimate four to six early-exit launches per iteration)
     double need = 0.0;
     bool rated = true;
     for (int d = 0; d < b.ndev; ++d) {
