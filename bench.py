@@ -577,6 +577,83 @@ def _cpu_diode(idx):
     return np.asarray(cur[-1]).tolist(), time.perf_counter() - t0
 
 
+def bench_transient(t_end=1e-9, dt0=1e-12, cpu_t_end=3e-12):
+    """Transient: the 200x120 MOSFET at Vg = 1.0 V, Vd = 0.05 V, its drain
+    stepped to 0.5 V at t = 0, backward Euler with the reference's adaptive
+    dt (transient2d.py:186-306) to t_end: accepted steps per second.  The
+    FP64 oracle (NumPy + SciPy SuperLU, the reference's direct solve) steps
+    the same device from the same state to cpu_t_end; the GPU's run to that
+    time gives the drain current and Newton counts compared, the counts
+    with the transient tests' bar: within +-1 of the oracle with the
+    equilibrated direct solve (the exact step the GPU's equilibrated Krylov
+    solve converges to) and never above the reference-method oracle's + 1."""
+    from oracle import transient as otr
+    from oracle_build import oracle_device
+    from tcadhip import configs
+    from tcadhip import transient as tr
+    from tcadhip.device import NewtonOptions, device_from_spec
+    spec = configs.config2_mosfet()
+    dev = device_from_spec(spec)
+    dev.solve_equilibrium(NewtonOptions())
+    for bias in spec.sweep:
+        dev.solve_bias(bias, NewtonOptions())
+    psi0, n0, p0 = dev.psi, dev.n, dev.p
+    vd = dev.bcs["drain"].V
+
+    def reset():
+        dev.bcs["drain"].V = vd
+        dev.set_state(psi0, n0, p0)
+
+    # the comparison run, which also warms the path up
+    short = tr.solve_transient(dev, {"drain": tr.StepWaveform(vd, 0.5)}, cpu_t_end, dt0)
+    reset()
+    _sync()
+    t0 = time.perf_counter()
+    r = tr.solve_transient(dev, {"drain": tr.StepWaveform(vd, 0.5)}, t_end, dt0,
+                           keep_history=False)
+    _sync()
+    dt = time.perf_counter() - t0
+    steps = len(r.iterations)
+    def oracle_at_start():
+        odev = oracle_device(spec)
+        odev.psi, odev.n, odev.p = psi0, n0, p0
+        for nm, bc in dev.bcs.items():
+            obc = odev.bcs[nm]
+            if hasattr(bc, "Vg"):
+                obc.Vg = bc.Vg
+            else:
+                obc.V = vd if nm == "drain" else bc.V
+        odev._update_surface_mobility(odev.psi)
+        return odev
+
+    odev = oracle_at_start()
+    t1 = time.perf_counter()
+    ores = otr.solve_transient(odev, {"drain": otr.StepWaveform(vd, 0.5)}, cpu_t_end, dt0)
+    dt_cpu = time.perf_counter() - t1
+    eres = otr.solve_transient(oracle_at_start(), {"drain": otr.StepWaveform(vd, 0.5)},
+                               cpu_t_end, dt0, equilibrated=True)
+    ng, nr, ne = (list(map(int, x.iterations)) for x in (short, ores, eres))
+    g_i = np.asarray(short.terminal_current["drain"])
+    o_i = np.asarray(ores.terminal_current["drain"])
+    k = min(len(g_i), len(o_i))
+    scale = float(np.abs(o_i[:k]).max())
+    return {"device": "mosfet_200x120 (Vg 1.0 V), drain 0.05 -> 0.5 V step at t = 0",
+            "scheme": "backward Euler, adaptive dt", "t_end_s": t_end, "dt0_s": dt0,
+            "steps": steps, "newton_total": int(np.sum(r.iterations)),
+            "seconds": round(dt, 3), "steps_per_s": round(steps / dt, 3),
+            "compared_to_s": cpu_t_end,
+            "times_equal_to_oracle": bool(len(short.times) == len(ores.times) and
+                                          np.allclose(short.times, ores.times, rtol=1e-12)),
+            "newton_gpu": ng, "newton_oracle": nr, "newton_oracle_equilibrated": ne,
+            "newton_within_bar": bool(len(ng) == len(ne) == len(nr) and
+                                      all(abs(a - e) <= 1 and a <= r + 1
+                                          for a, r, e in zip(ng, nr, ne))),
+            "max_rel_drain_err_vs_oracle": float(np.abs(g_i[:k] - o_i[:k]).max() / scale),
+            "cpu_oracle_steps_per_s": round(len(ores.iterations) / dt_cpu, 4),
+            "cpu_sample": f"FP64 oracle (NumPy + SciPy SuperLU, 1 core), the first "
+                          f"{len(ores.iterations)} steps to {cpu_t_end:g} s, {dt_cpu:.1f} s"}
+
+
 def bench_batch(lanes=2, sample=8, modes=None):
     """256 seeded diodes (40x30..160x100 nodes, -2.0..0.8 V, 0.1 V
     continuation steps), batched, both precisions; the plain FP64 path (one
@@ -848,7 +925,7 @@ def main():
     ap.add_argument("--cpu-cap", type=float, default=30.0,
                     help="wall-clock cap [s] of the full-size CPU baseline")
     ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--extras", default="diode,mosfet,ac,batch,serve",
+    ap.add_argument("--extras", default="diode,mosfet,ac,transient,batch,serve",
                     help="extra workloads (comma-separated; '' for none)")
     ap.add_argument("--batch-variants", default="",
                     help="only the batched diodes, with these option sets (comma-separated of "
@@ -991,12 +1068,13 @@ def main():
     torch.cuda.empty_cache()
     extras = [e for e in args.extras.split(",") if e]
     for name, fn in (("diode", bench_diode), ("mosfet", bench_mosfet), ("ac", bench_ac),
-                     ("batch", bench_batch), ("serve", bench_serve)):
+                     ("transient", bench_transient), ("batch", bench_batch),
+                     ("serve", bench_serve)):
         if name in extras:
             _say(f"extra: {name}")
             key = {"diode": "diode_60x40", "mosfet": "mosfet_200x120",
-                   "ac": "ac_mosfet_200x120", "batch": "batched_diodes_256",
-                   "serve": "serving"}[name]
+                   "ac": "ac_mosfet_200x120", "transient": "transient_mosfet_200x120",
+                   "batch": "batched_diodes_256", "serve": "serving"}[name]
             line[key] = fn()
             torch.cuda.empty_cache()
     print(json.dumps(line), flush=True)
