@@ -96,6 +96,10 @@ struct CArgs {
   ACScal *acs;
   int parity;
   int first;
+  // red-black tile ordering (tile != 0): k_cb_init / k_cb_s / k_cb_u apply
+  // the red tiles' inverses only, k_cb_black the black tiles' after them
+  // (v[rb_src] -> v[rb_dst])
+  int rb, rb_src, rb_dst;
 };
 
 #define CPROLOGUE                                   \
@@ -618,14 +622,61 @@ __global__ void __launch_bounds__(BS) k_cb_init(CArgs K) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) bb += K.b[c * N + H.g] * K.b[c * N + H.g];
   }
-  capply(K, I.dim, H.valid, H.g, r, z);
-  if (H.valid) cstore(K.v[CPH], N, H.g, z);
+  const bool on = H.valid && !(K.tile && K.rb && tile_black_at(I.dim, H.i, H.j, H.kk));
+  capply(K, I.dim, on, H.g, r, z);
+  if (on) cstore(K.v[CPH], N, H.g, z);
   double in[2] = {rr, bb}, s[2];
   block_sums<2>(in, sh, s);
   if (threadIdx.x == 0) {
     K.part[blockIdx.x] = s[0];
     K.part[(long long)K.nblk + blockIdx.x] = s[1];
   }
+}
+
+// The black tiles of the red-black tile ordering, after a kernel that
+// applied the red ones: z = T^-1 (x - C z_red) on every black tile, x =
+// v[rb_src], z = v[rb_dst], C the (real) stencil couplings of the tile's
+// nodes to the neighbouring tiles, all red (the storage diagonal jwD has no
+// couplings).  Forward block Gauss-Seidel over the two colours, as in the
+// real solver.
+__global__ void __launch_bounds__(BS) k_cb_black(CArgs K) {
+  CPROLOGUE
+  (void)lead;
+  (void)sh;
+  const NodeHood H = hood(I, blockIdx.x);
+  const long long N = K.N;
+  const bool on = H.valid && tile_black_at(I.dim, H.i, H.j, H.kk);
+  const double *src = K.v[K.rb_src];
+  double *dst = K.v[K.rb_dst];
+  double x[6] = {0, 0, 0, 0, 0, 0};
+  if (on) {
+    cload(src, N, H.g, x);
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      if (!H.nb[dir] || !leaves_tile_at(I.dim, H.i, H.j, H.kk, dir)) continue;
+      const long long gn = H.nbg[dir];
+      const double *o = K.A + (long long)(9 + 5 * dir) * N + H.g;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {              // real, imaginary halves
+        const double zp = dst[(3 * h) * N + gn], zn = dst[(3 * h + 1) * N + gn],
+                     zq = dst[(3 * h + 2) * N + gn];
+        x[3 * h] -= o[0] * zp;
+        x[3 * h + 1] -= o[N] * zp + o[2 * N] * zn;
+        x[3 * h + 2] -= o[3 * N] * zp + o[4 * N] * zq;
+      }
+    }
+  }
+  double z[6];
+  capply(K, I.dim, on, H.g, x, z);
+  if (on) cstore(dst, N, H.g, z);
+}
+
+static void launch_cblack(Batch &b, CArgs K, int src, int dst) {
+  if (!K.tile || !K.rb) return;
+  K.rb_src = src;
+  K.rb_dst = dst;
+  hipLaunchKernelGGL(k_cb_black, dim3(b.nblk), dim3(BS), 0, b.stream, K);
+  b.launches++;
 }
 
 // dst = (A + j w D) src; partial sum conj(a) dst at q0, q0 + 1
@@ -744,8 +795,9 @@ __global__ void __launch_bounds__(BS) k_cb_s(CArgs K) {
     }
     cstore(K.v[CS], N, H.g, s6);
   }
-  capply(K, I.dim, H.valid, H.g, s6, z);
-  if (H.valid) cstore(K.v[CSH], N, H.g, z);
+  const bool on = H.valid && !(K.tile && K.rb && tile_black_at(I.dim, H.i, H.j, H.kk));
+  capply(K, I.dim, on, H.g, s6, z);
+  if (on) cstore(K.v[CSH], N, H.g, z);
   if (lead) {
     S.alpha[0] = alpha.re;
     S.alpha[1] = alpha.im;
@@ -819,8 +871,9 @@ __global__ void __launch_bounds__(BS) k_cb_u(CArgs K) {
     cstore(K.v[CR], N, H.g, s6);
     cstore(K.v[CP], N, H.g, pn);
   }
-  capply(K, I.dim, H.valid, H.g, pn, z);
-  if (H.valid) cstore(K.v[CPH], N, H.g, z);
+  const bool on = H.valid && !(K.tile && K.rb && tile_black_at(I.dim, H.i, H.j, H.kk));
+  capply(K, I.dim, on, H.g, pn, z);
+  if (on) cstore(K.v[CPH], N, H.g, z);
   const double a = block_sum(rr, sh);
   if (threadIdx.x == 0) K.part[blockIdx.x] = a;
   if (lead) {
@@ -989,6 +1042,7 @@ CArgs make_cargs(Batch &b, const AcWork &w) {
   K.b = b.bs;
   K.part = w.part;
   K.acs = w.acs;
+  K.rb = b.tile_rb;
   return K;
 }
 
@@ -997,8 +1051,10 @@ void launch_citeration(Batch &b, CArgs K, int parity) {
   K.parity = parity;
   hipLaunchKernelGGL(k_cb_spmv, g, t, 0, b.stream, K, (int)CPH, (int)CV, (int)CRH, 2);
   hipLaunchKernelGGL(k_cb_s, g, t, 0, b.stream, K);
+  launch_cblack(b, K, CS, CSH);
   hipLaunchKernelGGL(k_cb_spmv_t, g, t, 0, b.stream, K);
   hipLaunchKernelGGL(k_cb_u, g, t, 0, b.stream, K);
+  launch_cblack(b, K, CP, CPH);
   b.launches += 4;
 }
 
@@ -1031,6 +1087,7 @@ int complex_solve(Batch &b, AcWork &w, std::vector<int> act, double rtol, int ma
     K.first = sweep == 0 && !warm ? 1 : 0;
     hipLaunchKernelGGL(k_cb_init, dim3(b.nblk), dim3(BS), 0, b.stream, K);
     b.launches += 2;
+    launch_cblack(b, K, CP, CPH);
     // x as this sweep starts (k_cb_init has zeroed it on a cold start)
     TCAD_HIP_CHECK(hipMemcpyAsync(w.xbak, w.v[CX], sizeof(double) * 6 * (size_t)b.ntot,
                                   hipMemcpyDeviceToDevice, b.stream));

@@ -155,14 +155,6 @@ __device__ __forceinline__ void prec3(const T *Di, long long N, long long g, con
 // ---------------------------------------------------------------------------
 constexpr int TU = TILE_U;
 
-// Red-black tile ordering: a tile is black when the sum of its tile
-// coordinates is odd.  Tiles that share a face differ in colour, so every
-// coupling that leaves a tile joins a red and a black one.
-__device__ __forceinline__ bool tile_black_at(int dim, int i, int j, int kk) {
-  if (dim == 3) return ((i / T3X + j / T3Y + kk / T3Z) & 1) != 0;
-  return ((i / TILE_X + j / TILE_Y) & 1) != 0;
-}
-
 // Storage slot of the tile-inverse rows of the node at lane t of workgroup
 // (patch) blk, whose tile has colour `black`: the patch's BS slots hold its
 // red tiles' nodes, then its black tiles' (each in lane order: every x row
@@ -175,20 +167,6 @@ __device__ __forceinline__ long long tile_row_slot(int dim, long long blk, int t
   const int il = t % px, row = t / px;
   const int rank = row * (px / 2) + ((il / tw) >> 1) * tw + il % tw;
   return blk * BS + (black ? BS / 2 : 0) + rank;
-}
-
-// Whether the neighbour of node (i, j, kk) in direction dir (x-, x+, y-,
-// y+, z-, z+) lies in another tile (at the device's edge: where it would).
-__device__ __forceinline__ bool leaves_tile_at(int dim, int i, int j, int kk, int dir) {
-  const int tx = dim == 3 ? T3X : TILE_X, ty = dim == 3 ? T3Y : TILE_Y;
-  switch (dir) {
-    case 0: return i % tx == 0;
-    case 1: return i % tx == tx - 1;
-    case 2: return j % ty == 0;
-    case 3: return j % ty == ty - 1;
-    case 4: return dim == 3 && kk % T3Z == 0;
-    default: return dim == 3 && kk % T3Z == T3Z - 1;
-  }
 }
 
 // The black tiles' couplings to the red tiles, FP32, for the black-tile
