@@ -927,6 +927,11 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--extras", default="diode,mosfet,ac,transient,batch,serve",
                     help="extra workloads (comma-separated; '' for none)")
+    ap.add_argument("--extras-only", action="store_true",
+                    help="run the --extras workloads without the headline (profiling runs); "
+                         "prints one JSON object of them")
+    ap.add_argument("--mosfet-modes", default="bicgstab,bicgstab_mixed,tuned",
+                    help="the MOSFET extra's legs (comma-separated)")
     ap.add_argument("--batch-variants", default="",
                     help="only the batched diodes, with these option sets (comma-separated of "
                          "bicgstab, forcing, quad, tuned, forcing_graph, graph, queue, queue2, queue4)")
@@ -971,6 +976,12 @@ def main():
                 "queue2": (NewtonOptions(), dict(queue=True, queue_depth=2)),
                 "queue4": (NewtonOptions(), dict(queue=True, queue_depth=4))}
         out = bench_batch(modes={k: allm[k] for k in args.batch_variants.split(",")})
+        print(json.dumps(out), flush=True)
+        return
+    if args.extras_only:
+        out = {}
+        for result, key in run_extras(args):
+            out[key] = result
         print(json.dumps(out), flush=True)
         return
 
@@ -1066,8 +1077,18 @@ def main():
                                                     gpu_newton)
     del r
     torch.cuda.empty_cache()
+    for result, key in run_extras(args):
+        line[key] = result
+    print(json.dumps(line), flush=True)
+
+
+def run_extras(args):
+    """The --extras workloads in a fixed order: yields (result, JSON key)."""
+    import torch
     extras = [e for e in args.extras.split(",") if e]
-    for name, fn in (("diode", bench_diode), ("mosfet", bench_mosfet), ("ac", bench_ac),
+    mosfet_modes = tuple(m for m in args.mosfet_modes.split(",") if m)
+    for name, fn in (("diode", bench_diode),
+                     ("mosfet", lambda: bench_mosfet(modes=mosfet_modes)), ("ac", bench_ac),
                      ("transient", bench_transient), ("batch", bench_batch),
                      ("serve", bench_serve)):
         if name in extras:
@@ -1075,9 +1096,8 @@ def main():
             key = {"diode": "diode_60x40", "mosfet": "mosfet_200x120",
                    "ac": "ac_mosfet_200x120", "transient": "transient_mosfet_200x120",
                    "batch": "batched_diodes_256", "serve": "serving"}[name]
-            line[key] = fn()
+            yield fn(), key
             torch.cuda.empty_cache()
-    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":
