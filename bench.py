@@ -790,9 +790,16 @@ def bench_serve(count=16, lanes=2, check=3, budget_gib=32):
     continuous batching in an HBM-budgeted pool with `lanes` lanes.
     First the saturation throughput (every request present at t = 0), then
     the same stream at 30% and 60% of that rate, where the server keeps up;
-    `check` requests re-solved alone must give the same bits."""
+    `check` requests re-solved alone must give the same bits.  The server
+    starts every continued point from the secant prediction and applies
+    quadratic forcing terms (0.1, 1e-3) to the devices whose start was
+    predicted (84 -> 102-106 points/s at saturation against the plain
+    options, the slopes 0.03-0.1 and linear forcing within 4% of each
+    other)."""
     from tcadhip import serve
     from tcadhip.device import NewtonOptions
+    sopts = NewtonOptions(forcing=True, forcing_pow=2, forcing_slope=0.1, forcing_max=1e-3,
+                          forcing_predicted=True)
     cap = serve.nodes_for_budget(budget_gib << 30)
     base = serve.request_stream(count, rate=1.0, seed=20261203)
     pts_total = sum(len(r.spec.sweep) for r in base)
@@ -804,7 +811,7 @@ def bench_serve(count=16, lanes=2, check=3, budget_gib=32):
     def run(rate):
         reqs = serve.request_stream(count, rate=rate, seed=20261203) if rate else \
             [serve.Request(r.rid, r.spec, 0.0, r.kind) for r in base]
-        S = serve.Server(cap, opts=NewtonOptions(), lanes=lanes, predictor=True)
+        S = serve.Server(cap, opts=sopts, lanes=lanes, predictor=True)
         _sync()
         t0 = time.perf_counter()
         served = S.run(reqs, realtime=True)
@@ -831,7 +838,7 @@ def bench_serve(count=16, lanes=2, check=3, budget_gib=32):
         out[f"load_{int(frac * 100)}pct"] = r
     same = True
     for o, r in list(zip(served, reqs))[:check]:
-        a = serve.serve_alone(r.spec, NewtonOptions(), predictor=True)
+        a = serve.serve_alone(r.spec, sopts, predictor=True)
         same = same and a.newton == o.result.newton and all(
             x[k] == y[k] for x, y in zip(a.currents, o.result.currents) for k in x)
     # the plain FP64 path on the same requests: one at a time, each point
@@ -843,6 +850,9 @@ def bench_serve(count=16, lanes=2, check=3, budget_gib=32):
         alone_pts += len(serve.serve_alone(r.spec, NewtonOptions()).currents)
     _sync()
     out["sweep_start"] = "secant predictor"
+    out["options"] = {k: getattr(sopts, k) for k in
+                      ("forcing", "forcing_pow", "forcing_slope", "forcing_max",
+                       "forcing_predicted")}
     out["alone_equals_mix_on_first"] = check
     out["alone_equals_mix"] = bool(same)
     out["plain_fp64_points_per_s_on_those"] = round(alone_pts / (time.perf_counter() - t1), 4)

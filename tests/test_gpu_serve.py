@@ -74,28 +74,8 @@ def test_serving_at_workload_sizes():
     requests match the oracle (Newton +-1, currents to the bar of
     tests/current_bar.py)."""
     serve, NewtonOptions = _mods()
-    from tcadhip import configs
     from current_bar import check, oracle_sweep
-    rng = np.random.default_rng(20261204)
-    arrivals = np.cumsum(rng.exponential(0.05, size=4))
-    specs = [
-        ("diode", configs.diode_2d(50, 40, Na=3e17, Nd=2e16, xj_frac=0.45,
-                                   biases=[-0.4 * (s + 1) for s in range(5)],
-                                   models=dict(doping_mobility=True, srh=True, auger=True,
-                                               bgn=True, surface_mobility=False),
-                                   material={}, name="req0_diode_50x40")),
-        ("mosfet", configs.mosfet_2d(100, 60, Vd=0.5, vg_list=np.linspace(0.0, 1.0, 5),
-                                     name="req1_mosfet_100x60")),
-        ("trigate", configs.trigate_3d(32, 24, 24, Vd=0.7, vg_list=np.linspace(0.0, 0.8, 5),
-                                       name="req2_trigate_32x24x24")),
-        ("diode", configs.diode_2d(60, 40, Na=1e16, Nd=1e18, xj_frac=0.6,
-                                   biases=[0.12 * (s + 1) for s in range(5)],
-                                   models=dict(doping_mobility=True, srh=True, auger=True,
-                                               bgn=True, surface_mobility=False),
-                                   material={}, name="req3_diode_60x40")),
-    ]
-    reqs = [serve.Request(i, sp, float(t), kind) for i, ((kind, sp), t)
-            in enumerate(zip(specs, arrivals))]
+    reqs = _size_stream(serve)
     cap = reqs[2].spec.N + reqs[1].spec.N + 100
     print(f"[serve-sizes] {[r.spec.N for r in reqs]} nodes, pool {cap}", flush=True)
     opts = NewtonOptions()
@@ -118,3 +98,60 @@ def test_serving_at_workload_sizes():
             print(f"  {r.spec.name}: Newton {mix.newton} oracle {oits.tolist()}", flush=True)
             assert np.all(np.abs(np.array(mix.newton) - oits) <= 1)
             check(got, ocur, r.spec.name, say=lambda m: print(m, flush=True))
+
+
+def test_serving_with_predicted_forcing():
+    """The stream of test_serving_at_workload_sizes served as bench.py
+    serves: the secant predictor, and quadratic Newton forcing terms
+    (0.1, 1e-3) on the devices whose start it predicted.  Every request
+    gives the same bits alone (with the same options) and in the mix; the
+    2D requests' currents match the oracle to the bar of
+    tests/current_bar.py (their Newton counts follow the predicted starts
+    and the forcing terms, so no +-1 bar applies)."""
+    serve, NewtonOptions = _mods()
+    from current_bar import check, oracle_sweep
+    reqs = _size_stream(serve)
+    cap = reqs[2].spec.N + reqs[1].spec.N + 100
+    opts = NewtonOptions(forcing=True, forcing_pow=2, forcing_slope=0.1, forcing_max=1e-3,
+                         forcing_predicted=True)
+    S = serve.Server(capacity_nodes=cap, opts=opts, predictor=True)
+    served = S.run(reqs, realtime=False)
+    for r, sv in zip(reqs, served):
+        mix = sv.result
+        assert not mix.failed, r.spec.name
+        alone = serve.serve_alone(r.spec, opts, predictor=True)
+        assert mix.newton == alone.newton, r.spec.name
+        for a, b in zip(mix.currents, alone.currents):
+            assert all(a[k] == b[k] for k in a), (r.spec.name, a, b)
+        if r.spec.dim == 2:
+            oits, ocur = oracle_sweep(r.spec)
+            names = [c.name for c in r.spec.contacts]
+            got = np.array([[c[nm] for nm in names] for c in mix.currents])
+            print(f"  {r.spec.name}: Newton {mix.newton} oracle {oits.tolist()}", flush=True)
+            check(got, ocur, r.spec.name, say=lambda m: print(m, flush=True))
+
+
+def _size_stream(serve):
+    """Four seeded requests at the server's stated sizes (see
+    test_serving_at_workload_sizes)."""
+    from tcadhip import configs
+    rng = np.random.default_rng(20261204)
+    arrivals = np.cumsum(rng.exponential(0.05, size=4))
+    specs = [
+        ("diode", configs.diode_2d(50, 40, Na=3e17, Nd=2e16, xj_frac=0.45,
+                                   biases=[-0.4 * (s + 1) for s in range(5)],
+                                   models=dict(doping_mobility=True, srh=True, auger=True,
+                                               bgn=True, surface_mobility=False),
+                                   material={}, name="req0_diode_50x40")),
+        ("mosfet", configs.mosfet_2d(100, 60, Vd=0.5, vg_list=np.linspace(0.0, 1.0, 5),
+                                     name="req1_mosfet_100x60")),
+        ("trigate", configs.trigate_3d(32, 24, 24, Vd=0.7, vg_list=np.linspace(0.0, 0.8, 5),
+                                       name="req2_trigate_32x24x24")),
+        ("diode", configs.diode_2d(60, 40, Na=1e16, Nd=1e18, xj_frac=0.6,
+                                   biases=[0.12 * (s + 1) for s in range(5)],
+                                   models=dict(doping_mobility=True, srh=True, auger=True,
+                                               bgn=True, surface_mobility=False),
+                                   material={}, name="req3_diode_60x40")),
+    ]
+    return [serve.Request(i, sp, float(t), kind) for i, ((kind, sp), t)
+            in enumerate(zip(specs, arrivals))]
