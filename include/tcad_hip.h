@@ -60,10 +60,11 @@ extern "C" {
 #define TCAD_LIN_FORCING 0x100
 /* Flag OR'd into lin_mode (tcad_solve_bias, tcad_solve_linear): precondition
  * every device with the reference's node block-Jacobi (3x3 block per node,
- * linsolve.py:363-427).  Without it, every device uses tile block-Jacobi:
- * the exact inverse of the matrix restricted to each 16-node tile (4 x 4
- * in 2D, 2 x 1 x 8 in 3D; 48 unknowns), stored as FP16 rows with an FP32
- * scale per row. */
+ * linsolve.py:363-427).  Without it, every device uses the exact inverse
+ * of the matrix restricted to each 16-node tile (4 x 4 in 2D, 2 x 1 x 8 in
+ * 3D; 48 unknowns), stored as FP16 rows with an FP32 scale per row, and
+ * applied in red-black order over the tile checkerboard (or as tile
+ * block-Jacobi: TCAD_TUNE_TILE_RB). */
 #define TCAD_PRECOND_NODE 0x200
 
 /* Per-device outcome of a Newton solve (h_status). */
